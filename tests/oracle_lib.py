@@ -1,0 +1,154 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle46.so) and to the
+reference's own Fortran built in place (oracle/_ref/libref46.so).
+
+Test infrastructure only: the product never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle46.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref46.so")
+
+RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OracleClaw(C.Structure):
+    _fields_ = [("solver", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
+                ("mwaves", C.c_int), ("method", C.c_int * 7), ("mthlim", C.c_int * 10),
+                ("grav", C.c_double), ("gamma", C.c_double)]
+
+
+class OracleDomain(C.Structure):
+    _fields_ = [("npatch", C.c_int), ("level", C.POINTER(C.c_int)),
+                ("gx", C.POINTER(C.c_int)), ("gy", C.POINTER(C.c_int)),
+                ("nbx", C.c_int), ("nby", C.c_int), ("periodic_x", C.c_int),
+                ("periodic_y", C.c_int), ("mx", C.c_int), ("my", C.c_int),
+                ("mbc", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
+                ("mthbc", C.c_int * 4), ("ax", C.c_double), ("bx", C.c_double),
+                ("ay", C.c_double), ("by", C.c_double)]
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle library missing: run `make -C oracle`")
+        lib = C.CDLL(ORACLE_SO)
+        lib.oracle_step2_patch.argtypes = [C.POINTER(OracleClaw), C.c_int, C.c_int, C.c_int,
+                                           _dp, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                           C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
+        lib.oracle_step2_patch.restype = C.c_int
+        lib.oracle_ghost_update.argtypes = [C.POINTER(OracleDomain), _dp]
+        lib.oracle_update_all.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw), _dp,
+                                          C.c_void_p, C.c_double]
+        lib.oracle_update_all.restype = C.c_double
+        lib.oracle_advance.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw), _dp,
+                                       C.c_void_p, C.c_double, C.c_int]
+        lib.oracle_advance.restype = C.c_double
+        lib.oracle_mass.argtypes = [C.POINTER(OracleDomain), _dp, _dp]
+        lib.oracle_bc2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip]
+        lib.oracle_inlinelimiter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             _dp, _dp, _ip]
+        for name in ("oracle_copy_face", "oracle_copy_corner", "oracle_average_face",
+                     "oracle_average_corner"):
+            getattr(lib, name).argtypes = ([C.c_int] * 4 + [_dp, _dp] +
+                                           [C.c_int] * (4 if "face" in name else 3))
+        lib.oracle_interpolate_face.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 4
+        lib.oracle_interpolate_corner.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 3
+        _oracle = lib
+    return _oracle
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF_SO)
+        lib.ref46_step2.argtypes = [C.c_int] * 7 + [_ip, _ip, _dp, C.c_void_p] + \
+            [C.c_double] * 5 + [C.POINTER(C.c_double), _dp, _dp, _dp, _dp]
+        lib.ref46_step2.restype = C.c_int
+        lib.ref46_bc2.argtypes = [C.c_int] * 4 + [_dp, _ip]
+        lib.ref46_inlinelimiter.argtypes = [C.c_int] * 5 + [_dp, _dp, _ip]
+        _ref = lib
+    return _ref
+
+
+def make_claw(solver, meqn, maux, mwaves, order=(2, 2), mthlim=None, grav=1.0, gamma=1.4):
+    c = OracleClaw()
+    c.solver, c.meqn, c.maux, c.mwaves = solver, meqn, maux, mwaves
+    method = [0, order[0], order[1], 0, 0, 0, maux]
+    for k in range(7):
+        c.method[k] = method[k]
+    lim = list(mthlim) if mthlim is not None else [4] * mwaves
+    for k in range(10):
+        c.mthlim[k] = lim[k] if k < len(lim) else 0
+    c.grav, c.gamma = grav, gamma
+    return c
+
+
+def step2_oracle(claw, q, aux, dx, dy, dt, want_fluxes=False):
+    """q: (meqn, my+2mbc, mx+2mbc) float64, updated copy returned with cfl."""
+    meqn, ny, nx = q.shape
+    mbc = 2
+    mx, my = nx - 2 * mbc, ny - 2 * mbc
+    qq = np.ascontiguousarray(q, dtype=np.float64).copy()
+    cfl = C.c_double(0.0)
+    fl = [np.zeros_like(qq) for _ in range(4)] if want_fluxes else [None] * 4
+    auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    rc = oracle().oracle_step2_patch(C.byref(claw), mx, my, mbc, qq,
+                                     None if auxp is None else auxp.ctypes.data,
+                                     dx, dy, dt, C.byref(cfl),
+                                     *[None if f is None else f.ctypes.data for f in fl])
+    assert rc == 0
+    if want_fluxes:
+        return qq, cfl.value, fl
+    return qq, cfl.value
+
+
+def step2_ref(claw, q, aux, dx, dy, dt, want_fluxes=False):
+    meqn, ny, nx = q.shape
+    mbc = 2
+    mx, my = nx - 2 * mbc, ny - 2 * mbc
+    qq = np.ascontiguousarray(q, dtype=np.float64).copy()
+    cfl = C.c_double(0.0)
+    fm, fp, gm, gp = (np.zeros_like(qq) for _ in range(4))
+    method = np.array(list(claw.method), dtype=np.int32)
+    mthlim = np.array(list(claw.mthlim), dtype=np.int32)
+    auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    rc = ref().ref46_step2(claw.solver, mx, my, mbc, meqn, claw.maux, claw.mwaves, method,
+                           mthlim, qq, None if auxp is None else auxp.ctypes.data,
+                           dx, dy, dt, claw.grav, claw.gamma, C.byref(cfl), fm, fp, gm, gp)
+    assert rc == 0
+    if want_fluxes:
+        return qq, cfl.value, [fm, fp, gm, gp]
+    return qq, cfl.value
+
+
+def domain_struct(level, gx, gy, nbx, nby, periodic, mx, my, mbc, meqn, maux, mthbc, extent):
+    """Build an OracleDomain; keeps the int arrays alive on the returned object."""
+    d = OracleDomain()
+    arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (level, gx, gy)]
+    d.npatch = len(arrs[0])
+    d.level, d.gx, d.gy = (a.ctypes.data_as(C.POINTER(C.c_int)) for a in arrs)
+    d.nbx, d.nby = nbx, nby
+    d.periodic_x, d.periodic_y = int(periodic[0]), int(periodic[1])
+    d.mx, d.my, d.mbc, d.meqn, d.maux = mx, my, mbc, meqn, maux
+    for k in range(4):
+        d.mthbc[k] = mthbc[k]
+    d.ax, d.bx, d.ay, d.by = extent
+    d._keep = arrs
+    return d
