@@ -1,0 +1,293 @@
+"""forestclaw_amd: MI355X-native batched patch update for ForestClaw.
+
+Python access to the in-tree library libforestclaw_amd.so (HIP kernels for
+gfx950 + the C host layer).  The library is required: importing the package
+works without it, but every object that computes raises if it is missing --
+there is no CPU fallback.
+
+Main objects
+  Forest  -- leaves of a 2:1-balanced forest of quadtrees over a brick of
+             blocks, and the device ghost-fill program for it
+  Solver  -- one resident solution on the GPU (include/hipclaw.h)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libforestclaw_amd.so")
+
+RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
+FP64, FP32_BULK = 0, 1
+SOLVER_SHAPES = {RP_ADVECTION: (1, 1, 2), RP_SHALLOW: (3, 3, 0), RP_EULER: (4, 4, 0)}
+
+_lib = None
+
+
+class HipclawConfig(C.Structure):
+    _fields_ = [("solver", C.c_int), ("precision", C.c_int), ("mx", C.c_int),
+                ("my", C.c_int), ("mbc", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
+                ("mwaves", C.c_int), ("order", C.c_int * 2), ("mthlim", C.c_int * 4),
+                ("mthbc", C.c_int * 4), ("grav", C.c_double), ("gamma", C.c_double),
+                ("tile", C.c_int)]
+
+
+def _dp():
+    return np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def _ip():
+    return np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libforestclaw_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("forestclaw_amd: %s is missing; build it with "
+                           "`python -m forestclaw_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "hipclaw_last_error": ([], C.c_char_p),
+        "hipclaw_initialize_device": ([i, C.c_char_p, C.c_size_t], i),
+        "hipclaw_create": ([C.POINTER(HipclawConfig), i, C.POINTER(vp)], i),
+        "hipclaw_destroy": ([vp], None),
+        "hipclaw_tile": ([vp], i),
+        "hipclaw_set_geometry": ([vp, i, _dp(), _dp()], i),
+        "hipclaw_set_q": ([vp, _dp(), i, i], i),
+        "hipclaw_get_q": ([vp, _dp(), i, i], i),
+        "hipclaw_set_aux": ([vp, _dp(), i, i], i),
+        "hipclaw_set_ghost_program": ([vp, _ip(), i, _ip()], i),
+        "hipclaw_ghost_fill": ([vp], i),
+        "hipclaw_step_batch": ([vp, _ip(), i, d, d, C.POINTER(d)], i),
+        "hipclaw_advance": ([vp, d, i, C.POINTER(d)], i),
+        "hipclaw_advance_async": ([vp, d, i], i),
+        "hipclaw_sync": ([vp], i),
+        "hipclaw_capture_graph": ([vp, d, i], i),
+        "hipclaw_replay_graph": ([vp], i),
+        "hipclaw_mass": ([vp, _dp(), _dp()], i),
+        "hipclaw_timing": ([vp, i, C.POINTER(d), C.POINTER(d)], i),
+        "fclaw2d_hip_forest_new": ([i, i, i, i, i], vp),
+        "fclaw2d_hip_forest_from_leaves": ([i, i, i, i, i, _ip(), _ip(), _ip(), i], vp),
+        "fclaw2d_hip_forest_destroy": ([vp], None),
+        "fclaw2d_hip_forest_num_patches": ([vp], i),
+        "fclaw2d_hip_forest_min_level": ([vp], i),
+        "fclaw2d_hip_forest_max_level": ([vp], i),
+        "fclaw2d_hip_forest_leaves": ([vp, _ip(), _ip(), _ip()], None),
+        "fclaw2d_hip_forest_refine": ([vp, _ip(), i], i),
+        "fclaw2d_hip_forest_ghost_program": ([vp, i, i, i, _ip(), vp, i, _ip()], i),
+        "fclaw2d_hip_forest_face_relation": ([vp, i, i, _ip()], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+class HipclawError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().hipclaw_last_error()
+        raise HipclawError("%s failed (code %d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def initialize_device(device=0):
+    """Select the GPU (fc2d_cudaclaw_initialize_GPUs); returns its name."""
+    buf = C.create_string_buffer(256)
+    _check(lib().hipclaw_initialize_device(device, buf, 256), "hipclaw_initialize_device")
+    return buf.value.decode()
+
+
+class Forest:
+    """Leaves of a balanced forest of quadtrees over an nbx x nby brick."""
+
+    def __init__(self, nbx=1, nby=1, periodic=(False, False), minlevel=0, leaves=None,
+                 balance=True):
+        L = lib()
+        self.nbx, self.nby = nbx, nby
+        self.periodic = (bool(periodic[0]), bool(periodic[1]))
+        if leaves is None:
+            self._h = L.fclaw2d_hip_forest_new(nbx, nby, int(periodic[0]), int(periodic[1]),
+                                               minlevel)
+        else:
+            lv, gx, gy = (np.ascontiguousarray(a, dtype=np.int32) for a in leaves)
+            self._h = L.fclaw2d_hip_forest_from_leaves(nbx, nby, int(periodic[0]),
+                                                       int(periodic[1]), len(lv), lv, gx, gy,
+                                                       int(balance))
+        if not self._h:
+            raise ValueError("invalid forest")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fclaw2d_hip_forest_destroy(self._h)
+            self._h = None
+
+    @property
+    def npatch(self):
+        return lib().fclaw2d_hip_forest_num_patches(self._h)
+
+    @property
+    def minlevel(self):
+        return lib().fclaw2d_hip_forest_min_level(self._h)
+
+    @property
+    def maxlevel(self):
+        return lib().fclaw2d_hip_forest_max_level(self._h)
+
+    def leaves(self):
+        n = self.npatch
+        lv, gx, gy = (np.zeros(n, np.int32) for _ in range(3))
+        lib().fclaw2d_hip_forest_leaves(self._h, lv, gx, gy)
+        return lv, gx, gy
+
+    def refine(self, flags, maxlevel):
+        flags = np.ascontiguousarray(flags, dtype=np.int32)
+        assert len(flags) == self.npatch
+        n = lib().fclaw2d_hip_forest_refine(self._h, flags, maxlevel)
+        if n < 0:
+            raise RuntimeError("forest refinement failed")
+        return n
+
+    def ghost_program(self, mx, my, mbc, mthbc):
+        mth = np.ascontiguousarray(mthbc, dtype=np.int32)
+        ps = np.zeros(7, np.int32)
+        n = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth, None, 0, ps)
+        if n < 0:
+            raise RuntimeError("ghost program failed")
+        ops = np.zeros((max(n, 1), 8), np.int32)
+        n2 = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth,
+                                                    ops.ctypes.data, n, ps)
+        assert n2 == n
+        return ops[:n], ps
+
+    def geometry(self, extent, mx, my):
+        """Per-patch xlower, ylower, dx, dy for physical extent (ax, bx, ay, by)."""
+        ax, bx, ay, by = extent
+        lv, gx, gy = self.leaves()
+        wx, wy = (bx - ax) / self.nbx, (by - ay) / self.nby
+        scale = np.ldexp(1.0, -lv.astype(np.int64))
+        xlower = ax + gx * wx * scale
+        ylower = ay + gy * wy * scale
+        return xlower, ylower, wx * scale / mx, wy * scale / my
+
+
+class Solver:
+    """A solution resident on the GPU (include/hipclaw.h)."""
+
+    def __init__(self, solver, mx, my, mbc=2, precision=FP64, order=(2, 2), mthlim=None,
+                 mthbc=(1, 1, 1, 1), grav=1.0, gamma=1.4, tile=0, max_patches=1):
+        meqn, mwaves, maux = SOLVER_SHAPES[solver]
+        cfg = HipclawConfig()
+        cfg.solver, cfg.precision = solver, precision
+        cfg.mx, cfg.my, cfg.mbc = mx, my, mbc
+        cfg.meqn, cfg.maux, cfg.mwaves = meqn, maux, mwaves
+        cfg.order[0], cfg.order[1] = order
+        lim = list(mthlim) if mthlim is not None else [4] * mwaves
+        for k in range(4):
+            cfg.mthlim[k] = lim[k] if k < len(lim) else 0
+            cfg.mthbc[k] = mthbc[k]
+        cfg.grav, cfg.gamma, cfg.tile = grav, gamma, tile
+        self.cfg = cfg
+        self.meqn, self.maux, self.mwaves = meqn, maux, mwaves
+        self.mx, self.my, self.mbc = mx, my, mbc
+        self.mthbc = tuple(mthbc)
+        self.max_patches = max_patches
+        self.npatch = 0
+        h = C.c_void_p()
+        _check(lib().hipclaw_create(C.byref(cfg), max_patches, C.byref(h)), "hipclaw_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hipclaw_destroy(self._h)
+            self._h = None
+
+    @property
+    def tile(self):
+        return lib().hipclaw_tile(self._h)
+
+    @property
+    def patch_shape(self):
+        return (self.meqn, self.my + 2 * self.mbc, self.mx + 2 * self.mbc)
+
+    def set_forest(self, forest, extent):
+        """Geometry and ghost-fill program of `forest` over physical `extent`."""
+        _, _, dx, dy = forest.geometry(extent, self.mx, self.my)
+        self.set_geometry(dx, dy)
+        ops, ps = forest.ghost_program(self.mx, self.my, self.mbc, self.mthbc)
+        self.set_ghost_program(ops, ps)
+
+    def set_geometry(self, dx, dy):
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        dy = np.ascontiguousarray(dy, dtype=np.float64)
+        _check(lib().hipclaw_set_geometry(self._h, len(dx), dx, dy), "hipclaw_set_geometry")
+        self.npatch = len(dx)
+
+    def set_ghost_program(self, ops, phase_start):
+        ops = np.ascontiguousarray(ops, dtype=np.int32).reshape(-1)
+        ps = np.ascontiguousarray(phase_start, dtype=np.int32)
+        _check(lib().hipclaw_set_ghost_program(self._h, ops, len(ops) // 8, ps),
+               "hipclaw_set_ghost_program")
+
+    def set_q(self, q, first=0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        _check(lib().hipclaw_set_q(self._h, q.reshape(-1), first, q.shape[0]), "hipclaw_set_q")
+
+    def get_q(self, first=0, count=None):
+        count = self.npatch - first if count is None else count
+        out = np.zeros((count,) + self.patch_shape, np.float64)
+        _check(lib().hipclaw_get_q(self._h, out.reshape(-1), first, count), "hipclaw_get_q")
+        return out
+
+    def set_aux(self, aux, first=0):
+        aux = np.ascontiguousarray(aux, dtype=np.float64)
+        _check(lib().hipclaw_set_aux(self._h, aux.reshape(-1), first, aux.shape[0]),
+               "hipclaw_set_aux")
+
+    def ghost_fill(self):
+        _check(lib().hipclaw_ghost_fill(self._h), "hipclaw_ghost_fill")
+
+    def step_batch(self, slots, dt, t=0.0):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        cfl = C.c_double(0.0)
+        _check(lib().hipclaw_step_batch(self._h, slots, len(slots), t, dt, C.byref(cfl)),
+               "hipclaw_step_batch")
+        return cfl.value
+
+    def advance(self, dt, nsteps=1):
+        cfl = C.c_double(0.0)
+        _check(lib().hipclaw_advance(self._h, dt, nsteps, C.byref(cfl)), "hipclaw_advance")
+        return cfl.value
+
+    def advance_async(self, dt, nsteps=1):
+        _check(lib().hipclaw_advance_async(self._h, dt, nsteps), "hipclaw_advance_async")
+
+    def sync(self):
+        _check(lib().hipclaw_sync(self._h), "hipclaw_sync")
+
+    def capture_graph(self, dt, nsteps):
+        _check(lib().hipclaw_capture_graph(self._h, dt, nsteps), "hipclaw_capture_graph")
+
+    def replay_graph(self):
+        _check(lib().hipclaw_replay_graph(self._h), "hipclaw_replay_graph")
+
+    def timing(self, nsteps):
+        tot, ker = C.c_double(0.0), C.c_double(0.0)
+        _check(lib().hipclaw_timing(self._h, nsteps, C.byref(tot), C.byref(ker)),
+               "hipclaw_timing")
+        return tot.value, ker.value
+
+    def mass(self):
+        hi = np.zeros(self.meqn)
+        lo = np.zeros(self.meqn)
+        _check(lib().hipclaw_mass(self._h, hi, lo), "hipclaw_mass")
+        return hi, lo

@@ -1,0 +1,541 @@
+/*
+ * Host forest and ghost-fill program builder (declared in
+ * include/fclaw2d_hip_forest.h).
+ *
+ * Neighbour relations follow p4est's semantics as ForestClaw uses them:
+ *   faces   : same size, half size (two neighbours) or double size
+ *             (src/fclaw2d_face_neighbors.c:63-190)
+ *   corners : a corner neighbour exists unless the corner is a hanging node,
+ *             i.e. lies inside a coarser patch's face
+ *             (src/fclaw2d_corner_neighbors.c:95-330)
+ * Every block of the brick has the same orientation, so all index transforms
+ * (src/forestclaw2d.c:608-977, 1128-1380) reduce to translations.
+ */
+#include "../../include/fclaw2d_hip_forest.h"
+
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct fclaw2d_hip_forest {
+    int nbx, nby, per_x, per_y;
+    int n, cap;
+    int *level, *gx, *gy;
+    /* open-addressing map (level, gx, gy) -> leaf */
+    size_t hcap;
+    uint64_t *hkey;
+    int *hval;
+};
+
+/* ---------------------------------------------------------------- hashing */
+
+static uint64_t fkey(int level, int64_t gx, int64_t gy)
+{
+    return ((uint64_t) level << 58) | ((uint64_t) gx << 29) | (uint64_t) gy;
+}
+
+static size_t fslot(uint64_t k, size_t cap)
+{
+    return (size_t) ((k * 0x9E3779B97F4A7C15ull) >> 17) & (cap - 1);
+}
+
+static int rehash(fclaw2d_hip_forest_t *f)
+{
+    size_t cap = 64, p;
+    int i;
+    while (cap < (size_t) f->n * 3) cap <<= 1;
+    if (cap != f->hcap) {
+        free(f->hkey);
+        free(f->hval);
+        f->hkey = (uint64_t *) malloc(cap * sizeof(uint64_t));
+        f->hval = (int *) malloc(cap * sizeof(int));
+        if (!f->hkey || !f->hval) return -1;
+        f->hcap = cap;
+    }
+    for (p = 0; p < cap; p++) f->hval[p] = -1;
+    for (i = 0; i < f->n; i++) {
+        uint64_t k = fkey(f->level[i], f->gx[i], f->gy[i]);
+        p = fslot(k, cap);
+        while (f->hval[p] >= 0) {
+            if (f->hkey[p] == k) return -1; /* duplicate leaf */
+            p = (p + 1) & (cap - 1);
+        }
+        f->hkey[p] = k;
+        f->hval[p] = i;
+    }
+    return 0;
+}
+
+static int lookup(const fclaw2d_hip_forest_t *f, int level, int64_t gx, int64_t gy)
+{
+    uint64_t k;
+    size_t p;
+    if (level < 0 || gx < 0 || gy < 0) return -1;
+    k = fkey(level, gx, gy);
+    p = fslot(k, f->hcap);
+    while (f->hval[p] >= 0) {
+        if (f->hkey[p] == k) return f->hval[p];
+        p = (p + 1) & (f->hcap - 1);
+    }
+    return -1;
+}
+
+/* Map a level-L patch coordinate into the brick; 0 if it falls outside a
+ * non-periodic side. */
+static int wrap(const fclaw2d_hip_forest_t *f, int L, int64_t *x, int64_t *y)
+{
+    const int64_t nx = (int64_t) f->nbx << L, ny = (int64_t) f->nby << L;
+    if (*x < 0 || *x >= nx) {
+        if (!f->per_x) return 0;
+        *x = ((*x % nx) + nx) % nx;
+    }
+    if (*y < 0 || *y >= ny) {
+        if (!f->per_y) return 0;
+        *y = ((*y % ny) + ny) % ny;
+    }
+    return 1;
+}
+
+/* Leaf covering level-L cell (x, y) at level <= L, or -1 if that region is
+ * refined beyond L (or outside the domain). */
+static int covering(const fclaw2d_hip_forest_t *f, int L, int64_t x, int64_t y)
+{
+    int l;
+    if (!wrap(f, L, &x, &y)) return -1;
+    for (l = L; l >= 0; l--) {
+        int q = lookup(f, l, x >> (L - l), y >> (L - l));
+        if (q >= 0) return q;
+    }
+    return -1;
+}
+
+/* ---------------------------------------------------------------- ordering */
+
+static uint64_t spread(uint32_t v)
+{
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+typedef struct { uint64_t block, morton; int level, gx, gy; } leaf_key_t;
+
+static int cmp_leaf(const void *a, const void *b)
+{
+    const leaf_key_t *x = (const leaf_key_t *) a, *y = (const leaf_key_t *) b;
+    if (x->block != y->block) return x->block < y->block ? -1 : 1;
+    if (x->morton != y->morton) return x->morton < y->morton ? -1 : 1;
+    return x->level - y->level;
+}
+
+static int sort_leaves(fclaw2d_hip_forest_t *f)
+{
+    leaf_key_t *k = (leaf_key_t *) malloc((size_t) f->n * sizeof(leaf_key_t));
+    int i;
+    if (!k) return -1;
+    for (i = 0; i < f->n; i++) {
+        const int L = f->level[i];
+        const uint32_t bx = (uint32_t) (f->gx[i] >> L), by = (uint32_t) (f->gy[i] >> L);
+        const uint32_t lx = (uint32_t) (f->gx[i] - ((int64_t) bx << L));
+        const uint32_t ly = (uint32_t) (f->gy[i] - ((int64_t) by << L));
+        k[i].block = (uint64_t) by * f->nbx + bx;
+        k[i].morton = spread(lx << (28 - L)) | (spread(ly << (28 - L)) << 1);
+        k[i].level = L; k[i].gx = f->gx[i]; k[i].gy = f->gy[i];
+    }
+    qsort(k, (size_t) f->n, sizeof(leaf_key_t), cmp_leaf);
+    for (i = 0; i < f->n; i++) { f->level[i] = k[i].level; f->gx[i] = k[i].gx; f->gy[i] = k[i].gy; }
+    free(k);
+    return rehash(f);
+}
+
+static int reserve(fclaw2d_hip_forest_t *f, int n)
+{
+    if (n <= f->cap) return 0;
+    {
+        int cap = f->cap ? f->cap : 64;
+        while (cap < n) cap *= 2;
+        int *l = (int *) realloc(f->level, (size_t) cap * sizeof(int));
+        if (!l) return -1;
+        f->level = l;
+        l = (int *) realloc(f->gx, (size_t) cap * sizeof(int));
+        if (!l) return -1;
+        f->gx = l;
+        l = (int *) realloc(f->gy, (size_t) cap * sizeof(int));
+        if (!l) return -1;
+        f->gy = l;
+        f->cap = cap;
+    }
+    return 0;
+}
+
+/* Replace every marked leaf by its four children. */
+static int split_marked(fclaw2d_hip_forest_t *f, const char *mark)
+{
+    int i, add = 0, n0 = f->n;
+    for (i = 0; i < n0; i++) add += mark[i] ? 3 : 0;
+    if (reserve(f, n0 + add)) return -1;
+    for (i = 0; i < n0; i++) {
+        if (!mark[i]) continue;
+        const int L = f->level[i] + 1, x = 2 * f->gx[i], y = 2 * f->gy[i];
+        f->level[i] = L; f->gx[i] = x; f->gy[i] = y;
+        f->level[f->n] = L; f->gx[f->n] = x + 1; f->gy[f->n] = y;     f->n++;
+        f->level[f->n] = L; f->gx[f->n] = x;     f->gy[f->n] = y + 1; f->n++;
+        f->level[f->n] = L; f->gx[f->n] = x + 1; f->gy[f->n] = y + 1; f->n++;
+    }
+    return 0;
+}
+
+static int balance(fclaw2d_hip_forest_t *f)
+{
+    static const int dx8[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    static const int dy8[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    for (;;) {
+        char *mark;
+        int i, d, any = 0;
+        if (rehash(f)) return -1;
+        mark = (char *) calloc((size_t) f->n, 1);
+        if (!mark) return -1;
+        for (i = 0; i < f->n; i++) {
+            const int L = f->level[i];
+            if (L < 2) continue;
+            for (d = 0; d < 8; d++) {
+                const int q = covering(f, L, (int64_t) f->gx[i] + dx8[d], (int64_t) f->gy[i] + dy8[d]);
+                if (q >= 0 && f->level[q] < L - 1 && !mark[q]) { mark[q] = 1; any = 1; }
+            }
+        }
+        if (any && split_marked(f, mark)) { free(mark); return -1; }
+        free(mark);
+        if (!any) break;
+    }
+    return sort_leaves(f);
+}
+
+/* ---------------------------------------------------------------- public */
+
+static fclaw2d_hip_forest_t *alloc_forest(int nbx, int nby, int px, int py)
+{
+    fclaw2d_hip_forest_t *f;
+    if (nbx <= 0 || nby <= 0) return NULL;
+    f = (fclaw2d_hip_forest_t *) calloc(1, sizeof(*f));
+    if (!f) return NULL;
+    f->nbx = nbx; f->nby = nby; f->per_x = px != 0; f->per_y = py != 0;
+    return f;
+}
+
+fclaw2d_hip_forest_t *fclaw2d_hip_forest_new(int nbx, int nby, int periodic_x, int periodic_y,
+                                             int minlevel)
+{
+    fclaw2d_hip_forest_t *f;
+    int x, y;
+    if (minlevel < 0 || minlevel > 20) return NULL;
+    f = alloc_forest(nbx, nby, periodic_x, periodic_y);
+    if (!f) return NULL;
+    if (reserve(f, (nbx * nby) << (2 * minlevel))) { fclaw2d_hip_forest_destroy(f); return NULL; }
+    for (y = 0; y < (nby << minlevel); y++)
+        for (x = 0; x < (nbx << minlevel); x++) {
+            f->level[f->n] = minlevel; f->gx[f->n] = x; f->gy[f->n] = y; f->n++;
+        }
+    if (sort_leaves(f)) { fclaw2d_hip_forest_destroy(f); return NULL; }
+    return f;
+}
+
+fclaw2d_hip_forest_t *fclaw2d_hip_forest_from_leaves(int nbx, int nby, int periodic_x,
+                                                     int periodic_y, int n, const int *level,
+                                                     const int *gx, const int *gy, int bal)
+{
+    fclaw2d_hip_forest_t *f = alloc_forest(nbx, nby, periodic_x, periodic_y);
+    double area = 0.0;
+    int i;
+    if (!f || n <= 0 || reserve(f, n)) { fclaw2d_hip_forest_destroy(f); return NULL; }
+    for (i = 0; i < n; i++) {
+        if (level[i] < 0 || level[i] > 28 || gx[i] < 0 || gy[i] < 0 ||
+            gx[i] >= (nbx << level[i]) || gy[i] >= (nby << level[i])) {
+            fclaw2d_hip_forest_destroy(f);
+            return NULL;
+        }
+        f->level[i] = level[i]; f->gx[i] = gx[i]; f->gy[i] = gy[i];
+        area += 1.0 / (double) (1ull << (2 * level[i]));
+    }
+    f->n = n;
+    /* leaves must be distinct and cover the brick exactly */
+    if (area != (double) (nbx * nby) || rehash(f)) { fclaw2d_hip_forest_destroy(f); return NULL; }
+    for (i = 0; i < n; i++) {
+        int l;
+        for (l = level[i] - 1; l >= 0; l--)
+            if (lookup(f, l, gx[i] >> (level[i] - l), gy[i] >> (level[i] - l)) >= 0) {
+                fclaw2d_hip_forest_destroy(f);
+                return NULL;
+            }
+    }
+    if ((bal ? balance(f) : sort_leaves(f)) != 0) { fclaw2d_hip_forest_destroy(f); return NULL; }
+    return f;
+}
+
+void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
+{
+    if (!f) return;
+    free(f->level); free(f->gx); free(f->gy); free(f->hkey); free(f->hval);
+    free(f);
+}
+
+int fclaw2d_hip_forest_num_patches(const fclaw2d_hip_forest_t *f) { return f ? f->n : 0; }
+
+int fclaw2d_hip_forest_min_level(const fclaw2d_hip_forest_t *f)
+{
+    int i, m = 1 << 30;
+    for (i = 0; i < f->n; i++) if (f->level[i] < m) m = f->level[i];
+    return m;
+}
+
+int fclaw2d_hip_forest_max_level(const fclaw2d_hip_forest_t *f)
+{
+    int i, m = -1;
+    for (i = 0; i < f->n; i++) if (f->level[i] > m) m = f->level[i];
+    return m;
+}
+
+void fclaw2d_hip_forest_leaves(const fclaw2d_hip_forest_t *f, int *level, int *gx, int *gy)
+{
+    memcpy(level, f->level, (size_t) f->n * sizeof(int));
+    memcpy(gx, f->gx, (size_t) f->n * sizeof(int));
+    memcpy(gy, f->gy, (size_t) f->n * sizeof(int));
+}
+
+int fclaw2d_hip_forest_refine(fclaw2d_hip_forest_t *f, const int *flags, int maxlevel)
+{
+    char *mark;
+    int i, rc;
+    if (!f || !flags) return -1;
+    mark = (char *) calloc((size_t) f->n, 1);
+    if (!mark) return -1;
+    for (i = 0; i < f->n; i++) mark[i] = flags[i] != 0 && f->level[i] < maxlevel;
+    rc = split_marked(f, mark);
+    free(mark);
+    if (rc || balance(f)) return -1;
+    return f->n;
+}
+
+/* ---------------------------------------------------------------- neighbours */
+
+enum { REL_BOUNDARY = 0, REL_SAME = 1, REL_HALF = 2, REL_DOUBLE = 3, REL_NONE = 4 };
+
+typedef struct {
+    int rel;
+    int nb[2];
+    int ti[2], tj[2];
+} nbinfo_t;
+
+static const int fdx[4] = {-1, 1, 0, 0}, fdy[4] = {0, 0, -1, 1};
+static const int cdx[4] = {-1, 1, -1, 1}, cdy[4] = {-1, -1, 1, 1};
+
+static void physical_faces(const fclaw2d_hip_forest_t *f, int p, int phys[4])
+{
+    const int L = f->level[p];
+    phys[0] = !f->per_x && f->gx[p] == 0;
+    phys[1] = !f->per_x && f->gx[p] == (f->nbx << L) - 1;
+    phys[2] = !f->per_y && f->gy[p] == 0;
+    phys[3] = !f->per_y && f->gy[p] == (f->nby << L) - 1;
+}
+
+static void face_rel(const fclaw2d_hip_forest_t *f, int p, int iface, int mx, int my, nbinfo_t *r)
+{
+    const int L = f->level[p];
+    const int64_t x = (int64_t) f->gx[p] + fdx[iface], y = (int64_t) f->gy[p] + fdy[iface];
+    int64_t wx = x, wy = y;
+    int q, k;
+    r->rel = REL_BOUNDARY;
+    if (!wrap(f, L, &wx, &wy)) return;
+    if ((q = lookup(f, L, wx, wy)) >= 0) {
+        r->rel = REL_SAME; r->nb[0] = q;
+        r->ti[0] = -fdx[iface] * mx; r->tj[0] = -fdy[iface] * my;
+        return;
+    }
+    if (L > 0 && (q = lookup(f, L - 1, wx >> 1, wy >> 1)) >= 0) {
+        r->rel = REL_DOUBLE; r->nb[0] = q;
+        return;
+    }
+    r->rel = REL_HALF;
+    for (k = 0; k < 2; k++) {
+        int64_t cx, cy;  /* unwrapped child coordinates adjacent to this face */
+        if (iface == 0) { cx = 2 * x + 1; cy = 2 * (int64_t) f->gy[p] + k; }
+        else if (iface == 1) { cx = 2 * x; cy = 2 * (int64_t) f->gy[p] + k; }
+        else if (iface == 2) { cx = 2 * (int64_t) f->gx[p] + k; cy = 2 * y + 1; }
+        else { cx = 2 * (int64_t) f->gx[p] + k; cy = 2 * y; }
+        r->ti[k] = (int) ((2 * (int64_t) f->gx[p] - cx) * mx);
+        r->tj[k] = (int) ((2 * (int64_t) f->gy[p] - cy) * my);
+        if (!wrap(f, L + 1, &cx, &cy) || (q = lookup(f, L + 1, cx, cy)) < 0) { r->rel = REL_NONE; return; }
+        r->nb[k] = q;
+    }
+}
+
+static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int my, nbinfo_t *r)
+{
+    const int L = f->level[p];
+    const int64_t x = (int64_t) f->gx[p] + cdx[ic], y = (int64_t) f->gy[p] + cdy[ic];
+    int64_t wx = x, wy = y;
+    int q;
+    r->rel = REL_NONE;
+    if (!wrap(f, L, &wx, &wy)) return;
+    if ((q = lookup(f, L, wx, wy)) >= 0) {
+        r->rel = REL_SAME; r->nb[0] = q;
+        r->ti[0] = -cdx[ic] * mx; r->tj[0] = -cdy[ic] * my;
+        return;
+    }
+    if (L > 0 && (q = lookup(f, L - 1, wx >> 1, wy >> 1)) >= 0) {
+        /* coarser diagonal leaf: a corner neighbour only if the corner point is
+         * one of its corners; otherwise this corner is a hanging node */
+        const int64_t px = (int64_t) f->gx[p] + (cdx[ic] > 0), py = (int64_t) f->gy[p] + (cdy[ic] > 0);
+        const int64_t ox = 2 * (x >> 1), oy = 2 * (y >> 1);  /* unwrapped coarse origin, level L units */
+        if ((px == ox || px == ox + 2) && (py == oy || py == oy + 2)) { r->rel = REL_DOUBLE; r->nb[0] = q; }
+        return;
+    }
+    {
+        int64_t cx = 2 * x + (cdx[ic] < 0), cy = 2 * y + (cdy[ic] < 0);
+        r->ti[0] = (int) ((2 * (int64_t) f->gx[p] - cx) * mx);
+        r->tj[0] = (int) ((2 * (int64_t) f->gy[p] - cy) * my);
+        if (wrap(f, L + 1, &cx, &cy) && (q = lookup(f, L + 1, cx, cy)) >= 0) {
+            r->rel = REL_HALF; r->nb[0] = q;
+        }
+    }
+}
+
+int fclaw2d_hip_forest_face_relation(const fclaw2d_hip_forest_t *f, int p, int iface, int nb[2])
+{
+    nbinfo_t r;
+    if (!f || p < 0 || p >= f->n || iface < 0 || iface > 3) return -1;
+    face_rel(f, p, iface, 1, 1, &r);
+    if (nb) { nb[0] = (r.rel == REL_SAME || r.rel == REL_HALF || r.rel == REL_DOUBLE) ? r.nb[0] : -1;
+              nb[1] = r.rel == REL_HALF ? r.nb[1] : -1; }
+    return r.rel == REL_NONE ? -1 : r.rel;
+}
+
+/* ---------------------------------------------------------------- ghost program */
+
+enum { OP_COPY_FACE, OP_COPY_CORNER, OP_AVG_FACE, OP_AVG_CORNER, OP_INTERP_FACE,
+       OP_INTERP_CORNER, OP_PHYSBC };
+
+typedef struct { int *ops; int cap; int n; } opbuf_t;
+
+static void emit(opbuf_t *b, int type, int dst, int src, int which, int ti, int tj, int bc)
+{
+    if (b->n < b->cap) {
+        int *o = b->ops + 8 * b->n;
+        o[0] = type; o[1] = dst; o[2] = src; o[3] = which; o[4] = ti; o[5] = tj; o[6] = bc; o[7] = 0;
+    }
+    b->n++;
+}
+
+static int corner_interior(const int phys[4], int ic)
+{
+    const int fa = (ic & 1) ? 1 : 0, fb = (ic & 2) ? 3 : 2;
+    return !phys[fa] && !phys[fb];
+}
+
+static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, const int mthbc[4])
+{
+    int p, k;
+    for (p = 0; p < f->n; p++) {
+        int phys[4], bc = 0, any = 0;
+        if (f->level[p] != level) continue;
+        physical_faces(f, p, phys);
+        for (k = 0; k < 4; k++) {
+            const int m = phys[k] ? mthbc[k] : -1;
+            any |= phys[k];
+            bc |= (m & 0xff) << (8 * k);
+        }
+        if (any) emit(b, OP_PHYSBC, p, 0, 0, 0, 0, bc);
+    }
+}
+
+int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int my, int mbc,
+                                     const int mthbc[4], int *ops, int cap, int phase_start[7])
+{
+    opbuf_t b;
+    int minl, maxl, L, p, k;
+    (void) mbc;
+    if (!f || f->n == 0 || !phase_start || !mthbc) return -1;
+    b.ops = ops; b.cap = ops ? cap : 0; b.n = 0;
+    minl = fclaw2d_hip_forest_min_level(f);
+    maxl = fclaw2d_hip_forest_max_level(f);
+
+    /* phase 0: average fine -> coarse ghost cells (coarse levels, finest first) */
+    phase_start[0] = b.n;
+    for (L = maxl - 1; L >= minl; L--)
+        for (p = 0; p < f->n; p++) {
+            int phys[4];
+            if (f->level[p] != L) continue;
+            physical_faces(f, p, phys);
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (phys[k]) continue;
+                face_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_HALF) {
+                    emit(&b, OP_AVG_FACE, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
+                    emit(&b, OP_AVG_FACE, p, r.nb[1], k, r.ti[1], r.tj[1], 0);
+                }
+            }
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (!corner_interior(phys, k)) continue;
+                corner_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_HALF) emit(&b, OP_AVG_CORNER, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
+            }
+        }
+    /* phase 1: copy between same-size neighbours (all levels) */
+    phase_start[1] = b.n;
+    for (L = maxl; L >= minl; L--)
+        for (p = 0; p < f->n; p++) {
+            int phys[4];
+            if (f->level[p] != L) continue;
+            physical_faces(f, p, phys);
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (phys[k]) continue;
+                face_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_SAME) emit(&b, OP_COPY_FACE, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
+            }
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (!corner_interior(phys, k)) continue;
+                corner_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_SAME) emit(&b, OP_COPY_CORNER, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
+            }
+        }
+    /* phase 2: physical BC on coarse levels */
+    phase_start[2] = b.n;
+    for (L = maxl - 1; L >= minl; L--) emit_physbc(f, &b, L, mthbc);
+    /* phase 3: interpolate coarse -> fine ghost cells */
+    phase_start[3] = b.n;
+    for (L = maxl - 1; L >= minl; L--)
+        for (p = 0; p < f->n; p++) {
+            int phys[4];
+            if (f->level[p] != L) continue;
+            physical_faces(f, p, phys);
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (phys[k]) continue;
+                face_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_HALF) {
+                    emit(&b, OP_INTERP_FACE, r.nb[0], p, k, r.ti[0], r.tj[0], 0);
+                    emit(&b, OP_INTERP_FACE, r.nb[1], p, k, r.ti[1], r.tj[1], 0);
+                }
+            }
+            for (k = 0; k < 4; k++) {
+                nbinfo_t r;
+                if (!corner_interior(phys, k)) continue;
+                corner_rel(f, p, k, mx, my, &r);
+                if (r.rel == REL_HALF) emit(&b, OP_INTERP_CORNER, r.nb[0], p, k, r.ti[0], r.tj[0], 0);
+            }
+        }
+    /* phase 4: physical BC on fine levels; phase 5: on all levels */
+    phase_start[4] = b.n;
+    for (L = maxl; L >= minl + 1; L--) emit_physbc(f, &b, L, mthbc);
+    phase_start[5] = b.n;
+    for (L = maxl; L >= minl; L--) emit_physbc(f, &b, L, mthbc);
+    phase_start[6] = b.n;
+    return b.n;
+}

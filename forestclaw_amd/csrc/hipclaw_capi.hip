@@ -1,0 +1,514 @@
+// C ABI of the batched patch-update backend (declared in include/hipclaw.h).
+//
+// Device memory model: one allocation per slab, sized once for max_patches.
+// q lives in two slabs (ping-pong) unless a single tile covers a patch, in
+// which case the step is done in place.  All launches go to one
+// non-blocking stream of default priority owned by the state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hipclaw.h"
+#include "hipclaw_internal.h"
+
+using hipclaw::GhostOp;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, const char* detail = "") {
+    std::snprintf(g_err, sizeof(g_err), fmt, detail);
+    return code;
+}
+
+int hipfail(hipError_t e, const char* where) {
+    std::snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+    return HIPCLAW_EDEVICE;
+}
+
+#define HIP_TRY(call)                                  \
+    do {                                               \
+        hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return hipfail(_e, #call); \
+    } while (0)
+
+}  // namespace
+
+struct hipclaw_state {
+    hipclaw_config_t cfg;
+    int tile = 16, ntx = 1, nty = 1;
+    bool inplace = false;
+    int max_patches = 0, npatch = 0;
+    long long pstride = 0, astride = 0;
+    double* q[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* aux = nullptr;
+    double* dxs = nullptr;
+    double* dys = nullptr;
+    int* all_slots = nullptr;
+    int* batch_slots = nullptr;
+    GhostOp* ops = nullptr;
+    int nops = 0, ops_cap = 0;
+    int phase_start[7] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned long long* cfl = nullptr;
+    double* mass_buf = nullptr;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
+    // timing of the update kernel inside advance_async
+    bool time_kernels = true;
+    std::vector<hipEvent_t> kev;  // pairs (before, after) per step
+    int timed_steps = 0;
+    hipEvent_t region[2] = {nullptr, nullptr};
+};
+
+extern "C" {
+
+const char* hipclaw_last_error(void) { return g_err; }
+
+int hipclaw_initialize_device(int device, char* name, size_t name_len) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(HIPCLAW_ENODEV, "no HIP device visible%s");
+    if (device < 0 || device >= count) return fail(HIPCLAW_EINVAL, "device index out of range%s");
+    HIP_TRY(hipSetDevice(device));
+    if (name && name_len > 0) {
+        hipDeviceProp_t p;
+        HIP_TRY(hipGetDeviceProperties(&p, device));
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    return HIPCLAW_OK;
+}
+
+static int default_tile(const hipclaw_config_t* c) {
+    const int m = std::min(c->mx, c->my);
+    int t = (c->precision == HIPCLAW_FP64) ? 16 : 32;
+    while (t > m) t >>= 1;
+    return t;
+}
+
+int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t** out) {
+    if (!cfg || !out || max_patches <= 0) return fail(HIPCLAW_EINVAL, "hipclaw_create: bad arguments%s");
+    *out = nullptr;
+    const int solver = cfg->solver;
+    static const int meqn_of[3] = {1, 3, 4}, mw_of[3] = {1, 3, 4}, maux_of[3] = {2, 0, 0};
+    if (solver < 0 || solver > 2) return fail(HIPCLAW_EINVAL, "unknown Riemann solver%s");
+    if (cfg->meqn != meqn_of[solver] || cfg->mwaves != mw_of[solver] || cfg->maux != maux_of[solver])
+        return fail(HIPCLAW_EINVAL, "meqn/mwaves/maux do not match the Riemann solver%s");
+    if (cfg->mbc != 2) return fail(HIPCLAW_EINVAL, "the batched step requires mbc = 2%s");
+    if (cfg->precision != HIPCLAW_FP64 && cfg->precision != HIPCLAW_FP32_BULK)
+        return fail(HIPCLAW_EINVAL, "unknown precision%s");
+    int tile = cfg->tile > 0 ? cfg->tile : default_tile(cfg);
+    if (!hipclaw::step_supported(solver, cfg->precision, tile))
+        return fail(HIPCLAW_EINVAL, "tile size not available for this solver and precision%s");
+    if (cfg->mx % tile != 0 || cfg->my % tile != 0)
+        return fail(HIPCLAW_EINVAL, "mx and my must be multiples of the tile size%s");
+    if (cfg->order[0] < 1 || cfg->order[0] > 2 || cfg->order[1] < 0 || cfg->order[1] > 2)
+        return fail(HIPCLAW_EINVAL, "order must be (1|2, 0|1|2)%s");
+    for (int k = 0; k < cfg->mwaves; k++)
+        if (cfg->mthlim[k] < 0 || cfg->mthlim[k] > 6) return fail(HIPCLAW_EINVAL, "mthlim out of range%s");
+
+    hipclaw_state* s = new hipclaw_state();
+    s->cfg = *cfg;
+    s->tile = tile;
+    s->ntx = cfg->mx / tile;
+    s->nty = cfg->my / tile;
+    s->inplace = (s->ntx == 1 && s->nty == 1);
+    s->max_patches = max_patches;
+    const long long cells = (long long) (cfg->mx + 2 * cfg->mbc) * (cfg->my + 2 * cfg->mbc);
+    s->pstride = cells * cfg->meqn;
+    s->astride = cells * cfg->maux;
+    auto cleanup = [&](int rc) { hipclaw_destroy(s); return rc; };
+#define ALLOC(p, bytes)                                                         \
+    do {                                                                        \
+        if (hipMalloc((void**) &(p), (bytes)) != hipSuccess) {                  \
+            return cleanup(fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", #p)); \
+        }                                                                       \
+    } while (0)
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(HIPCLAW_EDEVICE, "hipStreamCreate failed%s"));
+    ALLOC(s->q[0], sizeof(double) * s->pstride * max_patches);
+    if (!s->inplace) ALLOC(s->q[1], sizeof(double) * s->pstride * max_patches);
+    if (cfg->maux > 0) ALLOC(s->aux, sizeof(double) * s->astride * max_patches);
+    ALLOC(s->dxs, sizeof(double) * max_patches);
+    ALLOC(s->dys, sizeof(double) * max_patches);
+    ALLOC(s->all_slots, sizeof(int) * max_patches);
+    ALLOC(s->batch_slots, sizeof(int) * max_patches);
+    ALLOC(s->cfl, sizeof(unsigned long long));
+    ALLOC(s->mass_buf, sizeof(double) * 2 * cfg->meqn * (size_t) max_patches);
+#undef ALLOC
+    std::vector<int> iota(max_patches);
+    for (int p = 0; p < max_patches; p++) iota[p] = p;
+    if (hipMemcpy(s->all_slots, iota.data(), sizeof(int) * max_patches, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(s->q[0], 0, sizeof(double) * s->pstride * max_patches) != hipSuccess)
+        return cleanup(fail(HIPCLAW_EDEVICE, "initial device copies failed%s"));
+    if (s->q[1] && hipMemset(s->q[1], 0, sizeof(double) * s->pstride * max_patches) != hipSuccess)
+        return cleanup(fail(HIPCLAW_EDEVICE, "initial device copies failed%s"));
+    if (hipEventCreate(&s->region[0]) != hipSuccess || hipEventCreate(&s->region[1]) != hipSuccess)
+        return cleanup(fail(HIPCLAW_EDEVICE, "hipEventCreate failed%s"));
+    *out = s;
+    return HIPCLAW_OK;
+}
+
+void hipclaw_destroy(hipclaw_state_t* s) {
+    if (!s) return;
+    if (s->st) hipStreamSynchronize(s->st);
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    for (hipEvent_t e : s->kev) hipEventDestroy(e);
+    for (hipEvent_t e : s->region) if (e) hipEventDestroy(e);
+    hipFree(s->q[0]);
+    hipFree(s->q[1]);
+    hipFree(s->aux);
+    hipFree(s->dxs);
+    hipFree(s->dys);
+    hipFree(s->all_slots);
+    hipFree(s->batch_slots);
+    hipFree(s->ops);
+    hipFree(s->cfl);
+    hipFree(s->mass_buf);
+    if (s->st) hipStreamDestroy(s->st);
+    delete s;
+}
+
+int hipclaw_tile(const hipclaw_state_t* s) { return s ? s->tile : 0; }
+
+int hipclaw_set_geometry(hipclaw_state_t* s, int npatch, const double* dx, const double* dy) {
+    if (!s || npatch < 0 || npatch > s->max_patches || (npatch > 0 && (!dx || !dy)))
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_geometry: bad arguments%s");
+    for (int p = 0; p < npatch; p++)
+        if (!(dx[p] > 0.0) || !(dy[p] > 0.0)) return fail(HIPCLAW_EINVAL, "non-positive mesh width%s");
+    s->npatch = npatch;
+    if (npatch > 0) {
+        HIP_TRY(hipMemcpyAsync(s->dxs, dx, sizeof(double) * npatch, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->dys, dy, sizeof(double) * npatch, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
+    return HIPCLAW_OK;
+}
+
+static int check_range(hipclaw_state_t* s, const void* p, int first, int count) {
+    if (!s || !p || first < 0 || count < 0 || first + count > s->max_patches)
+        return fail(HIPCLAW_EINVAL, "patch range out of bounds%s");
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_q(hipclaw_state_t* s, const double* q, int first, int count) {
+    if (int rc = check_range(s, q, first, count)) return rc;
+    const size_t bytes = sizeof(double) * s->pstride * count;
+    HIP_TRY(hipMemcpyAsync(s->q[s->cur] + s->pstride * first, q, bytes, hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_get_q(hipclaw_state_t* s, double* q, int first, int count) {
+    if (int rc = check_range(s, q, first, count)) return rc;
+    const size_t bytes = sizeof(double) * s->pstride * count;
+    HIP_TRY(hipMemcpyAsync(q, s->q[s->cur] + s->pstride * first, bytes, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_aux(hipclaw_state_t* s, const double* aux, int first, int count) {
+    if (int rc = check_range(s, aux, first, count)) return rc;
+    if (s->cfg.maux == 0) return HIPCLAW_OK;
+    const size_t bytes = sizeof(double) * s->astride * count;
+    HIP_TRY(hipMemcpyAsync(s->aux + s->astride * first, aux, bytes, hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start) {
+    if (!s || nops < 0 || (nops > 0 && !ops) || !phase_start)
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
+    if (phase_start[0] != 0 || phase_start[6] != nops)
+        return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
+    for (int k = 0; k < 6; k++)
+        if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
+    // validate every op on the host: a bad slot or translation would fault the GPU
+    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    for (int o = 0; o < nops; o++) {
+        const int* op = ops + 8 * o;
+        const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
+        if (type < 0 || type > hipclaw::GHOST_PHYSBC || dst < 0 || dst >= s->npatch)
+            return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
+        if (type == hipclaw::GHOST_PHYSBC) continue;
+        if (src < 0 || src >= s->npatch || which < 0 || which > 3)
+            return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
+        // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
+        // neighbour's array; check the extreme cells the op can touch
+        int lo_i, hi_i, lo_j, hi_j;  // range of the translated index
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
+            lo_i = 1 - mbc + ti; hi_i = mx + mbc + ti; lo_j = 1 - mbc + tj; hi_j = my + mbc + tj;
+            // copy reads the neighbour's interior band only
+            int ci0, ci1, cj0, cj1;
+            if (type == hipclaw::GHOST_COPY_FACE) {
+                if (which == 0) { ci0 = 1 - mbc; ci1 = 0; cj0 = 1; cj1 = my; }
+                else if (which == 1) { ci0 = mx + 1; ci1 = mx + mbc; cj0 = 1; cj1 = my; }
+                else if (which == 2) { ci0 = 1; ci1 = mx; cj0 = 1 - mbc; cj1 = 0; }
+                else { ci0 = 1; ci1 = mx; cj0 = my + 1; cj1 = my + mbc; }
+            } else {
+                ci0 = (which & 1) ? mx + 1 : 1 - mbc; ci1 = ci0 + mbc - 1;
+                cj0 = (which & 2) ? my + 1 : 1 - mbc; cj1 = cj0 + mbc - 1;
+            }
+            lo_i = ci0 + ti; hi_i = ci1 + ti; lo_j = cj0 + tj; hi_j = cj1 + tj;
+            if (lo_i < 1 || hi_i > mx || lo_j < 1 || hi_j > my)
+                return fail(HIPCLAW_EINVAL, "copy op reads outside the neighbour's interior%s");
+        } else if (type == hipclaw::GHOST_AVERAGE_CORNER) {
+            const int ic0 = (which & 1) ? mx + 1 : 1 - mbc, jc0 = (which & 2) ? my + 1 : 1 - mbc;
+            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + mbc - 1) + ti;
+            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + mbc - 1) + tj;
+            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+                return fail(HIPCLAW_EINVAL, "average-corner op reads outside the fine patch%s");
+        } else if (type == hipclaw::GHOST_INTERP_CORNER) {
+            const int hb = mbc / 2;
+            const int ic0 = (which & 1) ? mx - hb + 1 : 1, jc0 = (which & 2) ? my - hb + 1 : 1;
+            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + hb - 1) + ti;
+            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + hb - 1) + tj;
+            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+                return fail(HIPCLAW_EINVAL, "interpolate-corner op writes outside the fine patch%s");
+        }
+        // average/interpolate faces test every cell against the neighbour's bounds in the kernel
+    }
+    if (nops > s->ops_cap) {
+        hipFree(s->ops);
+        s->ops = nullptr;
+        s->ops_cap = 0;
+        if (hipMalloc((void**) &s->ops, sizeof(GhostOp) * nops) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost ops failed%s");
+        s->ops_cap = nops;
+    }
+    if (nops > 0) {
+        HIP_TRY(hipMemcpyAsync(s->ops, ops, sizeof(GhostOp) * nops, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
+    s->nops = nops;
+    std::memcpy(s->phase_start, phase_start, sizeof(s->phase_start));
+    return HIPCLAW_OK;
+}
+
+static int ghost_fill_async(hipclaw_state_t* s) {
+    hipclaw::GhostArgs g;
+    g.q = s->q[s->cur];
+    g.pstride = s->pstride;
+    g.mx = s->cfg.mx; g.my = s->cfg.my; g.mbc = s->cfg.mbc; g.meqn = s->cfg.meqn;
+    for (int ph = 0; ph < 6; ph++) {
+        g.ops = s->ops + s->phase_start[ph];
+        g.nops = s->phase_start[ph + 1] - s->phase_start[ph];
+        if (int e = hipclaw::launch_ghost_phase(g, s->st))
+            return hipfail((hipError_t) e, "ghost_phase launch");
+    }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_ghost_fill(hipclaw_state_t* s) {
+    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
+    if (int rc = ghost_fill_async(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double* dst,
+                                   const int* slots, int n, double dt) {
+    hipclaw::StepArgs a;
+    a.qsrc = src;
+    a.qdst = dst;
+    a.aux = s->aux;
+    a.work = slots;
+    a.dxs = s->dxs;
+    a.dys = s->dys;
+    a.cfl = s->cfl;
+    a.pstride = s->pstride;
+    a.astride = s->astride;
+    a.nwork = n;
+    a.mx = s->cfg.mx; a.my = s->cfg.my; a.mbc = s->cfg.mbc;
+    a.ntx = s->ntx; a.nty = s->nty;
+    a.order2 = s->cfg.order[0] == 2;
+    a.transverse = s->cfg.order[1];
+    for (int k = 0; k < 4; k++) a.mthlim[k] = k < s->cfg.mwaves ? s->cfg.mthlim[k] : 0;
+    a.dt = dt;
+    a.pp.grav = s->cfg.grav;
+    a.pp.gamma = s->cfg.gamma;
+    return a;
+}
+
+static int launch_step(hipclaw_state_t* s, const hipclaw::StepArgs& a) {
+    int e = (s->cfg.precision == HIPCLAW_FP64)
+                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, a, s->st)
+                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, a, s->st);
+    if (e) return hipfail((hipError_t) e, "step_tiles launch");
+    return HIPCLAW_OK;
+}
+
+static double read_cfl(hipclaw_state_t* s, int* rc) {
+    unsigned long long bits = 0;
+    hipError_t e = hipMemcpyAsync(&bits, s->cfl, sizeof(bits), hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) { *rc = hipfail(e, "reading the CFL number"); return 0.0; }
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *rc = HIPCLAW_OK;
+    return v;
+}
+
+int hipclaw_step_batch(hipclaw_state_t* s, const int* slots, int nslots, double t, double dt,
+                       double* maxcfl) {
+    (void) t;
+    if (!s || nslots < 0 || nslots > s->max_patches || (nslots > 0 && !slots) || !(dt > 0.0))
+        return fail(HIPCLAW_EINVAL, "hipclaw_step_batch: bad arguments%s");
+    for (int k = 0; k < nslots; k++)
+        if (slots[k] < 0 || slots[k] >= s->npatch) return fail(HIPCLAW_EINVAL, "slot out of range%s");
+    if (maxcfl) *maxcfl = 0.0;
+    if (nslots == 0) return HIPCLAW_OK;
+    HIP_TRY(hipMemcpyAsync(s->batch_slots, slots, sizeof(int) * nslots, hipMemcpyHostToDevice, s->st));
+    if (int e = hipclaw::launch_reset_scalar(s->cfl, s->st)) return hipfail((hipError_t) e, "reset");
+    double* src = s->q[s->cur];
+    double* dst = s->inplace ? src : s->q[1 - s->cur];
+    if (int rc = launch_step(s, step_args(s, src, dst, s->batch_slots, nslots, dt))) return rc;
+    if (!s->inplace) {
+        int e = hipclaw::launch_copy_interior(dst, src, s->batch_slots, nslots, s->pstride,
+                                              s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn, s->st);
+        if (e) return hipfail((hipError_t) e, "copy_interior launch");
+    }
+    int rc;
+    const double c = read_cfl(s, &rc);
+    if (rc) return rc;
+    if (maxcfl) *maxcfl = c;
+    return HIPCLAW_OK;
+}
+
+static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool timed) {
+    if (timed) {
+        const size_t need = 2 * (size_t) nsteps;
+        while (s->kev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            s->kev.push_back(e);
+        }
+    }
+    if (int e = hipclaw::launch_reset_scalar(s->cfl, s->st)) return hipfail((hipError_t) e, "reset");
+    for (int n = 0; n < nsteps; n++) {
+        double* src = s->q[s->cur];
+        double* dst = s->inplace ? src : s->q[1 - s->cur];
+        if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n], s->st));
+        if (int rc = launch_step(s, step_args(s, src, dst, s->all_slots, s->npatch, dt))) return rc;
+        if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n + 1], s->st));
+        if (!s->inplace) s->cur = 1 - s->cur;
+        if (int rc = ghost_fill_async(s)) return rc;
+    }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_advance_async(hipclaw_state_t* s, double dt, int nsteps) {
+    if (!s || nsteps < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance: bad arguments%s");
+    HIP_TRY(hipEventRecord(s->region[0], s->st));
+    if (int rc = advance_launches(s, dt, nsteps, s->time_kernels)) return rc;
+    HIP_TRY(hipEventRecord(s->region[1], s->st));
+    s->timed_steps = s->time_kernels ? nsteps : 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_sync(hipclaw_state_t* s) {
+    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_advance(hipclaw_state_t* s, double dt, int nsteps, double* maxcfl) {
+    if (int rc = hipclaw_advance_async(s, dt, nsteps)) return rc;
+    int rc;
+    const double c = read_cfl(s, &rc);
+    if (rc) return rc;
+    if (maxcfl) *maxcfl = c;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_capture_graph(hipclaw_state_t* s, double dt, int nsteps) {
+    if (!s || nsteps <= 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_capture_graph: bad arguments%s");
+    if (!s->inplace && (nsteps % 2) != 0)
+        return fail(HIPCLAW_EINVAL, "ping-pong states need an even number of captured steps%s");
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    const int cur0 = s->cur;
+    HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    int rc = advance_launches(s, dt, nsteps, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->st, &g);
+    s->cur = cur0;  // capture only recorded the launches
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+    s->graph = g;
+    HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    s->graph_steps = nsteps;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_replay_graph(hipclaw_state_t* s) {
+    if (!s || !s->gexec) return fail(HIPCLAW_EINVAL, "no captured graph%s");
+    HIP_TRY(hipEventRecord(s->region[0], s->st));
+    HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+    HIP_TRY(hipEventRecord(s->region[1], s->st));
+    s->timed_steps = 0;
+    // an even number of ping-pong steps leaves the current slab unchanged
+    return HIPCLAW_OK;
+}
+
+int hipclaw_timing(hipclaw_state_t* s, int nsteps, double* total_ms, double* step_kernel_ms) {
+    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->region[0], s->region[1]));
+    if (total_ms) *total_ms = ms;
+    if (step_kernel_ms) {
+        *step_kernel_ms = 0.0;
+        const int n = std::min(nsteps, s->timed_steps);
+        double sum = 0.0;
+        for (int k = 0; k < n; k++) {
+            float m = 0.f;
+            HIP_TRY(hipEventElapsedTime(&m, s->kev[2 * k], s->kev[2 * k + 1]));
+            sum += m;
+        }
+        if (n > 0) *step_kernel_ms = sum / n;
+    }
+    return HIPCLAW_OK;
+}
+
+static void two_sum_host(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+int hipclaw_mass(hipclaw_state_t* s, double* sum_hi, double* sum_lo) {
+    if (!s || !sum_hi) return fail(HIPCLAW_EINVAL, "hipclaw_mass: bad arguments%s");
+    const int meqn = s->cfg.meqn;
+    int e = hipclaw::launch_mass(s->q[s->cur], s->all_slots, s->npatch, s->dxs, s->dys, s->pstride,
+                                 s->cfg.mx, s->cfg.my, s->cfg.mbc, meqn, s->mass_buf, s->st);
+    if (e) return hipfail((hipError_t) e, "patch_mass launch");
+    std::vector<double> buf(2 * (size_t) meqn * s->npatch);
+    if (!buf.empty())
+        HIP_TRY(hipMemcpyAsync(buf.data(), s->mass_buf, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    for (int m = 0; m < meqn; m++) {
+        double hi = 0.0, lo = 0.0;
+        for (int p = 0; p < s->npatch; p++) {
+            double t, err;
+            two_sum_host(hi, buf[((size_t) p * meqn + m) * 2], t, err);
+            hi = t;
+            lo += err + buf[((size_t) p * meqn + m) * 2 + 1];
+        }
+        double t, err;
+        two_sum_host(hi, lo, t, err);
+        sum_hi[m] = t;
+        if (sum_lo) sum_lo[m] = err;
+    }
+    return HIPCLAW_OK;
+}
+
+}  // extern "C"

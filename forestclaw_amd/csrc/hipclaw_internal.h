@@ -1,0 +1,57 @@
+// Internal interfaces between the C-ABI layer (hipclaw_capi.hip) and the two
+// kernel translation units (FP64 parity form, FP32-bulk form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hipclaw_step.h"
+
+namespace hipclaw {
+
+// One ghost-fill operation; every op touches only ghost cells of `dst`.
+// Layout is shared with the host op-list builder (int32 x 8).
+struct GhostOp {
+    int type;   // GHOST_* below
+    int dst;    // slot written
+    int src;    // slot read (unused for physical BC)
+    int which;  // face or corner of `dst` (copy) / of the coarse patch (average, interpolate)
+    int ti, tj; // index translation (see the op builder)
+    int bc;     // physical BC: mthbc of faces 0..3 packed as 4 signed bytes
+    int pad;
+};
+
+enum {
+    GHOST_COPY_FACE = 0,
+    GHOST_COPY_CORNER = 1,
+    GHOST_AVERAGE_FACE = 2,
+    GHOST_AVERAGE_CORNER = 3,
+    GHOST_INTERP_FACE = 4,
+    GHOST_INTERP_CORNER = 5,
+    GHOST_PHYSBC = 6,
+};
+
+struct GhostArgs {
+    const GhostOp* ops;
+    int nops;
+    double* q;
+    long long pstride;
+    int mx, my, mbc, meqn;
+};
+
+// Launchers (return hipError_t as int).
+int launch_ghost_phase(const GhostArgs& g, hipStream_t st);
+int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
+int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);
+int launch_reset_scalar(unsigned long long* p, hipStream_t st);
+int launch_copy_interior(const double* src, double* dst, const int* slots, int n,
+                         long long pstride, int mx, int my, int mbc, int meqn,
+                         hipStream_t st);
+// Per-slot compensated FP64 sums of q*dx*dy over interior cells: out[slot][m] = (hi, lo)
+int launch_mass(const double* q, const int* slots, int n, const double* dxs, const double* dys,
+                long long pstride, int mx, int my, int mbc, int meqn, double* out,
+                hipStream_t st);
+
+// Largest tile the FP64 / FP32 kernels are instantiated for, per solver.
+bool step_supported(int solver, int precision, int tile);
+
+}  // namespace hipclaw

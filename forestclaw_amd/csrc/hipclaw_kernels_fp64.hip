@@ -1,0 +1,294 @@
+// FP64 kernels: the oracle-parity patch step, the ghost fill, and the
+// compensated mass sums.  This translation unit is compiled with
+// -ffp-contract=off so that every FP64 operation rounds exactly as the
+// reference's CPU Fortran does.
+#include "hipclaw_internal.h"
+
+namespace hipclaw {
+
+#define QIX(i, j, m) (((long long) (m) * nyp + ((j) + mbc - 1)) * nxp + ((i) + mbc - 1))
+
+// fclaw2d_clawpatch_compute_slopes with mth = 5
+// (src/patches/clawpatch/fclaw2d_clawpatch_utils.f:45-55)
+__device__ __forceinline__ double amr_slope(double sl, double sr) {
+    const double sc = (sl + sr) / 2.0;
+    const double mn = fmin(fmin(2 * fabs(sl), 2 * fabs(sr)), fabs(sc));
+    return mn * fmax(0.0, copysign(1.0, sl * sr)) * copysign(1.0, sc);
+}
+
+__device__ __forceinline__ void corner_cell(int mx, int my, int icorner, int ibc, int jbc,
+                                            int& i1, int& j1) {
+    i1 = (icorner & 1) ? mx + ibc : 1 - ibc;
+    j1 = (icorner & 2) ? my + jbc : 1 - jbc;
+}
+
+// Interpolate one coarse cell (ic,jc) of `qc` into the 2x2 fine cells whose first
+// child is (fi,fj) in `qf` (fclaw2d_clawpatch46_interpolate.f:120-137).
+__device__ __forceinline__ void interp_cell(const double* qc, double* qf, int m, int ic, int jc,
+                                            int fi, int fj, int nxp, int nyp, int mbc) {
+    const double c = qc[QIX(ic, jc, m)];
+    const double gx = amr_slope(c - qc[QIX(ic - 1, jc, m)], qc[QIX(ic + 1, jc, m)] - c);
+    const double gy = amr_slope(c - qc[QIX(ic, jc - 1, m)], qc[QIX(ic, jc + 1, m)] - c);
+    qf[QIX(fi, fj, m)] = c + gx * -0.25 + gy * -0.25;
+    qf[QIX(fi + 1, fj, m)] = c + gx * 0.25 + gy * -0.25;
+    qf[QIX(fi, fj + 1, m)] = c + gx * -0.25 + gy * 0.25;
+    qf[QIX(fi + 1, fj + 1, m)] = c + gx * 0.25 + gy * 0.25;
+}
+
+// One workgroup (one wave) per ghost op.  Ops of one phase write disjoint
+// ghost cells and read only cells no op of the phase writes.
+__global__ void __launch_bounds__(64) ghost_phase(GhostArgs g) {
+    const int o = blockIdx.x;
+    if (o >= g.nops) return;
+    const GhostOp op = g.ops[o];
+    const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    double* qd = g.q + (long long) op.dst * g.pstride;
+    const double* qs = g.q + (long long) op.src * g.pstride;
+    const int t = threadIdx.x;
+
+    switch (op.type) {
+    case GHOST_COPY_FACE: {  // fclaw2d_clawpatch46_copy.f:40-84
+        const int len = (op.which < 2) ? my : mx;
+        for (int e = t; e < meqn * mbc * len; e += 64) {
+            const int m = e / (mbc * len), r = e - m * mbc * len;
+            const int l = r / mbc, b = r - l * mbc;  // b: ghost layer 0..mbc-1
+            int i1, j1;
+            if (op.which == 0) { i1 = 1 - mbc + b; j1 = 1 + l; }
+            else if (op.which == 1) { i1 = mx + 1 + b; j1 = 1 + l; }
+            else if (op.which == 2) { i1 = 1 + l; j1 = 1 - mbc + b; }
+            else { i1 = 1 + l; j1 = my + 1 + b; }
+            qd[QIX(i1, j1, m)] = qs[QIX(i1 + op.ti, j1 + op.tj, m)];
+        }
+        break;
+    }
+    case GHOST_COPY_CORNER: {  // fclaw2d_clawpatch46_copy.f:95-133
+        for (int e = t; e < meqn * mbc * mbc; e += 64) {
+            const int m = e / (mbc * mbc), r = e - m * mbc * mbc;
+            int i1, j1;
+            corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, i1, j1);
+            qd[QIX(i1, j1, m)] = qs[QIX(i1 + op.ti, j1 + op.tj, m)];
+        }
+        break;
+    }
+    case GHOST_AVERAGE_FACE: {  // fclaw2d_clawpatch46_average.f:61-174 (manifold = 0)
+        const int len = (op.which < 2) ? my : mx;
+        for (int e = t; e < meqn * mbc * len; e += 64) {
+            const int m = e / (mbc * len), r = e - m * mbc * len;
+            const int l = r / mbc, b = r - l * mbc + 1;  // b = ibc/jbc 1..mbc
+            int ic, jc;
+            if (op.which < 2) { ic = (op.which == 0) ? 1 - b : mx + b; jc = 1 + l; }
+            else { ic = 1 + l; jc = (op.which == 2) ? 1 - b : my + b; }
+            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+            // only cells whose four children are interior to this fine grid
+            if (fi < 1 || fi + 1 > mx || fj < 1 || fj + 1 > my) continue;
+            double sum = 0;
+            sum = sum + qs[QIX(fi, fj, m)];
+            sum = sum + qs[QIX(fi + 1, fj, m)];
+            sum = sum + qs[QIX(fi, fj + 1, m)];
+            sum = sum + qs[QIX(fi + 1, fj + 1, m)];
+            qd[QIX(ic, jc, m)] = sum / 4.0;
+        }
+        break;
+    }
+    case GHOST_AVERAGE_CORNER: {  // fclaw2d_clawpatch46_average.f:225-270
+        for (int e = t; e < meqn * mbc * mbc; e += 64) {
+            const int m = e / (mbc * mbc), r = e - m * mbc * mbc;
+            int ic, jc;
+            corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, ic, jc);
+            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+            double sum = 0;
+            sum = sum + qs[QIX(fi, fj, m)];
+            sum = sum + qs[QIX(fi + 1, fj, m)];
+            sum = sum + qs[QIX(fi, fj + 1, m)];
+            sum = sum + qs[QIX(fi + 1, fj + 1, m)];
+            qd[QIX(ic, jc, m)] = sum / 4.0;
+        }
+        break;
+    }
+    case GHOST_INTERP_FACE: {  // fclaw2d_clawpatch46_interpolate.f:91-197; dst = fine, src = coarse
+        const int hb = mbc / 2;
+        for (int e = t; e < meqn * hb * mx; e += 64) {  // the reference bounds the face loop by mx
+            const int m = e / (hb * mx), r = e - m * hb * mx;
+            const int l = r / hb, b = r - l * hb + 1;
+            int ic, jc;
+            if (op.which < 2) { ic = (op.which == 0) ? b : mx - b + 1; jc = 1 + l; }
+            else { ic = 1 + l; jc = (op.which == 2) ? b : my - b + 1; }
+            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+            if (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc) continue;
+            interp_cell(qs, qd, m, ic, jc, fi, fj, nxp, nyp, mbc);
+        }
+        break;
+    }
+    case GHOST_INTERP_CORNER: {  // fclaw2d_clawpatch46_interpolate.f:263-311
+        const int hb = mbc / 2;
+        for (int e = t; e < meqn * hb * hb; e += 64) {
+            const int m = e / (hb * hb), r = e - m * hb * hb;
+            const int ibc = r / hb + 1, jbc = r % hb + 1;
+            const int ic = (op.which & 1) ? mx - ibc + 1 : ibc;
+            const int jc = (op.which & 2) ? my - jbc + 1 : jbc;
+            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+            interp_cell(qs, qd, m, ic, jc, fi, fj, nxp, nyp, mbc);
+        }
+        break;
+    }
+    case GHOST_PHYSBC: {  // clawpack46_bc2_default.f, faces in order 0,1,2,3
+        for (int face = 0; face < 4; face++) {
+            const int mth = (signed char) ((op.bc >> (8 * face)) & 0xff);
+            if (mth == 1 || mth == 3) {
+                const int len = (face < 2) ? my + 2 * mbc : mx + 2 * mbc;
+                for (int e = t; e < meqn * mbc * len; e += 64) {
+                    const int m = e / (mbc * len), r = e - m * mbc * len;
+                    const int l = r / mbc, b = r - l * mbc + 1;
+                    int id, jd, is, js;
+                    if (face < 2) {
+                        jd = js = 1 - mbc + l;
+                        if (face == 0) { id = 1 - b; is = (mth == 1) ? 1 : b; }
+                        else { id = mx + b; is = (mth == 1) ? mx : mx + 1 - b; }
+                    } else {
+                        id = is = 1 - mbc + l;
+                        if (face == 2) { jd = 1 - b; js = (mth == 1) ? 1 : b; }
+                        else { jd = my + b; js = (mth == 1) ? my : my + 1 - b; }
+                    }
+                    const int mn = (face < 2) ? 1 : 2;  // normal momentum negated on walls
+                    const double v = qd[QIX(is, js, m)];
+                    qd[QIX(id, jd, m)] = (mth == 3 && m == mn) ? -v : v;
+                }
+            }
+            __syncthreads();
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+__global__ void reset_scalar(unsigned long long* p) { *p = 0ull; }
+
+__global__ void copy_interior(const double* __restrict__ src, double* __restrict__ dst,
+                              const int* __restrict__ slots, int n, long long pstride,
+                              int mx, int my, int mbc, int meqn) {
+    const int w = blockIdx.y;
+    if (w >= n) return;
+    const long long base = (long long) slots[w] * pstride;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < meqn * mx * my;
+         e += gridDim.x * blockDim.x) {
+        const int m = e / (mx * my), r = e - m * mx * my;
+        const int j = r / mx + 1, i = r - (j - 1) * mx + 1;
+        dst[base + QIX(i, j, m)] = src[base + QIX(i, j, m)];
+    }
+}
+
+// Error-free transformation a + b = s + e (Knuth TwoSum).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated per-patch sums of q*dx*dy: each thread keeps a double-double
+// accumulator, then the workgroup combines them with TwoSum.
+__global__ void __launch_bounds__(256)
+patch_mass(const double* __restrict__ q, const int* __restrict__ slots, int n,
+           const double* __restrict__ dxs, const double* __restrict__ dys, long long pstride,
+           int mx, int my, int mbc, int meqn, double* __restrict__ out) {
+    __shared__ double sh[2][256];
+    const int w = blockIdx.x;
+    if (w >= n) return;
+    const int slot = slots[w];
+    const double area = dxs[slot] * dys[slot];
+    const double* qp = q + (long long) slot * pstride;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    for (int m = 0; m < meqn; m++) {
+        double hi = 0.0, lo = 0.0;
+        for (int e = threadIdx.x; e < mx * my; e += 256) {
+            const int j = e / mx + 1, i = e - (j - 1) * mx + 1;
+            double s, err;
+            two_sum(hi, qp[QIX(i, j, m)] * area, s, err);
+            hi = s;
+            lo += err;
+        }
+        sh[0][threadIdx.x] = hi;
+        sh[1][threadIdx.x] = lo;
+        __syncthreads();
+        for (int st = 128; st >= 1; st >>= 1) {
+            if ((int) threadIdx.x < st) {
+                double s, err;
+                two_sum(sh[0][threadIdx.x], sh[0][threadIdx.x + st], s, err);
+                sh[0][threadIdx.x] = s;
+                sh[1][threadIdx.x] = sh[1][threadIdx.x] + sh[1][threadIdx.x + st] + err;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[((long long) slot * meqn + m) * 2 + 0] = sh[0][0];
+            out[((long long) slot * meqn + m) * 2 + 1] = sh[1][0];
+        }
+        __syncthreads();
+    }
+}
+
+int launch_ghost_phase(const GhostArgs& g, hipStream_t st) {
+    if (g.nops <= 0) return 0;
+    hipLaunchKernelGGL(ghost_phase, dim3(g.nops), dim3(64), 0, st, g);
+    return (int) hipGetLastError();
+}
+
+int launch_reset_scalar(unsigned long long* p, hipStream_t st) {
+    hipLaunchKernelGGL(reset_scalar, dim3(1), dim3(1), 0, st, p);
+    return (int) hipGetLastError();
+}
+
+int launch_copy_interior(const double* src, double* dst, const int* slots, int n,
+                         long long pstride, int mx, int my, int mbc, int meqn,
+                         hipStream_t st) {
+    if (n <= 0) return 0;
+    const int per = meqn * mx * my;
+    dim3 grid((per + 255) / 256, n);
+    hipLaunchKernelGGL(copy_interior, grid, dim3(256), 0, st, src, dst, slots, n, pstride, mx,
+                       my, mbc, meqn);
+    return (int) hipGetLastError();
+}
+
+int launch_mass(const double* q, const int* slots, int n, const double* dxs, const double* dys,
+                long long pstride, int mx, int my, int mbc, int meqn, double* out,
+                hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(patch_mass, dim3(n), dim3(256), 0, st, q, slots, n, dxs, dys, pstride,
+                       mx, my, mbc, meqn, out);
+    return (int) hipGetLastError();
+}
+
+// ---------------------------------------------------------------- FP64 step
+template <class RP, int T, int NW>
+static int launch_one(const StepArgs& a, hipStream_t st) {
+    const int nblocks = a.nwork * a.ntx * a.nty;
+    if (nblocks <= 0) return 0;
+    hipLaunchKernelGGL((step_tiles<RP, double, T, NW, false>), dim3(nblocks), dim3(NW * 64), 0,
+                       st, a);
+    return (int) hipGetLastError();
+}
+
+template <class RP>
+static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
+    switch (tile) {
+    case 8: return launch_one<RP, 8, 2>(a, st);
+    case 16: return launch_one<RP, 16, 6>(a, st);
+    default: return (int) hipErrorInvalidValue;
+    }
+}
+
+int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st) {
+    switch (solver) {
+    case 0: return launch_tile<RPAdvection<double>>(tile, a, st);
+    case 1:
+        if (tile == 32) return launch_one<RPShallow<double>, 32, 16>(a, st);
+        return launch_tile<RPShallow<double>>(tile, a, st);
+    case 2: return launch_tile<RPEuler<double>>(tile, a, st);
+    default: return (int) hipErrorInvalidValue;
+    }
+}
+
+}  // namespace hipclaw

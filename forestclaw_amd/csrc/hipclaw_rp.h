@@ -1,0 +1,328 @@
+// Device Riemann solvers for the batched patch update.
+//
+// Each solver works on ONE interface held in registers: qL is the cell to the
+// left of the interface (the reference's qr(i-1,:)), qR the cell to the right
+// (ql(i,:)).  IXY = 1 for an x-interface, 2 for a y-interface.  The arithmetic
+// is written in the same order as the reference Fortran so that, compiled
+// with FMA contraction off, the FP64 instantiation reproduces it bit for bit:
+//
+//   shallow water : applications/clawpack/shallow/2d/rp/clawpack46_rpn2.f:91-242,
+//                   clawpack46_rpt2.f:287-319 (Roe, entropy fix on)
+//   Euler         : applications/clawpack/euler/2d/rp/clawpack46_rpn2_euler4.f:83-285,
+//                   clawpack46_rpt2_euler4.f:345-383 (Roe, entropy fix on)
+//   advection     : applications/clawpack/advection/2d/all/clawpack46_rpn2adv.f:28-58,
+//                   clawpack46_rpt2adv.f:79-86
+//
+// Solvers are compile-time functors (no device function pointers) so the
+// compiler can keep waves, speeds and fluctuations in VGPRs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hipclaw {
+
+struct PhysParams {
+    double grav;   // shallow water gravity
+    double gamma;  // Euler ratio of specific heats
+};
+
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+
+// ----------------------------------------------------------------- shallow water
+template <typename R>
+struct RPShallow {
+    static constexpr int MEQN = 3, MWAVES = 3, MAUX = 0;
+    static constexpr int NROE = 3;  // u, v, a
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxR*/,
+                                                const PhysParams& pp, R wave[MWAVES][MEQN],
+                                                R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
+                                                R roe[NROE]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R grav = (R) pp.grav;
+        const R h = (qL[0] + qR[0]) / R(2);
+        const R hsqrtl = sqrt(qL[0]);
+        const R hsqrtr = sqrt(qR[0]);
+        const R hsq2 = hsqrtl + hsqrtr;
+        const R u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+        const R v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+        const R a = sqrt(grav * h);
+        roe[0] = u; roe[1] = v; roe[2] = a;
+
+        const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv];
+        const R a1 = ((u + a) * d0 - d1) / (R(2) * a);
+        const R a2 = -(v * d0) + d2;
+        const R a3 = (-((u - a) * d0) + d1) / (R(2) * a);
+
+        wave[0][0] = a1;   wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
+        wave[1][0] = R(0); wave[1][mu] = R(0);         wave[1][mv] = a2;     s[1] = u;
+        wave[2][0] = a3;   wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;
+
+        // entropy fix
+        const R him1 = qL[0];
+        const R s0 = qL[mu] / him1 - sqrt(grav * him1);
+        if (s0 > R(0) && s[0] > R(0)) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) amdq[m] = R(0);
+        } else {
+            const R h1 = qL[0] + wave[0][0];
+            const R hu1 = qL[mu] + wave[0][mu];
+            const R s1 = hu1 / h1 - sqrt(grav * h1);
+            R sfract;
+            if (s0 < R(0) && s1 > R(0)) sfract = s0 * ((s1 - s[0]) / (s1 - s0));
+            else if (s[0] < R(0)) sfract = s[0];
+            else sfract = R(0);
+#pragma unroll
+            for (int m = 0; m < 3; m++) amdq[m] = sfract * wave[0][m];
+            if (!(s[1] > R(0))) {
+#pragma unroll
+                for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
+                const R hi = qR[0];
+                const R s03 = qR[mu] / hi + sqrt(grav * hi);
+                const R h3 = qR[0] - wave[2][0];
+                const R hu3 = qR[mu] - wave[2][mu];
+                const R s3 = hu3 / h3 + sqrt(grav * h3);
+                bool add = true;
+                if (s3 < R(0) && s03 > R(0)) sfract = s3 * ((s03 - s[2]) / (s03 - s3));
+                else if (s[2] < R(0)) sfract = s[2];
+                else add = false;
+                if (add) {
+#pragma unroll
+                    for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + sfract * wave[2][m];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            R df = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 3; mw++) df = df + s[mw] * wave[mw][m];
+            apdq[m] = df - amdq[m];
+        }
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpt2(const R roe[NROE], const R asdq[MEQN],
+                                                R /*aux_lo*/, R /*aux_hi*/, R bm[MEQN],
+                                                R bp[MEQN]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R u = roe[0], v = roe[1], a = roe[2];
+        const R a1 = ((v + a) * asdq[0] - asdq[mv]) / (R(2) * a);
+        const R a2 = asdq[mu] - u * asdq[0];
+        const R a3 = (-((v - a) * asdq[0]) + asdq[mv]) / (R(2) * a);
+        R wb[3][3], sb[3];
+        wb[0][0] = a1;   wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
+        wb[1][0] = R(0); wb[1][mu] = a2;     wb[1][mv] = R(0);         sb[1] = v;
+        wb[2][0] = a3;   wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            R m_ = R(0), p_ = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 3; mw++) {
+                m_ = m_ + rmin(sb[mw], R(0)) * wb[mw][m];
+                p_ = p_ + rmax(sb[mw], R(0)) * wb[mw][m];
+            }
+            bm[m] = m_;
+            bp[m] = p_;
+        }
+    }
+
+    // physical flux f(q) (IXY = 1) or g(q) (IXY = 2), used by the flux form
+    template <int IXY>
+    __device__ __forceinline__ static void flux(const double* q, const double* /*aux*/,
+                                                const PhysParams& pp, double* f) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const double h = q[0], un = q[mu] / h;
+        f[0] = q[mu];
+        f[mu] = q[mu] * un + 0.5 * pp.grav * h * h;
+        f[mv] = q[mv] * un;
+    }
+};
+
+// ----------------------------------------------------------------- Euler
+template <typename R>
+struct RPEuler {
+    static constexpr int MEQN = 4, MWAVES = 4, MAUX = 0;
+    static constexpr int NROE = 7;  // u, v, a, enth, u2v2, g1a2, euv
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxR*/,
+                                                const PhysParams& pp, R wave[MWAVES][MEQN],
+                                                R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
+                                                R roe[NROE]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R gamma = (R) pp.gamma;
+        const R gamma1 = (R) (pp.gamma - 1.0);
+        const R rhsqrtl = sqrt(qL[0]);
+        const R rhsqrtr = sqrt(qR[0]);
+        const R pl = gamma1 * (qL[3] - R(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]) / qL[0]);
+        const R pr = gamma1 * (qR[3] - R(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]) / qR[0]);
+        const R rhsq2 = rhsqrtl + rhsqrtr;
+        const R u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+        const R v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+        const R enth = (((qL[3] + pl) / rhsqrtl + (qR[3] + pr) / rhsqrtr)) / rhsq2;
+        const R u2v2 = u * u + v * v;
+        const R a2s = gamma1 * (enth - R(.5) * u2v2);
+        const R a = sqrt(a2s);
+        const R g1a2 = gamma1 / a2s;
+        const R euv = enth - u2v2;
+        roe[0] = u; roe[1] = v; roe[2] = a; roe[3] = enth; roe[4] = u2v2; roe[5] = g1a2; roe[6] = euv;
+
+        const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[3] - qL[3];
+        const R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
+        const R a2 = d2 - v * d0;
+        const R a4 = (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
+        const R a1 = d0 - a3 - a4;
+
+        wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
+        wave[0][3] = a1 * (enth - u * a); s[0] = u - a;
+        wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2;
+        wave[1][3] = a2 * v; s[1] = u;
+        wave[2][0] = a3; wave[2][mu] = a3 * u; wave[2][mv] = a3 * v;
+        wave[2][3] = a3 * R(0.5) * u2v2; s[2] = u;
+        wave[3][0] = a4; wave[3][mu] = a4 * (u + a); wave[3][mv] = a4 * v;
+        wave[3][3] = a4 * (enth + u * a); s[3] = u + a;
+
+        // entropy fix
+        const R rhoim1 = qL[0];
+        const R pim1 = gamma1 * (qL[3] - R(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
+        const R cim1 = sqrt(gamma * pim1 / rhoim1);
+        const R s0 = qL[mu] / rhoim1 - cim1;
+        if (s0 >= R(0) && s[0] > R(0)) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) amdq[m] = R(0);
+        } else {
+            const R rho1 = qL[0] + wave[0][0];
+            const R rhou1 = qL[mu] + wave[0][mu];
+            const R rhov1 = qL[mv] + wave[0][mv];
+            const R en1 = qL[3] + wave[0][3];
+            const R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+            const R c1 = sqrt(gamma * p1 / rho1);
+            const R s1 = rhou1 / rho1 - c1;
+            R sfract;
+            if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            else if (s[0] < R(0)) sfract = s[0];
+            else sfract = R(0);
+#pragma unroll
+            for (int m = 0; m < 4; m++) amdq[m] = sfract * wave[0][m];
+            if (!(s[1] >= R(0))) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    amdq[m] = amdq[m] + s[1] * wave[1][m];
+                    amdq[m] = amdq[m] + s[2] * wave[2][m];
+                }
+                const R rhoi = qR[0];
+                const R pi = gamma1 * (qR[3] - R(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
+                const R ci = sqrt(gamma * pi / rhoi);
+                const R s3 = qR[mu] / rhoi + ci;
+                const R rho2 = qR[0] - wave[3][0];
+                const R rhou2 = qR[mu] - wave[3][mu];
+                const R rhov2 = qR[mv] - wave[3][mv];
+                const R en2 = qR[3] - wave[3][3];
+                const R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+                const R c2 = sqrt(gamma * p2 / rho2);
+                const R s2 = rhou2 / rho2 + c2;
+                bool add = true;
+                if (s2 < R(0) && s3 > R(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
+                else if (s[3] < R(0)) sfract = s[3];
+                else add = false;
+                if (add) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + sfract * wave[3][m];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            R df = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 4; mw++) df = df + s[mw] * wave[mw][m];
+            apdq[m] = df - amdq[m];
+        }
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpt2(const R roe[NROE], const R asdq[MEQN],
+                                                R /*aux_lo*/, R /*aux_hi*/, R bm[MEQN],
+                                                R bp[MEQN]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R u = roe[0], v = roe[1], a = roe[2], enth = roe[3], u2v2 = roe[4];
+        const R g1a2 = roe[5], euv = roe[6];
+        const R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
+        const R a2 = asdq[mu] - u * asdq[0];
+        const R a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
+        const R a1 = asdq[0] - a3 - a4;
+        R wb[3][4], sb[3];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
+        wb[0][3] = a1 * (enth - v * a); sb[0] = v - a;
+        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v;
+        wb[1][3] = a3 * R(0.5) * u2v2 + a2 * u; sb[1] = v;
+        wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a);
+        wb[2][3] = a4 * (enth + v * a); sb[2] = v + a;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            R m_ = R(0), p_ = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 3; mw++) {
+                m_ = m_ + rmin(sb[mw], R(0)) * wb[mw][m];
+                p_ = p_ + rmax(sb[mw], R(0)) * wb[mw][m];
+            }
+            bm[m] = m_;
+            bp[m] = p_;
+        }
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void flux(const double* q, const double* /*aux*/,
+                                                const PhysParams& pp, double* f) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const double rho = q[0], un = q[mu] / rho;
+        const double p = (pp.gamma - 1.0) * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / rho);
+        f[0] = q[mu];
+        f[mu] = q[mu] * un + p;
+        f[mv] = q[mv] * un;
+        f[3] = (q[3] + p) * un;
+    }
+};
+
+// ----------------------------------------------------------------- advection
+// aux(:,:,1) = u, aux(:,:,2) = v.  The normal speed is the right cell's aux
+// component IXY; rpt2 receives the transverse velocity at the two cells it
+// needs (aux_lo = aux2(i1,kv), aux_hi = aux3(i1,kv)).
+template <typename R>
+struct RPAdvection {
+    static constexpr int MEQN = 1, MWAVES = 1, MAUX = 2;
+    static constexpr int NROE = 1;
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* auxR,
+                                                const PhysParams&, R wave[MWAVES][MEQN],
+                                                R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
+                                                R roe[NROE]) {
+        const R delta = qR[0] - qL[0];
+        wave[0][0] = delta;
+        s[0] = auxR[IXY - 1];
+        amdq[0] = R(0) + rmin(s[0], R(0)) * wave[0][0];
+        apdq[0] = R(0) + rmax(s[0], R(0)) * wave[0][0];
+        roe[0] = R(0);
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpt2(const R* /*roe*/, const R asdq[MEQN], R aux_lo,
+                                                R aux_hi, R bm[MEQN], R bp[MEQN]) {
+        bm[0] = rmin(aux_lo, R(0)) * asdq[0];
+        bp[0] = rmax(aux_hi, R(0)) * asdq[0];
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void flux(const double* q, const double* aux,
+                                                const PhysParams&, double* f) {
+        f[0] = aux[IXY - 1] * q[0];
+    }
+};
+
+}  // namespace hipclaw

@@ -1,0 +1,59 @@
+/*
+ * Minimal host forest for the batched patch-update path: the leaves of a
+ * forest of quadtrees over a brick of nbx x nby unit blocks (optionally
+ * periodic), kept 2:1 balanced across faces and corners, in p4est's order
+ * (block, then Morton index).  It stands in for the topology queries the
+ * reference takes from p4est (src/forestclaw2d.c: patch face/corner neighbours
+ * and transforms) and produces the device ghost-fill program.
+ *
+ * A patch is identified by (level, gx, gy): its column and row at its level
+ * counted over the whole brick (block bx holds columns bx*2^level ..).
+ */
+#ifndef FCLAW2D_HIP_FOREST_H
+#define FCLAW2D_HIP_FOREST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fclaw2d_hip_forest fclaw2d_hip_forest_t;
+
+/* Uniform forest at `minlevel` over nbx x nby blocks. */
+fclaw2d_hip_forest_t *fclaw2d_hip_forest_new(int nbx, int nby, int periodic_x,
+                                             int periodic_y, int minlevel);
+/* Forest from an explicit leaf list (must tile the brick); it is balanced if
+ * `balance` is nonzero.  Returns NULL on an invalid leaf set. */
+fclaw2d_hip_forest_t *fclaw2d_hip_forest_from_leaves(int nbx, int nby, int periodic_x,
+                                                     int periodic_y, int n, const int *level,
+                                                     const int *gx, const int *gy,
+                                                     int balance);
+void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f);
+
+int fclaw2d_hip_forest_num_patches(const fclaw2d_hip_forest_t *f);
+int fclaw2d_hip_forest_min_level(const fclaw2d_hip_forest_t *f);
+int fclaw2d_hip_forest_max_level(const fclaw2d_hip_forest_t *f);
+/* Copy out the leaves in forest order. */
+void fclaw2d_hip_forest_leaves(const fclaw2d_hip_forest_t *f, int *level, int *gx, int *gy);
+
+/* Refine every flagged leaf (flags[p] != 0) below maxlevel by one level, then
+ * restore the 2:1 face and corner balance by refining further leaves.
+ * Returns the new number of leaves, or -1 on error. */
+int fclaw2d_hip_forest_refine(fclaw2d_hip_forest_t *f, const int *flags, int maxlevel);
+
+/* Ghost-fill program for the current leaves (see hipclaw_set_ghost_program).
+ * mthbc gives the physical BC of the four domain faces.  Writes up to `cap`
+ * ops of 8 ints and phase_start[7]; returns the number of ops needed (call
+ * with cap = 0 to size the buffer), or -1 on error. */
+int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int my, int mbc,
+                                     const int mthbc[4], int *ops, int cap,
+                                     int phase_start[7]);
+
+/* Face neighbour relation of patch p (0 boundary, 1 same size, 2 half size,
+ * 3 double size) -- exposed for tests and the host plugin layer. */
+int fclaw2d_hip_forest_face_relation(const fclaw2d_hip_forest_t *f, int p, int iface,
+                                     int nb[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

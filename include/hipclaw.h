@@ -1,0 +1,148 @@
+/*
+ * hipclaw: C ABI of the MI355X batched patch-update backend.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * HIPCLAW_E* code; hipclaw_last_error() gives the message for the calling
+ * thread.  Patch data keep the reference's FArrayBox layout: for each patch
+ * q(1-mbc:mx+mbc, 1-mbc:my+mbc, meqn), i fastest, then j, then m
+ * (src/patches/clawpatch/fclaw2d_clawpatch.hpp:42-80).
+ *
+ * The backend keeps every patch resident in HBM in one slab per solution
+ * ("slots"); the host exchanges data with it only on request.  This replaces
+ * the reference's per-batch host<->device copies
+ * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_step2.cu:99-129, 245-266).
+ */
+#ifndef HIPCLAW_H
+#define HIPCLAW_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HIPCLAW_OK = 0,
+    HIPCLAW_EINVAL = 1,   /* bad argument */
+    HIPCLAW_ENOMEM = 2,   /* device allocation failed */
+    HIPCLAW_EDEVICE = 3,  /* HIP runtime or kernel launch error */
+    HIPCLAW_ENODEV = 4    /* no usable GPU */
+};
+
+/* Riemann solvers compiled into the backend (device functors, FP32 and FP64).
+ * They replace the device function pointers cuda_rpn2 / cuda_rpt2 of
+ * fc2d_cudaclaw_vtable_t (src/solvers/fc2d_cudaclaw/fc2d_cudaclaw.h:159-162). */
+enum {
+    HIPCLAW_RP_ADVECTION = 0, /* clawpack46_rpn2adv.f / _rpt2adv.f, maux = 2 (u, v) */
+    HIPCLAW_RP_SHALLOW = 1,   /* clawpack46_rpn2.f / _rpt2.f, Roe + entropy fix */
+    HIPCLAW_RP_EULER = 2      /* clawpack46_rpn2_euler4.f / _rpt2_euler4.f, Roe + entropy fix */
+};
+
+enum {
+    HIPCLAW_FP64 = 0,      /* all FP64; bit-faithful to the CPU Clawpack-4.6 path */
+    HIPCLAW_FP32_BULK = 1  /* FP32 Riemann/limiter/transverse, FP64 q, fluxes, CFL, mass */
+};
+
+typedef struct hipclaw_config {
+    int solver;      /* HIPCLAW_RP_* */
+    int precision;   /* HIPCLAW_FP64 or HIPCLAW_FP32_BULK */
+    int mx, my, mbc; /* cells per patch and ghost layers (mbc = 2) */
+    int meqn, maux, mwaves;
+    int order[2];    /* Clawpack method(2:3): order and transverse propagation */
+    int mthlim[4];   /* limiter per wave family */
+    int mthbc[4];    /* physical BC per domain face: 1 extrapolation, 3 wall */
+    double grav;     /* shallow water */
+    double gamma;    /* Euler */
+    int tile;        /* cells per tile side (8, 16, 32); 0 = tuned default */
+} hipclaw_config_t;
+
+typedef struct hipclaw_state hipclaw_state_t;
+
+/* Thread-local message for the last failing call. */
+const char *hipclaw_last_error(void);
+
+/* Select the GPU and report its name.
+ * Replaces fc2d_cudaclaw_initialize_GPUs
+ * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_initialize.cu:37-74). */
+int hipclaw_initialize_device(int device, char *name, size_t name_len);
+
+/* Create a solver state with room for max_patches patches.  One allocation per
+ * slab (q, its ping-pong partner, aux, geometry); no per-patch allocation.
+ * Replaces fc2d_cudaclaw_allocate_buffers and the per-patch
+ * cudaclaw_allocate_fluxes pool
+ * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_allocate.cu:21-132) and
+ * cudaclaw_set_method_parameters
+ * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_flux2.cu:48-54). */
+int hipclaw_create(const hipclaw_config_t *cfg, int max_patches, hipclaw_state_t **out);
+
+/* Release all device memory of the state.
+ * Replaces fc2d_cudaclaw_deallocate_buffers / cudaclaw_deallocate_fluxes. */
+void hipclaw_destroy(hipclaw_state_t *s);
+
+/* Tile size actually used (after defaults). */
+int hipclaw_tile(const hipclaw_state_t *s);
+
+/* Number of patches in use and their mesh widths (slot p has dx[p], dy[p]). */
+int hipclaw_set_geometry(hipclaw_state_t *s, int npatch, const double *dx, const double *dy);
+
+/* Host <-> device patch data for slots [first, first+count).
+ * hipclaw_set_q / hipclaw_get_q replace the packing copies of
+ * cudaclaw_step2_batch (cudaclaw_step2.cu:99-129, 245-266). */
+int hipclaw_set_q(hipclaw_state_t *s, const double *q, int first, int count);
+int hipclaw_get_q(hipclaw_state_t *s, double *q, int first, int count);
+int hipclaw_set_aux(hipclaw_state_t *s, const double *aux, int first, int count);
+
+/* Ghost-fill program: nops ops of 8 int32 each (type, dst, src, which, ti, tj,
+ * bc, 0), grouped into 6 phases by phase_start[0..6]:
+ *   0 average fine->coarse ghost, 1 copy same level, 2 physical BC on coarse
+ *   levels, 3 interpolate coarse->fine ghost, 4 physical BC on fine levels,
+ *   5 physical BC on all levels
+ * (the order of fclaw2d_ghost_update_async, src/fclaw2d_ghost_fill.c:611-830). */
+int hipclaw_set_ghost_program(hipclaw_state_t *s, const int *ops, int nops,
+                              const int *phase_start);
+
+/* Run the ghost-fill program on the current solution.
+ * Replaces the per-face host callbacks of fclaw2d_ghost_update
+ * (src/fclaw2d_ghost_fill.c:838-856) and the Fortran kernels
+ * fclaw2d_clawpatch46_fort_{copy,average,interpolate}_{face,corner}. */
+int hipclaw_ghost_fill(hipclaw_state_t *s);
+
+/* One explicit step of the listed patches (the batch of
+ * fclaw2d_update_single_step), in place, at time step dt.  *maxcfl receives
+ * the batch's maximum Courant number (FP64).
+ * Replaces cudaclaw_step2_batch
+ * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_step2.cu:49-269). */
+int hipclaw_step_batch(hipclaw_state_t *s, const int *slots, int nslots, double t,
+                       double dt, double *maxcfl);
+
+/* nsteps of (update every patch at every level, ghost fill), all resident on
+ * the GPU; *maxcfl receives the maximum over the steps.  Equivalent to
+ * fclaw2d_advance_all_levels with subcycle = F
+ * (src/fclaw2d_advance.c:247-357). */
+int hipclaw_advance(hipclaw_state_t *s, double dt, int nsteps, double *maxcfl);
+
+/* Same as hipclaw_advance but does not wait for the GPU and does not return
+ * the CFL number; hipclaw_sync() waits.  For timing and graph replay. */
+int hipclaw_advance_async(hipclaw_state_t *s, double dt, int nsteps);
+int hipclaw_sync(hipclaw_state_t *s);
+
+/* Capture nsteps of hipclaw_advance_async in a HIP graph (replaces any graph
+ * captured before) and replay it. */
+int hipclaw_capture_graph(hipclaw_state_t *s, double dt, int nsteps);
+int hipclaw_replay_graph(hipclaw_state_t *s);
+
+/* Compensated FP64 sums of q*dx*dy over interior cells, per field:
+ * sum[m] = hi + lo of a double-double accumulation.
+ * Replaces fclaw2d_clawpatch46_fort_conservation_check
+ * (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_diagnostics.f:13-55). */
+int hipclaw_mass(hipclaw_state_t *s, double *sum_hi, double *sum_lo);
+
+/* Timing of the last hipclaw_advance_async/replay region, measured with HIP
+ * events on the state's stream: total milliseconds and milliseconds spent in
+ * the patch-update kernel alone (per launch average). */
+int hipclaw_timing(hipclaw_state_t *s, int nsteps, double *total_ms, double *step_kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

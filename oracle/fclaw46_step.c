@@ -415,8 +415,9 @@ void oracle_inlinelimiter(int maxm, int meqn, int mwaves, int mbc, int mx,
                 wlimitr = dmax(0.0, dmin(dmin(th * r, (1.0 + r) / 2.0), th));
                 break;
             }
-            default:
-                continue; /* computed GOTO out of range falls through to 170 */
+            default: /* computed GOTO out of range falls through to label 10 (minmod) */
+                wlimitr = dmax(0.0, dmin(1.0, r));
+                break;
             }
             for (m = 1; m <= meqn; m++) WV(i, m, mw) = wlimitr * WV(i, m, mw);
         }
