@@ -10,7 +10,7 @@ template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {
     const int nblocks = a.nwork * a.ntx * a.nty;
     if (nblocks <= 0) return 0;
-    hipLaunchKernelGGL((step_tiles<RP, float, T, NW, true>), dim3(nblocks), dim3(NW * 64), 0,
+    hipLaunchKernelGGL((step_tiles<RP, float, T, NW, RP::CONSERVATIVE>), dim3(nblocks), dim3(NW * 64), 0,
                        st, a);
     return (int) hipGetLastError();
 }

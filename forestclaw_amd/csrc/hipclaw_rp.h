@@ -31,10 +31,24 @@ __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
 
+// Division and square root.  FP64 keeps IEEE-correct operations (the parity
+// form); the FP32-bulk form uses the hardware reciprocal and square root
+// (v_rcp_f32, v_sqrt_f32, 1 ulp), whose error is far below the form's
+// stated tolerance.
+template <typename R>
+__device__ __forceinline__ R dv(R a, R b) { return a / b; }
+template <>
+__device__ __forceinline__ float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R>
+__device__ __forceinline__ R sq(R a) { return sqrt(a); }
+template <>
+__device__ __forceinline__ float sq<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // ----------------------------------------------------------------- shallow water
 template <typename R>
 struct RPShallow {
     static constexpr int MEQN = 3, MWAVES = 3, MAUX = 0;
+    static constexpr bool CONSERVATIVE = true;  // amdq + apdq = f(qR) - f(qL)
     static constexpr int NROE = 3;  // u, v, a
 
     template <int IXY>
@@ -44,19 +58,19 @@ struct RPShallow {
                                                 R roe[NROE]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R grav = (R) pp.grav;
-        const R h = (qL[0] + qR[0]) / R(2);
-        const R hsqrtl = sqrt(qL[0]);
-        const R hsqrtr = sqrt(qR[0]);
+        const R h = dv(qL[0] + qR[0], R(2));
+        const R hsqrtl = sq(qL[0]);
+        const R hsqrtr = sq(qR[0]);
         const R hsq2 = hsqrtl + hsqrtr;
-        const R u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
-        const R v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
-        const R a = sqrt(grav * h);
+        const R u = dv(dv(qL[mu], hsqrtl) + dv(qR[mu], hsqrtr), hsq2);
+        const R v = dv(dv(qL[mv], hsqrtl) + dv(qR[mv], hsqrtr), hsq2);
+        const R a = sq(grav * h);
         roe[0] = u; roe[1] = v; roe[2] = a;
 
         const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv];
-        const R a1 = ((u + a) * d0 - d1) / (R(2) * a);
+        const R a1 = dv((u + a) * d0 - d1, R(2) * a);
         const R a2 = -(v * d0) + d2;
-        const R a3 = (-((u - a) * d0) + d1) / (R(2) * a);
+        const R a3 = dv(-((u - a) * d0) + d1, R(2) * a);
 
         wave[0][0] = a1;   wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
         wave[1][0] = R(0); wave[1][mu] = R(0);         wave[1][mv] = a2;     s[1] = u;
@@ -64,16 +78,16 @@ struct RPShallow {
 
         // entropy fix
         const R him1 = qL[0];
-        const R s0 = qL[mu] / him1 - sqrt(grav * him1);
+        const R s0 = dv(qL[mu], him1) - sq(grav * him1);
         if (s0 > R(0) && s[0] > R(0)) {
 #pragma unroll
             for (int m = 0; m < 3; m++) amdq[m] = R(0);
         } else {
             const R h1 = qL[0] + wave[0][0];
             const R hu1 = qL[mu] + wave[0][mu];
-            const R s1 = hu1 / h1 - sqrt(grav * h1);
+            const R s1 = dv(hu1, h1) - sq(grav * h1);
             R sfract;
-            if (s0 < R(0) && s1 > R(0)) sfract = s0 * ((s1 - s[0]) / (s1 - s0));
+            if (s0 < R(0) && s1 > R(0)) sfract = s0 * dv(s1 - s[0], s1 - s0);
             else if (s[0] < R(0)) sfract = s[0];
             else sfract = R(0);
 #pragma unroll
@@ -82,12 +96,12 @@ struct RPShallow {
 #pragma unroll
                 for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
                 const R hi = qR[0];
-                const R s03 = qR[mu] / hi + sqrt(grav * hi);
+                const R s03 = dv(qR[mu], hi) + sq(grav * hi);
                 const R h3 = qR[0] - wave[2][0];
                 const R hu3 = qR[mu] - wave[2][mu];
-                const R s3 = hu3 / h3 + sqrt(grav * h3);
+                const R s3 = dv(hu3, h3) + sq(grav * h3);
                 bool add = true;
-                if (s3 < R(0) && s03 > R(0)) sfract = s3 * ((s03 - s[2]) / (s03 - s3));
+                if (s3 < R(0) && s03 > R(0)) sfract = s3 * dv(s03 - s[2], s03 - s3);
                 else if (s[2] < R(0)) sfract = s[2];
                 else add = false;
                 if (add) {
@@ -111,9 +125,9 @@ struct RPShallow {
                                                 R bp[MEQN]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R u = roe[0], v = roe[1], a = roe[2];
-        const R a1 = ((v + a) * asdq[0] - asdq[mv]) / (R(2) * a);
+        const R a1 = dv((v + a) * asdq[0] - asdq[mv], R(2) * a);
         const R a2 = asdq[mu] - u * asdq[0];
-        const R a3 = (-((v - a) * asdq[0]) + asdq[mv]) / (R(2) * a);
+        const R a3 = dv(-((v - a) * asdq[0]) + asdq[mv], R(2) * a);
         R wb[3][3], sb[3];
         wb[0][0] = a1;   wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2;     wb[1][mv] = R(0);         sb[1] = v;
@@ -147,6 +161,7 @@ struct RPShallow {
 template <typename R>
 struct RPEuler {
     static constexpr int MEQN = 4, MWAVES = 4, MAUX = 0;
+    static constexpr bool CONSERVATIVE = true;
     static constexpr int NROE = 7;  // u, v, a, enth, u2v2, g1a2, euv
 
     template <int IXY>
@@ -157,25 +172,25 @@ struct RPEuler {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R gamma = (R) pp.gamma;
         const R gamma1 = (R) (pp.gamma - 1.0);
-        const R rhsqrtl = sqrt(qL[0]);
-        const R rhsqrtr = sqrt(qR[0]);
-        const R pl = gamma1 * (qL[3] - R(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]) / qL[0]);
-        const R pr = gamma1 * (qR[3] - R(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]) / qR[0]);
+        const R rhsqrtl = sq(qL[0]);
+        const R rhsqrtr = sq(qR[0]);
+        const R pl = gamma1 * (qL[3] - dv(R(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]), qL[0]));
+        const R pr = gamma1 * (qR[3] - dv(R(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]), qR[0]));
         const R rhsq2 = rhsqrtl + rhsqrtr;
-        const R u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
-        const R v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
-        const R enth = (((qL[3] + pl) / rhsqrtl + (qR[3] + pr) / rhsqrtr)) / rhsq2;
+        const R u = dv(dv(qL[mu], rhsqrtl) + dv(qR[mu], rhsqrtr), rhsq2);
+        const R v = dv(dv(qL[mv], rhsqrtl) + dv(qR[mv], rhsqrtr), rhsq2);
+        const R enth = dv(dv(qL[3] + pl, rhsqrtl) + dv(qR[3] + pr, rhsqrtr), rhsq2);
         const R u2v2 = u * u + v * v;
         const R a2s = gamma1 * (enth - R(.5) * u2v2);
-        const R a = sqrt(a2s);
-        const R g1a2 = gamma1 / a2s;
+        const R a = sq(a2s);
+        const R g1a2 = dv(gamma1, a2s);
         const R euv = enth - u2v2;
         roe[0] = u; roe[1] = v; roe[2] = a; roe[3] = enth; roe[4] = u2v2; roe[5] = g1a2; roe[6] = euv;
 
         const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[3] - qL[3];
         const R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
         const R a2 = d2 - v * d0;
-        const R a4 = (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
+        const R a4 = dv(d1 + (a - u) * d0 - a * a3, R(2) * a);
         const R a1 = d0 - a3 - a4;
 
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
@@ -189,9 +204,9 @@ struct RPEuler {
 
         // entropy fix
         const R rhoim1 = qL[0];
-        const R pim1 = gamma1 * (qL[3] - R(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
-        const R cim1 = sqrt(gamma * pim1 / rhoim1);
-        const R s0 = qL[mu] / rhoim1 - cim1;
+        const R pim1 = gamma1 * (qL[3] - dv(R(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]), rhoim1));
+        const R cim1 = sq(dv(gamma * pim1, rhoim1));
+        const R s0 = dv(qL[mu], rhoim1) - cim1;
         if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
             for (int m = 0; m < 4; m++) amdq[m] = R(0);
@@ -200,11 +215,11 @@ struct RPEuler {
             const R rhou1 = qL[mu] + wave[0][mu];
             const R rhov1 = qL[mv] + wave[0][mv];
             const R en1 = qL[3] + wave[0][3];
-            const R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
-            const R c1 = sqrt(gamma * p1 / rho1);
-            const R s1 = rhou1 / rho1 - c1;
+            const R p1 = gamma1 * (en1 - dv(R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1), rho1));
+            const R c1 = sq(dv(gamma * p1, rho1));
+            const R s1 = dv(rhou1, rho1) - c1;
             R sfract;
-            if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            if (s0 < R(0) && s1 > R(0)) sfract = dv(s0 * (s1 - s[0]), s1 - s0);
             else if (s[0] < R(0)) sfract = s[0];
             else sfract = R(0);
 #pragma unroll
@@ -216,18 +231,18 @@ struct RPEuler {
                     amdq[m] = amdq[m] + s[2] * wave[2][m];
                 }
                 const R rhoi = qR[0];
-                const R pi = gamma1 * (qR[3] - R(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
-                const R ci = sqrt(gamma * pi / rhoi);
-                const R s3 = qR[mu] / rhoi + ci;
+                const R pi = gamma1 * (qR[3] - dv(R(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]), rhoi));
+                const R ci = sq(dv(gamma * pi, rhoi));
+                const R s3 = dv(qR[mu], rhoi) + ci;
                 const R rho2 = qR[0] - wave[3][0];
                 const R rhou2 = qR[mu] - wave[3][mu];
                 const R rhov2 = qR[mv] - wave[3][mv];
                 const R en2 = qR[3] - wave[3][3];
-                const R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
-                const R c2 = sqrt(gamma * p2 / rho2);
-                const R s2 = rhou2 / rho2 + c2;
+                const R p2 = gamma1 * (en2 - dv(R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2), rho2));
+                const R c2 = sq(dv(gamma * p2, rho2));
+                const R s2 = dv(rhou2, rho2) + c2;
                 bool add = true;
-                if (s2 < R(0) && s3 > R(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
+                if (s2 < R(0) && s3 > R(0)) sfract = dv(s2 * (s3 - s[3]), s3 - s2);
                 else if (s[3] < R(0)) sfract = s[3];
                 else add = false;
                 if (add) {
@@ -254,7 +269,7 @@ struct RPEuler {
         const R g1a2 = roe[5], euv = roe[6];
         const R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         const R a2 = asdq[mu] - u * asdq[0];
-        const R a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
+        const R a4 = dv(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
         const R a1 = asdq[0] - a3 - a4;
         R wb[3][4], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
@@ -296,6 +311,9 @@ struct RPEuler {
 template <typename R>
 struct RPAdvection {
     static constexpr int MEQN = 1, MWAVES = 1, MAUX = 2;
+    // colour equation q_t + u q_x + v q_y = 0: the fluctuations are not a flux
+    // difference unless the velocity is uniform, so no flux form exists
+    static constexpr bool CONSERVATIVE = false;
     static constexpr int NROE = 1;
 
     template <int IXY>

@@ -53,15 +53,15 @@ template <typename R>
 __device__ __forceinline__ R philim(int mth, R r) {
     switch (mth) {
     case 2: return rmax(rmax(R(0), rmin(R(1), R(2) * r)), rmin(R(2), r));
-    case 3: return (r + fabs(r)) / (R(1) + fabs(r));
+    case 3: return dv(r + fabs(r), R(1) + fabs(r));
     case 4: {
-        const R c = (R(1) + r) / R(2);
+        const R c = dv(R(1) + r, R(2));
         return rmax(R(0), rmin(rmin(c, R(2)), R(2) * r));
     }
     case 5: return r;
     case 6: {
         const R th = (R) 1.3f;  // REAL*4 literal in clawpack46_inlinelimiter.f:101
-        return rmax(R(0), rmin(rmin(th * r, (R(1) + r) / R(2)), th));
+        return rmax(R(0), rmin(rmin(th * r, dv(R(1) + r, R(2))), th));
     }
     default: return rmax(R(0), rmin(R(1), r));  // 1 (and the computed-GOTO fall-through)
     }
@@ -70,15 +70,28 @@ __device__ __forceinline__ R philim(int mth, R r) {
 template <typename R>
 __device__ __forceinline__ R shfl(R v, int src) { return __shfl(v, src, 64); }
 
+// The reference skips the limiter for a wave of zero norm
+// (clawpack46_inlinelimiter.f:51).  FP64 keeps that exact test; in FP32 a norm
+// in the denormal range is treated as zero, since the hardware reciprocal
+// flushes it and dotl/wn would become 0 * inf.
+__device__ __forceinline__ bool wave_nonzero(double wn) { return wn != 0.0; }
+__device__ __forceinline__ bool wave_nonzero(float wn) { return wn >= 1.17549435e-38f; }
+
+// LDS layout.  Rows that a wave walks across lanes in the transverse direction
+// get an odd stride so that the 64 lanes of a y sweep (or of the final stage)
+// fall into distinct banks.
 template <int T, int MEQN, int MAUX, typename R, bool FLUXFORM>
 struct TileSmem {
-    static constexpr int QN = (T + 4) * (T + 4);
-    R q[MEQN * QN];
-    R aux[MAUX > 0 ? MAUX * QN : 1];
+    static constexpr int QW = T + 4;            // q tile width (cells i0-2 .. i0+T+1)
+    static constexpr int QS = QW | 1;           // padded row stride
+    static constexpr int QP = QW * QS;          // one field
+    static constexpr int GS = T + 1;            // padded row stride of gt and ab
+    R q[MEQN * QP];
+    R aux[MAUX > 0 ? MAUX * QP : 1];
     R fm[MEQN * T * (T + 1)];       // x-interface slot s = i - i0 in [0, T], row b
     R fp[FLUXFORM ? 1 : MEQN * T * (T + 1)];
-    R gt[MEQN * (T + 1) * T];       // y-interface slot t = j - j0 in [0, T], col a
-    R ab[2 * MEQN * (T + 2) * T];   // transverse corrections (x sweep, then y sweep)
+    R gt[MEQN * (T + 1) * GS];      // y-interface slot t = j - j0 in [0, T], col a
+    R ab[2 * MEQN * (T + 2) * GS];  // transverse corrections (x sweep, then y sweep)
     double fx[FLUXFORM ? MEQN * T * (T + 1) : 1];
     double fy[FLUXFORM ? MEQN * (T + 1) * T : 1];
 };
@@ -99,11 +112,11 @@ template <class RP, typename R, int T, int NW, bool FLUXFORM>
 __global__ void __launch_bounds__(NW * 64)
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
-    constexpr int QW = T + 4;              // q tile width
+    using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM>;
+    constexpr int QW = Smem::QW, QS = Smem::QS, QP = Smem::QP, GS = Smem::GS;
     constexpr int L = T + 3;               // lanes per row/column of interfaces
     constexpr int SEG = 64 / L;            // rows (columns) per wave
     static_assert(SEG >= 1, "tile too wide for one wave");
-    using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM>;
     __shared__ Smem sm;
 
     const int tiles_per_patch = a.ntx * a.nty;
@@ -127,7 +140,7 @@ step_tiles(StepArgs a) {
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
         const int i = i0 - 2 + qi, j = j0 - 2 + qj;  // 1-based patch indices
-        sm.q[e] = (R) qp[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
+        sm.q[m * QP + qj * QS + qi] = (R) qp[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
     }
     if constexpr (MAUX > 0) {
         const double* __restrict__ ap = a.aux + (long long) slot * a.astride;
@@ -135,7 +148,7 @@ step_tiles(StepArgs a) {
             const int m = e / (QW * QW), rem = e - m * QW * QW;
             const int qj = rem / QW, qi = rem - qj * QW;
             const int i = i0 - 2 + qi, j = j0 - 2 + qj;
-            sm.aux[e] = (R) ap[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
+            sm.aux[m * QP + qj * QS + qi] = (R) ap[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
         }
     }
     __syncthreads();
@@ -144,15 +157,17 @@ step_tiles(StepArgs a) {
     const int seg = lane / L, k = lane - seg * L;
     double cfl = 0.0;
 
-    auto SQ = [&](int m, int qi, int qj) -> R { return sm.q[m * QW * QW + qj * QW + qi]; };
+    auto SQ = [&](int m, int qi, int qj) -> R { return sm.q[m * QP + qj * QS + qi]; };
     auto SA = [&](int m, int qi, int qj) -> R {
-        if constexpr (MAUX > 0) return sm.aux[m * QW * QW + qj * QW + qi];
+        if constexpr (MAUX > 0) return sm.aux[m * QP + qj * QS + qi];
         else return R(0);
     };
     // ab[w][m][r][a]
     auto AB = [&](int w, int m, int r, int c) -> R& {
-        return sm.ab[((w * MEQN + m) * (T + 2) + r) * T + c];
+        return sm.ab[((w * MEQN + m) * (T + 2) + r) * GS + c];
     };
+    // gt[m][t][a]
+    auto GT = [&](int m, int t, int c) -> R& { return sm.gt[(m * (T + 1) + t) * GS + c]; };
 
     // ============================ x sweep ============================
     // lane (seg, k): row j = j0-1+r, interface i = i0-1+k between cells i-1 and i
@@ -189,8 +204,8 @@ step_tiles(StepArgs a) {
                     dotl = dotl + wl * wave_[mw][m];
                 }
                 const R dotr = shfl(dotl, lane + 1);
-                if (k >= 1 && k <= T + 1 && wn != R(0)) {
-                    const R rr = (s[mw] > R(0) ? dotl : dotr) / wn;
+                if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
+                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                     const R phi = philim<R>(mth, rr);
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
@@ -247,7 +262,7 @@ step_tiles(StepArgs a) {
     for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NW * 64) {
         const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
         const int t = rem / T, c = rem - t * T;
-        sm.gt[e] = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+        GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
     }
     __syncthreads();
 
@@ -286,8 +301,8 @@ step_tiles(StepArgs a) {
                     dotl = dotl + wl * wave_[mw][m];
                 }
                 const R dotr = shfl(dotl, lane + 1);
-                if (k >= 1 && k <= T + 1 && wn != R(0)) {
-                    const R rr = (s[mw] > R(0) ? dotl : dotr) / wn;
+                if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
+                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                     const R phi = philim<R>(mth, rr);
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
@@ -308,7 +323,7 @@ step_tiles(StepArgs a) {
             if (act && own_col && k >= 1 && k <= T + 1) {
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    R& g = sm.gt[(m * (T + 1) + (k - 1)) * T + (c - 1)];
+                    R& g = GT(m, k - 1, c - 1);
                     g = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
                 }
             }
@@ -318,7 +333,7 @@ step_tiles(StepArgs a) {
             const int ccol = own_col ? c - 1 : 0;
 #pragma unroll
             for (int m = 0; m < MEQN; m++) {
-                const R g = sm.gt[(m * (T + 1) + kk) * T + ccol];
+                const R g = GT(m, kk, ccol);
                 gm[m] = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
                 gp[m] = g + ((R(0) - apdq[m]) + R(0.5) * cqxx[m]);
             }
@@ -326,7 +341,7 @@ step_tiles(StepArgs a) {
             for (int m = 0; m < MEQN; m++) {
                 const R gm_up = shfl(gm[m], lane + 1);
                 if (act && own_col && k >= 1 && k <= T)
-                    sm.gt[(m * (T + 1) + (k - 1)) * T + (c - 1)] = gm_up - gp[m];  // Dy(i, j)
+                    GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
             }
         }
         R ty1[MEQN], ty2[MEQN];
@@ -372,11 +387,16 @@ step_tiles(StepArgs a) {
             const R fmR = ((R(0) + sm.fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
             const R fpL = ((R(0) + sm.fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
             const R Dx = fmR - fpL;
-            const R Dy = sm.gt[(m * (T + 1) + b) * T + c];
-            const R qc = SQ(m, c + 2, b + 2);
-            const R qn = (qc - dtdx * Dx) - dtdy * Dy;
+            const R Dy = GT(m, b, c);
             const int i = i0 + c, j = j0 + b;
-            qo[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)] = (double) qn;
+            const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
+            if constexpr (sizeof(R) == sizeof(double)) {
+                const R qc = SQ(m, c + 2, b + 2);
+                qo[off] = (qc - dtdx * Dx) - dtdy * Dy;
+            } else {
+                // FP32 fluctuations, FP64 state: q itself never passes through FP32
+                qo[off] = (qp[off] - dtdx64 * (double) Dx) - dtdy64 * (double) Dy;
+            }
         }
     } else {
         // conservative FP64 interface fluxes, shared by both neighbours
@@ -411,7 +431,7 @@ step_tiles(StepArgs a) {
                 RP::template flux<2>(qcell, acell, a.pp, fl);
 #pragma unroll
                 for (int m = 0; m < MEQN; m++)
-                    sm.fy[(m * (T + 1) + t) * T + c] = fl[m] + (double) sm.gt[(m * (T + 1) + t) * T + c];
+                    sm.fy[(m * (T + 1) + t) * T + c] = fl[m] + (double) GT(m, t, c);
             }
         }
         __syncthreads();

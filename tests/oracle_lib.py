@@ -60,10 +60,9 @@ def oracle():
         lib.oracle_bc2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip]
         lib.oracle_inlinelimiter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                              _dp, _dp, _ip]
-        for name in ("oracle_copy_face", "oracle_copy_corner", "oracle_average_face",
-                     "oracle_average_corner"):
-            getattr(lib, name).argtypes = ([C.c_int] * 4 + [_dp, _dp] +
-                                           [C.c_int] * (4 if "face" in name else 3))
+        for name, nint in (("oracle_copy_face", 3), ("oracle_copy_corner", 3),
+                           ("oracle_average_face", 4), ("oracle_average_corner", 3)):
+            getattr(lib, name).argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * nint
         lib.oracle_interpolate_face.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 4
         lib.oracle_interpolate_corner.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 3
         _oracle = lib

@@ -1,0 +1,52 @@
+"""The C-ABI library loads and exports every function declared in include/*.h
+(no GPU calls)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import forestclaw_amd as fa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"typedef[^;]*;", "", text)
+        text = re.sub(r"(struct|enum)\s+\w*\s*\{[^}]*\}\s*\w*\s*;", "", text, flags=re.S)
+        for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            start = text.rfind("\n", 0, m.start())
+            line = text[start + 1:m.start()]
+            if line.strip().startswith("#") or name in ("sizeof", "defined"):
+                continue
+            if re.search(r"(^|[\s\*])(int|void|double|char|const|fclaw2d_\w+|fc2d_\w+|hipclaw_\w+)[\s\*]*$",
+                         line.rstrip() + " ") or line.strip() == "":
+                names.add(name)
+    return names
+
+
+def test_library_exports_all_declared_symbols():
+    lib = fa.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    import importlib
+    monkeypatch.setattr(fa, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(fa, "_lib", None)
+    try:
+        fa.lib()
+    except RuntimeError as e:
+        assert "missing" in str(e)
+    else:
+        raise AssertionError("expected a RuntimeError")
+    finally:
+        importlib.reload(fa)
