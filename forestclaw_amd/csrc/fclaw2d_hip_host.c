@@ -1,0 +1,546 @@
+/*
+ * Host side of the batched patch update, in C: the reference's plugin surface
+ * (declared in include/fclaw2d_hip_core.h and include/fc2d_hipclaw.h) on top
+ * of the device C ABI (include/hipclaw.h).
+ *
+ *   fclaw2d_update_single_step : src/fclaw2d_update_single_step.c:32-103
+ *   hipclaw_update (the single_step_update slot) : fc2d_cudaclaw.cpp:272-368
+ *   fc2d_hipclaw_solver_initialize : fc2d_cudaclaw.cpp:405-465
+ *   options defaults and checks : fc2d_cudaclaw_options.c:36-115
+ *
+ * Error behaviour follows the reference: a failing device call prints the
+ * backend's message and exits with the error code
+ * (src/solvers/fc2d_cudaclaw/fc2d_cudaclaw_check.h:3-11); option checks
+ * return an error code.
+ */
+#include "../../include/fc2d_hipclaw.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void device_check(int rc, const char *what)
+{
+    if (rc != HIPCLAW_OK) {
+        fprintf(stderr, "fc2d_hipclaw: %s failed: %s\n", what, hipclaw_last_error());
+        exit(rc);
+    }
+}
+
+/* ------------------------------------------------------------------ globals */
+
+fclaw2d_global_t *fclaw2d_global_new(void)
+{
+    return (fclaw2d_global_t *) calloc(1, sizeof(fclaw2d_global_t));
+}
+
+void fclaw2d_global_destroy(fclaw2d_global_t *glob)
+{
+    if (!glob) return;
+    fc2d_hipclaw_deallocate_buffers(glob);
+    fclaw2d_domain_destroy(glob->domain);
+    free(glob->hipclaw_vt);
+    free(glob->hipclaw_opt);
+    free(glob);
+}
+
+fclaw2d_domain_t *fclaw2d_domain_new_from_forest(fclaw2d_hip_forest_t *forest, int nbx, int nby,
+                                                 int periodic_x, int periodic_y, double ax,
+                                                 double bx, double ay, double by)
+{
+    fclaw2d_domain_t *d;
+    int n, p, b, *lv, *gx, *gy;
+    if (!forest) return NULL;
+    n = fclaw2d_hip_forest_num_patches(forest);
+    d = (fclaw2d_domain_t *) calloc(1, sizeof(*d));
+    lv = (int *) malloc(sizeof(int) * n);
+    gx = (int *) malloc(sizeof(int) * n);
+    gy = (int *) malloc(sizeof(int) * n);
+    if (!d || !lv || !gx || !gy) { free(d); free(lv); free(gx); free(gy); return NULL; }
+    fclaw2d_hip_forest_leaves(forest, lv, gx, gy);
+    d->num_blocks = nbx * nby;
+    d->blocks = (fclaw2d_block_t *) calloc((size_t) d->num_blocks, sizeof(fclaw2d_block_t));
+    d->local_num_patches = n;
+    d->global_minlevel = fclaw2d_hip_forest_min_level(forest);
+    d->global_maxlevel = fclaw2d_hip_forest_max_level(forest);
+    d->nbx = nbx; d->nby = nby; d->periodic_x = periodic_x; d->periodic_y = periodic_y;
+    d->ax = ax; d->bx = bx; d->ay = ay; d->by = by;
+    d->forest = forest;
+    /* leaves are ordered by block, then Morton index: count, then fill */
+    for (p = 0; p < n; p++) {
+        const int blk = (gx[p] >> lv[p]) + nbx * (gy[p] >> lv[p]);
+        d->blocks[blk].num_patches++;
+    }
+    for (b = 0, p = 0; b < d->num_blocks; b++) {
+        d->blocks[b].patchno_offset = p;
+        d->blocks[b].patches = (fclaw2d_patch_t *) calloc((size_t) d->blocks[b].num_patches + 1,
+                                                          sizeof(fclaw2d_patch_t));
+        p += d->blocks[b].num_patches;
+    }
+    for (p = 0; p < n; p++) {
+        const int L = lv[p], bxi = gx[p] >> L, byi = gy[p] >> L, blk = bxi + nbx * byi;
+        const int k = p - d->blocks[blk].patchno_offset;
+        fclaw2d_patch_t *pt = &d->blocks[blk].patches[k];
+        const double w = ldexp(1.0, -L);
+        pt->level = L;
+        pt->xlower = (gx[p] - (bxi << L)) * w; pt->xupper = pt->xlower + w;
+        pt->ylower = (gy[p] - (byi << L)) * w; pt->yupper = pt->ylower + w;
+        pt->blockno = blk;
+        pt->patchno = k;
+        pt->slot = p;
+    }
+    free(lv); free(gx); free(gy);
+    return d;
+}
+
+void fclaw2d_domain_destroy(fclaw2d_domain_t *d)
+{
+    int b;
+    if (!d) return;
+    for (b = 0; b < d->num_blocks; b++) free(d->blocks[b].patches);
+    free(d->blocks);
+    fclaw2d_hip_forest_destroy(d->forest);
+    free(d);
+}
+
+void fclaw2d_global_store_domain(fclaw2d_global_t *glob, fclaw2d_domain_t *domain)
+{
+    glob->domain = domain;
+}
+
+fclaw2d_patch_vtable_t *fclaw2d_patch_vt(fclaw2d_global_t *glob) { return &glob->patch_vt; }
+
+/* ------------------------------------------------------------------ clawpatch views */
+
+static size_t patch_size(const fclaw2d_global_t *glob, int fields)
+{
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    return (size_t) fields * (o->mx + 2 * o->mbc) * (o->my + 2 * o->mbc);
+}
+
+void fclaw2d_clawpatch_grid_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int *mx,
+                                 int *my, int *mbc, double *xlower, double *ylower, double *dx,
+                                 double *dy)
+{
+    const fclaw2d_domain_t *d = glob->domain;
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
+    const int bxi = patch->blockno % d->nbx, byi = patch->blockno / d->nbx;
+    *mx = o->mx; *my = o->my; *mbc = o->mbc;
+    *xlower = d->ax + (bxi + patch->xlower) * wx;
+    *ylower = d->ay + (byi + patch->ylower) * wy;
+    *dx = (patch->xupper - patch->xlower) * wx / o->mx;
+    *dy = (patch->yupper - patch->ylower) * wy / o->my;
+}
+
+void fclaw2d_clawpatch_soln_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, double **q,
+                                 int *meqn)
+{
+    *meqn = glob->clawpatch_opt.meqn;
+    *q = glob->qhost ? glob->qhost + patch_size(glob, *meqn) * patch->slot : NULL;
+}
+
+void fclaw2d_clawpatch_aux_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, double **aux,
+                                int *maux)
+{
+    *maux = glob->clawpatch_opt.maux;
+    *aux = (glob->auxhost && *maux > 0) ? glob->auxhost + patch_size(glob, *maux) * patch->slot
+                                        : NULL;
+}
+
+/* ------------------------------------------------------------------ options */
+
+void fc2d_hipclaw_options_default(fc2d_hipclaw_options_t *opt, int mwaves)
+{
+    int k;
+    memset(opt, 0, sizeof(*opt));
+    opt->mwaves = mwaves;
+    opt->order[0] = 2; opt->order[1] = 2;
+    for (k = 0; k < FC2D_HIPCLAW_MWAVES; k++) opt->mthlim[k] = 1;
+    for (k = 0; k < 4; k++) opt->mthbc[k] = 1;
+    opt->buffer_len = 1;
+    opt->precision = HIPCLAW_FP64;
+    opt->grav = 1.0;
+    opt->gamma = 1.4;
+    opt->is_registered = 1;
+}
+
+int hipclaw_check_parameters(int mwaves)
+{
+    if (mwaves > FC2D_HIPCLAW_MWAVES) {
+        fprintf(stderr, "fc2d_hipclaw : mwaves > FC2D_HIPCLAW_MWAVES (%d)\n", FC2D_HIPCLAW_MWAVES);
+        return 1;
+    }
+    return 0;
+}
+
+int fc2d_hipclaw_check(const fc2d_hipclaw_options_t *opt)
+{
+    if (hipclaw_check_parameters(opt->mwaves)) return 1;
+    if (opt->buffer_len < 1) return 1;
+    if (opt->mcapa != 0 || opt->use_fwaves != 0) {
+        fprintf(stderr, "fc2d_hipclaw : mcapa and f-waves are not supported\n");
+        return 1;
+    }
+    return 0;
+}
+
+void fc2d_hipclaw_options_store(fclaw2d_global_t *glob, fc2d_hipclaw_options_t *opt)
+{
+    if (!glob->hipclaw_opt) glob->hipclaw_opt = (fc2d_hipclaw_options_t *) malloc(sizeof(*opt));
+    *glob->hipclaw_opt = *opt;
+}
+
+fc2d_hipclaw_options_t *fc2d_hipclaw_get_options(fclaw2d_global_t *glob) { return glob->hipclaw_opt; }
+
+void hipclaw_set_method_parameters(fclaw2d_global_t *glob, const int order[], const int mthlim[],
+                                   int mwaves, int use_fwaves)
+{
+    fc2d_hipclaw_options_t *o = glob->hipclaw_opt;
+    int k;
+    o->order[0] = order[0]; o->order[1] = order[1];
+    for (k = 0; k < mwaves && k < FC2D_HIPCLAW_MWAVES; k++) o->mthlim[k] = mthlim[k];
+    o->mwaves = mwaves;
+    o->use_fwaves = use_fwaves;
+}
+
+/* ------------------------------------------------------------------ buffering */
+
+void hipclaw_store_buffer(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int patchno,
+                          int count, int iter, int *slot_buffer)
+{
+    (void) patchno; (void) count;
+    slot_buffer[iter % glob->hipclaw_opt->buffer_len] = patch->slot;
+}
+
+double hipclaw_step2_batch(fclaw2d_global_t *glob, const int *slot_buffer, int batch_size,
+                           double t, double dt)
+{
+    double maxcfl = 0.0;
+    device_check(hipclaw_step_batch(glob->state, slot_buffer, batch_size, t, dt, &maxcfl),
+                 "hipclaw_step_batch");
+    return maxcfl;
+}
+
+static fclaw2d_patch_t *patch_of_slot(fclaw2d_global_t *glob, int slot, int *blockno, int *patchno)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    int b;
+    for (b = d->num_blocks - 1; b >= 0; b--)
+        if (slot >= d->blocks[b].patchno_offset) break;
+    *blockno = b;
+    *patchno = slot - d->blocks[b].patchno_offset;
+    return &d->blocks[b].patches[*patchno];
+}
+
+/* The single_step_update slot (cudaclaw_update, fc2d_cudaclaw.cpp:272-368):
+ * buffer the patch; on a full buffer or the last patch, step the batch on the
+ * device and return its CFL number (0 otherwise).  b4step2 is not called, as
+ * on the reference's GPU path. */
+static double hipclaw_update(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                             int patchno, double t, double dt, void *user)
+{
+    fclaw2d_single_step_buffer_data_t *bd = (fclaw2d_single_step_buffer_data_t *) user;
+    const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
+    const int len = opt->buffer_len, iter = bd->iter, total = bd->total_count;
+    int *buf, n = 0, k;
+    double maxcfl = 0.0;
+    (void) blockno;
+    if (iter == 0) {
+        const int size = total < len ? total : len;
+        bd->user = malloc(sizeof(int) * (size_t) (size > 0 ? size : 1));
+    }
+    buf = (int *) bd->user;
+    hipclaw_store_buffer(glob, patch, patchno, total, iter, buf);
+    if ((iter + 1) % len == 0) n = len;
+    else if (iter + 1 == total) n = total % len;
+    if (n > 0) {
+        maxcfl = hipclaw_step2_batch(glob, buf, n, t, dt);
+        if (opt->src_term > 0 && glob->hipclaw_vt->src2) {
+            /* source term on the host views of the batch just stepped */
+            for (k = 0; k < n; k++) {
+                int b, p;
+                fclaw2d_patch_t *pt = patch_of_slot(glob, buf[k], &b, &p);
+                device_check(hipclaw_get_q(glob->state, glob->qhost + patch_size(glob, glob->clawpatch_opt.meqn) * buf[k], buf[k], 1), "hipclaw_get_q");
+                glob->hipclaw_vt->src2(glob, pt, b, p, t, dt);
+                device_check(hipclaw_set_q(glob->state, glob->qhost + patch_size(glob, glob->clawpatch_opt.meqn) * buf[k], buf[k], 1), "hipclaw_set_q");
+            }
+        }
+    }
+    if (iter == total - 1) {
+        free(bd->user);
+        bd->user = NULL;
+    }
+    return maxcfl;
+}
+
+/* ------------------------------------------------------------------ core drivers */
+
+double fclaw2d_update_single_step(fclaw2d_global_t *glob, int level, double t, double dt)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    fclaw2d_single_step_buffer_data_t bd;
+    double maxcfl = 0.0;
+    int b, p, count = 0;
+    bd.total_count = 0; bd.iter = 0; bd.user = NULL;
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++)
+            if (d->blocks[b].patches[p].level == level) count++;
+    bd.total_count = count;
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++) {
+            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+            double cfl;
+            if (pt->level != level) continue;
+            cfl = glob->patch_vt.single_step_update(glob, pt, b, p, t, dt, &bd);
+            bd.iter++;
+            glob->count_single_step++;
+            maxcfl = fmax(cfl, maxcfl);
+        }
+    return maxcfl;
+}
+
+void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel, double sync_time,
+                          int time_interp, int running)
+{
+    (void) sync_time; (void) running;
+    if (time_interp) {
+        fprintf(stderr, "fclaw2d_ghost_update: time interpolation (subcycling) is not supported\n");
+        exit(1);
+    }
+    if (minlevel > glob->domain->global_minlevel || maxlevel < glob->domain->global_maxlevel) {
+        fprintf(stderr, "fclaw2d_ghost_update: partial level ranges are not supported\n");
+        exit(1);
+    }
+    device_check(hipclaw_ghost_fill(glob->state), "hipclaw_ghost_fill");
+    glob->count_ghost_update++;
+}
+
+double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    double maxcfl = 0.0;
+    int level;
+    if (glob->patch_vt.single_step_update == hipclaw_update) {
+        /* every level in one batched launch, then the device ghost fill */
+        device_check(hipclaw_advance(glob->state, dt, 1, &maxcfl), "hipclaw_advance");
+        glob->count_single_step += d->local_num_patches;
+        glob->count_ghost_update++;
+    } else {
+        for (level = d->global_maxlevel; level >= d->global_minlevel; level--)
+            maxcfl = fmax(maxcfl, fclaw2d_update_single_step(glob, level, t_curr, dt));
+        fclaw2d_ghost_update(glob, d->global_minlevel, d->global_maxlevel, t_curr + dt, 0, 0);
+    }
+    glob->curr_time = t_curr + dt;
+    glob->curr_dt = dt;
+    return maxcfl;
+}
+
+/* ------------------------------------------------------------------ solver plugin */
+
+static void hipclaw_qinit_slot(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                               int patchno)
+{
+    fc2d_hipclaw_qinit(glob, patch, blockno, patchno);
+}
+
+static void hipclaw_setaux_slot(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                int patchno)
+{
+    fc2d_hipclaw_setaux(glob, patch, blockno, patchno);
+}
+
+void fc2d_hipclaw_solver_initialize(fclaw2d_global_t *glob)
+{
+    fc2d_hipclaw_vtable_t *vt = (fc2d_hipclaw_vtable_t *) calloc(1, sizeof(*vt));
+    if (glob->hipclaw_vt) {
+        fprintf(stderr, "fc2d_hipclaw_solver_initialize: called twice\n");
+        exit(1);
+    }
+    vt->riemann_solver = HIPCLAW_RP_SHALLOW;
+    vt->is_set = 1;
+    glob->hipclaw_vt = vt;
+    glob->patch_vt.initialize = hipclaw_qinit_slot;
+    glob->patch_vt.setup = hipclaw_setaux_slot;
+    glob->patch_vt.physical_bc = fc2d_hipclaw_bc2;
+    glob->patch_vt.single_step_update = hipclaw_update;
+    glob->patch_vt.is_set = 1;
+}
+
+fc2d_hipclaw_vtable_t *fc2d_hipclaw_vt(fclaw2d_global_t *glob)
+{
+    if (!glob->hipclaw_vt || !glob->hipclaw_vt->is_set) {
+        fprintf(stderr, "fc2d_hipclaw_vt: solver not initialized\n");
+        exit(1);
+    }
+    return glob->hipclaw_vt;
+}
+
+void fc2d_hipclaw_initialize_GPUs(fclaw2d_global_t *glob)
+{
+    char name[256];
+    (void) glob;
+    device_check(hipclaw_initialize_device(0, name, sizeof(name)), "hipclaw_initialize_device");
+}
+
+void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
+{
+    const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
+    const fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    fclaw2d_domain_t *d = glob->domain;
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    hipclaw_config_t cfg;
+    double *dx, *dy;
+    int *ops, phase[7], nops, b, p, k;
+    if (!opt || !d) {
+        fprintf(stderr, "fc2d_hipclaw_allocate_buffers: options or domain missing\n");
+        exit(1);
+    }
+    if (fc2d_hipclaw_check(opt)) exit(1);
+    fc2d_hipclaw_deallocate_buffers(glob);
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.solver = vt->riemann_solver;
+    cfg.precision = opt->precision;
+    cfg.mx = o->mx; cfg.my = o->my; cfg.mbc = o->mbc;
+    cfg.meqn = o->meqn; cfg.maux = o->maux; cfg.mwaves = opt->mwaves;
+    cfg.order[0] = opt->order[0]; cfg.order[1] = opt->order[1];
+    for (k = 0; k < 4; k++) {
+        cfg.mthlim[k] = k < opt->mwaves ? opt->mthlim[k] : 0;
+        cfg.mthbc[k] = opt->mthbc[k];
+    }
+    cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
+    device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");
+    dx = (double *) malloc(sizeof(double) * d->local_num_patches);
+    dy = (double *) malloc(sizeof(double) * d->local_num_patches);
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++) {
+            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+            int mx, my, mbc;
+            double xl, yl;
+            fclaw2d_clawpatch_grid_data(glob, pt, &mx, &my, &mbc, &xl, &yl, &dx[pt->slot], &dy[pt->slot]);
+        }
+    device_check(hipclaw_set_geometry(glob->state, d->local_num_patches, dx, dy), "hipclaw_set_geometry");
+    free(dx); free(dy);
+    nops = fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, NULL, 0, phase);
+    ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
+    fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, nops, phase);
+    device_check(hipclaw_set_ghost_program(glob->state, ops, nops, phase), "hipclaw_set_ghost_program");
+    free(ops);
+    glob->qhost = (double *) calloc(patch_size(glob, o->meqn) * d->local_num_patches, sizeof(double));
+    if (o->maux > 0)
+        glob->auxhost = (double *) calloc(patch_size(glob, o->maux) * d->local_num_patches, sizeof(double));
+}
+
+void fc2d_hipclaw_deallocate_buffers(fclaw2d_global_t *glob)
+{
+    if (glob->state) hipclaw_destroy(glob->state);
+    glob->state = NULL;
+    free(glob->qhost); glob->qhost = NULL;
+    free(glob->auxhost); glob->auxhost = NULL;
+}
+
+void fc2d_hipclaw_upload(fclaw2d_global_t *glob)
+{
+    const int n = glob->domain->local_num_patches;
+    device_check(hipclaw_set_q(glob->state, glob->qhost, 0, n), "hipclaw_set_q");
+    if (glob->auxhost) device_check(hipclaw_set_aux(glob->state, glob->auxhost, 0, n), "hipclaw_set_aux");
+}
+
+void fc2d_hipclaw_download(fclaw2d_global_t *glob)
+{
+    device_check(hipclaw_get_q(glob->state, glob->qhost, 0, glob->domain->local_num_patches),
+                 "hipclaw_get_q");
+}
+
+void fc2d_hipclaw_setprob(fclaw2d_global_t *glob)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    if (vt->fort_setprob) vt->fort_setprob();
+}
+
+void fc2d_hipclaw_qinit(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno, int patchno)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    int mx, my, mbc, meqn, maux;
+    double xl, yl, dx, dy, *q, *aux;
+    (void) blockno; (void) patchno;
+    if (!vt->fort_qinit) return;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    fclaw2d_clawpatch_soln_data(glob, patch, &q, &meqn);
+    fclaw2d_clawpatch_aux_data(glob, patch, &aux, &maux);
+    vt->fort_qinit(&mx, &my, &meqn, &mbc, &mx, &my, &xl, &yl, &dx, &dy, q, &maux, aux);
+}
+
+void fc2d_hipclaw_setaux(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno, int patchno)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    int mx, my, mbc, maux;
+    double xl, yl, dx, dy, *aux;
+    (void) blockno; (void) patchno;
+    if (!vt->fort_setaux) return;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    fclaw2d_clawpatch_aux_data(glob, patch, &aux, &maux);
+    vt->fort_setaux(&mx, &my, &mbc, &mx, &my, &xl, &yl, &dx, &dy, &maux, aux);
+}
+
+void fc2d_hipclaw_b4step2(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                          int patchno, double t, double dt)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    if (vt->b4step2) vt->b4step2(glob, patch, blockno, patchno, t, dt);
+}
+
+/* Physical BC on a patch's host view (clawpack46_bc2_default semantics, or the
+ * user's fort_bc2); the device path applies the same BC inside the ghost fill. */
+void fc2d_hipclaw_bc2(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno, int patchno,
+                      double t, double dt, int intersects_bc[], int time_interp)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
+    int mx, my, mbc, meqn, maux, mthbc[4], k, m, i, j, b;
+    double xl, yl, dx, dy, *q, *aux;
+    (void) blockno; (void) patchno; (void) time_interp;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    fclaw2d_clawpatch_soln_data(glob, patch, &q, &meqn);
+    fclaw2d_clawpatch_aux_data(glob, patch, &aux, &maux);
+    for (k = 0; k < 4; k++) mthbc[k] = intersects_bc[k] ? opt->mthbc[k] : -1;
+    if (vt->fort_bc2) {
+        vt->fort_bc2(&mx, &my, &meqn, &mbc, &mx, &my, &xl, &yl, &dx, &dy, q, &maux, aux, &t, &dt, mthbc);
+        return;
+    }
+#define QV(i, j, m) q[((size_t) (m) * (my + 2 * mbc) + (j) + mbc - 1) * (mx + 2 * mbc) + (i) + mbc - 1]
+    for (k = 0; k < 4; k++) {
+        if (mthbc[k] != 1 && mthbc[k] != 3) continue;
+        for (m = 0; m < meqn; m++)
+            for (b = 1; b <= mbc; b++) {
+                if (k < 2)
+                    for (j = 1 - mbc; j <= my + mbc; j++) {
+                        const int id = k == 0 ? 1 - b : mx + b;
+                        const int is = mthbc[k] == 1 ? (k == 0 ? 1 : mx) : (k == 0 ? b : mx + 1 - b);
+                        QV(id, j, m) = (mthbc[k] == 3 && m == 1) ? -QV(is, j, m) : QV(is, j, m);
+                    }
+                else
+                    for (i = 1 - mbc; i <= mx + mbc; i++) {
+                        const int jd = k == 2 ? 1 - b : my + b;
+                        const int js = mthbc[k] == 1 ? (k == 2 ? 1 : my) : (k == 2 ? b : my + 1 - b);
+                        QV(i, jd, m) = (mthbc[k] == 3 && m == 2) ? -QV(i, js, m) : QV(i, js, m);
+                    }
+            }
+    }
+#undef QV
+}
+
+void fc2d_hipclaw_src2(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno, int patchno,
+                       double t, double dt)
+{
+    fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    int mx, my, mbc, meqn, maux;
+    double xl, yl, dx, dy, *q, *aux;
+    (void) blockno; (void) patchno;
+    if (!vt->fort_src2) return;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    fclaw2d_clawpatch_soln_data(glob, patch, &q, &meqn);
+    fclaw2d_clawpatch_aux_data(glob, patch, &aux, &maux);
+    vt->fort_src2(&mx, &my, &meqn, &mbc, &mx, &my, &xl, &yl, &dx, &dy, q, &maux, aux, &t, &dt);
+}

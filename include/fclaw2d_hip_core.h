@@ -1,0 +1,150 @@
+/*
+ * The parts of ForestClaw's core C interface that the batched patch update
+ * plugs into, declared for this build (the reference's own headers need
+ * p4est/libsc).  Names, argument order and meaning follow the reference:
+ *
+ *   fclaw2d_patch_t / fclaw2d_block_t / fclaw2d_domain_t
+ *                      src/forestclaw2d.h:97-170 (fields this path uses)
+ *   fclaw2d_patch_vtable_t, single_step_update typedef
+ *                      src/fclaw2d_patch.h:937-943, 1413-1570
+ *   fclaw2d_single_step_buffer_data_t, fclaw2d_update_single_step
+ *                      src/fclaw2d_update_single_step.h:47-95
+ *   fclaw2d_ghost_update
+ *                      src/fclaw2d_ghost_fill.h (src/fclaw2d_ghost_fill.c:838-856)
+ *   fclaw2d_clawpatch_{grid,soln,aux}_data
+ *                      src/patches/clawpatch/fclaw2d_clawpatch.h
+ *
+ * Patch data live on the GPU; the clawpatch accessors return a host view of a
+ * patch that fc2d_hipclaw_download / fc2d_hipclaw_upload keep in step with it.
+ */
+#ifndef FCLAW2D_HIP_CORE_H
+#define FCLAW2D_HIP_CORE_H
+
+#include "fclaw2d_hip_forest.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct fclaw2d_global;
+struct fc2d_hipclaw_vtable;
+struct fc2d_hipclaw_options;
+struct hipclaw_state;
+
+typedef struct fclaw2d_patch {
+    int level;          /* refinement level */
+    double xlower, xupper, ylower, yupper;  /* in [0,1] block coordinates */
+    int blockno;        /* owning block */
+    int patchno;        /* index within the block */
+    int slot;           /* index of the patch's data on the device */
+    void *user;
+} fclaw2d_patch_t;
+
+typedef struct fclaw2d_block {
+    int num_patches;
+    int patchno_offset;        /* slot of the block's first patch */
+    fclaw2d_patch_t *patches;
+} fclaw2d_block_t;
+
+typedef struct fclaw2d_domain {
+    int num_blocks;
+    fclaw2d_block_t *blocks;
+    int local_num_patches;
+    int global_minlevel, global_maxlevel;
+    int nbx, nby, periodic_x, periodic_y;
+    double ax, bx, ay, by;     /* physical extent of the brick */
+    fclaw2d_hip_forest_t *forest;
+} fclaw2d_domain_t;
+
+typedef double (*fclaw2d_patch_single_step_update_t)(struct fclaw2d_global *glob,
+                                                     fclaw2d_patch_t *this_patch,
+                                                     int this_block_idx, int this_patch_idx,
+                                                     double t, double dt, void *user);
+
+typedef void (*fclaw2d_patch_initialize_t)(struct fclaw2d_global *glob,
+                                           fclaw2d_patch_t *this_patch,
+                                           int this_block_idx, int this_patch_idx);
+
+typedef void (*fclaw2d_patch_physical_bc_t)(struct fclaw2d_global *glob,
+                                            fclaw2d_patch_t *this_patch,
+                                            int this_block_idx, int this_patch_idx,
+                                            double t, double dt, int intersects_bc[],
+                                            int time_interp);
+
+typedef struct fclaw2d_patch_vtable {
+    fclaw2d_patch_initialize_t initialize;  /* qinit */
+    fclaw2d_patch_initialize_t setup;       /* setaux */
+    fclaw2d_patch_physical_bc_t physical_bc;
+    fclaw2d_patch_single_step_update_t single_step_update;
+    int is_set;
+} fclaw2d_patch_vtable_t;
+
+typedef struct fclaw2d_clawpatch_options {
+    int mx, my, mbc, meqn, maux;
+} fclaw2d_clawpatch_options_t;
+
+typedef struct fclaw2d_global {
+    fclaw2d_domain_t *domain;
+    int count_single_step;         /* patch updates so far (src/fclaw_timer.c:184) */
+    int count_ghost_update;
+    double curr_time, curr_dt;
+    fclaw2d_patch_vtable_t patch_vt;
+    fclaw2d_clawpatch_options_t clawpatch_opt;
+    struct fc2d_hipclaw_vtable *hipclaw_vt;
+    struct fc2d_hipclaw_options *hipclaw_opt;
+    struct hipclaw_state *state;   /* device-resident solution */
+    double *qhost, *auxhost;       /* host views [slot][field][j][i] */
+} fclaw2d_global_t;
+
+/** Buffer contract between fclaw2d_update_single_step and single_step_update
+ *  (src/fclaw2d_update_single_step.h:47-55). */
+typedef struct fclaw2d_single_step_buffer_data {
+    int total_count;   /* patches updated by this call */
+    int iter;          /* index of the current patch, incremented by the caller */
+    void *user;        /* owned by the callee: allocated at iter == 0, freed at the last */
+} fclaw2d_single_step_buffer_data_t;
+
+/* ---- globals, domains ---- */
+fclaw2d_global_t *fclaw2d_global_new(void);
+void fclaw2d_global_destroy(fclaw2d_global_t *glob);
+
+/* Domain over the brick [ax,bx] x [ay,by] of nbx x nby blocks whose leaves are
+ * those of `forest` (the domain takes ownership of the forest). */
+fclaw2d_domain_t *fclaw2d_domain_new_from_forest(fclaw2d_hip_forest_t *forest, int nbx, int nby,
+                                                 int periodic_x, int periodic_y, double ax,
+                                                 double bx, double ay, double by);
+void fclaw2d_domain_destroy(fclaw2d_domain_t *domain);
+void fclaw2d_global_store_domain(fclaw2d_global_t *glob, fclaw2d_domain_t *domain);
+
+fclaw2d_patch_vtable_t *fclaw2d_patch_vt(fclaw2d_global_t *glob);
+
+/* ---- the hot path's callers ---- */
+
+/** Advance every patch at `level` by one explicit step; returns the maximum
+ *  Courant number (src/fclaw2d_update_single_step.c:73-103). */
+double fclaw2d_update_single_step(fclaw2d_global_t *glob, int level, double t, double dt);
+
+/** Fill ghost cells of all patches at levels minlevel..maxlevel
+ *  (src/fclaw2d_ghost_fill.c:838-856).  time_interp must be 0 (subcycling is
+ *  not supported by this build). */
+void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel,
+                          double sync_time, int time_interp, int running);
+
+/** One global time step of every level followed by a ghost update
+ *  (fclaw2d_advance_all_levels with subcycle = F, advance_one_step = T,
+ *  src/fclaw2d_advance.c:247-357).  Returns the maximum Courant number. */
+double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt);
+
+/* ---- clawpatch accessors (host views) ---- */
+void fclaw2d_clawpatch_grid_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int *mx,
+                                 int *my, int *mbc, double *xlower, double *ylower, double *dx,
+                                 double *dy);
+void fclaw2d_clawpatch_soln_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, double **q,
+                                 int *meqn);
+void fclaw2d_clawpatch_aux_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, double **aux,
+                                int *maux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -50,3 +50,20 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         raise AssertionError("expected a RuntimeError")
     finally:
         importlib.reload(fa)
+
+
+def test_options_checks_follow_cudaclaw():
+    """fc2d_cudaclaw_check rejects mwaves > FC2D_CUDACLAW_MWAVES
+    (fc2d_cudaclaw_options.c:110-115); defaults follow fc2d_cudaclaw_options.c:36-71."""
+    import ctypes as C
+    import dropin_lib as D
+    L = D.bind()
+    assert L.hipclaw_check_parameters(10) == 0
+    assert L.hipclaw_check_parameters(11) == 1
+    opt = D.Options()
+    L.fc2d_hipclaw_options_default(C.byref(opt), 3)
+    assert (opt.order[0], opt.order[1], opt.buffer_len, opt.mcapa, opt.src_term) == (2, 2, 1, 0, 0)
+    assert list(opt.mthbc) == [1, 1, 1, 1]
+    assert L.fc2d_hipclaw_check(C.byref(opt)) == 0
+    opt.buffer_len = 0
+    assert L.fc2d_hipclaw_check(C.byref(opt)) == 1
