@@ -88,7 +88,7 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
 static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
     int t = (c->precision == HIPCLAW_FP64) ? 16 : 32;
-    while (t > m) t >>= 1;
+    while (t > 8 && (t > m || !hipclaw::step_supported(c->solver, c->precision, t))) t >>= 1;
     return t;
 }
 

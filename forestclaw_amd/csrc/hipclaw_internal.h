@@ -51,6 +51,29 @@ int launch_mass(const double* q, const int* slots, int n, const double* dxs, con
                 long long pstride, int mx, int my, int mbc, int meqn, double* out,
                 hipStream_t st);
 
+// Workgroups of `kernel` (block size `block`, static LDS only) that the
+// current device holds at once: the grid of a persistent launch.
+inline int resident_blocks(const void* kernel, int block) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+// Launch a persistent step kernel: min(tiles, resident workgroups) of them.
+// `resident` caches the occupancy query; each kernel instantiation owns one.
+template <typename K>
+inline int launch_persistent(K kernel, int block, const StepArgs& a, hipStream_t st, int& resident) {
+    const int ntiles = a.nwork * a.ntx * a.nty;
+    if (ntiles <= 0) return 0;
+    if (resident <= 0) resident = resident_blocks((const void*) kernel, block);
+    if (resident <= 0) return (int) hipErrorInvalidDeviceFunction;
+    const int nblocks = ntiles < resident ? ntiles : resident;
+    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, st, a);
+    return (int) hipGetLastError();
+}
+
 // Largest tile the FP64 / FP32 kernels are instantiated for, per solver.
 bool step_supported(int solver, int precision, int tile);
 

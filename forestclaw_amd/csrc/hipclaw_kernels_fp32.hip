@@ -8,19 +8,18 @@ namespace hipclaw {
 
 template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {
-    const int nblocks = a.nwork * a.ntx * a.nty;
-    if (nblocks <= 0) return 0;
-    hipLaunchKernelGGL((step_tiles<RP, float, T, NW, RP::CONSERVATIVE>), dim3(nblocks), dim3(NW * 64), 0,
-                       st, a);
-    return (int) hipGetLastError();
+    static int resident = 0;
+    return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE>, NW * 64, a, st, resident);
 }
 
-template <class RP>
+template <class RP, bool WITH32 = true>
 static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
     case 16: return launch_one<RP, 16, 6>(a, st);
-    case 32: return launch_one<RP, 32, 16>(a, st);
+    case 32:
+        if constexpr (WITH32) return launch_one<RP, 32, 16>(a, st);
+        return (int) hipErrorInvalidValue;
     default: return (int) hipErrorInvalidValue;
     }
 }
@@ -29,14 +28,16 @@ int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st) {
     switch (solver) {
     case 0: return launch_tile<RPAdvection<float>>(tile, a, st);
     case 1: return launch_tile<RPShallow<float>>(tile, a, st);
-    case 2: return launch_tile<RPEuler<float>>(tile, a, st);
+    case 2:  // meqn = 4 at T = 32 needs more LDS than a workgroup may hold
+        return launch_tile<RPEuler<float>, false>(tile, a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
 
 bool step_supported(int solver, int precision, int tile) {
     if (tile != 8 && tile != 16 && tile != 32) return false;
-    if (precision == 0 && tile == 32) return solver == 1;  // FP64 T=32 LDS fits only meqn=3
+    // T = 32 fits the 160 KB of LDS for meqn = 3 (FP64) and meqn <= 3 (FP32)
+    if (tile == 32) return precision == 0 ? solver == 1 : solver != 2;
     return solver >= 0 && solver <= 2;
 }
 

@@ -264,11 +264,8 @@ int launch_mass(const double* q, const int* slots, int n, const double* dxs, con
 // ---------------------------------------------------------------- FP64 step
 template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {
-    const int nblocks = a.nwork * a.ntx * a.nty;
-    if (nblocks <= 0) return 0;
-    hipLaunchKernelGGL((step_tiles<RP, double, T, NW, false>), dim3(nblocks), dim3(NW * 64), 0,
-                       st, a);
-    return (int) hipGetLastError();
+    static int resident = 0;
+    return launch_persistent(step_tiles<RP, double, T, NW, false>, NW * 64, a, st, resident);
 }
 
 template <class RP>

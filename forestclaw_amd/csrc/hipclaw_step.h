@@ -1,5 +1,6 @@
-// Batched patch-update kernel: one workgroup updates one T x T tile of one
-// patch, for every tile of every patch in a work list, in a single launch.
+// Batched patch-update kernel.  A persistent workgroup loops over the T x T
+// tiles of every patch in a work list (all levels of the forest in one
+// launch) and updates each tile's interior cells.
 //
 // What it computes is the reference's Clawpack-4.6 step
 // (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_step2.f:5-239,
@@ -7,7 +8,10 @@
 //  clawpack46_step2_wrap.f:106-125) restricted to the interior cells of the
 // tile.  How it computes it is MI355X-first:
 //
-//  * the tile and its 2-cell halo are staged once in LDS;
+//  * the tile and its 2-cell halo are staged once in LDS (FP64);
+//  * while a tile is being computed, the next tile's cells are already in
+//    flight from HBM into registers (software pipelining across tiles), so
+//    the global-load latency is hidden behind the sweeps;
 //  * each lane owns ONE interface: a row of T+3 x-interfaces (or a column of
 //    T+3 y-interfaces) lives in T+3 consecutive lanes of a wave64, so the
 //    limiter's neighbour waves and the transverse fluctuations of the next
@@ -16,13 +20,14 @@
 //    accumulated in LDS in exactly the order the reference adds them, so the
 //    FP64 instantiation (compiled with FMA contraction off) is bit-identical to
 //    the CPU Fortran;
-//  * the per-step CFL number is reduced with wave64 shuffles and one FP64
-//    atomic max per wave.
+//  * the CFL number is reduced in registers over all tiles of a workgroup and
+//    with wave64 shuffles, then one FP64 atomic max per wave.
 //
-// FLUXFORM selects the FP32-bulk variant: the Riemann solves, limiter and
-// transverse splitting run in R = float, but the update is written in
-// conservative flux form with FP64 interface fluxes F = f(q_left) + fm shared
-// by both neighbours, so mass is conserved to FP64 rounding.
+// FLUXFORM selects the FP32-bulk variant for conservative solvers: the Riemann
+// solves, limiter and transverse splitting run in R = float, but the update is
+// written in conservative flux form with FP64 interface fluxes
+// F = f(q_left) + fm shared by both neighbours, so mass is conserved to FP64
+// rounding.
 #pragma once
 
 #include "hipclaw_rp.h"
@@ -86,7 +91,7 @@ struct TileSmem {
     static constexpr int QS = QW | 1;           // padded row stride
     static constexpr int QP = QW * QS;          // one field
     static constexpr int GS = T + 1;            // padded row stride of gt and ab
-    R q[MEQN * QP];
+    double q[MEQN * QP];
     R aux[MAUX > 0 ? MAUX * QP : 1];
     R fm[MEQN * T * (T + 1)];       // x-interface slot s = i - i0 in [0, T], row b
     R fp[FLUXFORM ? 1 : MEQN * T * (T + 1)];
@@ -96,15 +101,19 @@ struct TileSmem {
     double fy[FLUXFORM ? MEQN * (T + 1) * T : 1];
 };
 
+template <class RP, typename R, int T, bool FLUXFORM>
+constexpr size_t tile_smem_bytes() {
+    return sizeof(TileSmem<T, RP::MEQN, RP::MAUX, R, FLUXFORM>);
+}
+
 // Interface-in-CFL-set test.  The reference takes the CFL over x-interfaces
 // i = 1..mx+1 of rows j = 2-mbc..my+mbc-1 (clawpack46_flux2.f:129-137,
 // clawpack46_step2.f:113) and symmetrically in y; each tile owns a disjoint
 // share of that set.
-__device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool first_n, bool last_n,
-                                          bool first_t, bool last_t) {
+__device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool first_t,
+                                          bool last_t) {
     const bool kn = (k >= 1 && k <= T) || (k == T + 1 && last_n);
     const bool rt = (r >= 1 && r <= T) || (r == 0 && first_t) || (r == T + 1 && last_t);
-    (void) first_n;
     return kn && rt;
 }
 
@@ -116,48 +125,66 @@ step_tiles(StepArgs a) {
     constexpr int QW = Smem::QW, QS = Smem::QS, QP = Smem::QP, GS = Smem::GS;
     constexpr int L = T + 3;               // lanes per row/column of interfaces
     constexpr int SEG = 64 / L;            // rows (columns) per wave
+    constexpr int NT = NW * 64;
+    constexpr int NQ = MEQN * QW * QW;     // q values staged per tile
+    constexpr int NA = MAUX * QW * QW;     // aux values staged per tile
+    constexpr int KQ = (NQ + NT - 1) / NT; // per-thread prefetch registers
+    constexpr int KA = MAUX > 0 ? (NA + NT - 1) / NT : 1;
     static_assert(SEG >= 1, "tile too wide for one wave");
     __shared__ Smem sm;
 
-    const int tiles_per_patch = a.ntx * a.nty;
-    const int wid = blockIdx.x / tiles_per_patch;
-    const int tix = blockIdx.x - wid * tiles_per_patch;
-    if (wid >= a.nwork) return;
-    const int slot = a.work[wid];
-    const int tx = tix % a.ntx, ty = tix / a.ntx;
-    const int i0 = 1 + tx * T, j0 = 1 + ty * T;  // first interior cell of the tile (1-based)
+    const int ntiles = a.nwork * a.ntx * a.nty;
     const int mx = a.mx, my = a.my, mbc = a.mbc;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
-    const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride;
-    const double dx = a.dxs[slot], dy = a.dys[slot];
-    const double dtdx64 = a.dt / dx, dtdy64 = a.dt / dy;
-    const R dtdx = (R) dtdx64, dtdy = (R) dtdy64;
-    const bool first_x = tx == 0, last_x = tx == a.ntx - 1;
-    const bool first_y = ty == 0, last_y = ty == a.nty - 1;
-
-    // ---- stage 0: tile + 2-cell halo into LDS (cells i0-2..i0+T+1) ----
-    for (int e = threadIdx.x; e < MEQN * QW * QW; e += NW * 64) {
-        const int m = e / (QW * QW), rem = e - m * QW * QW;
-        const int qj = rem / QW, qi = rem - qj * QW;
-        const int i = i0 - 2 + qi, j = j0 - 2 + qj;  // 1-based patch indices
-        sm.q[m * QP + qj * QS + qi] = (R) qp[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
-    }
-    if constexpr (MAUX > 0) {
-        const double* __restrict__ ap = a.aux + (long long) slot * a.astride;
-        for (int e = threadIdx.x; e < MAUX * QW * QW; e += NW * 64) {
-            const int m = e / (QW * QW), rem = e - m * QW * QW;
-            const int qj = rem / QW, qi = rem - qj * QW;
-            const int i = i0 - 2 + qi, j = j0 - 2 + qj;
-            sm.aux[m * QP + qj * QS + qi] = (R) ap[((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1)];
-        }
-    }
-    __syncthreads();
-
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int seg = lane / L, k = lane - seg * L;
     double cfl = 0.0;
 
-    auto SQ = [&](int m, int qi, int qj) -> R { return sm.q[m * QP + qj * QS + qi]; };
+    auto tile_slot = [&](int tile, int& tx, int& ty) {
+        const int tpp = a.ntx * a.nty;
+        const int wid = tile / tpp, tix = tile - wid * tpp;
+        tx = tix % a.ntx;
+        ty = tix / a.ntx;
+        return a.work[wid];
+    };
+    // global offset of staged value e (field m, tile-local row qj, column qi)
+    auto staged_offset = [&](int e, int i0, int j0) -> long long {
+        const int m = e / (QW * QW), rem = e - m * QW * QW;
+        const int qj = rem / QW, qi = rem - qj * QW;
+        const int i = i0 - 2 + qi, j = j0 - 2 + qj;  // 1-based patch indices
+        return ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
+    };
+    auto staged_lds = [&](int e) -> int {
+        const int m = e / (QW * QW), rem = e - m * QW * QW;
+        const int qj = rem / QW, qi = rem - qj * QW;
+        return m * QP + qj * QS + qi;
+    };
+
+    // prefetch registers for the next tile
+    double pq[KQ];
+    R pa[KA];
+    auto prefetch = [&](int tile) {
+        int tx, ty;
+        const int slot = tile_slot(tile, tx, ty);
+        const int i0 = 1 + tx * T, j0 = 1 + ty * T;
+        const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride;
+#pragma unroll
+        for (int kk = 0; kk < KQ; kk++) {
+            const int e = threadIdx.x + kk * NT;
+            pq[kk] = e < NQ ? qp[staged_offset(e, i0, j0)] : 0.0;
+        }
+        if constexpr (MAUX > 0) {
+            const double* __restrict__ ap = a.aux + (long long) slot * a.astride;
+#pragma unroll
+            for (int kk = 0; kk < KA; kk++) {
+                const int e = threadIdx.x + kk * NT;
+                pa[kk] = e < NA ? (R) ap[staged_offset(e, i0, j0)] : R(0);
+            }
+        }
+    };
+
+    auto SQ = [&](int m, int qi, int qj) -> R { return (R) sm.q[m * QP + qj * QS + qi]; };
+    auto SQD = [&](int m, int qi, int qj) -> double { return sm.q[m * QP + qj * QS + qi]; };
     auto SA = [&](int m, int qi, int qj) -> R {
         if constexpr (MAUX > 0) return sm.aux[m * QP + qj * QS + qi];
         else return R(0);
@@ -169,283 +196,305 @@ step_tiles(StepArgs a) {
     // gt[m][t][a]
     auto GT = [&](int m, int t, int c) -> R& { return sm.gt[(m * (T + 1) + t) * GS + c]; };
 
-    // ============================ x sweep ============================
-    // lane (seg, k): row j = j0-1+r, interface i = i0-1+k between cells i-1 and i
-    for (int rbase = wave * SEG; rbase < T + 2; rbase += NW * SEG) {
-        const int r = rbase + seg;
-        const bool act = (seg < SEG) && (r < T + 2);
-        const int rc = r < T + 1 ? r : T + 1;
-        const int kc = k < L ? k : L - 1;
-        R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
-#pragma unroll
-        for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, kc, rc + 1); qR[m] = SQ(m, kc + 1, rc + 1); }
-#pragma unroll
-        for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, kc + 1, rc + 1);
-        R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-        RP::template rpn2<1>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
-        if (act && cfl_owned(k, r, T, first_x, last_x, first_y, last_y)) {
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdx64 * (double) s[mw]));
-        }
-        R cqxx[MEQN];
-#pragma unroll
-        for (int m = 0; m < MEQN; m++) cqxx[m] = R(0);
-        if (a.order2) {
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) {
-                const int mth = a.mthlim[mw];
-                if (mth == 0) continue;
-                R wn = R(0), dotl = R(0);
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) {
-                    const R wl = shfl(wave_[mw][m], lane - 1);
-                    dotl = dotl + wl * wave_[mw][m];
-                }
-                const R dotr = shfl(dotl, lane + 1);
-                if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
-                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
-                    const R phi = philim<R>(mth, rr);
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                R c = R(0);
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++)
-                    c = c + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdx) * wave_[mw][m];
-                cqxx[m] = c;
-            }
-        }
-        if (act && r >= 1 && r <= T && k >= 1 && k <= T + 1) {
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                const R fadm = (R(0) + amdq[m]) + R(0.5) * cqxx[m];
-                sm.fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = fadm;
-                if constexpr (!FLUXFORM)
-                    sm.fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) - apdq[m]) + R(0.5) * cqxx[m];
-            }
-        }
-        R tx1[MEQN], tx2[MEQN];
-        if (a.transverse) {
-            if (a.order2 && a.transverse == 2) {
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
-            }
-            R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
-            // rpt2adv reads aux(i1, j) and aux(i1, j+1) with i1 = i-1 (amdq) or i (apdq)
-            RP::template rpt2<1>(roe, amdq, SA(1, kc, rc + 1), SA(1, kc, rc + 2), bmA, bpA);
-            RP::template rpt2<1>(roe, apdq, SA(1, kc + 1, rc + 1), SA(1, kc + 1, rc + 2), bmP, bpP);
-            const R half = R(0.5) * dtdx;
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                const R gAd = half * bmA[m], gAu = half * bpA[m];
-                const R gPd = half * bmP[m], gPu = half * bpP[m];
-                tx1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
-                tx2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) { tx1[m] = R(0); tx2[m] = R(0); }
-        }
-        if (act && k >= 1 && k <= T) {
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) { AB(0, m, r, k - 1) = tx1[m]; AB(1, m, r, k - 1) = tx2[m]; }
-        }
-    }
-    __syncthreads();
+    // Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
+    // that, in every round of the persistent loop, each XCD works on one
+    // contiguous run of tiles: neighbouring tiles of a patch share halo cells,
+    // which then come from the same L2.
+    const int G = gridDim.x;
+    int tile = (G % 8 == 0) ? (int) (blockIdx.x % 8) * (G / 8) + (int) (blockIdx.x / 8) : (int) blockIdx.x;
+    if (tile < ntiles) prefetch(tile);
 
-    // gm(i,j) = gp(i,j) after the x sweeps: row j-1 adds gadd(.,2), then row j adds gadd(.,1)
-    for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NW * 64) {
-        const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
-        const int t = rem / T, c = rem - t * T;
-        GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
-    }
-    __syncthreads();
+    for (; tile < ntiles; tile += G) {
+        int tx, ty;
+        const int slot = tile_slot(tile, tx, ty);
+        const int i0 = 1 + tx * T, j0 = 1 + ty * T;  // first interior cell of the tile (1-based)
+        const double dtdx64 = a.dt / a.dxs[slot], dtdy64 = a.dt / a.dys[slot];
+        const R dtdx = (R) dtdx64, dtdy = (R) dtdy64;
+        const bool first_x = tx == 0, last_x = tx == a.ntx - 1;
+        const bool first_y = ty == 0, last_y = ty == a.nty - 1;
 
-    // ============================ y sweep ============================
-    // lane (seg, k): column i = i0-1+c, interface j = j0-1+k between cells j-1 and j
-    for (int cbase = wave * SEG; cbase < T + 2; cbase += NW * SEG) {
-        const int c = cbase + seg;
-        const bool act = (seg < SEG) && (c < T + 2);
-        const int cc = c < T + 1 ? c : T + 1;
-        const int kc = k < L ? k : L - 1;
-        R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
+        // ---- stage 0: the prefetched tile + 2-cell halo into LDS ----
 #pragma unroll
-        for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, cc + 1, kc); qR[m] = SQ(m, cc + 1, kc + 1); }
-#pragma unroll
-        for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, cc + 1, kc + 1);
-        R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-        RP::template rpn2<2>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
-        if (act && cfl_owned(k, c, T, first_y, last_y, first_x, last_x)) {
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdy64 * (double) s[mw]));
+        for (int kk = 0; kk < KQ; kk++) {
+            const int e = threadIdx.x + kk * NT;
+            if (e < NQ) sm.q[staged_lds(e)] = pq[kk];
         }
-        R cqxx[MEQN];
+        if constexpr (MAUX > 0) {
 #pragma unroll
-        for (int m = 0; m < MEQN; m++) cqxx[m] = R(0);
-        if (a.order2) {
-#pragma unroll
-            for (int mw = 0; mw < MW; mw++) {
-                const int mth = a.mthlim[mw];
-                if (mth == 0) continue;
-                R wn = R(0), dotl = R(0);
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) {
-                    const R wl = shfl(wave_[mw][m], lane - 1);
-                    dotl = dotl + wl * wave_[mw][m];
-                }
-                const R dotr = shfl(dotl, lane + 1);
-                if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
-                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
-                    const R phi = philim<R>(mth, rr);
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                R cq = R(0);
-#pragma unroll
-                for (int mw = 0; mw < MW; mw++)
-                    cq = cq + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdy) * wave_[mw][m];
-                cqxx[m] = cq;
-            }
-        }
-        // normal y-fluxes: gm = gt + faddm, gp = gt + faddp
-        const bool own_col = c >= 1 && c <= T;
-        if constexpr (FLUXFORM) {
-            if (act && own_col && k >= 1 && k <= T + 1) {
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) {
-                    R& g = GT(m, k - 1, c - 1);
-                    g = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
-                }
-            }
-        } else {
-            R gm[MEQN], gp[MEQN];
-            const int kk = (k >= 1 && k <= T + 1) ? k - 1 : 0;
-            const int ccol = own_col ? c - 1 : 0;
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                const R g = GT(m, kk, ccol);
-                gm[m] = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
-                gp[m] = g + ((R(0) - apdq[m]) + R(0.5) * cqxx[m]);
-            }
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                const R gm_up = shfl(gm[m], lane + 1);
-                if (act && own_col && k >= 1 && k <= T)
-                    GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
-            }
-        }
-        R ty1[MEQN], ty2[MEQN];
-        if (a.transverse) {
-            if (a.order2 && a.transverse == 2) {
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
-            }
-            R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
-            // y sweep: aux2 = column i, aux3 = column i+1, at j1 = j-1 (amdq) or j (apdq)
-            RP::template rpt2<2>(roe, amdq, SA(0, cc + 1, kc), SA(0, cc + 2, kc), bmA, bpA);
-            RP::template rpt2<2>(roe, apdq, SA(0, cc + 1, kc + 1), SA(0, cc + 2, kc + 1), bmP, bpP);
-            const R half = R(0.5) * dtdy;
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) {
-                const R gAd = half * bmA[m], gAu = half * bpA[m];
-                const R gPd = half * bmP[m], gPu = half * bpP[m];
-                ty1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
-                ty2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) { ty1[m] = R(0); ty2[m] = R(0); }
-        }
-        if (act && k >= 1 && k <= T) {
-#pragma unroll
-            for (int m = 0; m < MEQN; m++) { AB(0, m, c, k - 1) = ty1[m]; AB(1, m, c, k - 1) = ty2[m]; }
-        }
-    }
-
-    // ---- CFL: wave64 max, one FP64-bits atomic per wave ----
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, off, 64));
-    if (lane == 0 && cfl > 0.0) atomicMax(a.cfl, (unsigned long long) __double_as_longlong(cfl));
-    __syncthreads();
-
-    double* __restrict__ qo = a.qdst + (long long) slot * a.pstride;
-    // fm(i,j) = ((0 + faddm_x(i,j)) + ty2(i-1,j)) + ty1(i,j), same for fp
-    if constexpr (!FLUXFORM) {
-        for (int e = threadIdx.x; e < MEQN * T * T; e += NW * 64) {
-            const int m = e / (T * T), rem = e - m * T * T;
-            const int b = rem / T, c = rem - b * T;   // cell i = i0+c, j = j0+b
-            const R fmR = ((R(0) + sm.fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
-            const R fpL = ((R(0) + sm.fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
-            const R Dx = fmR - fpL;
-            const R Dy = GT(m, b, c);
-            const int i = i0 + c, j = j0 + b;
-            const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
-            if constexpr (sizeof(R) == sizeof(double)) {
-                const R qc = SQ(m, c + 2, b + 2);
-                qo[off] = (qc - dtdx * Dx) - dtdy * Dy;
-            } else {
-                // FP32 fluctuations, FP64 state: q itself never passes through FP32
-                qo[off] = (qp[off] - dtdx64 * (double) Dx) - dtdy64 * (double) Dy;
-            }
-        }
-    } else {
-        // conservative FP64 interface fluxes, shared by both neighbours
-        const double* __restrict__ ap = MAUX > 0 ? a.aux + (long long) slot * a.astride : nullptr;
-        for (int e = threadIdx.x; e < 2 * T * (T + 1); e += NW * 64) {
-            const bool isx = e < T * (T + 1);
-            const int f = isx ? e : e - T * (T + 1);
-            double qcell[MEQN], acell[MAUX > 0 ? MAUX : 1], fl[MEQN];
-            int i, j;
-            if (isx) {  // x-interface s at row b: F = f(q(i-1,j)) + fm(i,j)
-                const int b = f / (T + 1), s = f - b * (T + 1);
-                i = i0 + s; j = j0 + b;
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) qcell[m] = qp[((long long) m * nyp + (j + mbc - 1)) * nxp + (i - 1 + mbc - 1)];
-#pragma unroll
-                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++)
-                    acell[m] = MAUX > 0 ? ap[((long long) m * nyp + (j + mbc - 1)) * nxp + (i - 1 + mbc - 1)] : 0.0;
-                RP::template flux<1>(qcell, acell, a.pp, fl);
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) {
-                    const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s]) + AB(1, m, s, b)) + AB(0, m, s + 1, b);
-                    sm.fx[(m * T + b) * (T + 1) + s] = fl[m] + (double) fmv;
-                }
-            } else {    // y-interface t at column c: G = g(q(i,j-1)) + gm(i,j)
-                const int t = f / T, c = f - t * T;
-                i = i0 + c; j = j0 + t;
-#pragma unroll
-                for (int m = 0; m < MEQN; m++) qcell[m] = qp[((long long) m * nyp + (j - 1 + mbc - 1)) * nxp + (i + mbc - 1)];
-#pragma unroll
-                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++)
-                    acell[m] = MAUX > 0 ? ap[((long long) m * nyp + (j - 1 + mbc - 1)) * nxp + (i + mbc - 1)] : 0.0;
-                RP::template flux<2>(qcell, acell, a.pp, fl);
-#pragma unroll
-                for (int m = 0; m < MEQN; m++)
-                    sm.fy[(m * (T + 1) + t) * T + c] = fl[m] + (double) GT(m, t, c);
+            for (int kk = 0; kk < KA; kk++) {
+                const int e = threadIdx.x + kk * NT;
+                if (e < NA) sm.aux[staged_lds(e)] = pa[kk];
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < MEQN * T * T; e += NW * 64) {
-            const int m = e / (T * T), rem = e - m * T * T;
-            const int b = rem / T, c = rem - b * T;
-            const int i = i0 + c, j = j0 + b;
-            const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
-            const double qc = qp[off];
-            const double dfx = sm.fx[(m * T + b) * (T + 1) + c + 1] - sm.fx[(m * T + b) * (T + 1) + c];
-            const double dgy = sm.fy[(m * (T + 1) + b + 1) * T + c] - sm.fy[(m * (T + 1) + b) * T + c];
-            qo[off] = (qc - dtdx64 * dfx) - dtdy64 * dgy;
+        // the next tile's data travels while this one is computed
+        if (tile + G < ntiles) prefetch(tile + G);
+
+        // ============================ x sweep ============================
+        // lane (seg, k): row j = j0-1+r, interface i = i0-1+k between cells i-1 and i
+        for (int rbase = wave * SEG; rbase < T + 2; rbase += NW * SEG) {
+            const int r = rbase + seg;
+            const bool act = (seg < SEG) && (r < T + 2);
+            const int rc = r < T + 1 ? r : T + 1;
+            const int kc = k < L ? k : L - 1;
+            R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, kc, rc + 1); qR[m] = SQ(m, kc + 1, rc + 1); }
+#pragma unroll
+            for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, kc + 1, rc + 1);
+            R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
+            RP::template rpn2<1>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            if (act && cfl_owned(k, r, T, last_x, first_y, last_y)) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdx64 * (double) s[mw]));
+            }
+            R cqxx[MEQN];
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) cqxx[m] = R(0);
+            if (a.order2) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    const int mth = a.mthlim[mw];
+                    if (mth == 0) continue;
+                    R wn = R(0), dotl = R(0);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        const R wl = shfl(wave_[mw][m], lane - 1);
+                        dotl = dotl + wl * wave_[mw][m];
+                    }
+                    const R dotr = shfl(dotl, lane + 1);
+                    if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
+                        const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
+                        const R phi = philim<R>(mth, rr);
+#pragma unroll
+                        for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    R c = R(0);
+#pragma unroll
+                    for (int mw = 0; mw < MW; mw++)
+                        c = c + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdx) * wave_[mw][m];
+                    cqxx[m] = c;
+                }
+            }
+            if (act && r >= 1 && r <= T && k >= 1 && k <= T + 1) {
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    sm.fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) + amdq[m]) + R(0.5) * cqxx[m];
+                    if constexpr (!FLUXFORM)
+                        sm.fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) - apdq[m]) + R(0.5) * cqxx[m];
+                }
+            }
+            R tx1[MEQN], tx2[MEQN];
+            if (a.transverse) {
+                if (a.order2 && a.transverse == 2) {
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
+                }
+                R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
+                // rpt2adv reads aux(i1, j) and aux(i1, j+1) with i1 = i-1 (amdq) or i (apdq)
+                RP::template rpt2<1>(roe, amdq, SA(1, kc, rc + 1), SA(1, kc, rc + 2), bmA, bpA);
+                RP::template rpt2<1>(roe, apdq, SA(1, kc + 1, rc + 1), SA(1, kc + 1, rc + 2), bmP, bpP);
+                const R half = R(0.5) * dtdx;
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    const R gAd = half * bmA[m], gAu = half * bpA[m];
+                    const R gPd = half * bmP[m], gPu = half * bpP[m];
+                    tx1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
+                    tx2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) { tx1[m] = R(0); tx2[m] = R(0); }
+            }
+            if (act && k >= 1 && k <= T) {
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) { AB(0, m, r, k - 1) = tx1[m]; AB(1, m, r, k - 1) = tx2[m]; }
+            }
         }
+        __syncthreads();
+
+        // gm(i,j) = gp(i,j) after the x sweeps: row j-1 adds gadd(.,2), then row j adds gadd(.,1)
+        for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NT) {
+            const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
+            const int t = rem / T, c = rem - t * T;
+            GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+        }
+        __syncthreads();
+
+        // ============================ y sweep ============================
+        // lane (seg, k): column i = i0-1+c, interface j = j0-1+k between cells j-1 and j
+        for (int cbase = wave * SEG; cbase < T + 2; cbase += NW * SEG) {
+            const int c = cbase + seg;
+            const bool act = (seg < SEG) && (c < T + 2);
+            const int cc = c < T + 1 ? c : T + 1;
+            const int kc = k < L ? k : L - 1;
+            R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, cc + 1, kc); qR[m] = SQ(m, cc + 1, kc + 1); }
+#pragma unroll
+            for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, cc + 1, kc + 1);
+            R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
+            RP::template rpn2<2>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            if (act && cfl_owned(k, c, T, last_y, first_x, last_x)) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdy64 * (double) s[mw]));
+            }
+            R cqxx[MEQN];
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) cqxx[m] = R(0);
+            if (a.order2) {
+#pragma unroll
+                for (int mw = 0; mw < MW; mw++) {
+                    const int mth = a.mthlim[mw];
+                    if (mth == 0) continue;
+                    R wn = R(0), dotl = R(0);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        const R wl = shfl(wave_[mw][m], lane - 1);
+                        dotl = dotl + wl * wave_[mw][m];
+                    }
+                    const R dotr = shfl(dotl, lane + 1);
+                    if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
+                        const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
+                        const R phi = philim<R>(mth, rr);
+#pragma unroll
+                        for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    R cq = R(0);
+#pragma unroll
+                    for (int mw = 0; mw < MW; mw++)
+                        cq = cq + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdy) * wave_[mw][m];
+                    cqxx[m] = cq;
+                }
+            }
+            // normal y-fluxes: gm = gt + faddm, gp = gt + faddp
+            const bool own_col = c >= 1 && c <= T;
+            if constexpr (FLUXFORM) {
+                if (act && own_col && k >= 1 && k <= T + 1) {
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        R& g = GT(m, k - 1, c - 1);
+                        g = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
+                    }
+                }
+            } else {
+                R gm[MEQN], gp[MEQN];
+                const int kk = (k >= 1 && k <= T + 1) ? k - 1 : 0;
+                const int ccol = own_col ? c - 1 : 0;
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    const R g = GT(m, kk, ccol);
+                    gm[m] = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
+                    gp[m] = g + ((R(0) - apdq[m]) + R(0.5) * cqxx[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    const R gm_up = shfl(gm[m], lane + 1);
+                    if (act && own_col && k >= 1 && k <= T)
+                        GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
+                }
+            }
+            R ty1[MEQN], ty2[MEQN];
+            if (a.transverse) {
+                if (a.order2 && a.transverse == 2) {
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
+                }
+                R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
+                // y sweep: aux2 = column i, aux3 = column i+1, at j1 = j-1 (amdq) or j (apdq)
+                RP::template rpt2<2>(roe, amdq, SA(0, cc + 1, kc), SA(0, cc + 2, kc), bmA, bpA);
+                RP::template rpt2<2>(roe, apdq, SA(0, cc + 1, kc + 1), SA(0, cc + 2, kc + 1), bmP, bpP);
+                const R half = R(0.5) * dtdy;
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    const R gAd = half * bmA[m], gAu = half * bpA[m];
+                    const R gPd = half * bmP[m], gPu = half * bpP[m];
+                    ty1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
+                    ty2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) { ty1[m] = R(0); ty2[m] = R(0); }
+            }
+            if (act && k >= 1 && k <= T) {
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) { AB(0, m, c, k - 1) = ty1[m]; AB(1, m, c, k - 1) = ty2[m]; }
+            }
+        }
+        __syncthreads();
+
+        double* __restrict__ qo = a.qdst + (long long) slot * a.pstride;
+        // fm(i,j) = ((0 + faddm_x(i,j)) + ty2(i-1,j)) + ty1(i,j), same for fp
+        if constexpr (!FLUXFORM) {
+            for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
+                const int m = e / (T * T), rem = e - m * T * T;
+                const int b = rem / T, c = rem - b * T;   // cell i = i0+c, j = j0+b
+                const R fmR = ((R(0) + sm.fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
+                const R fpL = ((R(0) + sm.fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
+                const R Dx = fmR - fpL;
+                const R Dy = GT(m, b, c);
+                const double qc = SQD(m, c + 2, b + 2);
+                const int i = i0 + c, j = j0 + b;
+                const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
+                if constexpr (sizeof(R) == sizeof(double))
+                    qo[off] = (qc - dtdx * Dx) - dtdy * Dy;
+                else  // FP32 fluctuations, FP64 state: q never passes through FP32
+                    qo[off] = (qc - dtdx64 * (double) Dx) - dtdy64 * (double) Dy;
+            }
+        } else {
+            // conservative FP64 interface fluxes, shared by both neighbours
+            for (int e = threadIdx.x; e < 2 * T * (T + 1); e += NT) {
+                double qcell[MEQN], fl[MEQN];
+                if (e < T * (T + 1)) {  // x-interface s at row b: F = f(q(i-1,j)) + fm(i,j)
+                    const int b = e / (T + 1), s = e - b * (T + 1);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, s + 1, b + 2);
+                    RP::template flux<1>(qcell, nullptr, a.pp, fl);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s]) + AB(1, m, s, b)) + AB(0, m, s + 1, b);
+                        sm.fx[(m * T + b) * (T + 1) + s] = fl[m] + (double) fmv;
+                    }
+                } else {                // y-interface t at column c: G = g(q(i,j-1)) + gm(i,j)
+                    const int f = e - T * (T + 1);
+                    const int t = f / T, c = f - t * T;
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, c + 2, t + 1);
+                    RP::template flux<2>(qcell, nullptr, a.pp, fl);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++)
+                        sm.fy[(m * (T + 1) + t) * T + c] = fl[m] + (double) GT(m, t, c);
+                }
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
+                const int m = e / (T * T), rem = e - m * T * T;
+                const int b = rem / T, c = rem - b * T;
+                const int i = i0 + c, j = j0 + b;
+                const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
+                const double qc = SQD(m, c + 2, b + 2);
+                const double dfx = sm.fx[(m * T + b) * (T + 1) + c + 1] - sm.fx[(m * T + b) * (T + 1) + c];
+                const double dgy = sm.fy[(m * (T + 1) + b + 1) * T + c] - sm.fy[(m * (T + 1) + b) * T + c];
+                qo[off] = (qc - dtdx64 * dfx) - dtdy64 * dgy;
+            }
+        }
+        __syncthreads();  // LDS is rewritten by the next tile
     }
+
+    // ---- CFL: wave64 max over all tiles of this workgroup, one atomic per wave ----
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) cfl = fmax(cfl, __shfl_xor(cfl, off, 64));
+    if (lane == 0 && cfl > 0.0) atomicMax(a.cfl, (unsigned long long) __double_as_longlong(cfl));
 }
 
 }  // namespace hipclaw
