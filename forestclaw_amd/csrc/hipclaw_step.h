@@ -72,8 +72,27 @@ __device__ __forceinline__ R philim(int mth, R r) {
     }
 }
 
+// Neighbour-lane exchange with DPP wavefront shifts (one VALU move per 32
+// bits, no LDS round trip): prev_lane(v) in lane l is v of lane l-1
+// (wave_shr:1), next_lane(v) is v of lane l+1 (wave_shl:1).  The lanes at the
+// wave's ends read 0; no caller uses those values.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long b = (unsigned long long) __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_update_dpp(0, (int) (unsigned) b, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned) __builtin_amdgcn_update_dpp(0, (int) (unsigned) (b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
 template <typename R>
-__device__ __forceinline__ R shfl(R v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ R prev_lane(R v) { return dpp_move<DPP_WAVE_SHR1>(v); }
+template <typename R>
+__device__ __forceinline__ R next_lane(R v) { return dpp_move<DPP_WAVE_SHL1>(v); }
 
 // The reference skips the limiter for a wave of zero norm
 // (clawpack46_inlinelimiter.f:51).  FP64 keeps that exact test; in FP32 a norm
@@ -261,10 +280,10 @@ step_tiles(StepArgs a) {
                     for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
-                        const R wl = shfl(wave_[mw][m], lane - 1);
+                        const R wl = prev_lane(wave_[mw][m]);
                         dotl = dotl + wl * wave_[mw][m];
                     }
-                    const R dotr = shfl(dotl, lane + 1);
+                    const R dotr = next_lane(dotl);
                     if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
                         const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                         const R phi = philim<R>(mth, rr);
@@ -304,8 +323,8 @@ step_tiles(StepArgs a) {
                 for (int m = 0; m < MEQN; m++) {
                     const R gAd = half * bmA[m], gAu = half * bpA[m];
                     const R gPd = half * bmP[m], gPu = half * bpP[m];
-                    tx1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
-                    tx2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
+                    tx1[m] = (R(0) - next_lane(gAd)) - gPd;
+                    tx2[m] = (R(0) - next_lane(gAu)) - gPu;
                 }
             } else {
 #pragma unroll
@@ -357,10 +376,10 @@ step_tiles(StepArgs a) {
                     for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
-                        const R wl = shfl(wave_[mw][m], lane - 1);
+                        const R wl = prev_lane(wave_[mw][m]);
                         dotl = dotl + wl * wave_[mw][m];
                     }
-                    const R dotr = shfl(dotl, lane + 1);
+                    const R dotr = next_lane(dotl);
                     if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
                         const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                         const R phi = philim<R>(mth, rr);
@@ -399,7 +418,7 @@ step_tiles(StepArgs a) {
                 }
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    const R gm_up = shfl(gm[m], lane + 1);
+                    const R gm_up = next_lane(gm[m]);
                     if (act && own_col && k >= 1 && k <= T)
                         GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
                 }
@@ -419,8 +438,8 @@ step_tiles(StepArgs a) {
                 for (int m = 0; m < MEQN; m++) {
                     const R gAd = half * bmA[m], gAu = half * bpA[m];
                     const R gPd = half * bmP[m], gPu = half * bpP[m];
-                    ty1[m] = (R(0) - shfl(gAd, lane + 1)) - gPd;
-                    ty2[m] = (R(0) - shfl(gAu, lane + 1)) - gPu;
+                    ty1[m] = (R(0) - next_lane(gAd)) - gPd;
+                    ty2[m] = (R(0) - next_lane(gAu)) - gPu;
                 }
             } else {
 #pragma unroll
