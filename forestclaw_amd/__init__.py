@@ -30,7 +30,7 @@ class HipclawConfig(C.Structure):
                 ("my", C.c_int), ("mbc", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
                 ("mwaves", C.c_int), ("order", C.c_int * 2), ("mthlim", C.c_int * 4),
                 ("mthbc", C.c_int * 4), ("grav", C.c_double), ("gamma", C.c_double),
-                ("tile", C.c_int)]
+                ("tile", C.c_int), ("time_sync", C.c_int)]
 
 
 def _dp():
@@ -63,6 +63,8 @@ def lib():
         "hipclaw_set_aux": ([vp, _dp(), i, i], i),
         "hipclaw_set_ghost_program": ([vp, _ip(), i, _ip()], i),
         "hipclaw_ghost_fill": ([vp], i),
+        "hipclaw_set_sync_program": ([vp, _ip(), i], i),
+        "hipclaw_time_sync": ([vp], i),
         "hipclaw_step_batch": ([vp, _ip(), i, d, d, C.POINTER(d)], i),
         "hipclaw_advance": ([vp, d, i, C.POINTER(d)], i),
         "hipclaw_advance_async": ([vp, d, i], i),
@@ -81,6 +83,7 @@ def lib():
         "fclaw2d_hip_forest_refine": ([vp, _ip(), i], i),
         "fclaw2d_hip_forest_ghost_program": ([vp, i, i, i, _ip(), vp, i, _ip()], i),
         "fclaw2d_hip_forest_face_relation": ([vp, i, i, _ip()], i),
+        "fclaw2d_hip_forest_sync_program": ([vp, i, i, vp, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -169,6 +172,16 @@ class Forest:
         assert n2 == n
         return ops[:n], ps
 
+    def sync_program(self, mx, my):
+        """Time-synchronisation ops (hipclaw_set_sync_program)."""
+        n = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, None, 0)
+        if n < 0:
+            raise RuntimeError("sync program failed")
+        ops = np.zeros((max(n, 1), 8), np.int32)
+        n2 = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, ops.ctypes.data, n)
+        assert n2 == n
+        return ops[:n]
+
     def geometry(self, extent, mx, my):
         """Per-patch xlower, ylower, dx, dy for physical extent (ax, bx, ay, by)."""
         ax, bx, ay, by = extent
@@ -184,7 +197,8 @@ class Solver:
     """A solution resident on the GPU (include/hipclaw.h)."""
 
     def __init__(self, solver, mx, my, mbc=2, precision=FP64, order=(2, 2), mthlim=None,
-                 mthbc=(1, 1, 1, 1), grav=1.0, gamma=1.4, tile=0, max_patches=1):
+                 mthbc=(1, 1, 1, 1), grav=1.0, gamma=1.4, tile=0, max_patches=1,
+                 time_sync=False):
         meqn, mwaves, maux = SOLVER_SHAPES[solver]
         cfg = HipclawConfig()
         cfg.solver, cfg.precision = solver, precision
@@ -196,6 +210,8 @@ class Solver:
             cfg.mthlim[k] = lim[k] if k < len(lim) else 0
             cfg.mthbc[k] = mthbc[k]
         cfg.grav, cfg.gamma, cfg.tile = grav, gamma, tile
+        cfg.time_sync = int(bool(time_sync))
+        self.time_sync = bool(time_sync)
         self.cfg = cfg
         self.meqn, self.maux, self.mwaves = meqn, maux, mwaves
         self.mx, self.my, self.mbc = mx, my, mbc
@@ -225,6 +241,17 @@ class Solver:
         self.set_geometry(dx, dy)
         ops, ps = forest.ghost_program(self.mx, self.my, self.mbc, self.mthbc)
         self.set_ghost_program(ops, ps)
+        if self.time_sync:
+            self.set_sync_program(forest.sync_program(self.mx, self.my))
+
+    def set_sync_program(self, ops):
+        ops = np.ascontiguousarray(ops, dtype=np.int32).reshape(-1)
+        _check(lib().hipclaw_set_sync_program(self._h, ops, len(ops) // 8),
+               "hipclaw_set_sync_program")
+
+    def time_sync_now(self):
+        """Apply the conservation fix-up from the registers of the last update."""
+        _check(lib().hipclaw_time_sync(self._h), "hipclaw_time_sync")
 
     def set_geometry(self, dx, dy):
         dx = np.ascontiguousarray(dx, dtype=np.float64)

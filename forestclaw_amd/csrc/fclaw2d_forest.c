@@ -539,3 +539,35 @@ int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int 
     phase_start[6] = b.n;
     return b.n;
 }
+
+/* ---------------------------------------------------------------- sync program */
+
+/* Time-synchronisation ops (see hipclaw_set_sync_program), grouped by the
+ * patch they correct, in the order correct_coarse_cells applies them
+ * (src/fclaw2d_time_sync.c:118-158 via cb_face_fill,
+ * src/fclaw2d_face_neighbors.c:321-383): same-size faces 0..3, then faces
+ * with half-size neighbours, lower child first. */
+int fclaw2d_hip_forest_sync_program(const fclaw2d_hip_forest_t *f, int mx, int my, int *ops,
+                                    int cap)
+{
+    opbuf_t b;
+    int p, k, g;
+    if (!f || mx != my || cap < 0) return -1;
+    b.ops = ops;
+    b.cap = ops ? cap : 0;
+    b.n = 0;
+    for (p = 0; p < f->n; p++) {
+        nbinfo_t r;
+        for (k = 0; k < 4; k++) {
+            face_rel(f, p, k, mx, my, &r);
+            if (r.rel == REL_NONE) return -1;
+            if (r.rel == REL_SAME) emit(&b, 0, p, r.nb[0], k, 0, 0, 0);
+        }
+        for (k = 0; k < 4; k++) {
+            face_rel(f, p, k, mx, my, &r);
+            if (r.rel != REL_HALF) continue;
+            for (g = 0; g < 2; g++) emit(&b, 1, p, r.nb[g], k, r.ti[g], r.tj[g], 0);
+        }
+    }
+    return b.n;
+}

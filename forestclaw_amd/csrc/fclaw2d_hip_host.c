@@ -317,6 +317,16 @@ void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel, do
     glob->count_ghost_update++;
 }
 
+void fclaw2d_time_sync(fclaw2d_global_t *glob, int minlevel, int maxlevel)
+{
+    if (!glob->time_sync) return;
+    if (minlevel > glob->domain->global_minlevel || maxlevel < glob->domain->global_maxlevel) {
+        fprintf(stderr, "fclaw2d_time_sync: partial level ranges are not supported\n");
+        exit(1);
+    }
+    device_check(hipclaw_time_sync(glob->state), "hipclaw_time_sync");
+}
+
 double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt)
 {
     fclaw2d_domain_t *d = glob->domain;
@@ -331,6 +341,7 @@ double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double 
         for (level = d->global_maxlevel; level >= d->global_minlevel; level--)
             maxcfl = fmax(maxcfl, fclaw2d_update_single_step(glob, level, t_curr, dt));
         fclaw2d_ghost_update(glob, d->global_minlevel, d->global_maxlevel, t_curr + dt, 0, 0);
+        if (glob->time_sync) fclaw2d_time_sync(glob, d->global_minlevel, d->global_maxlevel);
     }
     glob->curr_time = t_curr + dt;
     glob->curr_dt = dt;
@@ -410,6 +421,7 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
         cfg.mthbc[k] = opt->mthbc[k];
     }
     cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
+    cfg.time_sync = glob->time_sync;
     device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");
     dx = (double *) malloc(sizeof(double) * d->local_num_patches);
     dy = (double *) malloc(sizeof(double) * d->local_num_patches);
@@ -427,6 +439,17 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
     fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, nops, phase);
     device_check(hipclaw_set_ghost_program(glob->state, ops, nops, phase), "hipclaw_set_ghost_program");
     free(ops);
+    if (glob->time_sync) {
+        nops = fclaw2d_hip_forest_sync_program(d->forest, o->mx, o->my, NULL, 0);
+        if (nops < 0) {
+            fprintf(stderr, "fc2d_hipclaw_allocate_buffers: time sync needs mx = my\n");
+            exit(1);
+        }
+        ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
+        fclaw2d_hip_forest_sync_program(d->forest, o->mx, o->my, ops, nops);
+        device_check(hipclaw_set_sync_program(glob->state, ops, nops), "hipclaw_set_sync_program");
+        free(ops);
+    }
     glob->qhost = (double *) calloc(patch_size(glob, o->meqn) * d->local_num_patches, sizeof(double));
     if (o->maux > 0)
         glob->auxhost = (double *) calloc(patch_size(glob, o->maux) * d->local_num_patches, sizeof(double));

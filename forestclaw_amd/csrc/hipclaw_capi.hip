@@ -56,6 +56,12 @@ struct hipclaw_state {
     int phase_start[7] = {0, 0, 0, 0, 0, 0, 0};
     unsigned long long* cfl = nullptr;
     double* mass_buf = nullptr;
+    // time synchronisation: flux registers and the per-destination op groups
+    double* reg = nullptr;
+    long long rstride = 0;
+    int* sync_ops = nullptr;
+    int* sync_start = nullptr;
+    int nsync_groups = 0, sync_ops_cap = 0, sync_groups_cap = 0;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -112,6 +118,7 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
         return fail(HIPCLAW_EINVAL, "order must be (1|2, 0|1|2)%s");
     for (int k = 0; k < cfg->mwaves; k++)
         if (cfg->mthlim[k] < 0 || cfg->mthlim[k] > 6) return fail(HIPCLAW_EINVAL, "mthlim out of range%s");
+    if (cfg->mx != cfg->my) return fail(HIPCLAW_EINVAL, "the batched step requires mx = my%s");
 
     hipclaw_state* s = new hipclaw_state();
     s->cfg = *cfg;
@@ -141,6 +148,12 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     ALLOC(s->batch_slots, sizeof(int) * max_patches);
     ALLOC(s->cfl, sizeof(unsigned long long));
     ALLOC(s->mass_buf, sizeof(double) * 2 * cfg->meqn * (size_t) max_patches);
+    if (cfg->time_sync) {
+        s->rstride = 16LL * cfg->mx * cfg->meqn;
+        ALLOC(s->reg, sizeof(double) * s->rstride * max_patches);
+        if (hipMemset(s->reg, 0, sizeof(double) * s->rstride * max_patches) != hipSuccess)
+            return cleanup(fail(HIPCLAW_EDEVICE, "initial device copies failed%s"));
+    }
 #undef ALLOC
     std::vector<int> iota(max_patches);
     for (int p = 0; p < max_patches; p++) iota[p] = p;
@@ -172,6 +185,9 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->ops);
     hipFree(s->cfl);
     hipFree(s->mass_buf);
+    hipFree(s->reg);
+    hipFree(s->sync_ops);
+    hipFree(s->sync_start);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
@@ -292,6 +308,77 @@ int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, cons
     return HIPCLAW_OK;
 }
 
+int hipclaw_set_sync_program(hipclaw_state_t* s, const int* ops, int nops) {
+    if (!s || nops < 0 || (nops > 0 && !ops))
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_sync_program: bad arguments%s");
+    if (!s->reg && nops > 0) return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
+    // validate on the host and group by destination (groups must be contiguous)
+    std::vector<int> start;
+    std::vector<char> seen(s->npatch, 0);
+    for (int o = 0; o < nops; o++) {
+        const int* op = ops + 8 * o;
+        const int type = op[0], dst = op[1], src = op[2], face = op[3];
+        if ((type != hipclaw::SYNC_SAMESIZE && type != hipclaw::SYNC_F2C) || dst < 0 ||
+            dst >= s->npatch || src < 0 || src >= s->npatch || face < 0 || face > 3)
+            return fail(HIPCLAW_EINVAL, "sync op with bad type, slot or face%s");
+        if (o == 0 || op[1] != ops[8 * (o - 1) + 1]) {
+            if (seen[dst]) return fail(HIPCLAW_EINVAL, "sync ops of one destination must be contiguous%s");
+            seen[dst] = 1;
+            start.push_back(o);
+        }
+    }
+    start.push_back(nops);
+    const int ngroups = (int) start.size() - 1;
+    if (nops > s->sync_ops_cap) {
+        hipFree(s->sync_ops);
+        s->sync_ops = nullptr;
+        s->sync_ops_cap = 0;
+        if (hipMalloc((void**) &s->sync_ops, sizeof(int) * 8 * (size_t) nops) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of sync ops failed%s");
+        s->sync_ops_cap = nops;
+    }
+    if (ngroups + 1 > s->sync_groups_cap) {
+        hipFree(s->sync_start);
+        s->sync_start = nullptr;
+        s->sync_groups_cap = 0;
+        if (hipMalloc((void**) &s->sync_start, sizeof(int) * (size_t) (ngroups + 1)) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of sync groups failed%s");
+        s->sync_groups_cap = ngroups + 1;
+    }
+    if (nops > 0) {
+        HIP_TRY(hipMemcpyAsync(s->sync_ops, ops, sizeof(int) * 8 * (size_t) nops, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->sync_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
+    s->nsync_groups = ngroups;
+    return HIPCLAW_OK;
+}
+
+static int time_sync_async(hipclaw_state_t* s) {
+    if (!s->reg || s->nsync_groups == 0) return HIPCLAW_OK;
+    hipclaw::SyncArgs a;
+    a.ops = s->sync_ops;
+    a.start = s->sync_start;
+    a.ngroups = s->nsync_groups;
+    a.q = s->q[s->cur];
+    a.reg = s->reg;
+    a.dxs = s->dxs;
+    a.dys = s->dys;
+    a.pstride = s->pstride;
+    a.rstride = s->rstride;
+    a.mx = s->cfg.mx; a.mbc = s->cfg.mbc; a.meqn = s->cfg.meqn;
+    a.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->cfg.solver != HIPCLAW_RP_ADVECTION;
+    if (int e = hipclaw::launch_time_sync(a, s->st)) return hipfail((hipError_t) e, "time_sync launch");
+    return HIPCLAW_OK;
+}
+
+int hipclaw_time_sync(hipclaw_state_t* s) {
+    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
+    if (int rc = time_sync_async(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
 static int ghost_fill_async(hipclaw_state_t* s) {
     hipclaw::GhostArgs g;
     g.q = s->q[s->cur];
@@ -334,6 +421,8 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
     a.dt = dt;
     a.pp.grav = s->cfg.grav;
     a.pp.gamma = s->cfg.gamma;
+    a.reg = s->reg;
+    a.rstride = s->rstride;
     return a;
 }
 
@@ -400,6 +489,7 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
         if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n + 1], s->st));
         if (!s->inplace) s->cur = 1 - s->cur;
         if (int rc = ghost_fill_async(s)) return rc;
+        if (int rc = time_sync_async(s)) return rc;
     }
     return HIPCLAW_OK;
 }

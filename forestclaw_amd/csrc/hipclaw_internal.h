@@ -38,7 +38,24 @@ struct GhostArgs {
     int mx, my, mbc, meqn;
 };
 
+// One time-synchronisation op (int32 x 8), grouped by destination slot.
+enum { SYNC_SAMESIZE = 0, SYNC_F2C = 1 };
+
+struct SyncArgs {
+    const int* ops;      // [nops][8]: type, dst, src, face, ti, tj, -, -
+    const int* start;    // [ngroups + 1]: ops of group g are start[g] .. start[g+1]-1
+    int ngroups;
+    double* q;
+    const double* reg;   // flux registers, layout of StepArgs::reg
+    const double* dxs;
+    const double* dys;
+    long long pstride, rstride;
+    int mx, mbc, meqn;   // mx == my
+    int fluxreg;         // registers hold one shared edge flux (FP32 flux form)
+};
+
 // Launchers (return hipError_t as int).
+int launch_time_sync(const SyncArgs& a, hipStream_t st);
 int launch_ghost_phase(const GhostArgs& g, hipStream_t st);
 int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);

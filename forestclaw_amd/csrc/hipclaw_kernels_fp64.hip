@@ -261,6 +261,70 @@ int launch_mass(const double* q, const int* slots, int n, const double* dxs, con
     return (int) hipGetLastError();
 }
 
+// ---------------------------------------------------------------- time sync
+// The reference's fclaw2d_time_sync (src/fclaw2d_time_sync.c:118-202) with its
+// Fortran kernels clawpack46_fort_time_sync_samesize / _f2c
+// (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_time_sync.f:249-700):
+// the coarse patch's edge flux is replaced by the sum of the fine fluxes
+// across each coarse/fine face, and half the mismatch between same-size
+// neighbours goes to each side.  One workgroup per destination patch runs its
+// ops in order (same-size faces 0..3, then coarse/fine faces), so a corner
+// cell receives its corrections in the reference's order.  Registers are
+// read only; every op writes cells of its own destination patch.
+__global__ void __launch_bounds__(128) time_sync_groups(SyncArgs a) {
+    const int g = blockIdx.x;
+    if (g >= a.ngroups) return;
+    const int mx = a.mx, mbc = a.mbc, meqn = a.meqn, nx = mx + 2 * mbc;
+    const int kout = a.fluxreg ? REG_EFIN : REG_EFOUT;
+    for (int o = a.start[g]; o < a.start[g + 1]; o++) {
+        const int* op = a.ops + 8 * o;
+        const int type = op[0], dst = op[1], src = op[2], face = op[3], ti = op[4], tj = op[5];
+        double* __restrict__ q = a.q + (long long) dst * a.pstride;
+        const double* __restrict__ rt = a.reg + (long long) dst * a.rstride;
+        const double* __restrict__ rn = a.reg + (long long) src * a.rstride;
+        const double area = a.dxs[dst] * a.dys[dst];
+        const int kn = face ^ 1;              // the neighbour's face on this interface
+        const int kfl = (face == 0 || face == 2) ? REG_FP : REG_FM;
+        for (int e = threadIdx.x; e < meqn * mx; e += blockDim.x) {
+            const int m = e / mx, c = e - m * mx + 1;   // c = 1..mx along the face
+            auto R = [&](const double* r, int k, int kind, int pos) -> double {
+                return r[((k * 4 + kind) * meqn + m) * mx + pos];
+            };
+            int i, j;
+            if (face == 0) { i = 1; j = c; }
+            else if (face == 1) { i = mx; j = c; }
+            else if (face == 2) { i = c; j = 1; }
+            else { i = c; j = mx; }
+            double* qc = q + ((long long) m * nx + (j + mbc - 1)) * nx + (i + mbc - 1);
+            const double ein = R(rt, face, REG_EFIN, c - 1);
+            const double ft = a.fluxreg ? 0.0 : R(rt, face, kfl, c - 1);
+            if (type == SYNC_SAMESIZE) {
+                const double ef = R(rn, kn, kout, c - 1);
+                const double fn = a.fluxreg ? 0.0 : R(rn, kn, kfl, c - 1);
+                const double delta = (face == 0 || face == 2) ? (ef - ein) - (fn - ft)
+                                                              : (ein - ef) - (fn - ft);
+                *qc = *qc + 0.5 * delta / area;
+            } else {
+                const int f0 = 2 * c - 1 + (face < 2 ? tj : ti);  // fine rows/columns f0, f0+1
+                double deltac = 0.0;
+                if (f0 >= 1 && f0 + 1 <= mx) {
+                    const double ef = R(rn, kn, kout, f0 - 1) + R(rn, kn, kout, f0);
+                    const double flu = a.fluxreg ? 0.0 : R(rn, kn, kfl, f0 - 1) + R(rn, kn, kfl, f0);
+                    deltac = (face == 0 || face == 2) ? (ef - ein) - (flu - ft) : (ein - ef) - (flu - ft);
+                }
+                *qc = *qc + deltac / area;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+int launch_time_sync(const SyncArgs& a, hipStream_t st) {
+    if (a.ngroups <= 0) return 0;
+    hipLaunchKernelGGL(time_sync_groups, dim3(a.ngroups), dim3(128), 0, st, a);
+    return (int) hipGetLastError();
+}
+
 // ---------------------------------------------------------------- FP64 step
 template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {

@@ -155,6 +155,19 @@ struct RPShallow {
         f[mu] = q[mu] * un + 0.5 * pp.grav * h * h;
         f[mv] = q[mv] * un;
     }
+
+    // conservative flux stored in the time-sync registers, operation for
+    // operation as applications/clawpack/shallow/2d/radialdam/rpn2cons_update.f:27-46
+    template <int IXY>
+    __device__ __forceinline__ static void consflux(const double* q, const double* /*aux*/,
+                                                    const PhysParams& pp, double* f) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const double hun = q[mu], hut = q[mv], h = q[0];
+        const double un = hun / h;
+        f[0] = hun;
+        f[mu] = (hun * hun) / h + (0.5 * pp.grav) * (h * h);
+        f[mv] = un * hut;
+    }
 };
 
 // ----------------------------------------------------------------- Euler
@@ -302,6 +315,20 @@ struct RPEuler {
         f[mv] = q[mv] * un;
         f[3] = (q[3] + p) * un;
     }
+
+    // conservative flux stored in the time-sync registers (the reference has
+    // no Euler rpn2_cons; this is the physical flux, evaluated as the oracle does)
+    template <int IXY>
+    __device__ __forceinline__ static void consflux(const double* q, const double* /*aux*/,
+                                                    const PhysParams& pp, double* f) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const double rho = q[0], un = q[mu] / rho, ut = q[mv] / rho, en = q[3];
+        const double p = (pp.gamma - 1.0) * (en - 0.5 * rho * (un * un + ut * ut));
+        f[0] = q[mu];
+        f[mu] = q[mu] * un + p;
+        f[mv] = q[mu] * ut;
+        f[3] = un * (en + p);
+    }
 };
 
 // ----------------------------------------------------------------- advection
@@ -339,6 +366,14 @@ struct RPAdvection {
     template <int IXY>
     __device__ __forceinline__ static void flux(const double* q, const double* aux,
                                                 const PhysParams&, double* f) {
+        f[0] = aux[IXY - 1] * q[0];
+    }
+
+    // conservative flux stored in the time-sync registers: u_n q with the
+    // cell's velocity (applications/clawpack/advection/2d/adv_order3/user_4.6/rpn2cons_update.f:14-26)
+    template <int IXY>
+    __device__ __forceinline__ static void consflux(const double* q, const double* aux,
+                                                    const PhysParams&, double* f) {
         f[0] = aux[IXY - 1] * q[0];
     }
 };

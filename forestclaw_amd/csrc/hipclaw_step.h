@@ -52,7 +52,17 @@ struct StepArgs {
     int mthlim[4];
     double dt;
     PhysParams pp;
+    // Time-sync flux registers (null: not kept).  Per slot
+    // reg[((face * 4 + kind) * meqn + m) * mx + pos], face 0..3 = left, right,
+    // bottom, top; pos = j-1 (faces 0/1) or i-1 (faces 2/3).  The FP64 form
+    // writes the step's edge fluctuations (kinds REG_FM, REG_FP) and the
+    // conservative flux at the first interior / first ghost cell (REG_EFIN,
+    // REG_EFOUT); the FP32 flux form writes its shared FP64 edge flux in REG_EFIN.
+    double* reg;
+    long long rstride;
 };
+
+enum { REG_FM = 0, REG_FP = 1, REG_EFIN = 2, REG_EFOUT = 3 };
 
 template <typename R>
 __device__ __forceinline__ R philim(int mth, R r) {
@@ -422,6 +432,19 @@ step_tiles(StepArgs a) {
                     if (act && own_col && k >= 1 && k <= T)
                         GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
                 }
+                // time-sync registers: gm, gp at the patch's bottom (j = 1) and
+                // top (j = my+1) edges (clawpack46_time_sync.f:201-209)
+                if (a.reg != nullptr && act && own_col &&
+                    ((k == 1 && first_y) || (k == T + 1 && last_y))) {
+                    double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
+                    const double dtdx_el = a.dt * a.dxs[slot];
+                    const int face = k == 1 ? 2 : 3, pos = i0 + c - 2;
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        rg[((face * 4 + REG_FM) * MEQN + m) * mx + pos] = 0.0 + dtdx_el * (double) gm[m];
+                        rg[((face * 4 + REG_FP) * MEQN + m) * mx + pos] = 0.0 - dtdx_el * (double) gp[m];
+                    }
+                }
             }
             R ty1[MEQN], ty2[MEQN];
             if (a.transverse) {
@@ -470,6 +493,47 @@ step_tiles(StepArgs a) {
                 else  // FP32 fluctuations, FP64 state: q never passes through FP32
                     qo[off] = (qc - dtdx64 * (double) Dx) - dtdy64 * (double) Dy;
             }
+            if (a.reg != nullptr) {
+                double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
+                const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+                // fm, fp at the patch's left (i = 1) and right (i = mx+1) edges,
+                // summed as the update sums them (clawpack46_time_sync.f:192-199)
+                for (int e = threadIdx.x; e < 2 * MEQN * T; e += NT) {
+                    const int side = e / (MEQN * T), rem = e - side * (MEQN * T);
+                    const int m = rem / T, b = rem - m * T;
+                    if (side == 0 ? !first_x : !last_x) continue;
+                    const int s0 = side ? T : 0;
+                    const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    const R fpv = ((R(0) + sm.fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    rg[((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b] = 0.0 + dtdy_el * (double) fmv;
+                    rg[((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b] = 0.0 - dtdy_el * (double) fpv;
+                }
+                // conservative flux of the pre-step state at the first interior
+                // and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
+                for (int e = threadIdx.x; e < 8 * T; e += NT) {
+                    const int face = e / (2 * T), rem = e - face * (2 * T);
+                    const int kio = rem / T, c = rem - kio * T;
+                    bool on;
+                    int qi, qj;
+                    if (face == 0) { on = first_x; qi = 2 - kio; qj = c + 2; }
+                    else if (face == 1) { on = last_x; qi = T + 1 + kio; qj = c + 2; }
+                    else if (face == 2) { on = first_y; qi = c + 2; qj = 2 - kio; }
+                    else { on = last_y; qi = c + 2; qj = T + 1 + kio; }
+                    if (!on) continue;
+                    double qcell[MEQN], acell[MAUX > 0 ? MAUX : 1], fl[MEQN];
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
+#pragma unroll
+                    for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) acell[m] = (double) SA(m, qi, qj);
+                    if (face < 2) RP::template consflux<1>(qcell, acell, a.pp, fl);
+                    else RP::template consflux<2>(qcell, acell, a.pp, fl);
+                    const double el = face < 2 ? dtdy_el : dtdx_el;
+                    const int pos = (face < 2 ? j0 : i0) - 1 + c;
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++)
+                        rg[((face * 4 + REG_EFIN + kio) * MEQN + m) * mx + pos] = 0.0 + el * fl[m];
+                }
+            }
         } else {
             // conservative FP64 interface fluxes, shared by both neighbours
             for (int e = threadIdx.x; e < 2 * T * (T + 1); e += NT) {
@@ -496,6 +560,22 @@ step_tiles(StepArgs a) {
                 }
             }
             __syncthreads();
+            if (a.reg != nullptr) {
+                // the shared FP64 flux through each patch edge, times dt * edge length
+                double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
+                const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+                for (int e = threadIdx.x; e < 4 * MEQN * T; e += NT) {
+                    const int face = e / (MEQN * T), rem = e - face * (MEQN * T);
+                    const int m = rem / T, c = rem - m * T;
+                    double F;
+                    if (face == 0) { if (!first_x) continue; F = sm.fx[(m * T + c) * (T + 1)]; }
+                    else if (face == 1) { if (!last_x) continue; F = sm.fx[(m * T + c) * (T + 1) + T]; }
+                    else if (face == 2) { if (!first_y) continue; F = sm.fy[(m * (T + 1)) * T + c]; }
+                    else { if (!last_y) continue; F = sm.fy[(m * (T + 1) + T) * T + c]; }
+                    const int pos = (face < 2 ? j0 : i0) - 1 + c;
+                    rg[((face * 4 + REG_EFIN) * MEQN + m) * mx + pos] = (face < 2 ? dtdy_el : dtdx_el) * F;
+                }
+            }
             for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
                 const int m = e / (T * T), rem = e - m * T * T;
                 const int b = rem / T, c = rem - b * T;

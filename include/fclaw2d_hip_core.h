@@ -94,6 +94,7 @@ typedef struct fclaw2d_global {
     struct fc2d_hipclaw_options *hipclaw_opt;
     struct hipclaw_state *state;   /* device-resident solution */
     double *qhost, *auxhost;       /* host views [slot][field][j][i] */
+    int time_sync;                 /* fclaw_options time-sync (src/fclaw_options.c:110) */
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -134,6 +135,11 @@ void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel,
  *  (fclaw2d_advance_all_levels with subcycle = F, advance_one_step = T,
  *  src/fclaw2d_advance.c:247-357).  Returns the maximum Courant number. */
 double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt);
+
+/* Conservation fix-up after all levels reached the same time
+ * (src/fclaw2d_time_sync.c:161-202); needs glob->time_sync set before
+ * fc2d_hipclaw_allocate_buffers. */
+void fclaw2d_time_sync(fclaw2d_global_t *glob, int minlevel, int maxlevel);
 
 /* ---- clawpatch accessors (host views) ---- */
 void fclaw2d_clawpatch_grid_data(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int *mx,

@@ -48,6 +48,12 @@ int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int 
                                      const int mthbc[4], int *ops, int cap,
                                      int phase_start[7]);
 
+/* Time-synchronisation program for the current leaves (see
+ * hipclaw_set_sync_program): writes up to `cap` ops of 8 ints and returns the
+ * number needed (cap = 0 to size the buffer), or -1 on error (mx != my). */
+int fclaw2d_hip_forest_sync_program(const fclaw2d_hip_forest_t *f, int mx, int my, int *ops,
+                                    int cap);
+
 /* Face neighbour relation of patch p (0 boundary, 1 same size, 2 half size,
  * 3 double size) -- exposed for tests and the host plugin layer. */
 int fclaw2d_hip_forest_face_relation(const fclaw2d_hip_forest_t *f, int p, int iface,

@@ -54,6 +54,8 @@ typedef struct hipclaw_config {
     double grav;     /* shallow water */
     double gamma;    /* Euler */
     int tile;        /* cells per tile side (8, 16, 32); 0 = tuned default */
+    int time_sync;   /* keep flux registers for the conservation fix-up
+                        (fclaw_options time-sync, src/fclaw_options.c:110); needs mx == my */
 } hipclaw_config_t;
 
 typedef struct hipclaw_state hipclaw_state_t;
@@ -115,8 +117,24 @@ int hipclaw_ghost_fill(hipclaw_state_t *s);
 int hipclaw_step_batch(hipclaw_state_t *s, const int *slots, int nslots, double t,
                        double dt, double *maxcfl);
 
-/* nsteps of (update every patch at every level, ghost fill), all resident on
- * the GPU; *maxcfl receives the maximum over the steps.  Equivalent to
+/* Time-synchronisation program (config time_sync = 1): nops ops of 8 int32
+ * (type, dst, src, face, ti, tj, 0, 0), type 0 = same-size correction of dst
+ * from neighbour src across dst's face, 1 = fine-to-coarse correction of
+ * coarse dst from fine src (the first fine child of coarse cell (ic, jc) is
+ * (2ic-1+ti, 2jc-1+tj)).  Ops of one destination are contiguous, same-size
+ * faces first, as correct_coarse_cells applies them
+ * (src/fclaw2d_time_sync.c:118-158). */
+int hipclaw_set_sync_program(hipclaw_state_t *s, const int *ops, int nops);
+
+/* Apply the conservation fix-up from the flux registers of the last update of
+ * every patch.  Replaces fclaw2d_time_sync (src/fclaw2d_time_sync.c:161-202)
+ * with clawpack46_fort_time_sync_f2c / _samesize
+ * (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_time_sync.f:249-700). */
+int hipclaw_time_sync(hipclaw_state_t *s);
+
+/* nsteps of (update every patch at every level, ghost fill, and with
+ * time_sync the conservation fix-up), all resident on the GPU; *maxcfl
+ * receives the maximum over the steps.  Equivalent to
  * fclaw2d_advance_all_levels with subcycle = F
  * (src/fclaw2d_advance.c:247-357). */
 int hipclaw_advance(hipclaw_state_t *s, double dt, int nsteps, double *maxcfl);

@@ -574,12 +574,107 @@ double oracle_update_all(const oracle_domain_t *d, const oracle_claw_t *claw,
 double oracle_advance(const oracle_domain_t *d, const oracle_claw_t *claw,
                       double *q, const double *aux, double dt, int nsteps)
 {
-    double maxcfl = 0.0;
-    int n;
-    for (n = 0; n < nsteps; n++) {
-        maxcfl = fmax(maxcfl, oracle_update_all(d, claw, q, aux, dt));
-        oracle_ghost_update(d, q);
+    return oracle_advance_ts(d, claw, q, aux, dt, nsteps, 0);
+}
+
+/* correct_coarse_cells (src/fclaw2d_time_sync.c:118-158) with one process:
+ * per level, finest first, the same-size corrections of every patch face
+ * (cb_face_fill with FCLAW2D_TIME_SYNC_SAMESIZE, src/fclaw2d_face_neighbors.c:368-383),
+ * then, below the finest level, the fine-to-coarse corrections of every
+ * coarse face with half-size neighbours (:321-336).  Every register is reset
+ * by one of the three resets (SAMESIZE, F2C, PHYS), so all end zero. */
+int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg)
+{
+    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
+    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t rsize = oracle_sync_register_count(mx, meqn);
+    const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
+    ohash_t h;
+    int p, level, k, g, minlevel = 1 << 30, maxlevel = -1;
+    size_t e;
+    if (d->npatch <= 0) return 0;
+    if (mx != my) return -1;
+    hash_init(&h, d->npatch);
+    for (p = 0; p < d->npatch; p++) {
+        hash_put(&h, hkey(d->level[p], d->gx[p], d->gy[p]), p);
+        if (d->level[p] < minlevel) minlevel = d->level[p];
+        if (d->level[p] > maxlevel) maxlevel = d->level[p];
     }
+    for (level = maxlevel; level >= minlevel; level--) {
+        for (p = 0; p < d->npatch; p++) {
+            const double nl = (double) (1 << d->level[p]);
+            const double dx = wx / (nl * mx), dy = wy / (nl * my);
+            if (d->level[p] != level) continue;
+            for (k = 0; k < 4; k++) {
+                face_nb_t f;
+                face_neighbor(d, &h, p, k, &f);
+                if (f.type == NB_SAME)
+                    oracle_time_sync_samesize(mx, my, mbc, meqn, PATCH(q, p), reg + p * rsize,
+                                              reg + (size_t) f.nb[0] * rsize, k, dx, dy);
+            }
+        }
+        if (level == maxlevel) continue;
+        for (p = 0; p < d->npatch; p++) {
+            const double nl = (double) (1 << d->level[p]);
+            const double dx = wx / (nl * mx), dy = wy / (nl * my);
+            if (d->level[p] != level) continue;
+            for (k = 0; k < 4; k++) {
+                face_nb_t f;
+                face_neighbor(d, &h, p, k, &f);
+                if (f.type != NB_HALF) continue;
+                for (g = 0; g < 2; g++)
+                    oracle_time_sync_f2c(mx, my, mbc, meqn, PATCH(q, p), reg + p * rsize,
+                                         reg + (size_t) f.nb[g] * rsize, k, f.ti[g], f.tj[g], dx, dy);
+            }
+        }
+    }
+    for (e = 0; e < (size_t) d->npatch * rsize; e++) reg[e] = 0.0;
+    hash_free(&h);
+    return 0;
+}
+
+/* nsteps of fclaw2d_advance_all_levels with subcycle = F
+ * (src/fclaw2d_advance.c:247-357): every patch steps at dt (with, if
+ * time_sync, the flux registers of fc2d_clawpack46.cpp:317-392 filled before
+ * and after its step), then one ghost update over all levels, then, if
+ * time_sync, fclaw2d_time_sync. */
+double oracle_advance_ts(const oracle_domain_t *d, const oracle_claw_t *claw, double *q,
+                         const double *aux, double dt, int nsteps, int time_sync)
+{
+    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
+    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t asize = (size_t) (d->maux > 0 ? d->maux : 0) * (mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t rsize = oracle_sync_register_count(mx, meqn);
+    const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
+    double *fl = NULL, *reg = NULL, maxcfl = 0.0;
+    int n, p;
+    if (time_sync) {
+        fl = (double *) malloc(4 * psize * sizeof(double));
+        reg = (double *) calloc((size_t) d->npatch * rsize, sizeof(double));
+    }
+    for (n = 0; n < nsteps; n++) {
+        if (!time_sync) {
+            maxcfl = fmax(maxcfl, oracle_update_all(d, claw, q, aux, dt));
+            oracle_ghost_update(d, q);
+            continue;
+        }
+        for (p = 0; p < d->npatch; p++) {
+            const double nl = (double) (1 << d->level[p]);
+            const double dx = wx / (nl * mx), dy = wy / (nl * my);
+            const double *ap = aux ? aux + (size_t) p * asize : NULL;
+            double cfl;
+            oracle_store_flux(claw, mx, my, mbc, PATCH(q, p), ap, dt, dx, dy, reg + p * rsize);
+            oracle_step2_patch(claw, mx, my, mbc, PATCH(q, p), ap, dx, dy, dt, &cfl,
+                               fl, fl + psize, fl + 2 * psize, fl + 3 * psize);
+            oracle_accumulate_waves(mx, my, mbc, meqn, dt, dx, dy, fl + psize, fl,
+                                    fl + 3 * psize, fl + 2 * psize, reg + p * rsize);
+            maxcfl = fmax(cfl, maxcfl);
+        }
+        oracle_ghost_update(d, q);
+        oracle_time_sync(d, q, reg);
+    }
+    free(fl);
+    free(reg);
     return maxcfl;
 }
 

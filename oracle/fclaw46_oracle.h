@@ -14,6 +14,8 @@
 #ifndef FCLAW46_ORACLE_H
 #define FCLAW46_ORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -79,6 +81,33 @@ double oracle_update_all(const oracle_domain_t *d, const oracle_claw_t *claw,
 /* nsteps of (update all levels, ghost update) at fixed dt (subcycle = F). */
 double oracle_advance(const oracle_domain_t *d, const oracle_claw_t *claw,
                       double *q, const double *aux, double dt, int nsteps);
+
+/* Same with the reference's time synchronisation switched on or off. */
+double oracle_advance_ts(const oracle_domain_t *d, const oracle_claw_t *claw,
+                         double *q, const double *aux, double dt, int nsteps, int time_sync);
+
+/* ---------------- time synchronisation (conservation fix-up) ----------------
+ * Flux registers of one patch (mx == my): reg[((face*4 + kind)*meqn + m)*mx + pos]
+ * with face 0..3 (left, right, bottom, top), pos = j-1 (faces 0/1) or i-1
+ * (faces 2/3).  Kinds: the step's fm and fp at that edge (gm/gp on faces 2/3)
+ * and the conservative flux at the edge's interior and exterior cell. */
+enum { ORACLE_REG_FM = 0, ORACLE_REG_FP = 1, ORACLE_REG_EFIN = 2, ORACLE_REG_EFOUT = 3 };
+size_t oracle_sync_register_count(int mx, int meqn);
+void oracle_cons_flux(const oracle_claw_t *claw, int idir, const double *q, const double *auxc,
+                      double *flux);
+void oracle_store_flux(const oracle_claw_t *claw, int mx, int my, int mbc, const double *q,
+                       const double *aux, double dt, double dx, double dy, double *reg);
+void oracle_accumulate_waves(int mx, int my, int mbc, int meqn, double dt, double dx, double dy,
+                             const double *fp, const double *fm, const double *gp,
+                             const double *gm, double *reg);
+void oracle_time_sync_f2c(int mx, int my, int mbc, int meqn, double *qcoarse,
+                          const double *rcoarse, const double *rfine, int iface, int ti, int tj,
+                          double dx, double dy);
+void oracle_time_sync_samesize(int mx, int my, int mbc, int meqn, double *qthis,
+                               const double *rthis, const double *rnbr, int iface, double dx,
+                               double dy);
+/* fclaw2d_time_sync over all levels; registers are left zero. */
+int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg);
 
 /* Kahan mass sum per field in patch order:
  * src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_diagnostics.f:13-55 */

@@ -69,6 +69,18 @@ def oracle_advance(prob, forest, q, aux, mx, dt, nsteps):
     return out, cfl
 
 
+def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True):
+    """nsteps of (update all levels, ghost update, time sync) on the oracle."""
+    d = oracle_domain(prob, forest, mx)
+    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma)
+    out = np.ascontiguousarray(q, dtype=np.float64).copy()
+    auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    cfl = oracle().oracle_advance_ts(C.byref(d), C.byref(claw), out.reshape(-1),
+                                     None if auxp is None else auxp.ctypes.data, dt, nsteps,
+                                     int(time_sync))
+    return out, cfl
+
+
 def oracle_mass(prob, forest, q, mx):
     d = oracle_domain(prob, forest, mx)
     s = np.zeros(prob.meqn)
@@ -80,13 +92,14 @@ def interior(q, mbc=2):
     return q[..., mbc:-mbc, mbc:-mbc]
 
 
-def setup_problem(prob, precision=fa.FP64, tile=0):
+def setup_problem(prob, precision=fa.FP64, tile=0, time_sync=False):
     """Forest, initial data (ghost cells filled by the oracle) and a GPU solver."""
     forest = P.build_forest(prob)
     q0, aux = P.initial_data(prob, forest)
     q0 = oracle_ghost_update(prob, forest, q0, prob.mx)
     sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
-                   grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch)
+                   grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
+                   time_sync=time_sync)
     sv.set_forest(forest, prob.extent)
     sv.set_q(q0)
     if aux is not None:

@@ -56,6 +56,10 @@ def oracle():
         lib.oracle_advance.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw), _dp,
                                        C.c_void_p, C.c_double, C.c_int]
         lib.oracle_advance.restype = C.c_double
+        lib.oracle_advance_ts.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw), _dp,
+                                          C.c_void_p, C.c_double, C.c_int, C.c_int]
+        lib.oracle_advance_ts.restype = C.c_double
+        lib.oracle_cons_flux.argtypes = [C.POINTER(OracleClaw), C.c_int, _dp, _dp, _dp]
         lib.oracle_mass.argtypes = [C.POINTER(OracleDomain), _dp, _dp]
         lib.oracle_bc2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip]
         lib.oracle_inlinelimiter.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -82,6 +86,8 @@ def ref():
         lib.ref46_step2.restype = C.c_int
         lib.ref46_bc2.argtypes = [C.c_int] * 4 + [_dp, _ip]
         lib.ref46_inlinelimiter.argtypes = [C.c_int] * 5 + [_dp, _dp, _ip]
+        ip = C.POINTER(C.c_int)
+        lib.rpn2_cons_update_.argtypes = [ip, ip, ip, ip, _dp, _dp, _dp, _dp]
         _ref = lib
     return _ref
 
