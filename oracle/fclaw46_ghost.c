@@ -450,7 +450,11 @@ static int corner_is_interior(const oracle_domain_t *d, int p, int icorner)
 
 enum { FILL_AVERAGE, FILL_COPY, FILL_INTERP };
 
-static void face_fill_level(const oracle_domain_t *d, const ohash_t *h, double *q,
+/* One level of face exchanges: `qt` holds the patches of `level` (coarse side
+ * for average/interpolate), `qn` their neighbours (fine side, or same level).
+ * With time interpolation the coarse level's arrays are its time-interpolated
+ * copies (fclaw2d_clawpatch_timesync_data, src/patches/clawpatch/fclaw2d_clawpatch.cpp:1746-1754). */
+static void face_fill_level(const oracle_domain_t *d, const ohash_t *h, double *qt, double *qn,
                             int level, int filltype)
 {
     const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
@@ -464,21 +468,21 @@ static void face_fill_level(const oracle_domain_t *d, const ohash_t *h, double *
             if (f.type == NB_HALF) {
                 for (ig = 0; ig < 2; ig++) {
                     if (filltype == FILL_INTERP)
-                        oracle_interpolate_face(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[ig]),
+                        oracle_interpolate_face(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[ig]),
                                                 iface / 2, iface, f.ti[ig], f.tj[ig]);
                     else if (filltype == FILL_AVERAGE)
-                        oracle_average_face(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[ig]),
+                        oracle_average_face(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[ig]),
                                             iface / 2, iface, f.ti[ig], f.tj[ig]);
                 }
             } else if (f.type == NB_SAME && filltype == FILL_COPY) {
-                oracle_copy_face(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[0]), iface, f.ti[0], f.tj[0]);
+                oracle_copy_face(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[0]), iface, f.ti[0], f.tj[0]);
             }
         }
     }
 }
 
-static void corner_fill_level(const oracle_domain_t *d, const ohash_t *h, double *q,
-                              int level, int filltype)
+static void corner_fill_level(const oracle_domain_t *d, const ohash_t *h, double *qt,
+                              double *qn, int level, int filltype)
 {
     const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
     const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
@@ -491,11 +495,11 @@ static void corner_fill_level(const oracle_domain_t *d, const ohash_t *h, double
             corner_neighbor(d, h, p, ic, &f);
             if (f.type == NB_HALF) {
                 if (filltype == FILL_INTERP)
-                    oracle_interpolate_corner(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[0]), ic, f.ti[0], f.tj[0]);
+                    oracle_interpolate_corner(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[0]), ic, f.ti[0], f.tj[0]);
                 else if (filltype == FILL_AVERAGE)
-                    oracle_average_corner(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[0]), ic, f.ti[0], f.tj[0]);
+                    oracle_average_corner(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[0]), ic, f.ti[0], f.tj[0]);
             } else if (f.type == NB_SAME && filltype == FILL_COPY) {
-                oracle_copy_corner(mx, my, mbc, meqn, PATCH(q, p), PATCH(q, f.nb[0]), ic, f.ti[0], f.tj[0]);
+                oracle_copy_corner(mx, my, mbc, meqn, PATCH(qt, p), PATCH(qn, f.nb[0]), ic, f.ti[0], f.tj[0]);
             }
         }
     }
@@ -516,37 +520,75 @@ static void physbc_level(const oracle_domain_t *d, double *q, int level)
     }
 }
 
+/* fclaw2d_ghost_update_async over levels minlevel..maxlevel with one process
+ * (the INTERIOR_ONLY pass, then the final physical-BC sweep).  With
+ * time_interp, level minlevel-1 takes part through its time-interpolated
+ * arrays qti: fine level minlevel averages into their ghost cells, they copy
+ * among themselves, get physical BCs, and interpolate to level minlevel's
+ * ghost cells (src/fclaw2d_ghost_fill.c:341-442).  The reference also applies
+ * physical BCs to time-interpolated arrays that nothing reads afterwards
+ * (level minlevel in the fine sweep, level minlevel-1 in the final sweep);
+ * those are left out, since q does not depend on them. */
+static void ghost_update_range(const oracle_domain_t *d, const ohash_t *h, double *q, double *qti,
+                               int minlevel, int maxlevel, int time_interp)
+{
+    const int ti = minlevel - 1, mincoarse = minlevel, maxcoarse = maxlevel - 1;
+    int level;
+    /* average_fine2coarse_ghost */
+    for (level = maxcoarse; level >= mincoarse; level--) {
+        face_fill_level(d, h, q, q, level, FILL_AVERAGE);
+        corner_fill_level(d, h, q, q, level, FILL_AVERAGE);
+    }
+    if (time_interp) {
+        face_fill_level(d, h, qti, q, ti, FILL_AVERAGE);
+        corner_fill_level(d, h, qti, q, ti, FILL_AVERAGE);
+    }
+    /* copy_samelevel */
+    for (level = maxlevel; level >= minlevel; level--) {
+        face_fill_level(d, h, q, q, level, FILL_COPY);
+        corner_fill_level(d, h, q, q, level, FILL_COPY);
+    }
+    if (time_interp) {
+        face_fill_level(d, h, qti, qti, ti, FILL_COPY);
+        corner_fill_level(d, h, qti, qti, ti, FILL_COPY);
+    }
+    /* fill_physical_ghost on coarse levels */
+    for (level = maxcoarse; level >= mincoarse; level--) physbc_level(d, q, level);
+    if (time_interp) physbc_level(d, qti, ti);
+    /* interpolate_coarse2fine_ghost */
+    for (level = maxcoarse; level >= mincoarse; level--) {
+        face_fill_level(d, h, q, q, level, FILL_INTERP);
+        corner_fill_level(d, h, q, q, level, FILL_INTERP);
+    }
+    if (time_interp) {
+        face_fill_level(d, h, qti, q, ti, FILL_INTERP);
+        corner_fill_level(d, h, qti, q, ti, FILL_INTERP);
+    }
+    /* fill_physical_ghost on fine levels, then on all levels */
+    for (level = maxlevel; level >= mincoarse + 1; level--) physbc_level(d, q, level);
+    for (level = maxlevel; level >= minlevel; level--) physbc_level(d, q, level);
+}
+
+static void domain_levels(const oracle_domain_t *d, ohash_t *h, int *minlevel, int *maxlevel)
+{
+    int p;
+    *minlevel = 1 << 30;
+    *maxlevel = -1;
+    hash_init(h, d->npatch);
+    for (p = 0; p < d->npatch; p++) {
+        hash_put(h, hkey(d->level[p], d->gx[p], d->gy[p]), p);
+        if (d->level[p] < *minlevel) *minlevel = d->level[p];
+        if (d->level[p] > *maxlevel) *maxlevel = d->level[p];
+    }
+}
+
 int oracle_ghost_update(const oracle_domain_t *d, double *q)
 {
     ohash_t h;
-    int p, level, minlevel = 1 << 30, maxlevel = -1;
+    int minlevel, maxlevel;
     if (d->npatch <= 0) return 0;
-    hash_init(&h, d->npatch);
-    for (p = 0; p < d->npatch; p++) {
-        hash_put(&h, hkey(d->level[p], d->gx[p], d->gy[p]), p);
-        if (d->level[p] < minlevel) minlevel = d->level[p];
-        if (d->level[p] > maxlevel) maxlevel = d->level[p];
-    }
-    /* average_fine2coarse_ghost: coarse levels maxlevel-1 .. minlevel */
-    for (level = maxlevel - 1; level >= minlevel; level--) {
-        face_fill_level(d, &h, q, level, FILL_AVERAGE);
-        corner_fill_level(d, &h, q, level, FILL_AVERAGE);
-    }
-    /* copy_samelevel: maxlevel .. minlevel */
-    for (level = maxlevel; level >= minlevel; level--) {
-        face_fill_level(d, &h, q, level, FILL_COPY);
-        corner_fill_level(d, &h, q, level, FILL_COPY);
-    }
-    /* fill_physical_ghost on coarse levels */
-    for (level = maxlevel - 1; level >= minlevel; level--) physbc_level(d, q, level);
-    /* interpolate_coarse2fine_ghost */
-    for (level = maxlevel - 1; level >= minlevel; level--) {
-        face_fill_level(d, &h, q, level, FILL_INTERP);
-        corner_fill_level(d, &h, q, level, FILL_INTERP);
-    }
-    /* fill_physical_ghost on fine levels, then on all levels */
-    for (level = maxlevel; level >= minlevel + 1; level--) physbc_level(d, q, level);
-    for (level = maxlevel; level >= minlevel; level--) physbc_level(d, q, level);
+    domain_levels(d, &h, &minlevel, &maxlevel);
+    ghost_update_range(d, &h, q, NULL, minlevel, maxlevel, 0);
     hash_free(&h);
     return 0;
 }
@@ -577,29 +619,32 @@ double oracle_advance(const oracle_domain_t *d, const oracle_claw_t *claw,
     return oracle_advance_ts(d, claw, q, aux, dt, nsteps, 0);
 }
 
-/* correct_coarse_cells (src/fclaw2d_time_sync.c:118-158) with one process:
- * per level, finest first, the same-size corrections of every patch face
- * (cb_face_fill with FCLAW2D_TIME_SYNC_SAMESIZE, src/fclaw2d_face_neighbors.c:368-383),
- * then, below the finest level, the fine-to-coarse corrections of every
- * coarse face with half-size neighbours (:321-336).  Every register is reset
- * by one of the three resets (SAMESIZE, F2C, PHYS), so all end zero. */
-int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg)
+/* Zero the registers of one face: fclaw2d_clawpatch_time_sync_reset
+ * (src/patches/clawpatch/fclaw2d_clawpatch_conservation.c:207-278). */
+static void reset_face(double *reg, int mx, int meqn, int face)
+{
+    int kind, e;
+    for (kind = 0; kind < 4; kind++)
+        for (e = 0; e < meqn * mx; e++) reg[((size_t) face * 4 + kind) * meqn * mx + e] = 0.0;
+}
+
+/* correct_coarse_cells (src/fclaw2d_time_sync.c:118-158) with one process, for
+ * levels maxlevel..minlevel: per level, finest first, the same-size
+ * corrections of every patch face (cb_face_fill with
+ * FCLAW2D_TIME_SYNC_SAMESIZE, src/fclaw2d_face_neighbors.c:368-383) and the
+ * reset of those registers, then, below the finest level, the fine-to-coarse
+ * corrections of every coarse face with half-size neighbours (:321-336) and
+ * the reset of the registers on both sides of those faces; finally the reset
+ * of the registers on physical boundaries.  Registers of faces between level
+ * minlevel and minlevel-1 keep accumulating (subcycling). */
+static void time_sync_range(const oracle_domain_t *d, const ohash_t *h, double *q, double *reg,
+                            int minlevel, int maxlevel)
 {
     const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
     const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
     const size_t rsize = oracle_sync_register_count(mx, meqn);
     const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
-    ohash_t h;
-    int p, level, k, g, minlevel = 1 << 30, maxlevel = -1;
-    size_t e;
-    if (d->npatch <= 0) return 0;
-    if (mx != my) return -1;
-    hash_init(&h, d->npatch);
-    for (p = 0; p < d->npatch; p++) {
-        hash_put(&h, hkey(d->level[p], d->gx[p], d->gy[p]), p);
-        if (d->level[p] < minlevel) minlevel = d->level[p];
-        if (d->level[p] > maxlevel) maxlevel = d->level[p];
-    }
+    int p, level, k, g;
     for (level = maxlevel; level >= minlevel; level--) {
         for (p = 0; p < d->npatch; p++) {
             const double nl = (double) (1 << d->level[p]);
@@ -607,10 +652,18 @@ int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg)
             if (d->level[p] != level) continue;
             for (k = 0; k < 4; k++) {
                 face_nb_t f;
-                face_neighbor(d, &h, p, k, &f);
+                face_neighbor(d, h, p, k, &f);
                 if (f.type == NB_SAME)
                     oracle_time_sync_samesize(mx, my, mbc, meqn, PATCH(q, p), reg + p * rsize,
                                               reg + (size_t) f.nb[0] * rsize, k, dx, dy);
+            }
+        }
+        for (p = 0; p < d->npatch; p++) {
+            if (d->level[p] != level) continue;
+            for (k = 0; k < 4; k++) {
+                face_nb_t f;
+                face_neighbor(d, h, p, k, &f);
+                if (f.type == NB_SAME) reset_face(reg + p * rsize, mx, meqn, k);
             }
         }
         if (level == maxlevel) continue;
@@ -620,17 +673,73 @@ int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg)
             if (d->level[p] != level) continue;
             for (k = 0; k < 4; k++) {
                 face_nb_t f;
-                face_neighbor(d, &h, p, k, &f);
+                face_neighbor(d, h, p, k, &f);
                 if (f.type != NB_HALF) continue;
                 for (g = 0; g < 2; g++)
                     oracle_time_sync_f2c(mx, my, mbc, meqn, PATCH(q, p), reg + p * rsize,
                                          reg + (size_t) f.nb[g] * rsize, k, f.ti[g], f.tj[g], dx, dy);
             }
         }
+        for (p = 0; p < d->npatch; p++) {
+            if (d->level[p] != level && d->level[p] != level + 1) continue;
+            for (k = 0; k < 4; k++) {
+                face_nb_t f;
+                face_neighbor(d, h, p, k, &f);
+                if ((d->level[p] == level && f.type == NB_HALF) ||
+                    (d->level[p] == level + 1 && f.type == NB_DOUBLE))
+                    reset_face(reg + p * rsize, mx, meqn, k);
+            }
+        }
     }
-    for (e = 0; e < (size_t) d->npatch * rsize; e++) reg[e] = 0.0;
+    for (p = 0; p < d->npatch; p++) {
+        if (d->level[p] < minlevel) continue;
+        for (k = 0; k < 4; k++) {
+            face_nb_t f;
+            face_neighbor(d, h, p, k, &f);
+            if (f.type == NB_BOUNDARY) reset_face(reg + p * rsize, mx, meqn, k);
+        }
+    }
+}
+
+int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg)
+{
+    ohash_t h;
+    int minlevel, maxlevel;
+    if (d->npatch <= 0) return 0;
+    if (d->mx != d->my) return -1;
+    domain_levels(d, &h, &minlevel, &maxlevel);
+    time_sync_range(d, &h, q, reg, minlevel, maxlevel);
     hash_free(&h);
     return 0;
+}
+
+/* One patch update with, if reg is given, the flux registers filled before
+ * and after (fc2d_clawpack46.cpp:317-392); with qlast, the state is saved
+ * first (fclaw2d_clawpatch_save_current_step before every update). */
+static double update_patch(const oracle_domain_t *d, const oracle_claw_t *claw, double *q,
+                           const double *aux, double *qlast, double *reg, double *fl, int p,
+                           double dt)
+{
+    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
+    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t asize = (size_t) (d->maux > 0 ? d->maux : 0) * (mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t rsize = oracle_sync_register_count(mx, meqn);
+    const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
+    const double nl = (double) (1 << d->level[p]);
+    const double dx = wx / (nl * mx), dy = wy / (nl * my);
+    const double *ap = aux ? aux + (size_t) p * asize : NULL;
+    double cfl;
+    if (qlast) memcpy(PATCH(qlast, p), PATCH(q, p), psize * sizeof(double));
+    if (!reg) {
+        oracle_step2_patch(claw, mx, my, mbc, PATCH(q, p), ap, dx, dy, dt, &cfl, NULL, NULL, NULL, NULL);
+        return cfl;
+    }
+    oracle_store_flux(claw, mx, my, mbc, PATCH(q, p), ap, dt, dx, dy, reg + p * rsize);
+    oracle_step2_patch(claw, mx, my, mbc, PATCH(q, p), ap, dx, dy, dt, &cfl,
+                       fl, fl + psize, fl + 2 * psize, fl + 3 * psize);
+    oracle_accumulate_waves(mx, my, mbc, meqn, dt, dx, dy, fl + psize, fl, fl + 3 * psize,
+                            fl + 2 * psize, reg + p * rsize);
+    return cfl;
 }
 
 /* nsteps of fclaw2d_advance_all_levels with subcycle = F
@@ -643,9 +752,7 @@ double oracle_advance_ts(const oracle_domain_t *d, const oracle_claw_t *claw, do
 {
     const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
     const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
-    const size_t asize = (size_t) (d->maux > 0 ? d->maux : 0) * (mx + 2 * mbc) * (my + 2 * mbc);
     const size_t rsize = oracle_sync_register_count(mx, meqn);
-    const double wx = (d->bx - d->ax) / d->nbx, wy = (d->by - d->ay) / d->nby;
     double *fl = NULL, *reg = NULL, maxcfl = 0.0;
     int n, p;
     if (time_sync) {
@@ -653,28 +760,137 @@ double oracle_advance_ts(const oracle_domain_t *d, const oracle_claw_t *claw, do
         reg = (double *) calloc((size_t) d->npatch * rsize, sizeof(double));
     }
     for (n = 0; n < nsteps; n++) {
-        if (!time_sync) {
-            maxcfl = fmax(maxcfl, oracle_update_all(d, claw, q, aux, dt));
-            oracle_ghost_update(d, q);
-            continue;
-        }
-        for (p = 0; p < d->npatch; p++) {
-            const double nl = (double) (1 << d->level[p]);
-            const double dx = wx / (nl * mx), dy = wy / (nl * my);
-            const double *ap = aux ? aux + (size_t) p * asize : NULL;
-            double cfl;
-            oracle_store_flux(claw, mx, my, mbc, PATCH(q, p), ap, dt, dx, dy, reg + p * rsize);
-            oracle_step2_patch(claw, mx, my, mbc, PATCH(q, p), ap, dx, dy, dt, &cfl,
-                               fl, fl + psize, fl + 2 * psize, fl + 3 * psize);
-            oracle_accumulate_waves(mx, my, mbc, meqn, dt, dx, dy, fl + psize, fl,
-                                    fl + 3 * psize, fl + 2 * psize, reg + p * rsize);
-            maxcfl = fmax(cfl, maxcfl);
-        }
+        for (p = 0; p < d->npatch; p++)
+            maxcfl = fmax(update_patch(d, claw, q, aux, NULL, reg, fl, p, dt), maxcfl);
         oracle_ghost_update(d, q);
-        oracle_time_sync(d, q, reg);
+        if (time_sync) oracle_time_sync(d, q, reg);
     }
     free(fl);
     free(reg);
+    return maxcfl;
+}
+
+/* ---------------- subcycling: fclaw2d_advance_all_levels with subcycle = T ----------------
+ * src/fclaw2d_advance.c:54-357.  Level L steps 2^(L-minlevel) times with
+ * dt/2^(L-minlevel); a coarser level steps first (advance_level recursion)
+ * and its time-interpolated border band (fclaw2d_clawpatch46_fort_timeinterp,
+ * src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:1-58, called
+ * through clawpatch_setup_timeinterp, fclaw2d_clawpatch.cpp:474-526) feeds the
+ * ghost cells of the finer levels at intermediate times. */
+typedef struct sub_ctx {
+    const oracle_domain_t *d;
+    const oracle_claw_t *claw;
+    ohash_t h;
+    double *q, *qlast, *qti, *reg, *fl;
+    const double *aux;
+    int minlevel, maxlevel;
+    int last_step[32], step_inc[32], total_steps[32];
+    double dt_step[32], current_time[32];
+} sub_ctx_t;
+
+static void timeinterp_level(sub_ctx_t *c, int level, double alpha)
+{
+    const oracle_domain_t *d = c->d;
+    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn, mint = mbc;
+    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
+    int p, i, j, m;
+    for (p = 0; p < d->npatch; p++) {
+        const double *qc = PATCH(c->q, p), *ql = PATCH(c->qlast, p);
+        double *qi = PATCH(c->qti, p);
+        if (d->level[p] != level) continue;
+        for (m = 0; m < meqn; m++) {
+            for (j = 1; j <= my - mint; j++)
+                for (i = 1; i <= mint; i++)
+                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+            for (j = 1; j <= mint; j++)
+                for (i = mint + 1; i <= mx; i++)
+                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+            for (j = mint + 1; j <= my; j++)
+                for (i = mx - mint + 1; i <= mx; i++)
+                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+            for (j = my - mint + 1; j <= my; j++)
+                for (i = 1; i <= mx - mint; i++)
+                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+        }
+    }
+}
+
+static double advance_level_rec(sub_ctx_t *c, int level, int curr_fine_step, double maxcfl)
+{
+    const oracle_domain_t *d = c->d;
+    int p;
+    for (p = 0; p < d->npatch; p++)
+        if (d->level[p] == level)
+            maxcfl = fmax(maxcfl, update_patch(d, c->claw, c->q, c->aux, c->qlast, c->reg, c->fl,
+                                               p, c->dt_step[level]));
+    c->last_step[level] += c->step_inc[level];
+    c->current_time[level] += c->dt_step[level];
+    if (level > c->minlevel) {
+        const int coarser = level - 1;
+        if (c->last_step[coarser] == curr_fine_step) {
+            maxcfl = fmax(maxcfl, advance_level_rec(c, coarser, c->last_step[coarser], maxcfl));
+            /* compute_alpha (src/fclaw2d_advance.c:152-170) */
+            timeinterp_level(c, coarser, ((double) (c->last_step[level] % c->step_inc[coarser])) /
+                                             c->step_inc[coarser]);
+        }
+    }
+    return maxcfl;
+}
+
+double oracle_advance_subcycle(const oracle_domain_t *d, const oracle_claw_t *claw, double *q,
+                               const double *aux, double dt, int time_sync)
+{
+    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn;
+    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
+    sub_ctx_t c;
+    double maxcfl = 0.0, dtl;
+    int level, nf, inc, tot;
+    if (d->npatch <= 0) return 0.0;
+    memset(&c, 0, sizeof(c));
+    c.d = d;
+    c.claw = claw;
+    c.q = q;
+    c.aux = aux;
+    domain_levels(d, &c.h, &c.minlevel, &c.maxlevel);
+    if (c.maxlevel >= 31) { hash_free(&c.h); return -1.0; }
+    c.qlast = (double *) calloc((size_t) d->npatch * psize, sizeof(double));
+    c.qti = (double *) calloc((size_t) d->npatch * psize, sizeof(double));
+    if (time_sync) {
+        c.fl = (double *) malloc(4 * psize * sizeof(double));
+        c.reg = (double *) calloc((size_t) d->npatch * oracle_sync_register_count(mx, meqn),
+                                  sizeof(double));
+    }
+    /* initialize_timestep_counters (:54-110) with subcycle */
+    dtl = dt;
+    inc = 1 << (c.maxlevel - c.minlevel);
+    tot = 1;
+    for (level = c.minlevel; level <= c.maxlevel; level++) {
+        c.last_step[level] = 0;
+        c.dt_step[level] = dtl;
+        c.total_steps[level] = tot;
+        c.step_inc[level] = inc;
+        dtl /= 2;
+        inc /= 2;
+        tot *= 2;
+    }
+    for (nf = 0; nf < c.total_steps[c.maxlevel]; nf++) {
+        int last_step;
+        maxcfl = fmax(maxcfl, advance_level_rec(&c, c.maxlevel, nf, maxcfl));
+        last_step = c.last_step[c.maxlevel];
+        if (last_step < c.total_steps[c.maxlevel]) {
+            int ti = c.maxlevel - 1;
+            while (last_step % c.step_inc[ti] == 0) ti--;
+            ghost_update_range(d, &c.h, q, c.qti, ti + 1, c.maxlevel, 1);
+            if (time_sync) time_sync_range(d, &c.h, q, c.reg, ti + 1, c.maxlevel);
+        }
+    }
+    ghost_update_range(d, &c.h, q, NULL, c.minlevel, c.maxlevel, 0);
+    if (time_sync) time_sync_range(d, &c.h, q, c.reg, c.minlevel, c.maxlevel);
+    free(c.qlast);
+    free(c.qti);
+    free(c.fl);
+    free(c.reg);
+    hash_free(&c.h);
     return maxcfl;
 }
 

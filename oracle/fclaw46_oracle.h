@@ -86,6 +86,13 @@ double oracle_advance(const oracle_domain_t *d, const oracle_claw_t *claw,
 double oracle_advance_ts(const oracle_domain_t *d, const oracle_claw_t *claw,
                          double *q, const double *aux, double dt, int nsteps, int time_sync);
 
+/* One fclaw2d_advance_all_levels(t, dt) with subcycle = T: the coarsest level
+ * steps once with dt, level L 2^(L-minlevel) times with dt/2^(L-minlevel),
+ * with time-interpolated ghost exchanges and (if time_sync) the
+ * synchronisations in between.  Returns the maximum CFL number. */
+double oracle_advance_subcycle(const oracle_domain_t *d, const oracle_claw_t *claw, double *q,
+                               const double *aux, double dt, int time_sync);
+
 /* ---------------- time synchronisation (conservation fix-up) ----------------
  * Flux registers of one patch (mx == my): reg[((face*4 + kind)*meqn + m)*mx + pos]
  * with face 0..3 (left, right, bottom, top), pos = j-1 (faces 0/1) or i-1

@@ -81,6 +81,20 @@ def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True):
     return out, cfl
 
 
+def oracle_advance_subcycle(prob, forest, q, aux, mx, dt, ncoarse=1, time_sync=True):
+    """ncoarse calls of fclaw2d_advance_all_levels(dt) with subcycling on the oracle."""
+    d = oracle_domain(prob, forest, mx)
+    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma)
+    out = np.ascontiguousarray(q, dtype=np.float64).copy()
+    auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    cfl = 0.0
+    for _ in range(ncoarse):
+        cfl = max(cfl, oracle().oracle_advance_subcycle(
+            C.byref(d), C.byref(claw), out.reshape(-1),
+            None if auxp is None else auxp.ctypes.data, dt, int(time_sync)))
+    return out, cfl
+
+
 def oracle_mass(prob, forest, q, mx):
     d = oracle_domain(prob, forest, mx)
     s = np.zeros(prob.meqn)

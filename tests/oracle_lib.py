@@ -59,6 +59,9 @@ def oracle():
         lib.oracle_advance_ts.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw), _dp,
                                           C.c_void_p, C.c_double, C.c_int, C.c_int]
         lib.oracle_advance_ts.restype = C.c_double
+        lib.oracle_advance_subcycle.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw),
+                                                _dp, C.c_void_p, C.c_double, C.c_int]
+        lib.oracle_advance_subcycle.restype = C.c_double
         lib.oracle_cons_flux.argtypes = [C.POINTER(OracleClaw), C.c_int, _dp, _dp, _dp]
         lib.oracle_mass.argtypes = [C.POINTER(OracleDomain), _dp, _dp]
         lib.oracle_bc2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip]
