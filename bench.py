@@ -8,8 +8,9 @@ initial data, patches mx = my = 32, mbc = 2, Roe solver with entropy fix, MC
 limiter, order (2, 2), fixed dt at Courant number 0.9 on the finest level.
 
 A step is one pass of the hot path over the whole forest: the batched update
-of every leaf patch at every level followed by the device ghost fill
-(fclaw2d_advance_all_levels with subcycle = F).  The headline value is the
+of every leaf patch at every level, the device ghost fill and the
+coarse/fine conservation fix-up (fclaw2d_advance_all_levels with
+subcycle = F, time-sync = T).  The headline value is the
 FP32-bulk form's cell-updates/s (sum over steps of patches x mx x my / wall
 time); the FP64 form is timed the same way and reported beside it, together
 with the FP32 form's L1/max error against the FP64 form and its mass change.
@@ -38,9 +39,11 @@ def algorithmic_bytes_per_cell(meqn, maux, mx, mbc=2, s=8):
     return s * (meqn * (h + 1.0) + maux * h)
 
 
-def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_graph):
+def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_graph,
+             time_sync):
     sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
-                   grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch)
+                   grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
+                   time_sync=time_sync)
     sv.set_forest(forest, prob.extent)
     sv.set_q(q0)
     if aux is not None:
@@ -72,6 +75,95 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
                 graph=graph, total_steps=total_steps)
 
 
+PROFILE_STEPS = 6
+COPY_BYTES = 256 << 20
+
+
+def profile_run(fa, prob, forest, q0, aux, dt, tiles, time_sync):
+    """The program profiled by rocprofv3 (kernel trace, then one counter per
+    pass): PROFILE_STEPS steps of each form at the benchmark tiles, then three
+    device-to-device copies of COPY_BYTES (the HIP runtime's copy kernel) that
+    calibrate the byte counters.  Uses no torch."""
+    import ctypes as C
+    for precision, tile in ((fa.FP64, tiles[0]), (fa.FP32_BULK, tiles[1])):
+        sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
+                       grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
+                       time_sync=time_sync)
+        sv.set_forest(forest, prob.extent)
+        sv.set_q(q0)
+        sv.ghost_fill()
+        sv.advance_async(dt, PROFILE_STEPS)
+        sv.sync()
+        print("profiled form %d tile %d: %d steps" % (precision, sv.tile, PROFILE_STEPS), flush=True)
+        del sv
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) or hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 0, C.c_size_t(COPY_BYTES)):
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3):  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("calibration copies: 3 x %d bytes" % COPY_BYTES, flush=True)
+
+
+def pmc_summary(fetch_dir, write_dir, cells):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of every kernel in two
+    rocprofv3 --pmc runs of --profile-run, calibrated against the copy kernel:
+    factor = bytes the copy moves / bytes the counter reports for it."""
+    import csv
+    import glob as G
+
+    def load(d, counter):
+        out = {}
+        for f in G.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] != counter:
+                    continue
+                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return out
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise RuntimeError("no copy kernel in the counter runs")
+    nf = sum(fetch[copy[0]]) * 1024.0   # bytes reported over all copy dispatches
+    nw = sum(write[copy[0]]) * 1024.0
+    f_fetch = 3 * COPY_BYTES / nf
+    f_write = 3 * COPY_BYTES / nw
+    lines = ["calibration: 3 copies of %d bytes; FETCH_SIZE factor %.3f, WRITE_SIZE factor %.3f"
+             % (COPY_BYTES, f_fetch, f_write)]
+    for k in sorted(fetch):
+        if "step_tiles" not in k and "ghost_phase" not in k and "time_sync" not in k:
+            continue
+        fk = sum(fetch[k]) / len(fetch[k])
+        wk = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
+        traffic = fk * 1024 * f_fetch + wk * 1024 * f_write
+        name = k.split("(")[0][:120]
+        lines.append("kernel=%s dispatches=%d fetch_kib=%.1f write_kib=%.1f traffic_bytes=%.0f "
+                     "bytes_per_cell=%.3f" % (name, len(fetch[k]), fk, wk, traffic,
+                                              traffic / cells))
+    return "\n".join(lines)
+
+
+def committed_traffic(workload, kernel_tag):
+    """HBM bytes per launch of the step kernel from the committed counter summary
+    of this workload (None if there is none for this kernel)."""
+    path = os.path.join(ROOT, "profiles", "step_tiles_%s_pmc.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("kernel=") and kernel_tag in line:
+            for tok in line.split():
+                if tok.startswith("traffic_bytes="):
+                    return float(tok.split("=")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +177,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-time-sync", action="store_true",
+                    help="skip the coarse/fine conservation fix-up (on by default)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="run the workload's step kernels a few times for rocprofv3 (no JSON)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs of --profile-run")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,7 +195,6 @@ def main():
     import forestclaw_amd as fa
     from forestclaw_amd import problems as P
 
-    device = fa.initialize_device(0)
     prob = P.dambreak_problem(args.minlevel, args.maxlevel, mx=args.mx)
     forest = P.build_forest(prob)
     q0, aux = P.initial_data(prob, forest)
@@ -105,20 +202,32 @@ def main():
     npatch = forest.npatch
     cells = npatch * prob.mx * prob.mx
     lv = forest.leaves()[0]
+    workload = "swe_dambreak_amr_L%d-%d_mx%d" % (args.minlevel, args.maxlevel, prob.mx)
+    if args.pmc_summary:
+        print("workload=%s cells=%d" % (workload, cells))
+        print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], cells))
+        return
+
+    device = fa.initialize_device(0)
+    if args.profile_run:
+        profile_run(fa, prob, forest, q0, aux, dt, (args.tile64, args.tile32), not args.no_time_sync)
+        return
 
     r64 = run_form(fa, prob, forest, q0, aux, fa.FP64, dt, args.steps, args.warmup,
-                   args.tile64, not args.no_graph)
+                   args.tile64, not args.no_graph, not args.no_time_sync)
     r32 = run_form(fa, prob, forest, q0, aux, fa.FP32_BULK, dt, args.steps, args.warmup,
-                   args.tile32, not args.no_graph)
+                   args.tile32, not args.no_graph, not args.no_time_sync)
 
     B = algorithmic_bytes_per_cell(prob.meqn, prob.maux, prob.mx)
     value32 = cells * args.steps / r32["wall"]
     value64 = cells * args.steps / r64["wall"]
 
-    def roofline(r):
+    def roofline(r, rtype):
         achieved = B * cells / (r["kernel_ms"] * 1e-3) / 1e9
+        traffic = committed_traffic(workload, "%s, %d," % (rtype, r["tile"]))
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round(traffic),
                 "kernel": "step_tiles", "kernel_avg_ms": round(r["kernel_ms"], 5),
                 "bytes_per_cell_update": round(B, 4), "cell_updates_per_launch": cells}
 
@@ -154,18 +263,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "swe_dambreak_amr_L%d-%d_mx%d" % (args.minlevel, args.maxlevel, prob.mx),
+            "workload": workload,
             "form": "fp32-bulk (f32 Riemann/limiter/transverse; f64 q, fluxes, CFL, mass)",
             "patches": npatch, "patches_per_level": np.bincount(lv).tolist(),
             "cells": cells, "mx": prob.mx, "mbc": 2, "meqn": prob.meqn,
             "dt": dt, "tile_fp32": r32["tile"], "tile_fp64": r64["tile"],
-            "timed_with_hip_graph": bool(r32["graph"]), "device": device,
+            "timed_with_hip_graph": bool(r32["graph"]), "time_sync": not args.no_time_sync,
+            "device": device,
         },
-        "roofline": roofline(r32),
+        "roofline": roofline(r32, "RPShallow<float>, float"),
         "fp64": {
             "value": round(value64, 1), "unit": "cell-updates/s",
             "ms_per_step": round(r64["wall"] * 1e3 / args.steps, 5),
-            "roofline": roofline(r64),
+            "roofline": roofline(r64, "RPShallow<double>, double"),
         },
         "fp32_over_fp64": round(value32 / value64, 3),
         "accuracy": {

@@ -435,7 +435,8 @@ static int corner_interior(const int phys[4], int ic)
     return !phys[fa] && !phys[fb];
 }
 
-static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, const int mthbc[4])
+static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, const int mthbc[4],
+                        int tflag)
 {
     int p, k;
     for (p = 0; p < f->n; p++) {
@@ -447,117 +448,121 @@ static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, co
             any |= phys[k];
             bc |= (m & 0xff) << (8 * k);
         }
-        if (any) emit(b, OP_PHYSBC, p, 0, 0, 0, 0, bc);
+        if (any) emit(b, OP_PHYSBC, p | tflag, 0, 0, 0, 0, bc);
     }
+}
+
+/* Face and corner exchanges of the patches at `level` (this side) with their
+ * neighbours, as cb_face_fill / cb_corner_fill do for one exchange type.
+ * `tflag` marks this side's slots as time-interpolated copies; for
+ * interpolation the op writes the fine neighbour and reads this patch. */
+static void emit_exchanges(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, int mx, int my,
+                           int kind, int tflag)
+{
+    int p, k;
+    for (p = 0; p < f->n; p++) {
+        int phys[4];
+        if (f->level[p] != level) continue;
+        physical_faces(f, p, phys);
+        for (k = 0; k < 4; k++) {
+            nbinfo_t r;
+            if (phys[k]) continue;
+            face_rel(f, p, k, mx, my, &r);
+            if (kind == OP_COPY_FACE && r.rel == REL_SAME)
+                emit(b, OP_COPY_FACE, p | tflag, r.nb[0] | tflag, k, r.ti[0], r.tj[0], 0);
+            else if (kind == OP_AVG_FACE && r.rel == REL_HALF) {
+                emit(b, OP_AVG_FACE, p | tflag, r.nb[0], k, r.ti[0], r.tj[0], 0);
+                emit(b, OP_AVG_FACE, p | tflag, r.nb[1], k, r.ti[1], r.tj[1], 0);
+            } else if (kind == OP_INTERP_FACE && r.rel == REL_HALF) {
+                emit(b, OP_INTERP_FACE, r.nb[0], p | tflag, k, r.ti[0], r.tj[0], 0);
+                emit(b, OP_INTERP_FACE, r.nb[1], p | tflag, k, r.ti[1], r.tj[1], 0);
+            }
+        }
+        for (k = 0; k < 4; k++) {
+            nbinfo_t r;
+            if (!corner_interior(phys, k)) continue;
+            corner_rel(f, p, k, mx, my, &r);
+            if (kind == OP_COPY_FACE && r.rel == REL_SAME)
+                emit(b, OP_COPY_CORNER, p | tflag, r.nb[0] | tflag, k, r.ti[0], r.tj[0], 0);
+            else if (kind == OP_AVG_FACE && r.rel == REL_HALF)
+                emit(b, OP_AVG_CORNER, p | tflag, r.nb[0], k, r.ti[0], r.tj[0], 0);
+            else if (kind == OP_INTERP_FACE && r.rel == REL_HALF)
+                emit(b, OP_INTERP_CORNER, r.nb[0], p | tflag, k, r.ti[0], r.tj[0], 0);
+        }
+    }
+}
+
+int fclaw2d_hip_forest_ghost_program_range(const fclaw2d_hip_forest_t *f, int mx, int my,
+                                           int mbc, const int mthbc[4], int minlevel,
+                                           int maxlevel, int time_interp, int *ops, int cap,
+                                           int phase_start[7])
+{
+    opbuf_t b;
+    const int ti = minlevel - 1, TI = FCLAW2D_HIP_TIME_INTERP_SLOT;
+    int L;
+    (void) mbc;
+    if (!f || f->n == 0 || !phase_start || !mthbc || minlevel > maxlevel) return -1;
+    if (time_interp && ti < fclaw2d_hip_forest_min_level(f)) return -1;
+    b.ops = ops; b.cap = ops ? cap : 0; b.n = 0;
+
+    /* phase 0: average fine -> coarse ghost cells (coarse levels, finest first) */
+    phase_start[0] = b.n;
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_AVG_FACE, 0);
+    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_AVG_FACE, TI);
+    /* phase 1: copy between same-size neighbours */
+    phase_start[1] = b.n;
+    for (L = maxlevel; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_COPY_FACE, 0);
+    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_COPY_FACE, TI);
+    /* phase 2: physical BC on coarse levels */
+    phase_start[2] = b.n;
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_physbc(f, &b, L, mthbc, 0);
+    if (time_interp) emit_physbc(f, &b, ti, mthbc, TI);
+    /* phase 3: interpolate coarse -> fine ghost cells */
+    phase_start[3] = b.n;
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_INTERP_FACE, 0);
+    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_INTERP_FACE, TI);
+    /* phase 4: physical BC on fine levels; phase 5: on all levels */
+    phase_start[4] = b.n;
+    for (L = maxlevel; L >= minlevel + 1; L--) emit_physbc(f, &b, L, mthbc, 0);
+    phase_start[5] = b.n;
+    for (L = maxlevel; L >= minlevel; L--) emit_physbc(f, &b, L, mthbc, 0);
+    phase_start[6] = b.n;
+    return b.n;
 }
 
 int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int my, int mbc,
                                      const int mthbc[4], int *ops, int cap, int phase_start[7])
 {
-    opbuf_t b;
-    int minl, maxl, L, p, k;
-    (void) mbc;
-    if (!f || f->n == 0 || !phase_start || !mthbc) return -1;
-    b.ops = ops; b.cap = ops ? cap : 0; b.n = 0;
-    minl = fclaw2d_hip_forest_min_level(f);
-    maxl = fclaw2d_hip_forest_max_level(f);
-
-    /* phase 0: average fine -> coarse ghost cells (coarse levels, finest first) */
-    phase_start[0] = b.n;
-    for (L = maxl - 1; L >= minl; L--)
-        for (p = 0; p < f->n; p++) {
-            int phys[4];
-            if (f->level[p] != L) continue;
-            physical_faces(f, p, phys);
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (phys[k]) continue;
-                face_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_HALF) {
-                    emit(&b, OP_AVG_FACE, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
-                    emit(&b, OP_AVG_FACE, p, r.nb[1], k, r.ti[1], r.tj[1], 0);
-                }
-            }
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (!corner_interior(phys, k)) continue;
-                corner_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_HALF) emit(&b, OP_AVG_CORNER, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
-            }
-        }
-    /* phase 1: copy between same-size neighbours (all levels) */
-    phase_start[1] = b.n;
-    for (L = maxl; L >= minl; L--)
-        for (p = 0; p < f->n; p++) {
-            int phys[4];
-            if (f->level[p] != L) continue;
-            physical_faces(f, p, phys);
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (phys[k]) continue;
-                face_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_SAME) emit(&b, OP_COPY_FACE, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
-            }
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (!corner_interior(phys, k)) continue;
-                corner_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_SAME) emit(&b, OP_COPY_CORNER, p, r.nb[0], k, r.ti[0], r.tj[0], 0);
-            }
-        }
-    /* phase 2: physical BC on coarse levels */
-    phase_start[2] = b.n;
-    for (L = maxl - 1; L >= minl; L--) emit_physbc(f, &b, L, mthbc);
-    /* phase 3: interpolate coarse -> fine ghost cells */
-    phase_start[3] = b.n;
-    for (L = maxl - 1; L >= minl; L--)
-        for (p = 0; p < f->n; p++) {
-            int phys[4];
-            if (f->level[p] != L) continue;
-            physical_faces(f, p, phys);
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (phys[k]) continue;
-                face_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_HALF) {
-                    emit(&b, OP_INTERP_FACE, r.nb[0], p, k, r.ti[0], r.tj[0], 0);
-                    emit(&b, OP_INTERP_FACE, r.nb[1], p, k, r.ti[1], r.tj[1], 0);
-                }
-            }
-            for (k = 0; k < 4; k++) {
-                nbinfo_t r;
-                if (!corner_interior(phys, k)) continue;
-                corner_rel(f, p, k, mx, my, &r);
-                if (r.rel == REL_HALF) emit(&b, OP_INTERP_CORNER, r.nb[0], p, k, r.ti[0], r.tj[0], 0);
-            }
-        }
-    /* phase 4: physical BC on fine levels; phase 5: on all levels */
-    phase_start[4] = b.n;
-    for (L = maxl; L >= minl + 1; L--) emit_physbc(f, &b, L, mthbc);
-    phase_start[5] = b.n;
-    for (L = maxl; L >= minl; L--) emit_physbc(f, &b, L, mthbc);
-    phase_start[6] = b.n;
-    return b.n;
+    if (!f || f->n == 0) return -1;
+    return fclaw2d_hip_forest_ghost_program_range(f, mx, my, mbc, mthbc,
+                                                  fclaw2d_hip_forest_min_level(f),
+                                                  fclaw2d_hip_forest_max_level(f), 0, ops, cap,
+                                                  phase_start);
 }
 
 /* ---------------------------------------------------------------- sync program */
 
-/* Time-synchronisation ops (see hipclaw_set_sync_program), grouped by the
- * patch they correct, in the order correct_coarse_cells applies them
- * (src/fclaw2d_time_sync.c:118-158 via cb_face_fill,
- * src/fclaw2d_face_neighbors.c:321-383): same-size faces 0..3, then faces
- * with half-size neighbours, lower child first. */
-int fclaw2d_hip_forest_sync_program(const fclaw2d_hip_forest_t *f, int mx, int my, int *ops,
-                                    int cap)
+/* Time-synchronisation ops for fclaw2d_time_sync(minlevel, maxlevel) (see
+ * hipclaw_set_sync_program_k), grouped by the patch they correct, in the
+ * order correct_coarse_cells applies them (src/fclaw2d_time_sync.c:118-158
+ * via cb_face_fill, src/fclaw2d_face_neighbors.c:321-383): same-size faces
+ * 0..3, then faces with half-size neighbours, lower child first; patches
+ * below minlevel are not corrected.  The resets (slot, face) are those of
+ * fclaw2d_clawpatch_time_sync_reset (SAMESIZE at every level from minlevel,
+ * F2C for coarse levels from minlevel and their fine neighbours, PHYS). */
+int fclaw2d_hip_forest_sync_program_range(const fclaw2d_hip_forest_t *f, int mx, int my,
+                                          int minlevel, int *ops, int cap, int *resets,
+                                          int rcap, int *nresets)
 {
     opbuf_t b;
-    int p, k, g;
-    if (!f || mx != my || cap < 0) return -1;
+    int p, k, g, nr = 0;
+    if (!f || mx != my || cap < 0 || rcap < 0) return -1;
     b.ops = ops;
     b.cap = ops ? cap : 0;
     b.n = 0;
     for (p = 0; p < f->n; p++) {
         nbinfo_t r;
+        if (f->level[p] < minlevel) continue;
         for (k = 0; k < 4; k++) {
             face_rel(f, p, k, mx, my, &r);
             if (r.rel == REL_NONE) return -1;
@@ -569,5 +574,29 @@ int fclaw2d_hip_forest_sync_program(const fclaw2d_hip_forest_t *f, int mx, int m
             for (g = 0; g < 2; g++) emit(&b, 1, p, r.nb[g], k, r.ti[g], r.tj[g], 0);
         }
     }
+    for (p = 0; p < f->n; p++) {
+        nbinfo_t r;
+        for (k = 0; k < 4; k++) {
+            int reset;
+            face_rel(f, p, k, mx, my, &r);
+            if (f->level[p] >= minlevel)
+                reset = r.rel == REL_SAME || r.rel == REL_HALF || r.rel == REL_BOUNDARY ||
+                        (r.rel == REL_DOUBLE && f->level[p] - 1 >= minlevel);
+            else
+                reset = 0;
+            if (!reset) continue;
+            if (resets && nr < rcap) { resets[2 * nr] = p; resets[2 * nr + 1] = k; }
+            nr++;
+        }
+    }
+    if (nresets) *nresets = nr;
     return b.n;
+}
+
+int fclaw2d_hip_forest_sync_program(const fclaw2d_hip_forest_t *f, int mx, int my, int *ops,
+                                    int cap)
+{
+    if (!f || f->n == 0) return -1;
+    return fclaw2d_hip_forest_sync_program_range(f, mx, my, fclaw2d_hip_forest_min_level(f),
+                                                 ops, cap, NULL, 0, NULL);
 }

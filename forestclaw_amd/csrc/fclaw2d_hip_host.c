@@ -301,47 +301,136 @@ double fclaw2d_update_single_step(fclaw2d_global_t *glob, int level, double t, d
     return maxcfl;
 }
 
+/* Program numbers: 0 = every level; 1 + (ti - minlevel of the domain) = the
+ * levels above a time-interpolated level ti. */
+static int program_for(fclaw2d_global_t *glob, int minlevel, int maxlevel, int time_interp,
+                       const char *what)
+{
+    const fclaw2d_domain_t *d = glob->domain;
+    if (!time_interp && minlevel <= d->global_minlevel && maxlevel >= d->global_maxlevel) return 0;
+    if (time_interp && maxlevel >= d->global_maxlevel && minlevel - 1 >= d->global_minlevel &&
+        glob->subcycle)
+        return 1 + (minlevel - 1 - d->global_minlevel);
+    fprintf(stderr, "%s: level range %d..%d (time_interp %d) is not supported\n", what, minlevel,
+            maxlevel, time_interp);
+    exit(1);
+}
+
 void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel, double sync_time,
                           int time_interp, int running)
 {
+    const int k = program_for(glob, minlevel, maxlevel, time_interp, "fclaw2d_ghost_update");
     (void) sync_time; (void) running;
-    if (time_interp) {
-        fprintf(stderr, "fclaw2d_ghost_update: time interpolation (subcycling) is not supported\n");
-        exit(1);
-    }
-    if (minlevel > glob->domain->global_minlevel || maxlevel < glob->domain->global_maxlevel) {
-        fprintf(stderr, "fclaw2d_ghost_update: partial level ranges are not supported\n");
-        exit(1);
-    }
-    device_check(hipclaw_ghost_fill(glob->state), "hipclaw_ghost_fill");
+    device_check(hipclaw_ghost_fill_k(glob->state, k), "hipclaw_ghost_fill");
     glob->count_ghost_update++;
+}
+
+void fclaw2d_timeinterp(fclaw2d_global_t *glob, int level, double alpha)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    int *slots = (int *) malloc(sizeof(int) * (size_t) (d->local_num_patches + 1));
+    int b, p, n = 0;
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++)
+            if (d->blocks[b].patches[p].level == level) slots[n++] = d->blocks[b].patches[p].slot;
+    device_check(hipclaw_timeinterp(glob->state, slots, n, alpha), "hipclaw_timeinterp");
+    free(slots);
 }
 
 void fclaw2d_time_sync(fclaw2d_global_t *glob, int minlevel, int maxlevel)
 {
+    int k;
     if (!glob->time_sync) return;
-    if (minlevel > glob->domain->global_minlevel || maxlevel < glob->domain->global_maxlevel) {
-        fprintf(stderr, "fclaw2d_time_sync: partial level ranges are not supported\n");
-        exit(1);
-    }
-    device_check(hipclaw_time_sync(glob->state), "hipclaw_time_sync");
+    k = program_for(glob, minlevel, maxlevel, minlevel > glob->domain->global_minlevel,
+                    "fclaw2d_time_sync");
+    device_check(hipclaw_time_sync_k(glob->state, k), "hipclaw_time_sync");
 }
 
+/* ---- fclaw2d_advance_all_levels with subcycle = T (src/fclaw2d_advance.c:54-246) ---- */
+typedef struct {
+    int last_step, step_inc, total_steps;
+    double dt_step, current_time;
+} level_counter_t;
+
+static double advance_level(fclaw2d_global_t *glob, int level, int curr_fine_step, double maxcfl,
+                            level_counter_t *tc)
+{
+    const int minlevel = glob->domain->global_minlevel;
+    maxcfl = fmax(maxcfl, fclaw2d_update_single_step(glob, level, tc[level].current_time,
+                                                     tc[level].dt_step));
+    tc[level].last_step += tc[level].step_inc;
+    tc[level].current_time += tc[level].dt_step;
+    if (level > minlevel && tc[level - 1].last_step == curr_fine_step) {
+        maxcfl = fmax(maxcfl, advance_level(glob, level - 1, tc[level - 1].last_step, maxcfl, tc));
+        if (glob->subcycle) {
+            /* compute_alpha (:152-170) */
+            const double alpha = ((double) (tc[level].last_step % tc[level - 1].step_inc)) /
+                                 tc[level - 1].step_inc;
+            fclaw2d_timeinterp(glob, level - 1, alpha);
+        }
+    }
+    return maxcfl;
+}
+
+static double advance_all_levels_stepwise(fclaw2d_global_t *glob, double t_curr, double dt)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    const int minl = d->global_minlevel, maxl = d->global_maxlevel;
+    level_counter_t tc[32];
+    double maxcfl = 0.0;
+    int level, nf;
+    if (maxl >= 32) { fprintf(stderr, "fclaw2d_advance_all_levels: too many levels\n"); exit(1); }
+    /* initialize_timestep_counters (:54-110) */
+    for (level = minl; level <= maxl; level++) {
+        tc[level].last_step = 0;
+        tc[level].current_time = t_curr;
+        if (glob->subcycle) {
+            tc[level].dt_step = dt / (double) (1 << (level - minl));
+            tc[level].total_steps = 1 << (level - minl);
+            tc[level].step_inc = 1 << (maxl - level);
+        } else {
+            const int rf = glob->advance_one_step ? 1 : 1 << (maxl - minl);
+            tc[level].step_inc = 1;
+            tc[level].dt_step = dt / rf;
+            tc[level].total_steps = rf;
+        }
+    }
+    for (nf = 0; nf < tc[maxl].total_steps; nf++) {
+        int last_step;
+        maxcfl = advance_level(glob, maxl, nf, maxcfl, tc);
+        last_step = tc[maxl].last_step;
+        if (last_step < tc[maxl].total_steps) {
+            if (glob->subcycle) {
+                int ti = maxl - 1;
+                while (last_step % tc[ti].step_inc == 0) ti--;
+                fclaw2d_ghost_update(glob, ti + 1, maxl, tc[maxl].current_time, 1, 0);
+                if (glob->time_sync) fclaw2d_time_sync(glob, ti + 1, maxl);
+            } else {
+                fclaw2d_ghost_update(glob, minl, maxl, tc[maxl].current_time, 0, 0);
+                if (glob->time_sync) fclaw2d_time_sync(glob, minl, maxl);
+            }
+        }
+    }
+    fclaw2d_ghost_update(glob, minl, maxl, tc[maxl].current_time, 0, 0);
+    if (glob->time_sync) fclaw2d_time_sync(glob, minl, maxl);
+    return maxcfl;
+}
+
+/* fclaw2d_advance_all_levels (src/fclaw2d_advance.c:247-357).  Without
+ * subcycling every level takes the same steps, so all of them go in one
+ * batched launch per step (followed by the device ghost fill and the
+ * synchronisation), as one captured sequence. */
 double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt)
 {
     fclaw2d_domain_t *d = glob->domain;
     double maxcfl = 0.0;
-    int level;
-    if (glob->patch_vt.single_step_update == hipclaw_update) {
-        /* every level in one batched launch, then the device ghost fill */
-        device_check(hipclaw_advance(glob->state, dt, 1, &maxcfl), "hipclaw_advance");
-        glob->count_single_step += d->local_num_patches;
-        glob->count_ghost_update++;
+    if (!glob->subcycle && glob->patch_vt.single_step_update == hipclaw_update) {
+        const int rf = glob->advance_one_step ? 1 : 1 << (d->global_maxlevel - d->global_minlevel);
+        device_check(hipclaw_advance(glob->state, dt / rf, rf, &maxcfl), "hipclaw_advance");
+        glob->count_single_step += rf * d->local_num_patches;
+        glob->count_ghost_update += rf;
     } else {
-        for (level = d->global_maxlevel; level >= d->global_minlevel; level--)
-            maxcfl = fmax(maxcfl, fclaw2d_update_single_step(glob, level, t_curr, dt));
-        fclaw2d_ghost_update(glob, d->global_minlevel, d->global_maxlevel, t_curr + dt, 0, 0);
-        if (glob->time_sync) fclaw2d_time_sync(glob, d->global_minlevel, d->global_maxlevel);
+        maxcfl = advance_all_levels_stepwise(glob, t_curr, dt);
     }
     glob->curr_time = t_curr + dt;
     glob->curr_dt = dt;
@@ -422,6 +511,7 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
     }
     cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
     cfg.time_sync = glob->time_sync;
+    cfg.subcycle = glob->subcycle;
     device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");
     dx = (double *) malloc(sizeof(double) * d->local_num_patches);
     dy = (double *) malloc(sizeof(double) * d->local_num_patches);
@@ -439,16 +529,44 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
     fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, nops, phase);
     device_check(hipclaw_set_ghost_program(glob->state, ops, nops, phase), "hipclaw_set_ghost_program");
     free(ops);
-    if (glob->time_sync) {
-        nops = fclaw2d_hip_forest_sync_program(d->forest, o->mx, o->my, NULL, 0);
-        if (nops < 0) {
-            fprintf(stderr, "fc2d_hipclaw_allocate_buffers: time sync needs mx = my\n");
+    if (glob->subcycle) {
+        /* one program per time-interpolated level ti: levels ti+1..maxlevel */
+        int ti;
+        if (d->global_maxlevel - d->global_minlevel >= HIPCLAW_MAX_PROGRAMS) {
+            fprintf(stderr, "fc2d_hipclaw_allocate_buffers: too many levels\n");
             exit(1);
         }
-        ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
-        fclaw2d_hip_forest_sync_program(d->forest, o->mx, o->my, ops, nops);
-        device_check(hipclaw_set_sync_program(glob->state, ops, nops), "hipclaw_set_sync_program");
-        free(ops);
+        for (ti = d->global_minlevel; ti < d->global_maxlevel; ti++) {
+            nops = fclaw2d_hip_forest_ghost_program_range(d->forest, o->mx, o->my, o->mbc, opt->mthbc,
+                                                          ti + 1, d->global_maxlevel, 1, NULL, 0, phase);
+            ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
+            fclaw2d_hip_forest_ghost_program_range(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ti + 1,
+                                                   d->global_maxlevel, 1, ops, nops, phase);
+            device_check(hipclaw_set_ghost_program_k(glob->state, 1 + ti - d->global_minlevel, ops,
+                                                     nops, phase), "hipclaw_set_ghost_program");
+            free(ops);
+        }
+    }
+    if (glob->time_sync) {
+        /* program 0: every level; with subcycling also one per level range */
+        int minl;
+        for (minl = d->global_minlevel; minl <= (glob->subcycle ? d->global_maxlevel : d->global_minlevel); minl++) {
+            int nres = 0, *res;
+            nops = fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, NULL, 0, NULL, 0, &nres);
+            if (nops < 0) {
+                fprintf(stderr, "fc2d_hipclaw_allocate_buffers: time sync needs mx = my\n");
+                exit(1);
+            }
+            ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
+            res = (int *) malloc(sizeof(int) * 2 * (size_t) (nres > 0 ? nres : 1));
+            fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, ops, nops, res, nres, &nres);
+            /* without subcycling every register is rewritten by every update */
+            device_check(hipclaw_set_sync_program_k(glob->state, minl == d->global_minlevel ? 0 : minl - d->global_minlevel,
+                                                    ops, nops, res, glob->subcycle ? nres : 0),
+                         "hipclaw_set_sync_program");
+            free(ops);
+            free(res);
+        }
     }
     glob->qhost = (double *) calloc(patch_size(glob, o->meqn) * d->local_num_patches, sizeof(double));
     if (o->maux > 0)

@@ -38,6 +38,22 @@ int hipfail(hipError_t e, const char* where) {
 
 }  // namespace
 
+// One ghost-fill program (hipclaw_set_ghost_program_k).
+struct GhostProgram {
+    GhostOp* ops = nullptr;
+    int nops = 0, cap = 0;
+    int phase_start[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool phase_physbc[6] = {false, false, false, false, false, false};
+};
+
+// One time-synchronisation program (hipclaw_set_sync_program_k).
+struct SyncProgram {
+    int* ops = nullptr;       // [nops][8], grouped by destination
+    int* start = nullptr;     // [ngroups + 1]
+    int* resets = nullptr;    // [nresets][2] (slot, face)
+    int ngroups = 0, nresets = 0, ops_cap = 0, groups_cap = 0, resets_cap = 0;
+};
+
 struct hipclaw_state {
     hipclaw_config_t cfg;
     int tile = 16, ntx = 1, nty = 1;
@@ -51,17 +67,17 @@ struct hipclaw_state {
     double* dys = nullptr;
     int* all_slots = nullptr;
     int* batch_slots = nullptr;
-    GhostOp* ops = nullptr;
-    int nops = 0, ops_cap = 0;
-    int phase_start[7] = {0, 0, 0, 0, 0, 0, 0};
+    GhostProgram gp[HIPCLAW_MAX_PROGRAMS];
     unsigned long long* cfl = nullptr;
     double* mass_buf = nullptr;
     // time synchronisation: flux registers and the per-destination op groups
     double* reg = nullptr;
     long long rstride = 0;
-    int* sync_ops = nullptr;
-    int* sync_start = nullptr;
-    int nsync_groups = 0, sync_ops_cap = 0, sync_groups_cap = 0;
+    SyncProgram sp[HIPCLAW_MAX_PROGRAMS];
+    bool reg_accumulate = false;  // subcycling: registers add up over substeps
+    // subcycling: the state before each patch's last update, and time-interpolated copies
+    double* qlast = nullptr;
+    double* qti = nullptr;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -148,6 +164,14 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     ALLOC(s->batch_slots, sizeof(int) * max_patches);
     ALLOC(s->cfl, sizeof(unsigned long long));
     ALLOC(s->mass_buf, sizeof(double) * 2 * cfg->meqn * (size_t) max_patches);
+    if (cfg->subcycle) {
+        ALLOC(s->qlast, sizeof(double) * s->pstride * max_patches);
+        ALLOC(s->qti, sizeof(double) * s->pstride * max_patches);
+        if (hipMemset(s->qlast, 0, sizeof(double) * s->pstride * max_patches) != hipSuccess ||
+            hipMemset(s->qti, 0, sizeof(double) * s->pstride * max_patches) != hipSuccess)
+            return cleanup(fail(HIPCLAW_EDEVICE, "initial device copies failed%s"));
+        s->reg_accumulate = true;
+    }
     if (cfg->time_sync) {
         s->rstride = 16LL * cfg->mx * cfg->meqn;
         ALLOC(s->reg, sizeof(double) * s->rstride * max_patches);
@@ -182,12 +206,17 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->dys);
     hipFree(s->all_slots);
     hipFree(s->batch_slots);
-    hipFree(s->ops);
+    for (GhostProgram& g : s->gp) hipFree(g.ops);
+    for (SyncProgram& y : s->sp) {
+        hipFree(y.ops);
+        hipFree(y.start);
+        hipFree(y.resets);
+    }
+    hipFree(s->qlast);
+    hipFree(s->qti);
     hipFree(s->cfl);
     hipFree(s->mass_buf);
     hipFree(s->reg);
-    hipFree(s->sync_ops);
-    hipFree(s->sync_start);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
@@ -239,8 +268,16 @@ int hipclaw_set_aux(hipclaw_state_t* s, const double* aux, int first, int count)
     return HIPCLAW_OK;
 }
 
-int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start) {
-    if (!s || nops < 0 || (nops > 0 && !ops) || !phase_start)
+// A slot of a ghost op: the solution, or (flag) the time-interpolated copy.
+static bool op_slot_ok(const hipclaw_state_t* s, int slot) {
+    const int p = slot & ~HIPCLAW_TIME_INTERP_SLOT;
+    if ((slot & HIPCLAW_TIME_INTERP_SLOT) && !s->qti) return false;
+    return slot >= 0 && p < s->npatch;
+}
+
+int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int nops,
+                                const int* phase_start) {
+    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) || !phase_start)
         return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
     if (phase_start[0] != 0 || phase_start[6] != nops)
         return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
@@ -248,13 +285,25 @@ int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, cons
         if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
     // validate every op on the host: a bad slot or translation would fault the GPU
     const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    // the op types each phase may hold (bit mask over GHOST_*)
+    static const int phase_types[6] = {
+        (1 << hipclaw::GHOST_AVERAGE_FACE) | (1 << hipclaw::GHOST_AVERAGE_CORNER),
+        (1 << hipclaw::GHOST_COPY_FACE) | (1 << hipclaw::GHOST_COPY_CORNER),
+        1 << hipclaw::GHOST_PHYSBC,
+        (1 << hipclaw::GHOST_INTERP_FACE) | (1 << hipclaw::GHOST_INTERP_CORNER),
+        1 << hipclaw::GHOST_PHYSBC,
+        1 << hipclaw::GHOST_PHYSBC};
+    int ph = 0;
     for (int o = 0; o < nops; o++) {
         const int* op = ops + 8 * o;
         const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
-        if (type < 0 || type > hipclaw::GHOST_PHYSBC || dst < 0 || dst >= s->npatch)
+        while (o >= phase_start[ph + 1]) ph++;
+        if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
             return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
+        if (!(phase_types[ph] & (1 << type)))
+            return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
         if (type == hipclaw::GHOST_PHYSBC) continue;
-        if (src < 0 || src >= s->npatch || which < 0 || which > 3)
+        if (!op_slot_ok(s, src) || which < 0 || which > 3)
             return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
         // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
         // neighbour's array; check the extreme cells the op can touch
@@ -291,27 +340,53 @@ int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, cons
         }
         // average/interpolate faces test every cell against the neighbour's bounds in the kernel
     }
-    if (nops > s->ops_cap) {
-        hipFree(s->ops);
-        s->ops = nullptr;
-        s->ops_cap = 0;
-        if (hipMalloc((void**) &s->ops, sizeof(GhostOp) * nops) != hipSuccess)
+    GhostProgram& g = s->gp[k];
+    if (nops > g.cap) {
+        hipFree(g.ops);
+        g.ops = nullptr;
+        g.cap = 0;
+        if (hipMalloc((void**) &g.ops, sizeof(GhostOp) * nops) != hipSuccess)
             return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost ops failed%s");
-        s->ops_cap = nops;
+        g.cap = nops;
     }
     if (nops > 0) {
-        HIP_TRY(hipMemcpyAsync(s->ops, ops, sizeof(GhostOp) * nops, hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(g.ops, ops, sizeof(GhostOp) * nops, hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
     }
-    s->nops = nops;
-    std::memcpy(s->phase_start, phase_start, sizeof(s->phase_start));
+    g.nops = nops;
+    std::memcpy(g.phase_start, phase_start, sizeof(g.phase_start));
+    for (int ph = 0; ph < 6; ph++) {
+        g.phase_physbc[ph] = false;
+        for (int o = phase_start[ph]; o < phase_start[ph + 1]; o++)
+            if (ops[8 * o] == hipclaw::GHOST_PHYSBC) g.phase_physbc[ph] = true;
+    }
     return HIPCLAW_OK;
 }
 
-int hipclaw_set_sync_program(hipclaw_state_t* s, const int* ops, int nops) {
-    if (!s || nops < 0 || (nops > 0 && !ops))
+int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start) {
+    return hipclaw_set_ghost_program_k(s, 0, ops, nops, phase_start);
+}
+
+static int upload(hipclaw_state_t* s, int*& dev, int& cap, const int* host, int n, const char* what) {
+    if (n > cap) {
+        hipFree(dev);
+        dev = nullptr;
+        cap = 0;
+        if (hipMalloc((void**) &dev, sizeof(int) * (size_t) n) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
+        cap = n;
+    }
+    if (n > 0) HIP_TRY(hipMemcpyAsync(dev, host, sizeof(int) * (size_t) n, hipMemcpyHostToDevice, s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int nops,
+                               const int* resets, int nresets) {
+    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) ||
+        nresets < 0 || (nresets > 0 && !resets))
         return fail(HIPCLAW_EINVAL, "hipclaw_set_sync_program: bad arguments%s");
-    if (!s->reg && nops > 0) return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
+    if (!s->reg && (nops > 0 || nresets > 0))
+        return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
     // validate on the host and group by destination (groups must be contiguous)
     std::vector<int> start;
     std::vector<char> seen(s->npatch, 0);
@@ -328,38 +403,33 @@ int hipclaw_set_sync_program(hipclaw_state_t* s, const int* ops, int nops) {
         }
     }
     start.push_back(nops);
+    for (int r = 0; r < nresets; r++)
+        if (resets[2 * r] < 0 || resets[2 * r] >= s->npatch || resets[2 * r + 1] < 0 || resets[2 * r + 1] > 3)
+            return fail(HIPCLAW_EINVAL, "register reset with bad slot or face%s");
+    SyncProgram& y = s->sp[k];
     const int ngroups = (int) start.size() - 1;
-    if (nops > s->sync_ops_cap) {
-        hipFree(s->sync_ops);
-        s->sync_ops = nullptr;
-        s->sync_ops_cap = 0;
-        if (hipMalloc((void**) &s->sync_ops, sizeof(int) * 8 * (size_t) nops) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of sync ops failed%s");
-        s->sync_ops_cap = nops;
-    }
-    if (ngroups + 1 > s->sync_groups_cap) {
-        hipFree(s->sync_start);
-        s->sync_start = nullptr;
-        s->sync_groups_cap = 0;
-        if (hipMalloc((void**) &s->sync_start, sizeof(int) * (size_t) (ngroups + 1)) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of sync groups failed%s");
-        s->sync_groups_cap = ngroups + 1;
-    }
-    if (nops > 0) {
-        HIP_TRY(hipMemcpyAsync(s->sync_ops, ops, sizeof(int) * 8 * (size_t) nops, hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipMemcpyAsync(s->sync_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, s->st));
-        HIP_TRY(hipStreamSynchronize(s->st));
-    }
-    s->nsync_groups = ngroups;
+    if (int rc = upload(s, y.ops, y.ops_cap, ops, 8 * nops, "sync ops")) return rc;
+    if (int rc = upload(s, y.start, y.groups_cap, start.data(), ngroups + 1, "sync groups")) return rc;
+    if (int rc = upload(s, y.resets, y.resets_cap, resets, 2 * nresets, "register resets")) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    y.ngroups = ngroups;
+    y.nresets = nresets;
     return HIPCLAW_OK;
 }
 
-static int time_sync_async(hipclaw_state_t* s) {
-    if (!s->reg || s->nsync_groups == 0) return HIPCLAW_OK;
+int hipclaw_set_sync_program(hipclaw_state_t* s, const int* ops, int nops) {
+    return hipclaw_set_sync_program_k(s, 0, ops, nops, nullptr, 0);
+}
+
+static int time_sync_async(hipclaw_state_t* s, int k) {
+    const SyncProgram& y = s->sp[k];
+    if (!s->reg || (y.ngroups == 0 && y.nresets == 0)) return HIPCLAW_OK;
     hipclaw::SyncArgs a;
-    a.ops = s->sync_ops;
-    a.start = s->sync_start;
-    a.ngroups = s->nsync_groups;
+    a.ops = y.ops;
+    a.start = y.start;
+    a.ngroups = y.ngroups;
+    a.resets = y.resets;
+    a.nresets = y.nresets;
     a.q = s->q[s->cur];
     a.reg = s->reg;
     a.dxs = s->dxs;
@@ -372,30 +442,57 @@ static int time_sync_async(hipclaw_state_t* s) {
     return HIPCLAW_OK;
 }
 
-int hipclaw_time_sync(hipclaw_state_t* s) {
-    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
-    if (int rc = time_sync_async(s)) return rc;
+int hipclaw_time_sync_k(hipclaw_state_t* s, int k) {
+    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS) return fail(HIPCLAW_EINVAL, "hipclaw_time_sync: bad arguments%s");
+    if (int rc = time_sync_async(s, k)) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
     return HIPCLAW_OK;
 }
 
-static int ghost_fill_async(hipclaw_state_t* s) {
+int hipclaw_time_sync(hipclaw_state_t* s) { return hipclaw_time_sync_k(s, 0); }
+
+static int ghost_fill_async(hipclaw_state_t* s, int k) {
+    const GhostProgram& p = s->gp[k];
     hipclaw::GhostArgs g;
     g.q = s->q[s->cur];
+    g.qti = s->qti;
     g.pstride = s->pstride;
     g.mx = s->cfg.mx; g.my = s->cfg.my; g.mbc = s->cfg.mbc; g.meqn = s->cfg.meqn;
+    // Phases 0 (average fine to coarse) and 1 (copy) write disjoint ghost cells
+    // and read interior cells only, so they share one launch.
     for (int ph = 0; ph < 6; ph++) {
-        g.ops = s->ops + s->phase_start[ph];
-        g.nops = s->phase_start[ph + 1] - s->phase_start[ph];
-        if (int e = hipclaw::launch_ghost_phase(g, s->st))
+        const int last = (ph == 0) ? 1 : ph;
+        g.ops = p.ops + p.phase_start[ph];
+        g.nops = p.phase_start[last + 1] - p.phase_start[ph];
+        const bool physbc = p.phase_physbc[ph] || p.phase_physbc[last];
+        if (int e = hipclaw::launch_ghost_phase(g, physbc, s->st))
             return hipfail((hipError_t) e, "ghost_phase launch");
+        ph = last;
     }
     return HIPCLAW_OK;
 }
 
-int hipclaw_ghost_fill(hipclaw_state_t* s) {
-    if (!s) return fail(HIPCLAW_EINVAL, "null state%s");
-    if (int rc = ghost_fill_async(s)) return rc;
+int hipclaw_ghost_fill_k(hipclaw_state_t* s, int k) {
+    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS) return fail(HIPCLAW_EINVAL, "hipclaw_ghost_fill: bad arguments%s");
+    if (int rc = ghost_fill_async(s, k)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_ghost_fill(hipclaw_state_t* s) { return hipclaw_ghost_fill_k(s, 0); }
+
+int hipclaw_timeinterp(hipclaw_state_t* s, const int* slots, int nslots, double alpha) {
+    if (!s || nslots < 0 || nslots > s->max_patches || (nslots > 0 && !slots))
+        return fail(HIPCLAW_EINVAL, "hipclaw_timeinterp: bad arguments%s");
+    if (!s->qti) return fail(HIPCLAW_EINVAL, "the state was created without subcycle%s");
+    for (int k = 0; k < nslots; k++)
+        if (slots[k] < 0 || slots[k] >= s->npatch) return fail(HIPCLAW_EINVAL, "slot out of range%s");
+    if (nslots == 0) return HIPCLAW_OK;
+    HIP_TRY(hipMemcpyAsync(s->batch_slots, slots, sizeof(int) * nslots, hipMemcpyHostToDevice, s->st));
+    int e = hipclaw::launch_timeinterp(s->q[s->cur], s->qlast, s->qti, s->batch_slots, nslots,
+                                       s->pstride, s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn,
+                                       alpha, s->st);
+    if (e) return hipfail((hipError_t) e, "timeinterp launch");
     HIP_TRY(hipStreamSynchronize(s->st));
     return HIPCLAW_OK;
 }
@@ -423,6 +520,7 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
     a.pp.gamma = s->cfg.gamma;
     a.reg = s->reg;
     a.rstride = s->rstride;
+    a.reg_accumulate = s->reg_accumulate;
     return a;
 }
 
@@ -456,6 +554,11 @@ int hipclaw_step_batch(hipclaw_state_t* s, const int* slots, int nslots, double 
     if (nslots == 0) return HIPCLAW_OK;
     HIP_TRY(hipMemcpyAsync(s->batch_slots, slots, sizeof(int) * nslots, hipMemcpyHostToDevice, s->st));
     if (int e = hipclaw::launch_reset_scalar(s->cfl, s->st)) return hipfail((hipError_t) e, "reset");
+    if (s->qlast) {  // fclaw2d_clawpatch_save_current_step before every update
+        int e = hipclaw::launch_copy_patches(s->q[s->cur], s->qlast, s->batch_slots, nslots,
+                                             s->pstride, s->st);
+        if (e) return hipfail((hipError_t) e, "copy_patches launch");
+    }
     double* src = s->q[s->cur];
     double* dst = s->inplace ? src : s->q[1 - s->cur];
     if (int rc = launch_step(s, step_args(s, src, dst, s->batch_slots, nslots, dt))) return rc;
@@ -488,14 +591,16 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
         if (int rc = launch_step(s, step_args(s, src, dst, s->all_slots, s->npatch, dt))) return rc;
         if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n + 1], s->st));
         if (!s->inplace) s->cur = 1 - s->cur;
-        if (int rc = ghost_fill_async(s)) return rc;
-        if (int rc = time_sync_async(s)) return rc;
+        if (int rc = ghost_fill_async(s, 0)) return rc;
+        if (int rc = time_sync_async(s, 0)) return rc;
     }
     return HIPCLAW_OK;
 }
 
 int hipclaw_advance_async(hipclaw_state_t* s, double dt, int nsteps) {
     if (!s || nsteps < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance: bad arguments%s");
+    if (s->reg_accumulate && s->sp[0].nresets == 0 && s->sp[0].ngroups > 0)
+        return fail(HIPCLAW_EINVAL, "with subcycle, the full sync program needs its register resets%s");
     HIP_TRY(hipEventRecord(s->region[0], s->st));
     if (int rc = advance_launches(s, dt, nsteps, s->time_kernels)) return rc;
     HIP_TRY(hipEventRecord(s->region[1], s->st));

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/hipclaw.h"
+
 #include "hipclaw_step.h"
 
 namespace hipclaw {
@@ -34,6 +36,7 @@ struct GhostArgs {
     const GhostOp* ops;
     int nops;
     double* q;
+    double* qti;           // time-interpolated copies (slots flagged HIPCLAW_TIME_INTERP_SLOT)
     long long pstride;
     int mx, my, mbc, meqn;
 };
@@ -50,16 +53,25 @@ struct SyncArgs {
     const double* dxs;
     const double* dys;
     long long pstride, rstride;
+    const int* resets;   // [nresets][2]: (slot, face) registers zeroed after the corrections
+    int nresets;
     int mx, mbc, meqn;   // mx == my
     int fluxreg;         // registers hold one shared edge flux (FP32 flux form)
 };
 
 // Launchers (return hipError_t as int).
 int launch_time_sync(const SyncArgs& a, hipStream_t st);
-int launch_ghost_phase(const GhostArgs& g, hipStream_t st);
+int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st);
 int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_reset_scalar(unsigned long long* p, hipStream_t st);
+// Whole patch slabs src[slot] -> dst[slot].
+int launch_copy_patches(const double* src, double* dst, const int* slots, int n, long long pstride,
+                        hipStream_t st);
+// qti = qlast + alpha (q - qlast) on the interior band of width mbc.
+int launch_timeinterp(const double* q, const double* qlast, double* qti, const int* slots, int n,
+                      long long pstride, int mx, int my, int mbc, int meqn, double alpha,
+                      hipStream_t st);
 int launch_copy_interior(const double* src, double* dst, const int* slots, int n,
                          long long pstride, int mx, int my, int mbc, int meqn,
                          hipStream_t st);

@@ -35,17 +35,21 @@ __device__ __forceinline__ void interp_cell(const double* qc, double* qf, int m,
     qf[QIX(fi + 1, fj + 1, m)] = c + gx * 0.25 + gy * 0.25;
 }
 
-// One workgroup (one wave) per ghost op.  Ops of one phase write disjoint
-// ghost cells and read only cells no op of the phase writes.
-__global__ void __launch_bounds__(64) ghost_phase(GhostArgs g) {
-    const int o = blockIdx.x;
+// One wave per ghost op, OPW ops (waves) per workgroup.  Ops of one phase
+// write disjoint ghost cells and read only cells no op of the phase writes.
+// The physical-BC op sequences its faces with workgroup barriers, so phases
+// that hold such ops run with OPW = 1.
+template <int OPW>
+__global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
+    const int o = blockIdx.x * OPW + (OPW > 1 ? (int) (threadIdx.x >> 6) : 0);
     if (o >= g.nops) return;
     const GhostOp op = g.ops[o];
     const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
-    double* qd = g.q + (long long) op.dst * g.pstride;
-    const double* qs = g.q + (long long) op.src * g.pstride;
-    const int t = threadIdx.x;
+    const int TI = HIPCLAW_TIME_INTERP_SLOT;
+    double* qd = ((op.dst & TI) ? g.qti : g.q) + (long long) (op.dst & ~TI) * g.pstride;
+    const double* qs = ((op.src & TI) ? g.qti : g.q) + (long long) (op.src & ~TI) * g.pstride;
+    const int t = threadIdx.x & 63;
 
     switch (op.type) {
     case GHOST_COPY_FACE: {  // fclaw2d_clawpatch46_copy.f:40-84
@@ -133,6 +137,7 @@ __global__ void __launch_bounds__(64) ghost_phase(GhostArgs g) {
         break;
     }
     case GHOST_PHYSBC: {  // clawpack46_bc2_default.f, faces in order 0,1,2,3
+        if constexpr (OPW > 1) break;   // never launched this way (see launch_ghost_phase)
         for (int face = 0; face < 4; face++) {
             const int mth = (signed char) ((op.bc >> (8 * face)) & 0xff);
             if (mth == 1 || mth == 3) {
@@ -165,6 +170,45 @@ __global__ void __launch_bounds__(64) ghost_phase(GhostArgs g) {
 }
 
 __global__ void reset_scalar(unsigned long long* p) { *p = 0ull; }
+
+__global__ void copy_patches(const double* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ slots, int n, long long pstride) {
+    const int w = blockIdx.y;
+    if (w >= n) return;
+    const long long base = (long long) slots[w] * pstride;
+    for (long long e = blockIdx.x * (long long) blockDim.x + threadIdx.x; e < pstride;
+         e += (long long) gridDim.x * blockDim.x)
+        dst[base + e] = src[base + e];
+}
+
+// fclaw2d_clawpatch46_fort_timeinterp
+// (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:1-58):
+// qinterp = qlast + alpha (qcurr - qlast) on the interior cells within mbc of
+// the patch edge (the cells fine neighbours interpolate from).
+__global__ void timeinterp_band(const double* __restrict__ q, const double* __restrict__ ql,
+                                double* __restrict__ qi, const int* __restrict__ slots, int n,
+                                long long pstride, int mx, int my, int mbc, int meqn,
+                                double alpha) {
+    const int w = blockIdx.y;
+    if (w >= n) return;
+    const long long base = (long long) slots[w] * pstride;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < meqn * mx * my; e += gridDim.x * blockDim.x) {
+        const int m = e / (mx * my), r = e - m * mx * my;
+        const int j = r / mx + 1, i = r - (j - 1) * mx + 1;
+        if (i > mbc && i <= mx - mbc && j > mbc && j <= my - mbc) continue;  // the hole
+        const long long o = base + QIX(i, j, m);
+        qi[o] = ql[o] + alpha * (q[o] - ql[o]);
+    }
+}
+
+__global__ void reset_registers(double* __restrict__ reg, const int* __restrict__ resets,
+                                int nresets, long long rstride, int per_face) {
+    const int r = blockIdx.x;
+    if (r >= nresets) return;
+    double* p = reg + (long long) resets[2 * r] * rstride + (long long) resets[2 * r + 1] * per_face;
+    for (int e = threadIdx.x; e < per_face; e += blockDim.x) p[e] = 0.0;
+}
 
 __global__ void copy_interior(const double* __restrict__ src, double* __restrict__ dst,
                               const int* __restrict__ slots, int n, long long pstride,
@@ -230,9 +274,30 @@ patch_mass(const double* __restrict__ q, const int* __restrict__ slots, int n,
     }
 }
 
-int launch_ghost_phase(const GhostArgs& g, hipStream_t st) {
+int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st) {
     if (g.nops <= 0) return 0;
-    hipLaunchKernelGGL(ghost_phase, dim3(g.nops), dim3(64), 0, st, g);
+    if (has_physbc)
+        hipLaunchKernelGGL(ghost_phase<1>, dim3(g.nops), dim3(64), 0, st, g);
+    else
+        hipLaunchKernelGGL(ghost_phase<4>, dim3((g.nops + 3) / 4), dim3(256), 0, st, g);
+    return (int) hipGetLastError();
+}
+
+int launch_copy_patches(const double* src, double* dst, const int* slots, int n, long long pstride,
+                        hipStream_t st) {
+    if (n <= 0) return 0;
+    dim3 grid((unsigned) ((pstride + 255) / 256), n);
+    hipLaunchKernelGGL(copy_patches, grid, dim3(256), 0, st, src, dst, slots, n, pstride);
+    return (int) hipGetLastError();
+}
+
+int launch_timeinterp(const double* q, const double* qlast, double* qti, const int* slots, int n,
+                      long long pstride, int mx, int my, int mbc, int meqn, double alpha,
+                      hipStream_t st) {
+    if (n <= 0) return 0;
+    dim3 grid((meqn * mx * my + 255) / 256, n);
+    hipLaunchKernelGGL(timeinterp_band, grid, dim3(256), 0, st, q, qlast, qti, slots, n, pstride,
+                       mx, my, mbc, meqn, alpha);
     return (int) hipGetLastError();
 }
 
@@ -320,8 +385,16 @@ __global__ void __launch_bounds__(128) time_sync_groups(SyncArgs a) {
 }
 
 int launch_time_sync(const SyncArgs& a, hipStream_t st) {
-    if (a.ngroups <= 0) return 0;
-    hipLaunchKernelGGL(time_sync_groups, dim3(a.ngroups), dim3(128), 0, st, a);
+    if (a.ngroups > 0) {
+        hipLaunchKernelGGL(time_sync_groups, dim3(a.ngroups), dim3(128), 0, st, a);
+        if (hipError_t e = hipGetLastError()) return (int) e;
+    }
+    if (a.nresets > 0) {
+        // after every correction of this synchronisation (they read the registers)
+        const int per_face = 4 * a.meqn * a.mx;
+        hipLaunchKernelGGL(reset_registers, dim3(a.nresets), dim3(128), 0, st,
+                           const_cast<double*>(a.reg), a.resets, a.nresets, a.rstride, per_face);
+    }
     return (int) hipGetLastError();
 }
 

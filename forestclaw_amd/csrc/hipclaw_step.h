@@ -60,9 +60,21 @@ struct StepArgs {
     // REG_EFOUT); the FP32 flux form writes its shared FP64 edge flux in REG_EFIN.
     double* reg;
     long long rstride;
+    int reg_accumulate;   // add to the registers (subcycling) instead of overwriting them
 };
 
 enum { REG_FM = 0, REG_FP = 1, REG_EFIN = 2, REG_EFOUT = 3 };
+
+// Register update r = r + v (r = r - v): the reference accumulates onto
+// registers that start at zero (clawpack46_time_sync.f:75-147, 192-209); when
+// every register is reset at every synchronisation (no subcycling) the
+// previous value is zero and need not be read.
+__device__ __forceinline__ void reg_add(double* r, double v, int accumulate) {
+    *r = (accumulate ? *r : 0.0) + v;
+}
+__device__ __forceinline__ void reg_sub(double* r, double v, int accumulate) {
+    *r = (accumulate ? *r : 0.0) - v;
+}
 
 template <typename R>
 __device__ __forceinline__ R philim(int mth, R r) {
@@ -176,18 +188,28 @@ step_tiles(StepArgs a) {
         ty = tix / a.ntx;
         return a.work[wid];
     };
-    // global offset of staged value e (field m, tile-local row qj, column qi)
-    auto staged_offset = [&](int e, int i0, int j0) -> long long {
+    // Staging map, the same for every tile: element e = threadIdx.x + kk*NT of
+    // the (field m, tile row qj, tile column qi) halo tile sits at goff[kk] +
+    // (tile origin) in the patch slab and at loff[kk] in LDS (-1: no element).
+    int goff[KQ], loff[KQ];
+#pragma unroll
+    for (int kk = 0; kk < KQ; kk++) {
+        const int e = threadIdx.x + kk * NT;
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
-        const int i = i0 - 2 + qi, j = j0 - 2 + qj;  // 1-based patch indices
-        return ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
-    };
-    auto staged_lds = [&](int e) -> int {
+        // cell (i, j) = (i0 - 2 + qi, j0 - 2 + qj) of field m
+        goff[kk] = e < NQ ? (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2 : -1;
+        loff[kk] = e < NQ ? m * QP + qj * QS + qi : -1;
+    }
+    int gaoff[KA], laoff[KA];
+#pragma unroll
+    for (int kk = 0; kk < KA; kk++) {
+        const int e = threadIdx.x + kk * NT;
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
-        return m * QP + qj * QS + qi;
-    };
+        gaoff[kk] = (MAUX > 0 && e < NA) ? (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2 : -1;
+        laoff[kk] = (MAUX > 0 && e < NA) ? m * QP + qj * QS + qi : -1;
+    }
 
     // prefetch registers for the next tile
     double pq[KQ];
@@ -195,20 +217,14 @@ step_tiles(StepArgs a) {
     auto prefetch = [&](int tile) {
         int tx, ty;
         const int slot = tile_slot(tile, tx, ty);
-        const int i0 = 1 + tx * T, j0 = 1 + ty * T;
-        const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride;
+        const long long shift = (long long) (ty * T) * nxp + tx * T;  // tile origin in the slab
+        const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride + shift;
 #pragma unroll
-        for (int kk = 0; kk < KQ; kk++) {
-            const int e = threadIdx.x + kk * NT;
-            pq[kk] = e < NQ ? qp[staged_offset(e, i0, j0)] : 0.0;
-        }
+        for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
         if constexpr (MAUX > 0) {
-            const double* __restrict__ ap = a.aux + (long long) slot * a.astride;
+            const double* __restrict__ ap = a.aux + (long long) slot * a.astride + shift;
 #pragma unroll
-            for (int kk = 0; kk < KA; kk++) {
-                const int e = threadIdx.x + kk * NT;
-                pa[kk] = e < NA ? (R) ap[staged_offset(e, i0, j0)] : R(0);
-            }
+            for (int kk = 0; kk < KA; kk++) pa[kk] = gaoff[kk] >= 0 ? (R) ap[gaoff[kk]] : R(0);
         }
     };
 
@@ -244,16 +260,12 @@ step_tiles(StepArgs a) {
 
         // ---- stage 0: the prefetched tile + 2-cell halo into LDS ----
 #pragma unroll
-        for (int kk = 0; kk < KQ; kk++) {
-            const int e = threadIdx.x + kk * NT;
-            if (e < NQ) sm.q[staged_lds(e)] = pq[kk];
-        }
+        for (int kk = 0; kk < KQ; kk++)
+            if (loff[kk] >= 0) sm.q[loff[kk]] = pq[kk];
         if constexpr (MAUX > 0) {
 #pragma unroll
-            for (int kk = 0; kk < KA; kk++) {
-                const int e = threadIdx.x + kk * NT;
-                if (e < NA) sm.aux[staged_lds(e)] = pa[kk];
-            }
+            for (int kk = 0; kk < KA; kk++)
+                if (laoff[kk] >= 0) sm.aux[laoff[kk]] = pa[kk];
         }
         __syncthreads();
         // the next tile's data travels while this one is computed
@@ -441,8 +453,8 @@ step_tiles(StepArgs a) {
                     const int face = k == 1 ? 2 : 3, pos = i0 + c - 2;
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
-                        rg[((face * 4 + REG_FM) * MEQN + m) * mx + pos] = 0.0 + dtdx_el * (double) gm[m];
-                        rg[((face * 4 + REG_FP) * MEQN + m) * mx + pos] = 0.0 - dtdx_el * (double) gp[m];
+                        reg_add(rg + ((face * 4 + REG_FM) * MEQN + m) * mx + pos, dtdx_el * (double) gm[m], a.reg_accumulate);
+                        reg_sub(rg + ((face * 4 + REG_FP) * MEQN + m) * mx + pos, dtdx_el * (double) gp[m], a.reg_accumulate);
                     }
                 }
             }
@@ -505,8 +517,8 @@ step_tiles(StepArgs a) {
                     const int s0 = side ? T : 0;
                     const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
                     const R fpv = ((R(0) + sm.fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
-                    rg[((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b] = 0.0 + dtdy_el * (double) fmv;
-                    rg[((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b] = 0.0 - dtdy_el * (double) fpv;
+                    reg_add(rg + ((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fmv, a.reg_accumulate);
+                    reg_sub(rg + ((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fpv, a.reg_accumulate);
                 }
                 // conservative flux of the pre-step state at the first interior
                 // and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
@@ -531,7 +543,7 @@ step_tiles(StepArgs a) {
                     const int pos = (face < 2 ? j0 : i0) - 1 + c;
 #pragma unroll
                     for (int m = 0; m < MEQN; m++)
-                        rg[((face * 4 + REG_EFIN + kio) * MEQN + m) * mx + pos] = 0.0 + el * fl[m];
+                        reg_add(rg + ((face * 4 + REG_EFIN + kio) * MEQN + m) * mx + pos, el * fl[m], a.reg_accumulate);
                 }
             }
         } else {
@@ -573,7 +585,7 @@ step_tiles(StepArgs a) {
                     else if (face == 2) { if (!first_y) continue; F = sm.fy[(m * (T + 1)) * T + c]; }
                     else { if (!last_y) continue; F = sm.fy[(m * (T + 1) + T) * T + c]; }
                     const int pos = (face < 2 ? j0 : i0) - 1 + c;
-                    rg[((face * 4 + REG_EFIN) * MEQN + m) * mx + pos] = (face < 2 ? dtdy_el : dtdx_el) * F;
+                    reg_add(rg + ((face * 4 + REG_EFIN) * MEQN + m) * mx + pos, (face < 2 ? dtdy_el : dtdx_el) * F, a.reg_accumulate);
                 }
             }
             for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {

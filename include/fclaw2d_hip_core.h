@@ -95,6 +95,8 @@ typedef struct fclaw2d_global {
     struct hipclaw_state *state;   /* device-resident solution */
     double *qhost, *auxhost;       /* host views [slot][field][j][i] */
     int time_sync;                 /* fclaw_options time-sync (src/fclaw_options.c:110) */
+    int subcycle;                  /* fclaw_options subcycle (src/fclaw_options.c:99) */
+    int advance_one_step;          /* fclaw_options advance-one-step (src/fclaw_options.c:91) */
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -135,6 +137,10 @@ void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel,
  *  (fclaw2d_advance_all_levels with subcycle = F, advance_one_step = T,
  *  src/fclaw2d_advance.c:247-357).  Returns the maximum Courant number. */
 double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt);
+
+/* Time-interpolate the patches of `level` (fclaw2d_timeinterp,
+ * src/fclaw2d_timeinterp.c); needs glob->subcycle. */
+void fclaw2d_timeinterp(fclaw2d_global_t *glob, int level, double alpha);
 
 /* Conservation fix-up after all levels reached the same time
  * (src/fclaw2d_time_sync.c:161-202); needs glob->time_sync set before

@@ -48,6 +48,27 @@ int fclaw2d_hip_forest_ghost_program(const fclaw2d_hip_forest_t *f, int mx, int 
                                      const int mthbc[4], int *ops, int cap,
                                      int phase_start[7]);
 
+/* Slot flag in ghost-fill ops: the slot's time-interpolated copy (see
+ * hipclaw_timeinterp) rather than its solution. */
+#define FCLAW2D_HIP_TIME_INTERP_SLOT (1 << 30)
+
+/* Ghost-fill program of fclaw2d_ghost_update(minlevel, maxlevel, time_interp)
+ * (src/fclaw2d_ghost_fill.c:611-830): with time_interp, level minlevel-1
+ * takes part through its time-interpolated copies (ops flagged with
+ * FCLAW2D_HIP_TIME_INTERP_SLOT).  Same conventions as
+ * fclaw2d_hip_forest_ghost_program, which is the full range without it. */
+int fclaw2d_hip_forest_ghost_program_range(const fclaw2d_hip_forest_t *f, int mx, int my,
+                                           int mbc, const int mthbc[4], int minlevel,
+                                           int maxlevel, int time_interp, int *ops, int cap,
+                                           int phase_start[7]);
+
+/* Time-synchronisation program of fclaw2d_time_sync(minlevel, maxlevel):
+ * returns the number of ops (up to `cap` written) and the (slot, face)
+ * register resets in `resets` (up to rcap pairs; *nresets = number needed). */
+int fclaw2d_hip_forest_sync_program_range(const fclaw2d_hip_forest_t *f, int mx, int my,
+                                          int minlevel, int *ops, int cap, int *resets,
+                                          int rcap, int *nresets);
+
 /* Time-synchronisation program for the current leaves (see
  * hipclaw_set_sync_program): writes up to `cap` ops of 8 ints and returns the
  * number needed (cap = 0 to size the buffer), or -1 on error (mx != my). */

@@ -56,7 +56,15 @@ typedef struct hipclaw_config {
     int tile;        /* cells per tile side (8, 16, 32); 0 = tuned default */
     int time_sync;   /* keep flux registers for the conservation fix-up
                         (fclaw_options time-sync, src/fclaw_options.c:110); needs mx == my */
+    int subcycle;    /* keep the state before each update and time-interpolated
+                        copies for local time stepping (fclaw_options subcycle) */
 } hipclaw_config_t;
+
+/* Numbered ghost-fill and synchronisation programs per state; program 0 is
+ * the one hipclaw_advance uses. */
+#define HIPCLAW_MAX_PROGRAMS 32
+/* Flag on a ghost op's slot: the slot's time-interpolated copy. */
+#define HIPCLAW_TIME_INTERP_SLOT (1 << 30)
 
 typedef struct hipclaw_state hipclaw_state_t;
 
@@ -116,6 +124,30 @@ int hipclaw_ghost_fill(hipclaw_state_t *s);
  * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_step2.cu:49-269). */
 int hipclaw_step_batch(hipclaw_state_t *s, const int *slots, int nslots, double t,
                        double dt, double *maxcfl);
+
+/* Ghost-fill program number k (0 <= k < HIPCLAW_MAX_PROGRAMS); program 0 is
+ * hipclaw_set_ghost_program's.  Programs for fclaw2d_ghost_update over a
+ * partial level range with time interpolation (subcycling) reference the
+ * coarser level's time-interpolated copies with HIPCLAW_TIME_INTERP_SLOT. */
+int hipclaw_set_ghost_program_k(hipclaw_state_t *s, int k, const int *ops, int nops,
+                                const int *phase_start);
+int hipclaw_ghost_fill_k(hipclaw_state_t *s, int k);
+
+/* Time interpolation of the listed patches for the ghost fill of their finer
+ * neighbours at an intermediate time (config subcycle = 1): copies =
+ * qlast + alpha (q - qlast) on the interior band of width mbc, where qlast is
+ * the state saved by hipclaw_step_batch before the patch's last update.
+ * Replaces fclaw2d_timeinterp / clawpatch_setup_timeinterp
+ * (src/patches/clawpatch/fclaw2d_clawpatch.cpp:474-526). */
+int hipclaw_timeinterp(hipclaw_state_t *s, const int *slots, int nslots, double alpha);
+
+/* Time-synchronisation program number k, with the (slot, face) register
+ * resets applied after its corrections (nresets pairs).  Registers add up
+ * over substeps when the state was created with subcycle = 1; otherwise every
+ * update overwrites them and program 0 needs no resets. */
+int hipclaw_set_sync_program_k(hipclaw_state_t *s, int k, const int *ops, int nops,
+                               const int *resets, int nresets);
+int hipclaw_time_sync_k(hipclaw_state_t *s, int k);
 
 /* Time-synchronisation program (config time_sync = 1): nops ops of 8 int32
  * (type, dst, src, face, ti, tj, 0, 0), type 0 = same-size correction of dst

@@ -50,7 +50,7 @@ class Global(C.Structure):
                 ("clawpatch_opt", ClawpatchOptions), ("hipclaw_vt", C.c_void_p),
                 ("hipclaw_opt", C.c_void_p), ("state", C.c_void_p),
                 ("qhost", C.POINTER(C.c_double)), ("auxhost", C.POINTER(C.c_double)),
-                ("time_sync", C.c_int)]
+                ("time_sync", C.c_int), ("subcycle", C.c_int), ("advance_one_step", C.c_int)]
 
 
 class Options(C.Structure):
@@ -84,6 +84,7 @@ def bind():
         "fclaw2d_ghost_update": ([C.POINTER(Global), i, i, d, i, i], None),
         "fclaw2d_advance_all_levels": ([C.POINTER(Global), d, d], d),
         "fclaw2d_time_sync": ([C.POINTER(Global), i, i], None),
+        "fclaw2d_timeinterp": ([C.POINTER(Global), i, d], None),
         "fc2d_hipclaw_options_default": ([C.POINTER(Options), i], None),
         "fc2d_hipclaw_options_store": ([C.POINTER(Global), C.POINTER(Options)], None),
         "fc2d_hipclaw_check": ([C.POINTER(Options)], i),

@@ -96,6 +96,7 @@ def test_dropin_time_sync_per_level(gpu):
     dom = L.fclaw2d_domain_new_from_forest(fh, 1, 1, 0, 0, 0.0, 1.0, 0.0, 1.0)
     glob = L.fclaw2d_global_new()
     L.fclaw2d_global_store_domain(glob, dom)
+    glob.contents.advance_one_step = 1
     glob.contents.time_sync = 1
     cp = glob.contents.clawpatch_opt
     cp.mx, cp.my, cp.mbc, cp.meqn, cp.maux = 16, 16, 2, 3, 0
