@@ -685,3 +685,124 @@ void fc2d_hipclaw_src2(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int block
     fclaw2d_clawpatch_aux_data(glob, patch, &aux, &maux);
     vt->fort_src2(&mx, &my, &meqn, &mbc, &mx, &my, &xl, &yl, &dx, &dy, q, &maux, aux, &t, &dt);
 }
+
+/* ------------------------------------------------------------------ ASCII output */
+
+/* A value in Fortran Ew.d edit form (0.d1..dd E+xx, or +xxx for three-digit
+ * exponents), right-justified in w characters. */
+static void fortran_e(char *out, double x, int w, int d)
+{
+    char digits[64], body[96];
+    int e10 = 0, neg = x < 0.0, len;
+    if (x == 0.0) {
+        memset(digits, '0', (size_t) d);
+        digits[d] = 0;
+        e10 = 0;
+    } else {
+        char tmp[96];
+        char *p, *q;
+        snprintf(tmp, sizeof tmp, "%.*e", d - 1, neg ? -x : x);  /* d.ddd..e+XX */
+        q = digits;
+        for (p = tmp; *p && *p != 'e'; p++)
+            if (*p >= '0' && *p <= '9') *q++ = *p;
+        *q = 0;
+        e10 = atoi(p + 1) + 1;
+    }
+    if (e10 >= -99 && e10 <= 99)
+        len = snprintf(body, sizeof body, "%s0.%sE%c%02d", neg ? "-" : "", digits,
+                       e10 < 0 ? '-' : '+', e10 < 0 ? -e10 : e10);
+    else
+        len = snprintf(body, sizeof body, "%s0.%s%c%03d", neg ? "-" : "", digits,
+                       e10 < 0 ? '-' : '+', e10 < 0 ? -e10 : e10);
+    if (len >= w) {
+        memcpy(out, body, (size_t) len + 1);
+    } else {
+        memset(out, ' ', (size_t) (w - len));
+        memcpy(out + (w - len), body, (size_t) len + 1);
+    }
+}
+
+int fclaw2d_clawpatch_time_header_ascii_files(const char *fname_q, const char *fname_t,
+                                              double time, int meqn, int maux, int ngrids)
+{
+    char e[64];
+    FILE *f = fopen(fname_t, "w");
+    if (!f) return -1;
+    fortran_e(e, time, 30, 20);
+    fprintf(f, "%s    time\n%5d                 meqn\n%5d                 ngrids\n"
+               "%5d                 num_aux\n%5d                 num_dim\n",
+            e, meqn, ngrids, maux, 2);
+    fclose(f);
+    f = fopen(fname_q, "w");
+    if (!f) return -1;
+    fclose(f);
+    return 0;
+}
+
+int fclaw2d_clawpatch_output_ascii_patch(const char *fname_q, int mx, int my, int meqn, int mbc,
+                                         double xlower, double ylower, double dx, double dy,
+                                         const double *q, int patch_num, int level, int blockno,
+                                         int mpirank)
+{
+    const int nx = mx + 2 * mbc, ny = my + 2 * mbc;
+    char e[64];
+    int i, j, m;
+    FILE *f;
+    if (meqn > 20) return -1;
+    f = fopen(fname_q, "a");
+    if (!f) return -1;
+    fprintf(f, "%5d                 grid_number\n%5d                 AMR_level\n"
+               "%5d                 block_number\n%5d                 mpi_rank\n"
+               "%5d                 mx\n%5d                 my\n",
+            patch_num, level, blockno, mpirank, mx, my);
+    fortran_e(e, xlower, 24, 16); fprintf(f, "%s    xlow\n", e);
+    fortran_e(e, ylower, 24, 16); fprintf(f, "%s    ylow\n", e);
+    fortran_e(e, dx, 24, 16); fprintf(f, "%s    dx\n", e);
+    fortran_e(e, dy, 24, 16); fprintf(f, "%s    dy\n\n", e);
+    for (j = 1; j <= my; j++) {
+        for (i = 1; i <= mx; i++) {
+            for (m = 0; m < meqn; m++) {
+                double v = q[((size_t) m * ny + (size_t) (j + mbc - 1)) * nx + (size_t) (i + mbc - 1)];
+                if (fabs(v) < 1e-99) v = 0.0;  /* as the reference writer does */
+                fortran_e(e, v, 26, 16);
+                fputs(e, f);
+            }
+            fputc('\n', f);
+        }
+        fputs("  \n", f);
+    }
+    fclose(f);
+    return 0;
+}
+
+/* fclaw2d_clawpatch_output_ascii (src/patches/clawpatch/fclaw2d_clawpatch_output_ascii.c:139-160)
+ * with the writers of fclaw2d_clawpatch46_output_ascii.f: fort.tXXXX and
+ * fort.qXXXX in the current directory, from the device-resident solution. */
+void fclaw2d_clawpatch_output_ascii(fclaw2d_global_t *glob, int iframe)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    char fq[32], ft[32];
+    int b, p;
+    fc2d_hipclaw_download(glob);
+    snprintf(fq, sizeof fq, "fort.q%04d", iframe);
+    snprintf(ft, sizeof ft, "fort.t%04d", iframe);
+    if (fclaw2d_clawpatch_time_header_ascii_files(fq, ft, glob->curr_time, o->meqn, o->maux,
+                                                  d->local_num_patches)) {
+        fprintf(stderr, "fclaw2d_clawpatch_output_ascii: cannot write %s\n", ft);
+        exit(1);
+    }
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++) {
+            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+            int mx, my, mbc;
+            double xl, yl, dx, dy;
+            fclaw2d_clawpatch_grid_data(glob, pt, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+            if (fclaw2d_clawpatch_output_ascii_patch(fq, mx, my, o->meqn, mbc, xl, yl, dx, dy,
+                                                     glob->qhost + patch_size(glob, o->meqn) * (size_t) pt->slot,
+                                                     d->blocks[b].patchno_offset + p, pt->level, b, 0)) {
+                fprintf(stderr, "fclaw2d_clawpatch_output_ascii: cannot write %s\n", fq);
+                exit(1);
+            }
+        }
+}

@@ -142,6 +142,21 @@ double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double 
  * src/fclaw2d_timeinterp.c); needs glob->subcycle. */
 void fclaw2d_timeinterp(fclaw2d_global_t *glob, int level, double alpha);
 
+/* ---- output ----
+ * ASCII frames as the reference writes them: fort.tXXXX (time header) and
+ * fort.qXXXX (every patch), fclaw2d_clawpatch_output_ascii
+ * (src/patches/clawpatch/fclaw2d_clawpatch_output_ascii.c:112-160) with the
+ * formats of src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_output_ascii.f.
+ * The first downloads the device solution; the other two are its pieces
+ * (time header, then one patch appended), return 0 or -1 on a file error. */
+void fclaw2d_clawpatch_output_ascii(fclaw2d_global_t *glob, int iframe);
+int fclaw2d_clawpatch_time_header_ascii_files(const char *fname_q, const char *fname_t,
+                                              double time, int meqn, int maux, int ngrids);
+int fclaw2d_clawpatch_output_ascii_patch(const char *fname_q, int mx, int my, int meqn, int mbc,
+                                         double xlower, double ylower, double dx, double dy,
+                                         const double *q, int patch_num, int level, int blockno,
+                                         int mpirank);
+
 /* Conservation fix-up after all levels reached the same time
  * (src/fclaw2d_time_sync.c:161-202); needs glob->time_sync set before
  * fc2d_hipclaw_allocate_buffers. */
