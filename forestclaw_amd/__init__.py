@@ -30,7 +30,7 @@ class HipclawConfig(C.Structure):
                 ("my", C.c_int), ("mbc", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
                 ("mwaves", C.c_int), ("order", C.c_int * 2), ("mthlim", C.c_int * 4),
                 ("mthbc", C.c_int * 4), ("grav", C.c_double), ("gamma", C.c_double),
-                ("tile", C.c_int), ("time_sync", C.c_int)]
+                ("tile", C.c_int), ("time_sync", C.c_int), ("subcycle", C.c_int)]
 
 
 def _dp():

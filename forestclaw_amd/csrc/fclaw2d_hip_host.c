@@ -30,6 +30,16 @@ static void device_check(int rc, const char *what)
 
 /* ------------------------------------------------------------------ globals */
 
+void fclaw2d_hip_struct_sizes(size_t out[6])
+{
+    out[0] = sizeof(fclaw2d_global_t);
+    out[1] = sizeof(fclaw2d_domain_t);
+    out[2] = sizeof(fclaw2d_block_t);
+    out[3] = sizeof(fclaw2d_patch_t);
+    out[4] = sizeof(fc2d_hipclaw_options_t);
+    out[5] = sizeof(fc2d_hipclaw_vtable_t);
+}
+
 fclaw2d_global_t *fclaw2d_global_new(void)
 {
     return (fclaw2d_global_t *) calloc(1, sizeof(fclaw2d_global_t));

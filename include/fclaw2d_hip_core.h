@@ -20,6 +20,8 @@
 #ifndef FCLAW2D_HIP_CORE_H
 #define FCLAW2D_HIP_CORE_H
 
+#include <stddef.h>
+
 #include "fclaw2d_hip_forest.h"
 
 #ifdef __cplusplus
@@ -106,6 +108,11 @@ typedef struct fclaw2d_single_step_buffer_data {
     int iter;          /* index of the current patch, incremented by the caller */
     void *user;        /* owned by the callee: allocated at iter == 0, freed at the last */
 } fclaw2d_single_step_buffer_data_t;
+
+/* sizeof of fclaw2d_global_t, fclaw2d_domain_t, fclaw2d_block_t, fclaw2d_patch_t
+ * (out[0..3]) and of fc2d_hipclaw_options_t, fc2d_hipclaw_vtable_t (out[4..5]),
+ * for bindings to check their struct copies against this build. */
+void fclaw2d_hip_struct_sizes(size_t out[6]);
 
 /* ---- globals, domains ---- */
 fclaw2d_global_t *fclaw2d_global_new(void);

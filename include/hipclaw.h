@@ -71,6 +71,10 @@ typedef struct hipclaw_state hipclaw_state_t;
 /* Thread-local message for the last failing call. */
 const char *hipclaw_last_error(void);
 
+/* sizeof(hipclaw_config_t): lets a foreign-language binding check that its
+ * copy of the struct matches this build (no counterpart in the reference). */
+size_t hipclaw_config_size(void);
+
 /* Select the GPU and report its name.
  * Replaces fc2d_cudaclaw_initialize_GPUs
  * (src/solvers/fc2d_cudaclaw/cuda_source/cudaclaw_initialize.cu:37-74). */

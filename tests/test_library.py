@@ -67,3 +67,15 @@ def test_options_checks_follow_cudaclaw():
     assert L.fc2d_hipclaw_check(C.byref(opt)) == 0
     opt.buffer_len = 0
     assert L.fc2d_hipclaw_check(C.byref(opt)) == 1
+
+
+def test_struct_layouts_match_the_build():
+    """The ctypes copies of the C structs have the sizes the library was built with."""
+    import dropin_lib as D
+    L = fa.lib()
+    L.hipclaw_config_size.restype = C.c_size_t
+    assert C.sizeof(fa.HipclawConfig) == L.hipclaw_config_size()
+    sizes = (C.c_size_t * 6)()
+    L.fclaw2d_hip_struct_sizes(sizes)
+    assert list(sizes) == [C.sizeof(D.Global), C.sizeof(D.Domain), C.sizeof(D.Block),
+                           C.sizeof(D.Patch), C.sizeof(D.Options), C.sizeof(D.Vtable)]
