@@ -132,6 +132,8 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     int tile = cfg->tile > 0 ? cfg->tile : default_tile(cfg);
     if (!hipclaw::step_supported(solver, cfg->precision, tile))
         return fail(HIPCLAW_EINVAL, "tile size not available for this solver and precision%s");
+    if (cfg->time_sync && cfg->precision == HIPCLAW_FP64 && tile == 32)
+        return fail(HIPCLAW_EINVAL, "FP64 tiles of 32 leave no LDS for the time-sync edge buffers%s");
     if (cfg->mx % tile != 0 || cfg->my % tile != 0)
         return fail(HIPCLAW_EINVAL, "mx and my must be multiples of the tile size%s");
     if (cfg->order[0] < 1 || cfg->order[0] > 2 || cfg->order[1] < 0 || cfg->order[1] > 2)

@@ -8,8 +8,12 @@ namespace hipclaw {
 
 template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {
-    static int resident = 0;
-    return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE>, NW * 64, a, st, resident);
+    // kernels that keep time-sync registers are separate instantiations, so the
+    // plain update carries none of their code or register pressure
+    static int resident[2] = {0, 0};
+    if (a.reg)
+        return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE, true>, NW * 64, a, st, resident[1]);
+    return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE, false>, NW * 64, a, st, resident[0]);
 }
 
 template <class RP, bool WITH32 = true>

@@ -399,10 +399,17 @@ int launch_time_sync(const SyncArgs& a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- FP64 step
-template <class RP, int T, int NW>
+template <class RP, int T, int NW, bool WITH_REGS = true>
 static int launch_one(const StepArgs& a, hipStream_t st) {
-    static int resident = 0;
-    return launch_persistent(step_tiles<RP, double, T, NW, false>, NW * 64, a, st, resident);
+    // kernels that keep time-sync registers are separate instantiations, so the
+    // plain update carries none of their code or register pressure
+    static int resident[2] = {0, 0};
+    if (a.reg) {
+        if constexpr (WITH_REGS)
+            return launch_persistent(step_tiles<RP, double, T, NW, false, true>, NW * 64, a, st, resident[1]);
+        return (int) hipErrorInvalidValue;
+    }
+    return launch_persistent(step_tiles<RP, double, T, NW, false, false>, NW * 64, a, st, resident[0]);
 }
 
 template <class RP>
@@ -418,7 +425,8 @@ int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st) {
     switch (solver) {
     case 0: return launch_tile<RPAdvection<double>>(tile, a, st);
     case 1:
-        if (tile == 32) return launch_one<RPShallow<double>, 32, 16>(a, st);
+        // T = 32 fits the LDS only without the time-sync edge buffers
+        if (tile == 32) return launch_one<RPShallow<double>, 32, 16, false>(a, st);
         return launch_tile<RPShallow<double>>(tile, a, st);
     case 2: return launch_tile<RPEuler<double>>(tile, a, st);
     default: return (int) hipErrorInvalidValue;

@@ -126,7 +126,7 @@ __device__ __forceinline__ bool wave_nonzero(float wn) { return wn >= 1.17549435
 // LDS layout.  Rows that a wave walks across lanes in the transverse direction
 // get an odd stride so that the 64 lanes of a y sweep (or of the final stage)
 // fall into distinct banks.
-template <int T, int MEQN, int MAUX, typename R, bool FLUXFORM>
+template <int T, int MEQN, int MAUX, typename R, bool FLUXFORM, bool REGS = false>
 struct TileSmem {
     static constexpr int QW = T + 4;            // q tile width (cells i0-2 .. i0+T+1)
     static constexpr int QS = QW | 1;           // padded row stride
@@ -140,6 +140,8 @@ struct TileSmem {
     R ab[2 * MEQN * (T + 2) * GS];  // transverse corrections (x sweep, then y sweep)
     double fx[FLUXFORM ? MEQN * T * (T + 1) : 1];
     double fy[FLUXFORM ? MEQN * (T + 1) * T : 1];
+    // gm, gp at the patch's bottom and top edges, kept for the time-sync registers
+    R eg[(REGS && !FLUXFORM) ? 2 * 2 * MEQN * T : 1];
 };
 
 template <class RP, typename R, int T, bool FLUXFORM>
@@ -158,11 +160,11 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
     return kn && rt;
 }
 
-template <class RP, typename R, int T, int NW, bool FLUXFORM>
+template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS>
 __global__ void __launch_bounds__(NW * 64)
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
-    using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM>;
+    using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
     constexpr int QW = Smem::QW, QS = Smem::QS, QP = Smem::QP, GS = Smem::GS;
     constexpr int L = T + 3;               // lanes per row/column of interfaces
     constexpr int SEG = 64 / L;            // rows (columns) per wave
@@ -444,17 +446,16 @@ step_tiles(StepArgs a) {
                     if (act && own_col && k >= 1 && k <= T)
                         GT(m, k - 1, c - 1) = gm_up - gp[m];  // Dy(i, j)
                 }
-                // time-sync registers: gm, gp at the patch's bottom (j = 1) and
-                // top (j = my+1) edges (clawpack46_time_sync.f:201-209)
-                if (a.reg != nullptr && act && own_col &&
-                    ((k == 1 && first_y) || (k == T + 1 && last_y))) {
-                    double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
-                    const double dtdx_el = a.dt * a.dxs[slot];
-                    const int face = k == 1 ? 2 : 3, pos = i0 + c - 2;
+                // gm, gp at the tile's bottom (t = 0) and top (t = T) y-interfaces,
+                // kept in LDS for the time-sync registers of the patch edges
+                if constexpr (REGS) {
+                    if (act && own_col && (k == 1 || k == T + 1)) {
+                        const int side = k == 1 ? 0 : 1;
 #pragma unroll
-                    for (int m = 0; m < MEQN; m++) {
-                        reg_add(rg + ((face * 4 + REG_FM) * MEQN + m) * mx + pos, dtdx_el * (double) gm[m], a.reg_accumulate);
-                        reg_sub(rg + ((face * 4 + REG_FP) * MEQN + m) * mx + pos, dtdx_el * (double) gp[m], a.reg_accumulate);
+                        for (int m = 0; m < MEQN; m++) {
+                            sm.eg[((side * 2 + 0) * MEQN + m) * T + c - 1] = gm[m];
+                            sm.eg[((side * 2 + 1) * MEQN + m) * T + c - 1] = gp[m];
+                        }
                     }
                 }
             }
@@ -505,7 +506,7 @@ step_tiles(StepArgs a) {
                 else  // FP32 fluctuations, FP64 state: q never passes through FP32
                     qo[off] = (qc - dtdx64 * (double) Dx) - dtdy64 * (double) Dy;
             }
-            if (a.reg != nullptr) {
+            if constexpr (REGS) {
                 double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
                 const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
                 // fm, fp at the patch's left (i = 1) and right (i = mx+1) edges,
@@ -519,6 +520,18 @@ step_tiles(StepArgs a) {
                     const R fpv = ((R(0) + sm.fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
                     reg_add(rg + ((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fmv, a.reg_accumulate);
                     reg_sub(rg + ((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fpv, a.reg_accumulate);
+                }
+                // gm, gp at the patch's bottom (j = 1) and top (j = my+1) edges
+                // (clawpack46_time_sync.f:201-209)
+                for (int e = threadIdx.x; e < 2 * MEQN * T; e += NT) {
+                    const int side = e / (MEQN * T), rem = e - side * (MEQN * T);
+                    const int m = rem / T, c = rem - m * T;
+                    if (side == 0 ? !first_y : !last_y) continue;
+                    const int face = 2 + side, pos = i0 - 1 + c;
+                    reg_add(rg + ((face * 4 + REG_FM) * MEQN + m) * mx + pos,
+                            dtdx_el * (double) sm.eg[((side * 2 + 0) * MEQN + m) * T + c], a.reg_accumulate);
+                    reg_sub(rg + ((face * 4 + REG_FP) * MEQN + m) * mx + pos,
+                            dtdx_el * (double) sm.eg[((side * 2 + 1) * MEQN + m) * T + c], a.reg_accumulate);
                 }
                 // conservative flux of the pre-step state at the first interior
                 // and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
@@ -572,7 +585,7 @@ step_tiles(StepArgs a) {
                 }
             }
             __syncthreads();
-            if (a.reg != nullptr) {
+            if constexpr (REGS) {
                 // the shared FP64 flux through each patch edge, times dt * edge length
                 double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
                 const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
