@@ -160,8 +160,10 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
     return kn && rt;
 }
 
+// At least 4 waves per SIMD: the FP64 kernels that keep time-sync registers
+// would otherwise settle at 3 (measured faster at 4 despite a few spills).
 template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS>
-__global__ void __launch_bounds__(NW * 64)
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
     using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
@@ -270,6 +272,35 @@ step_tiles(StepArgs a) {
                 if (laoff[kk] >= 0) sm.aux[laoff[kk]] = pa[kk];
         }
         __syncthreads();
+        if constexpr (REGS && !FLUXFORM) {
+            double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
+            const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+            // time-sync registers: conservative flux of the pre-step state at the first
+            // interior and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
+            for (int e = threadIdx.x; e < 8 * T; e += NT) {
+                const int face = e / (2 * T), rem = e - face * (2 * T);
+                const int kio = rem / T, c = rem - kio * T;
+                bool on;
+                int qi, qj;
+                if (face == 0) { on = first_x; qi = 2 - kio; qj = c + 2; }
+                else if (face == 1) { on = last_x; qi = T + 1 + kio; qj = c + 2; }
+                else if (face == 2) { on = first_y; qi = c + 2; qj = 2 - kio; }
+                else { on = last_y; qi = c + 2; qj = T + 1 + kio; }
+                if (!on) continue;
+                double qcell[MEQN], acell[MAUX > 0 ? MAUX : 1], fl[MEQN];
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
+#pragma unroll
+                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) acell[m] = (double) SA(m, qi, qj);
+                if (face < 2) RP::template consflux<1>(qcell, acell, a.pp, fl);
+                else RP::template consflux<2>(qcell, acell, a.pp, fl);
+                const double el = face < 2 ? dtdy_el : dtdx_el;
+                const int pos = (face < 2 ? j0 : i0) - 1 + c;
+#pragma unroll
+                for (int m = 0; m < MEQN; m++)
+                    reg_add(rg + ((face * 4 + REG_EFIN + kio) * MEQN + m) * mx + pos, el * fl[m], a.reg_accumulate);
+            }
+        }
         // the next tile's data travels while this one is computed
         if (tile + G < ntiles) prefetch(tile + G);
 
@@ -532,31 +563,6 @@ step_tiles(StepArgs a) {
                             dtdx_el * (double) sm.eg[((side * 2 + 0) * MEQN + m) * T + c], a.reg_accumulate);
                     reg_sub(rg + ((face * 4 + REG_FP) * MEQN + m) * mx + pos,
                             dtdx_el * (double) sm.eg[((side * 2 + 1) * MEQN + m) * T + c], a.reg_accumulate);
-                }
-                // conservative flux of the pre-step state at the first interior
-                // and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
-                for (int e = threadIdx.x; e < 8 * T; e += NT) {
-                    const int face = e / (2 * T), rem = e - face * (2 * T);
-                    const int kio = rem / T, c = rem - kio * T;
-                    bool on;
-                    int qi, qj;
-                    if (face == 0) { on = first_x; qi = 2 - kio; qj = c + 2; }
-                    else if (face == 1) { on = last_x; qi = T + 1 + kio; qj = c + 2; }
-                    else if (face == 2) { on = first_y; qi = c + 2; qj = 2 - kio; }
-                    else { on = last_y; qi = c + 2; qj = T + 1 + kio; }
-                    if (!on) continue;
-                    double qcell[MEQN], acell[MAUX > 0 ? MAUX : 1], fl[MEQN];
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
-#pragma unroll
-                    for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) acell[m] = (double) SA(m, qi, qj);
-                    if (face < 2) RP::template consflux<1>(qcell, acell, a.pp, fl);
-                    else RP::template consflux<2>(qcell, acell, a.pp, fl);
-                    const double el = face < 2 ? dtdy_el : dtdx_el;
-                    const int pos = (face < 2 ? j0 : i0) - 1 + c;
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++)
-                        reg_add(rg + ((face * 4 + REG_EFIN + kio) * MEQN + m) * mx + pos, el * fl[m], a.reg_accumulate);
                 }
             }
         } else {
