@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "libforestclaw_amd.so")
 
 RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
 FP64, FP32_BULK = 0, 1
+# provenance of a leaf after a forest adapt (fclaw2d_hip_forest_provenance)
+REGRID_SAME, REGRID_CHILD, REGRID_PARENT = 0, 1, 2
 SOLVER_SHAPES = {RP_ADVECTION: (1, 1, 2), RP_SHALLOW: (3, 3, 0), RP_EULER: (4, 4, 0)}
 
 _lib = None
@@ -84,6 +86,10 @@ def lib():
         "fclaw2d_hip_forest_ghost_program": ([vp, i, i, i, _ip(), vp, i, _ip()], i),
         "fclaw2d_hip_forest_face_relation": ([vp, i, i, _ip()], i),
         "fclaw2d_hip_forest_sync_program": ([vp, i, i, vp, i], i),
+        "fclaw2d_hip_forest_adapt": ([vp, _ip(), _ip(), i, i], i),
+        "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
+        "hipclaw_tag_patches": ([vp, i, d, d, _ip(), _ip()], i),
+        "hipclaw_regrid": ([vp, i, _ip(), _ip(), _dp(), _dp()], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -172,6 +178,21 @@ class Forest:
         assert n2 == n
         return ops[:n], ps
 
+    def adapt(self, refine, coarsen, minlevel, maxlevel):
+        """One regrid of the leaves; returns (kind, src) provenance of the new
+        leaves (see fclaw2d_hip_forest_adapt)."""
+        r = np.ascontiguousarray(refine, dtype=np.int32)
+        c = np.ascontiguousarray(coarsen, dtype=np.int32)
+        assert len(r) == self.npatch and len(c) == self.npatch
+        n = lib().fclaw2d_hip_forest_adapt(self._h, r, c, minlevel, maxlevel)
+        if n < 0:
+            raise RuntimeError("forest adapt failed")
+        kind = np.zeros(n, np.int32)
+        src = np.zeros(4 * n, np.int32)
+        if lib().fclaw2d_hip_forest_provenance(self._h, kind, src) != n:
+            raise RuntimeError("forest provenance failed")
+        return kind, src.reshape(n, 4)
+
     def sync_program(self, mx, my):
         """Time-synchronisation ops (hipclaw_set_sync_program)."""
         n = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, None, 0)
@@ -252,6 +273,27 @@ class Solver:
     def time_sync_now(self):
         """Apply the conservation fix-up from the registers of the last update."""
         _check(lib().hipclaw_time_sync(self._h), "hipclaw_time_sync")
+
+    def tag(self, ivar, refine_threshold, coarsen_threshold):
+        """Per patch: (exceeds refine threshold, exceeds coarsen threshold)."""
+        er = np.zeros(self.npatch, np.int32)
+        ec = np.zeros(self.npatch, np.int32)
+        _check(lib().hipclaw_tag_patches(self._h, ivar, refine_threshold, coarsen_threshold, er, ec),
+               "hipclaw_tag_patches")
+        return er, ec
+
+    def regrid(self, forest, kind, src, extent):
+        """Move the solution onto the adapted `forest` (provenance kind, src),
+        then install its geometry and programs (aux must be set again)."""
+        _, _, dx, dy = forest.geometry(extent, self.mx, self.my)
+        kind = np.ascontiguousarray(kind, dtype=np.int32)
+        src = np.ascontiguousarray(src, dtype=np.int32).reshape(-1)
+        _check(lib().hipclaw_regrid(self._h, len(kind), kind, src,
+                                    np.ascontiguousarray(dx, dtype=np.float64),
+                                    np.ascontiguousarray(dy, dtype=np.float64)), "hipclaw_regrid")
+        self.npatch = len(kind)
+        self.max_patches = max(self.max_patches, self.npatch)
+        self.set_forest(forest, extent)
 
     def set_geometry(self, dx, dy):
         dx = np.ascontiguousarray(dx, dtype=np.float64)

@@ -21,6 +21,8 @@ struct fclaw2d_hip_forest {
     int nbx, nby, per_x, per_y;
     int n, cap;
     int *level, *gx, *gy;
+    /* provenance of the leaves after the last adapt: kind and 4 sources each */
+    int *pkind, *psrc, pn;
     /* open-addressing map (level, gx, gy) -> leaf */
     size_t hcap;
     uint64_t *hkey;
@@ -279,6 +281,7 @@ void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
 {
     if (!f) return;
     free(f->level); free(f->gx); free(f->gy); free(f->hkey); free(f->hval);
+    free(f->pkind); free(f->psrc);
     free(f);
 }
 
@@ -310,12 +313,131 @@ int fclaw2d_hip_forest_refine(fclaw2d_hip_forest_t *f, const int *flags, int max
     char *mark;
     int i, rc;
     if (!f || !flags) return -1;
+    f->pn = -1;  /* provenance describes adapt only */
     mark = (char *) calloc((size_t) f->n, 1);
     if (!mark) return -1;
     for (i = 0; i < f->n; i++) mark[i] = flags[i] != 0 && f->level[i] < maxlevel;
     rc = split_marked(f, mark);
     free(mark);
     if (rc || balance(f)) return -1;
+    return f->n;
+}
+
+/* ---------------------------------------------------------------- adapt */
+
+/* Can the four children of level-(L-1) patch (X, Y) be replaced by their
+ * parent without breaking the 2:1 face and corner balance?  Every level-L
+ * cell in the ring around the parent must be covered by a leaf of level <= L. */
+static int coarsen_keeps_balance(const fclaw2d_hip_forest_t *f, int L, int64_t X, int64_t Y)
+{
+    int64_t x, y;
+    for (y = 2 * Y - 1; y <= 2 * Y + 2; y++)
+        for (x = 2 * X - 1; x <= 2 * X + 2; x++) {
+            int64_t wx = x, wy = y;
+            if (x >= 2 * X && x <= 2 * X + 1 && y >= 2 * Y && y <= 2 * Y + 1) continue;
+            if (!wrap(f, L, &wx, &wy)) continue;          /* outside the domain */
+            if (covering(f, L, wx, wy) < 0) return 0;     /* refined beyond L */
+        }
+    return 1;
+}
+
+int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const int *coarsen,
+                             int minlevel, int maxlevel)
+{
+    fclaw2d_hip_forest_t *old;
+    char *mark;
+    int i, g, n0, ncoarse = 0;
+    int *fam = NULL;   /* parent (level, X, Y) of families to coarsen, 3 ints each */
+    if (!f || !refine || !coarsen) return -1;
+    n0 = f->n;
+    /* keep the old leaves for the provenance */
+    old = alloc_forest(f->nbx, f->nby, f->per_x, f->per_y);
+    if (!old || reserve(old, n0)) { fclaw2d_hip_forest_destroy(old); return -1; }
+    memcpy(old->level, f->level, (size_t) n0 * sizeof(int));
+    memcpy(old->gx, f->gx, (size_t) n0 * sizeof(int));
+    memcpy(old->gy, f->gy, (size_t) n0 * sizeof(int));
+    old->n = n0;
+    if (rehash(old)) { fclaw2d_hip_forest_destroy(old); return -1; }
+    /* families to coarsen: all four siblings flagged, none flagged to refine
+     * (cb_regrid_tag4coarsening, src/fclaw2d_regrid.c:61-93) */
+    fam = (int *) malloc(sizeof(int) * 3 * (size_t) (n0 / 4 + 1));
+    if (!fam) { fclaw2d_hip_forest_destroy(old); return -1; }
+    for (i = 0; i < n0; i++) {
+        const int L = f->level[i], x = f->gx[i], y = f->gy[i];
+        int sib[4], ok = L > minlevel && L > 0 && !(x & 1) && !(y & 1);
+        for (g = 0; g < 4 && ok; g++) {
+            sib[g] = lookup(old, L, x + (g & 1), y + (g >> 1));
+            ok = sib[g] >= 0 && coarsen[sib[g]] && !(refine[sib[g]] && L < maxlevel);
+        }
+        if (ok) { fam[3 * ncoarse] = L; fam[3 * ncoarse + 1] = x >> 1; fam[3 * ncoarse + 2] = y >> 1; ncoarse++; }
+    }
+    /* refine (cb_fclaw2d_regrid_tag4refinement) and restore the balance */
+    mark = (char *) calloc((size_t) n0 + 1, 1);
+    if (!mark) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+    for (i = 0; i < n0; i++) mark[i] = refine[i] != 0 && f->level[i] < maxlevel;
+    if (split_marked(f, mark) || balance(f)) { free(mark); free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+    free(mark);
+    /* coarsen the families that are still intact and keep the balance */
+    if (ncoarse > 0) {
+        int k, w;
+        char *drop = (char *) calloc((size_t) f->n + 1, 1);
+        int nadd = 0;
+        if (!drop) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+        for (k = 0; k < ncoarse; k++) {
+            const int L = fam[3 * k], X = fam[3 * k + 1], Y = fam[3 * k + 2];
+            int kids[4], ok = 1;
+            for (g = 0; g < 4 && ok; g++) {
+                kids[g] = lookup(f, L, 2 * X + (g & 1), 2 * Y + (g >> 1));
+                ok = kids[g] >= 0;
+            }
+            if (!ok || !coarsen_keeps_balance(f, L, X, Y)) { fam[3 * k] = -1; continue; }
+            for (g = 0; g < 4; g++) drop[kids[g]] = 1;
+            nadd++;
+        }
+        for (i = 0, w = 0; i < f->n; i++)
+            if (!drop[i]) { f->level[w] = f->level[i]; f->gx[w] = f->gx[i]; f->gy[w] = f->gy[i]; w++; }
+        f->n = w;
+        free(drop);
+        if (reserve(f, f->n + nadd)) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+        for (k = 0; k < ncoarse; k++)
+            if (fam[3 * k] >= 0) {
+                f->level[f->n] = fam[3 * k] - 1; f->gx[f->n] = fam[3 * k + 1]; f->gy[f->n] = fam[3 * k + 2];
+                f->n++;
+            }
+        if (sort_leaves(f)) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+    }
+    free(fam);
+    /* provenance of every new leaf (cb_fclaw2d_regrid_repopulate, src/fclaw2d_regrid.c:95-230) */
+    free(f->pkind); free(f->psrc);
+    f->pkind = (int *) malloc(sizeof(int) * (size_t) f->n);
+    f->psrc = (int *) malloc(sizeof(int) * 4 * (size_t) f->n);
+    if (!f->pkind || !f->psrc) { fclaw2d_hip_forest_destroy(old); return -1; }
+    f->pn = f->n;
+    for (i = 0; i < f->n; i++) {
+        const int L = f->level[i], x = f->gx[i], y = f->gy[i];
+        int q = lookup(old, L, x, y);
+        int *src = f->psrc + 4 * i;
+        src[0] = src[1] = src[2] = src[3] = -1;
+        if (q >= 0) { f->pkind[i] = FCLAW2D_HIP_REGRID_SAME; src[0] = q; continue; }
+        if (L > 0 && (q = lookup(old, L - 1, x >> 1, y >> 1)) >= 0) {
+            f->pkind[i] = FCLAW2D_HIP_REGRID_CHILD; src[0] = q; src[1] = (x & 1) + 2 * (y & 1);
+            continue;
+        }
+        f->pkind[i] = FCLAW2D_HIP_REGRID_PARENT;
+        for (g = 0; g < 4; g++) {
+            src[g] = lookup(old, L + 1, 2 * x + (g & 1), 2 * y + (g >> 1));
+            if (src[g] < 0) { fclaw2d_hip_forest_destroy(old); return -1; }
+        }
+    }
+    fclaw2d_hip_forest_destroy(old);
+    return f->n;
+}
+
+int fclaw2d_hip_forest_provenance(const fclaw2d_hip_forest_t *f, int *kind, int *src)
+{
+    if (!f || !f->pkind || f->pn != f->n) return -1;
+    memcpy(kind, f->pkind, sizeof(int) * (size_t) f->n);
+    memcpy(src, f->psrc, sizeof(int) * 4 * (size_t) f->n);
     return f->n;
 }
 

@@ -494,46 +494,13 @@ void fc2d_hipclaw_initialize_GPUs(fclaw2d_global_t *glob)
     device_check(hipclaw_initialize_device(0, name, sizeof(name)), "hipclaw_initialize_device");
 }
 
-void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
+/* Ghost-fill and time-sync programs of the current domain. */
+static void install_programs(fclaw2d_global_t *glob)
 {
     const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
-    const fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
     fclaw2d_domain_t *d = glob->domain;
     const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
-    hipclaw_config_t cfg;
-    double *dx, *dy;
-    int *ops, phase[7], nops, b, p, k;
-    if (!opt || !d) {
-        fprintf(stderr, "fc2d_hipclaw_allocate_buffers: options or domain missing\n");
-        exit(1);
-    }
-    if (fc2d_hipclaw_check(opt)) exit(1);
-    fc2d_hipclaw_deallocate_buffers(glob);
-    memset(&cfg, 0, sizeof(cfg));
-    cfg.solver = vt->riemann_solver;
-    cfg.precision = opt->precision;
-    cfg.mx = o->mx; cfg.my = o->my; cfg.mbc = o->mbc;
-    cfg.meqn = o->meqn; cfg.maux = o->maux; cfg.mwaves = opt->mwaves;
-    cfg.order[0] = opt->order[0]; cfg.order[1] = opt->order[1];
-    for (k = 0; k < 4; k++) {
-        cfg.mthlim[k] = k < opt->mwaves ? opt->mthlim[k] : 0;
-        cfg.mthbc[k] = opt->mthbc[k];
-    }
-    cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
-    cfg.time_sync = glob->time_sync;
-    cfg.subcycle = glob->subcycle;
-    device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");
-    dx = (double *) malloc(sizeof(double) * d->local_num_patches);
-    dy = (double *) malloc(sizeof(double) * d->local_num_patches);
-    for (b = 0; b < d->num_blocks; b++)
-        for (p = 0; p < d->blocks[b].num_patches; p++) {
-            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
-            int mx, my, mbc;
-            double xl, yl;
-            fclaw2d_clawpatch_grid_data(glob, pt, &mx, &my, &mbc, &xl, &yl, &dx[pt->slot], &dy[pt->slot]);
-        }
-    device_check(hipclaw_set_geometry(glob->state, d->local_num_patches, dx, dy), "hipclaw_set_geometry");
-    free(dx); free(dy);
+    int *ops, phase[7], nops;
     nops = fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, NULL, 0, phase);
     ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
     fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, nops, phase);
@@ -578,9 +545,71 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
             free(res);
         }
     }
+}
+
+/* Host views of q and aux for every patch (zeroed). */
+static void host_views(fclaw2d_global_t *glob)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    free(glob->qhost);
+    free(glob->auxhost);
+    glob->auxhost = NULL;
     glob->qhost = (double *) calloc(patch_size(glob, o->meqn) * d->local_num_patches, sizeof(double));
     if (o->maux > 0)
         glob->auxhost = (double *) calloc(patch_size(glob, o->maux) * d->local_num_patches, sizeof(double));
+}
+
+/* Mesh widths of every patch, by slot. */
+static void patch_widths(fclaw2d_global_t *glob, double *dx, double *dy)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    int b, p;
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++) {
+            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+            int mx, my, mbc;
+            double xl, yl;
+            fclaw2d_clawpatch_grid_data(glob, pt, &mx, &my, &mbc, &xl, &yl, &dx[pt->slot], &dy[pt->slot]);
+        }
+}
+
+void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
+{
+    const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
+    const fc2d_hipclaw_vtable_t *vt = fc2d_hipclaw_vt(glob);
+    fclaw2d_domain_t *d = glob->domain;
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    hipclaw_config_t cfg;
+    double *dx, *dy;
+    int k;
+    if (!opt || !d) {
+        fprintf(stderr, "fc2d_hipclaw_allocate_buffers: options or domain missing\n");
+        exit(1);
+    }
+    if (fc2d_hipclaw_check(opt)) exit(1);
+    fc2d_hipclaw_deallocate_buffers(glob);
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.solver = vt->riemann_solver;
+    cfg.precision = opt->precision;
+    cfg.mx = o->mx; cfg.my = o->my; cfg.mbc = o->mbc;
+    cfg.meqn = o->meqn; cfg.maux = o->maux; cfg.mwaves = opt->mwaves;
+    cfg.order[0] = opt->order[0]; cfg.order[1] = opt->order[1];
+    for (k = 0; k < 4; k++) {
+        cfg.mthlim[k] = k < opt->mwaves ? opt->mthlim[k] : 0;
+        cfg.mthbc[k] = opt->mthbc[k];
+    }
+    cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
+    cfg.time_sync = glob->time_sync;
+    cfg.subcycle = glob->subcycle;
+    device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");
+    dx = (double *) malloc(sizeof(double) * d->local_num_patches);
+    dy = (double *) malloc(sizeof(double) * d->local_num_patches);
+    patch_widths(glob, dx, dy);
+    device_check(hipclaw_set_geometry(glob->state, d->local_num_patches, dx, dy), "hipclaw_set_geometry");
+    free(dx); free(dy);
+    install_programs(glob);
+    host_views(glob);
 }
 
 void fc2d_hipclaw_deallocate_buffers(fclaw2d_global_t *glob)
@@ -815,4 +844,139 @@ void fclaw2d_clawpatch_output_ascii(fclaw2d_global_t *glob, int iframe)
                 exit(1);
             }
         }
+}
+
+/* ------------------------------------------------------------------ regrid */
+
+/* Replace the domain by one over the (adapted) leaves of its forest. */
+static void rebuild_domain(fclaw2d_global_t *glob)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    fclaw2d_domain_t *nd = fclaw2d_domain_new_from_forest(d->forest, d->nbx, d->nby, d->periodic_x,
+                                                          d->periodic_y, d->ax, d->bx, d->ay, d->by);
+    if (!nd) {
+        fprintf(stderr, "fclaw2d_regrid: out of memory\n");
+        exit(1);
+    }
+    d->forest = NULL;   /* now owned by the new domain */
+    fclaw2d_domain_destroy(d);
+    glob->domain = nd;
+}
+
+/* Tag every patch on the device; refine[]/coarsen[] by leaf (= slot).
+ * Coarsening is never asked for while the mesh is being initialized
+ * (fclaw2d_clawpatch46_tag4coarsening.f: initflag). */
+static void tag_leaves(fclaw2d_global_t *glob, int domain_init, int *refine, int *coarsen)
+{
+    const int n = glob->domain->local_num_patches;
+    int p, *ec = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    if (glob->threshold_variable < 1 || glob->threshold_variable > glob->clawpatch_opt.meqn) {
+        fprintf(stderr, "fclaw2d_regrid: threshold_variable %d is not in 1..meqn\n",
+                glob->threshold_variable);
+        exit(1);
+    }
+    device_check(hipclaw_tag_patches(glob->state, glob->threshold_variable - 1,
+                                     glob->refine_threshold, glob->coarsen_threshold, refine, ec),
+                 "hipclaw_tag_patches");
+    for (p = 0; p < n; p++) coarsen[p] = !domain_init && !ec[p];
+    free(ec);
+}
+
+/* Adapt the forest; returns 1 if the leaves changed.  kind/src receive the
+ * provenance (caller frees). */
+static int adapt_forest(fclaw2d_global_t *glob, const int *refine, const int *coarsen, int **kind,
+                        int **src)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    const int n0 = d->local_num_patches;
+    int p, changed, n = fclaw2d_hip_forest_adapt(d->forest, refine, coarsen, glob->minlevel,
+                                                 glob->maxlevel);
+    if (n <= 0) {
+        fprintf(stderr, "fclaw2d_regrid: forest adapt failed\n");
+        exit(1);
+    }
+    *kind = (int *) malloc(sizeof(int) * (size_t) n);
+    *src = (int *) malloc(sizeof(int) * 4 * (size_t) n);
+    if (fclaw2d_hip_forest_provenance(d->forest, *kind, *src) != n) {
+        fprintf(stderr, "fclaw2d_regrid: forest provenance failed\n");
+        exit(1);
+    }
+    changed = n != n0;
+    for (p = 0; p < n && !changed; p++)
+        changed = (*kind)[p] != FCLAW2D_HIP_REGRID_SAME || (*src)[4 * p] != p;
+    return changed;
+}
+
+/* setaux (patch setup) then qinit on every patch's host view, uploaded. */
+static void initialize_patches(fclaw2d_global_t *glob, int with_q)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    int b, p;
+    for (b = 0; b < d->num_blocks; b++)
+        for (p = 0; p < d->blocks[b].num_patches; p++) {
+            fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+            if (glob->patch_vt.setup) glob->patch_vt.setup(glob, pt, b, p);
+            if (with_q && glob->patch_vt.initialize) glob->patch_vt.initialize(glob, pt, b, p);
+        }
+    if (with_q)
+        device_check(hipclaw_set_q(glob->state, glob->qhost, 0, d->local_num_patches), "hipclaw_set_q");
+    if (glob->auxhost)
+        device_check(hipclaw_set_aux(glob->state, glob->auxhost, 0, d->local_num_patches),
+                     "hipclaw_set_aux");
+}
+
+void fclaw2d_initialize(fclaw2d_global_t *glob)
+{
+    int level;
+    fc2d_hipclaw_allocate_buffers(glob);
+    initialize_patches(glob, 1);
+    fclaw2d_ghost_update(glob, glob->domain->global_minlevel, glob->domain->global_maxlevel, 0.0, 0, 0);
+    for (level = glob->minlevel; level < glob->maxlevel; level++) {
+        const int n = glob->domain->local_num_patches;
+        int *refine = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+        int *coarsen = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+        int *kind, *src, changed;
+        tag_leaves(glob, 1, refine, coarsen);
+        changed = adapt_forest(glob, refine, coarsen, &kind, &src);
+        free(refine); free(coarsen); free(kind); free(src);
+        if (!changed) break;   /* no new initial refinement */
+        rebuild_domain(glob);
+        /* new patches are initialized, not interpolated (domain_init) */
+        fc2d_hipclaw_allocate_buffers(glob);
+        initialize_patches(glob, 1);
+        fclaw2d_ghost_update(glob, glob->domain->global_minlevel, glob->domain->global_maxlevel,
+                             0.0, 0, 0);
+    }
+    glob->curr_time = 0.0;
+}
+
+void fclaw2d_regrid(fclaw2d_global_t *glob)
+{
+    const int n = glob->domain->local_num_patches;
+    int *refine = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    int *coarsen = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    int *kind, *src, changed;
+    tag_leaves(glob, 0, refine, coarsen);
+    changed = adapt_forest(glob, refine, coarsen, &kind, &src);
+    free(refine); free(coarsen);
+    if (changed) {
+        fclaw2d_domain_t *d;
+        double *dx, *dy;
+        rebuild_domain(glob);
+        d = glob->domain;
+        dx = (double *) malloc(sizeof(double) * (size_t) d->local_num_patches);
+        dy = (double *) malloc(sizeof(double) * (size_t) d->local_num_patches);
+        patch_widths(glob, dx, dy);
+        device_check(hipclaw_regrid(glob->state, d->local_num_patches, kind, src, dx, dy),
+                     "hipclaw_regrid");
+        free(dx); free(dy);
+        install_programs(glob);
+        host_views(glob);
+        initialize_patches(glob, 0);
+        /* every level is at the same time when the mesh changes */
+        fclaw2d_ghost_update(glob, d->global_minlevel, d->global_maxlevel, glob->curr_time, 0, 0);
+        glob->count_amr_new_domain++;
+    }
+    free(kind); free(src);
+    glob->count_amr_regrid++;
 }

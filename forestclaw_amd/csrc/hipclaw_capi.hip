@@ -712,4 +712,131 @@ int hipclaw_mass(hipclaw_state_t* s, double* sum_hi, double* sum_lo) {
     return HIPCLAW_OK;
 }
 
+int hipclaw_tag_patches(hipclaw_state_t* s, int ivar, double refine_threshold,
+                        double coarsen_threshold, int* exceeds_refine, int* exceeds_coarsen) {
+    if (!s || ivar < 0 || ivar >= s->cfg.meqn || !exceeds_refine || !exceeds_coarsen)
+        return fail(HIPCLAW_EINVAL, "hipclaw_tag_patches: bad arguments%s");
+    const int n = s->npatch;
+    if (n == 0) return HIPCLAW_OK;
+    int* dev = nullptr;
+    HIP_TRY(hipMalloc((void**) &dev, sizeof(int) * 2 * (size_t) n));
+    int e = hipclaw::launch_tag_patches(s->q[s->cur], s->all_slots, n, s->dxs, s->dys, s->pstride,
+                                        s->cfg.mx, s->cfg.my, s->cfg.mbc, ivar, refine_threshold,
+                                        coarsen_threshold, dev, s->st);
+    std::vector<int> out(2 * (size_t) n);
+    hipError_t he = e ? (hipError_t) e
+                      : hipMemcpyAsync(out.data(), dev, sizeof(int) * 2 * (size_t) n, hipMemcpyDeviceToHost, s->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(s->st);
+    hipFree(dev);
+    if (he != hipSuccess) return hipfail(he, "tag_patches");
+    for (int p = 0; p < n; p++) {
+        exceeds_refine[p] = out[2 * p];
+        exceeds_coarsen[p] = out[2 * p + 1];
+    }
+    return HIPCLAW_OK;
+}
+
+// Grow every per-patch device buffer to `cap` patches.  q is replaced by the
+// caller; everything else starts at zero (registers, saved and interpolated
+// states, aux) or is rebuilt (slot list).
+static int grow_patch_buffers(hipclaw_state_t* s, int cap) {
+    const size_t P = (size_t) cap;
+    auto realloc0 = [&](double*& p, size_t elems) -> int {
+        hipFree(p);
+        p = nullptr;
+        if (hipMalloc((void**) &p, sizeof(double) * elems) != hipSuccess) return HIPCLAW_ENOMEM;
+        return hipMemset(p, 0, sizeof(double) * elems) == hipSuccess ? HIPCLAW_OK : HIPCLAW_EDEVICE;
+    };
+    int rc = HIPCLAW_OK;
+    if (s->cfg.maux > 0 && !rc) rc = realloc0(s->aux, s->astride * P);
+    if (!rc) rc = realloc0(s->dxs, P);
+    if (!rc) rc = realloc0(s->dys, P);
+    if (!rc) rc = realloc0(s->mass_buf, 2 * (size_t) s->cfg.meqn * P);
+    if (!rc && s->qlast) rc = realloc0(s->qlast, s->pstride * P);
+    if (!rc && s->qti) rc = realloc0(s->qti, s->pstride * P);
+    if (!rc && s->reg) rc = realloc0(s->reg, s->rstride * P);
+    if (rc) return fail(rc, "hipclaw_regrid: device allocation failed%s");
+    hipFree(s->all_slots);
+    hipFree(s->batch_slots);
+    s->all_slots = s->batch_slots = nullptr;
+    if (hipMalloc((void**) &s->all_slots, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void**) &s->batch_slots, sizeof(int) * P) != hipSuccess)
+        return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
+    std::vector<int> iota(P);
+    for (int p = 0; p < cap; p++) iota[p] = p;
+    HIP_TRY(hipMemcpy(s->all_slots, iota.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    s->max_patches = cap;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const int* src,
+                   const double* dx, const double* dy) {
+    if (!s || npatch_new <= 0 || !kind || !src || !dx || !dy)
+        return fail(HIPCLAW_EINVAL, "hipclaw_regrid: bad arguments%s");
+    const int nold = s->npatch;
+    std::vector<int> ops(8 * (size_t) npatch_new, 0);
+    for (int p = 0; p < npatch_new; p++) {
+        const int k = kind[p];
+        const int* sp = src + 4 * (size_t) p;
+        int* op = ops.data() + 8 * (size_t) p;
+        auto ok = [&](int v) { return v >= 0 && v < nold; };
+        if (!(dx[p] > 0.0) || !(dy[p] > 0.0)) return fail(HIPCLAW_EINVAL, "non-positive mesh width%s");
+        if (k == hipclaw::REGRID_SAME ? !ok(sp[0])
+            : k == hipclaw::REGRID_CHILD ? !(ok(sp[0]) && sp[1] >= 0 && sp[1] <= 3)
+            : k == hipclaw::REGRID_PARENT ? !(ok(sp[0]) && ok(sp[1]) && ok(sp[2]) && ok(sp[3]))
+            : true)
+            return fail(HIPCLAW_EINVAL, "regrid source with bad kind or slot%s");
+        op[0] = k;
+        op[1] = p;
+        for (int g = 0; g < 4; g++) op[2 + g] = sp[g];
+    }
+    const int cap = std::max(s->max_patches, npatch_new);
+    double* qnew = nullptr;
+    int* dops = nullptr;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    if (hipMalloc((void**) &qnew, sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
+        return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
+    if (hipMalloc((void**) &dops, sizeof(int) * ops.size()) != hipSuccess) {
+        hipFree(qnew);
+        return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
+    }
+    hipError_t he = hipMemsetAsync(qnew, 0, sizeof(double) * s->pstride * (size_t) cap, s->st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dops, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, s->st);
+    if (he == hipSuccess)
+        he = (hipError_t) hipclaw::launch_regrid_remap(s->q[s->cur], qnew, dops, npatch_new, s->pstride,
+                                                       s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn, s->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(s->st);
+    hipFree(dops);
+    if (he != hipSuccess) {
+        hipFree(qnew);
+        return hipfail(he, "regrid_remap");
+    }
+    hipFree(s->q[0]);
+    hipFree(s->q[1]);
+    s->q[0] = qnew;
+    s->q[1] = nullptr;
+    s->cur = 0;
+    if (!s->inplace) {
+        if (hipMalloc((void**) &s->q[1], sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
+        HIP_TRY(hipMemset(s->q[1], 0, sizeof(double) * s->pstride * (size_t) cap));
+    }
+    if (cap > s->max_patches) {
+        if (int rc = grow_patch_buffers(s, cap)) return rc;
+    } else if (s->reg) {
+        HIP_TRY(hipMemset(s->reg, 0, sizeof(double) * s->rstride * (size_t) s->max_patches));
+    }
+    // every program names old slots; the graph names old buffers
+    for (GhostProgram& g : s->gp) {
+        g.nops = 0;
+        for (int& v : g.phase_start) v = 0;
+        for (bool& b : g.phase_physbc) b = false;
+    }
+    for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+    return hipclaw_set_geometry(s, npatch_new, dx, dy);
+}
+
 }  // extern "C"

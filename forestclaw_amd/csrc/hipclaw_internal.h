@@ -59,7 +59,15 @@ struct SyncArgs {
     int fluxreg;         // registers hold one shared edge flux (FP32 flux form)
 };
 
+// Regrid op kinds (the forest's provenance kinds).
+enum { REGRID_SAME = 0, REGRID_CHILD = 1, REGRID_PARENT = 2 };
+
 // Launchers (return hipError_t as int).
+int launch_tag_patches(const double* q, const int* slots, int n, const double* dxs,
+                       const double* dys, long long pstride, int mx, int my, int mbc, int ivar,
+                       double thr_refine, double thr_coarsen, int* out, hipStream_t st);
+int launch_regrid_remap(const double* qold, double* qnew, const int* ops, int nops,
+                        long long pstride, int mx, int my, int mbc, int meqn, hipStream_t st);
 int launch_time_sync(const SyncArgs& a, hipStream_t st);
 int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st);
 int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);

@@ -171,6 +171,98 @@ __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
 
 __global__ void reset_scalar(unsigned long long* p) { *p = 0ull; }
 
+// ---------------------------------------------------------------- regrid
+// Gradient criterion of the reference's tag4refinement / tag4coarsening
+// (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_tag4refinement.f,
+// src/patches/clawpatch/fort_user/fclaw2d_clawpatch_gradient_exceeds_th.f90,
+// unmapped): does some interior cell have |(dq/dx, dq/dy, 0)| above the
+// threshold?  out[2w] for the refine threshold, out[2w+1] for the coarsen
+// threshold (a negative refine threshold or a coarsen threshold <= 0 always
+// counts as exceeded: always refine, never coarsen, as
+// src/patches/clawpatch/fclaw2d_clawpatch.cpp:813-817, 890-911 decide).
+__global__ void __launch_bounds__(256)
+tag_patches(const double* __restrict__ q, const int* __restrict__ slots, int n,
+            const double* __restrict__ dxs, const double* __restrict__ dys, long long pstride,
+            int mx, int my, int mbc, int ivar, double thr_refine, double thr_coarsen,
+            int* __restrict__ out) {
+    const int w = blockIdx.x;
+    if (w >= n) return;
+    const int slot = slots[w];
+    const double* qp = q + (long long) slot * pstride;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    const double dx2 = 2 * dxs[slot], dy2 = 2 * dys[slot];
+    int er = 0, ec = 0;
+    for (int e = threadIdx.x; e < mx * my; e += blockDim.x) {
+        const int j = e / mx + 1, i = e - (j - 1) * mx + 1;
+        const double dqx = (qp[QIX(i + 1, j, ivar)] - qp[QIX(i - 1, j, ivar)]) / dx2;
+        const double dqy = (qp[QIX(i, j + 1, ivar)] - qp[QIX(i, j - 1, ivar)]) / dy2;
+        const double g3 = 0.0;
+        const double ds = sqrt(dqx * dqx + dqy * dqy + g3 * g3);
+        er |= ds > thr_refine;
+        ec |= ds > thr_coarsen;
+    }
+    er = __syncthreads_or(er);
+    ec = __syncthreads_or(ec);
+    if (threadIdx.x == 0) {
+        out[2 * w] = thr_refine < 0 ? 1 : er;
+        out[2 * w + 1] = thr_coarsen <= 0 ? 1 : ec;
+    }
+}
+
+// Repopulate one new patch (cb_fclaw2d_regrid_repopulate,
+// src/fclaw2d_regrid.c:95-230): copy of the same old patch, interpolation from
+// the old parent (fclaw2d_clawpatch46_fort_interpolate2fine, over the
+// reference's loop ranges), or average of the four old children
+// (fclaw2d_clawpatch46_fort_average2coarse).  op = (kind, dst, src0..src3, -, -).
+__global__ void __launch_bounds__(256)
+regrid_remap(const double* __restrict__ qold, double* __restrict__ qnew,
+             const int* __restrict__ ops, int nops, long long pstride, int mx, int my, int mbc,
+             int meqn) {
+    const int o = blockIdx.x;
+    if (o >= nops) return;
+    const int* op = ops + 8 * o;
+    const int kind = op[0];
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    double* qd = qnew + (long long) op[1] * pstride;
+    if (kind == REGRID_SAME) {
+        const double* qs = qold + (long long) op[2] * pstride;
+        for (long long e = threadIdx.x; e < pstride; e += blockDim.x) qd[e] = qs[e];
+    } else if (kind == REGRID_CHILD) {
+        const double* qc = qold + (long long) op[2] * pstride;
+        const int igrid = op[3], ig = igrid % 2, jg = (igrid - ig) / 2;
+        const int i1 = 1 - ig, ni = mx / 2 + 1, ic_add = ig * mx / 2;
+        const int j1 = 1 - jg, nj = my / 2 + 1, jc_add = jg * my / 2;
+        for (int e = threadIdx.x; e < meqn * ni * nj; e += blockDim.x) {
+            const int m = e / (ni * nj), r = e - m * ni * nj;
+            const int j = j1 + r / ni, i = i1 + r % ni;
+            const int ic = i + ic_add, jc = j + jc_add;
+            const double c = qc[QIX(ic, jc, m)];
+            const double gradx = amr_slope(c - qc[QIX(ic - 1, jc, m)], qc[QIX(ic + 1, jc, m)] - c);
+            const double grady = amr_slope(c - qc[QIX(ic, jc - 1, m)], qc[QIX(ic, jc + 1, m)] - c);
+            for (int ii = 1; ii <= 2; ii++)
+                for (int jj = 1; jj <= 2; jj++) {
+                    const double shiftx = (ii - 2 / 2.0 - 0.5) / 2;
+                    const double shifty = (jj - 2 / 2.0 - 0.5) / 2;
+                    qd[QIX((i - 1) * 2 + ii, (j - 1) * 2 + jj, m)] = c + shiftx * gradx + shifty * grady;
+                }
+        }
+    } else {
+        for (int e = threadIdx.x; e < meqn * mx * my; e += blockDim.x) {
+            const int m = e / (mx * my), r = e - m * mx * my;
+            const int jc = r / mx + 1, ic = r - (jc - 1) * mx + 1;
+            const int ig = ic > mx / 2, jg = jc > my / 2, g = ig + 2 * jg;
+            const double* qf = qold + (long long) op[2 + g] * pstride;
+            const int i = ic - ig * mx / 2, j = jc - jg * mx / 2;  // mx, as the reference writes it
+            double sum = 0;
+            sum = sum + qf[QIX(2 * i - 1, 2 * j - 1, m)];
+            sum = sum + qf[QIX(2 * i, 2 * j - 1, m)];
+            sum = sum + qf[QIX(2 * i - 1, 2 * j, m)];
+            sum = sum + qf[QIX(2 * i, 2 * j, m)];
+            qd[QIX(ic, jc, m)] = sum / 4;
+        }
+    }
+}
+
 __global__ void copy_patches(const double* __restrict__ src, double* __restrict__ dst,
                              const int* __restrict__ slots, int n, long long pstride) {
     const int w = blockIdx.y;
@@ -298,6 +390,23 @@ int launch_timeinterp(const double* q, const double* qlast, double* qti, const i
     dim3 grid((meqn * mx * my + 255) / 256, n);
     hipLaunchKernelGGL(timeinterp_band, grid, dim3(256), 0, st, q, qlast, qti, slots, n, pstride,
                        mx, my, mbc, meqn, alpha);
+    return (int) hipGetLastError();
+}
+
+int launch_tag_patches(const double* q, const int* slots, int n, const double* dxs,
+                       const double* dys, long long pstride, int mx, int my, int mbc, int ivar,
+                       double thr_refine, double thr_coarsen, int* out, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(tag_patches, dim3(n), dim3(256), 0, st, q, slots, n, dxs, dys, pstride, mx,
+                       my, mbc, ivar, thr_refine, thr_coarsen, out);
+    return (int) hipGetLastError();
+}
+
+int launch_regrid_remap(const double* qold, double* qnew, const int* ops, int nops,
+                        long long pstride, int mx, int my, int mbc, int meqn, hipStream_t st) {
+    if (nops <= 0) return 0;
+    hipLaunchKernelGGL(regrid_remap, dim3(nops), dim3(256), 0, st, qold, qnew, ops, nops, pstride,
+                       mx, my, mbc, meqn);
     return (int) hipGetLastError();
 }
 

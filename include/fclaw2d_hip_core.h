@@ -99,6 +99,12 @@ typedef struct fclaw2d_global {
     int time_sync;                 /* fclaw_options time-sync (src/fclaw_options.c:110) */
     int subcycle;                  /* fclaw_options subcycle (src/fclaw_options.c:99) */
     int advance_one_step;          /* fclaw_options advance-one-step (src/fclaw_options.c:91) */
+    /* regrid options (src/fclaw_options.c:190-221) and the clawpatch
+     * threshold-variable, 1..meqn (src/patches/clawpatch/fclaw2d_clawpatch_options.c:112) */
+    int minlevel, maxlevel;
+    double refine_threshold, coarsen_threshold;
+    int threshold_variable;
+    int count_amr_regrid, count_amr_new_domain;   /* src/fclaw2d_regrid.c:321, 333 */
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -135,19 +141,36 @@ fclaw2d_patch_vtable_t *fclaw2d_patch_vt(fclaw2d_global_t *glob);
 double fclaw2d_update_single_step(fclaw2d_global_t *glob, int level, double t, double dt);
 
 /** Fill ghost cells of all patches at levels minlevel..maxlevel
- *  (src/fclaw2d_ghost_fill.c:838-856).  time_interp must be 0 (subcycling is
- *  not supported by this build). */
+ *  (src/fclaw2d_ghost_fill.c:838-856).  The level ranges are those the
+ *  advance asks for: every level, or (time_interp, with subcycling) the
+ *  levels above a time-interpolated one. */
 void fclaw2d_ghost_update(fclaw2d_global_t *glob, int minlevel, int maxlevel,
                           double sync_time, int time_interp, int running);
 
-/** One global time step of every level followed by a ghost update
- *  (fclaw2d_advance_all_levels with subcycle = F, advance_one_step = T,
- *  src/fclaw2d_advance.c:247-357).  Returns the maximum Courant number. */
+/** Advance every level from t_curr to t_curr + dt, ending with a ghost
+ *  update (and time sync) of every level (fclaw2d_advance_all_levels,
+ *  src/fclaw2d_advance.c:247-357, with glob's subcycle / advance_one_step /
+ *  time_sync).  Returns the maximum Courant number. */
 double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double dt);
 
 /* Time-interpolate the patches of `level` (fclaw2d_timeinterp,
  * src/fclaw2d_timeinterp.c); needs glob->subcycle. */
 void fclaw2d_timeinterp(fclaw2d_global_t *glob, int level, double alpha);
+
+/* ---- mesh adaptation ----
+ * fclaw2d_initialize (src/fclaw2d_initialize.c:75-240): device state for the
+ * current domain, setaux + qinit on every patch, ghost update, then for each
+ * level minlevel..maxlevel-1 tag (no coarsening), refine and initialize the
+ * new domain again, until no patch refines.  Ghost cells are filled before
+ * each tagging.
+ * fclaw2d_regrid (src/fclaw2d_regrid.c:235-340): gradient tags on the device
+ * (refine_threshold / coarsen_threshold on threshold_variable), forest adapt
+ * (refine below maxlevel, coarsen families above minlevel, 2:1 balance), the
+ * solution moved to the new leaves on the device (copy, interpolate2fine or
+ * average2coarse), setaux on the host, then a ghost update of every level.
+ * Nothing moves when the leaves do not change. */
+void fclaw2d_initialize(fclaw2d_global_t *glob);
+void fclaw2d_regrid(fclaw2d_global_t *glob);
 
 /* ---- output ----
  * ASCII frames as the reference writes them: fort.tXXXX (time header) and

@@ -40,6 +40,24 @@ void fclaw2d_hip_forest_leaves(const fclaw2d_hip_forest_t *f, int *level, int *g
  * Returns the new number of leaves, or -1 on error. */
 int fclaw2d_hip_forest_refine(fclaw2d_hip_forest_t *f, const int *flags, int maxlevel);
 
+/* One regrid step of the forest (the adapt of fclaw2d_regrid,
+ * src/fclaw2d_regrid.c:235-330): leaves flagged in refine[] (below maxlevel)
+ * are split, the forest is rebalanced, then every family of four siblings all
+ * flagged in coarsen[] (none refined, above minlevel) is replaced by its
+ * parent where that keeps the 2:1 balance.  Returns the new number of leaves
+ * or -1.  The provenance of each new leaf is then available (below). */
+int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const int *coarsen,
+                             int minlevel, int maxlevel);
+
+enum {
+    FCLAW2D_HIP_REGRID_SAME = 0,    /* src[0] = old leaf index */
+    FCLAW2D_HIP_REGRID_CHILD = 1,   /* src[0] = old parent, src[1] = child index 0..3 */
+    FCLAW2D_HIP_REGRID_PARENT = 2   /* src[0..3] = old children in z-order */
+};
+/* Provenance of the leaves after the last fclaw2d_hip_forest_adapt:
+ * kind[n] and src[4 n]; returns n or -1. */
+int fclaw2d_hip_forest_provenance(const fclaw2d_hip_forest_t *f, int *kind, int *src);
+
 /* Ghost-fill program for the current leaves (see hipclaw_set_ghost_program).
  * mthbc gives the physical BC of the four domain faces.  Writes up to `cap`
  * ops of 8 ints and phase_start[7]; returns the number of ops needed (call

@@ -168,6 +168,30 @@ int hipclaw_set_sync_program(hipclaw_state_t *s, const int *ops, int nops);
  * (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_time_sync.f:249-700). */
 int hipclaw_time_sync(hipclaw_state_t *s);
 
+/* Gradient refinement criterion of every patch (tag4refinement /
+ * tag4coarsening with the gradient criterion,
+ * src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_tag4refinement.f,
+ * src/patches/clawpatch/fort_user/fclaw2d_clawpatch_gradient_exceeds_th.f90):
+ * exceeds_refine[p] / exceeds_coarsen[p] = 1 if some interior cell of patch p
+ * has |grad q(ivar)| above the refine / coarsen threshold (always 1 for a
+ * negative refine threshold, "always refine", and for a coarsen threshold
+ * <= 0, "never coarsen": src/patches/clawpatch/fclaw2d_clawpatch.cpp:813-817,
+ * 890-911).  ivar is 0-based. */
+int hipclaw_tag_patches(hipclaw_state_t *s, int ivar, double refine_threshold,
+                        double coarsen_threshold, int *exceeds_refine, int *exceeds_coarsen);
+
+/* Move the solution onto a new set of npatch_new patches (the repopulate step
+ * of fclaw2d_regrid, src/fclaw2d_regrid.c:95-230): patch p is a copy of old
+ * patch src[4p] (kind 0), interpolated from old parent src[4p] as child
+ * src[4p+1] (kind 1, fclaw2d_clawpatch46_fort_interpolate2fine), or the
+ * average of old children src[4p..4p+3] (kind 2,
+ * fclaw2d_clawpatch46_fort_average2coarse) -- the provenance of
+ * fclaw2d_hip_forest_adapt.  New mesh widths dx, dy.  Buffers grow as needed;
+ * ghost and sync programs, a captured graph and aux are invalidated (set them
+ * again), registers start at zero. */
+int hipclaw_regrid(hipclaw_state_t *s, int npatch_new, const int *kind, const int *src,
+                   const double *dx, const double *dy);
+
 /* nsteps of (update every patch at every level, ghost fill, and with
  * time_sync the conservation fix-up), all resident on the GPU; *maxcfl
  * receives the maximum over the steps.  Equivalent to

@@ -141,7 +141,7 @@ void oracle_average_corner(int mx, int my, int mbc, int meqn, double *qcoarse,
 /* ---------------- interpolate: fclaw2d_clawpatch46_interpolate.f ---------------- */
 
 /* fclaw2d_clawpatch_compute_slopes with mth = 5 (AMRClaw slopes). */
-static double compute_slopes(double sl, double sr)
+double oracle_compute_slopes(double sl, double sr)
 {
     double sc = (sl + sr) / 2.0;
     double mn = fmin(fmin(2 * fabs(sl), 2 * fabs(sr)), fabs(sc));
@@ -159,10 +159,10 @@ static void interp_cell(int mx, int my, int mbc, const double *qcoarse, double *
     int m;
     sl = (qc - Q(qcoarse, ic - 1, jc, mq));
     sr = (Q(qcoarse, ic + 1, jc, mq) - qc);
-    gradx = compute_slopes(sl, sr);
+    gradx = oracle_compute_slopes(sl, sr);
     sl = (qc - Q(qcoarse, ic, jc - 1, mq));
     sr = (Q(qcoarse, ic, jc + 1, mq) - qc);
-    grady = compute_slopes(sl, sr);
+    grady = oracle_compute_slopes(sl, sr);
     for (m = 0; m < 4; m++) {
         int iff = i2[0] + dfi[m], jff = j2[0] + dfj[m];
         Q(qfine, iff, jff, mq) = qc + gradx * shx[m] + grady * shy[m];

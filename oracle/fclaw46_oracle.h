@@ -116,6 +116,21 @@ void oracle_time_sync_samesize(int mx, int my, int mbc, int meqn, double *qthis,
 /* fclaw2d_time_sync over all levels; registers are left zero. */
 int oracle_time_sync(const oracle_domain_t *d, double *q, double *reg);
 
+/* ---------------- regrid ---------------- */
+/* fclaw2d_clawpatch_compute_slopes, mth = 5 (fclaw2d_clawpatch_utils.f:45-63) */
+double oracle_compute_slopes(double sl, double sr);
+/* 1 if some interior cell's |grad q(ivar)| exceeds threshold (always 1 for a
+ * negative threshold): the gradient criterion of tag4refinement/coarsening. */
+int oracle_patch_exceeds(int mx, int my, int mbc, const double *q, int ivar, double dx,
+                         double dy, double threshold);
+void oracle_interpolate2fine(int mx, int my, int mbc, int meqn, const double *qcoarse,
+                             double *qfine, int igrid);
+void oracle_average2coarse(int mx, int my, int mbc, int meqn, double *qcoarse,
+                           const double *qfine, int igrid);
+int oracle_regrid_remap(int mx, int my, int mbc, int meqn, int nold, const int *lold,
+                        const int *xold, const int *yold, const double *qold, int nnew,
+                        const int *lnew, const int *xnew, const int *ynew, double *qnew);
+
 /* Kahan mass sum per field in patch order:
  * src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_diagnostics.f:13-55 */
 void oracle_mass(const oracle_domain_t *d, const double *q, double *sum);

@@ -50,7 +50,10 @@ class Global(C.Structure):
                 ("clawpatch_opt", ClawpatchOptions), ("hipclaw_vt", C.c_void_p),
                 ("hipclaw_opt", C.c_void_p), ("state", C.c_void_p),
                 ("qhost", C.POINTER(C.c_double)), ("auxhost", C.POINTER(C.c_double)),
-                ("time_sync", C.c_int), ("subcycle", C.c_int), ("advance_one_step", C.c_int)]
+                ("time_sync", C.c_int), ("subcycle", C.c_int), ("advance_one_step", C.c_int),
+                ("minlevel", C.c_int), ("maxlevel", C.c_int), ("refine_threshold", C.c_double),
+                ("coarsen_threshold", C.c_double), ("threshold_variable", C.c_int),
+                ("count_amr_regrid", C.c_int), ("count_amr_new_domain", C.c_int)]
 
 
 class Options(C.Structure):
@@ -85,6 +88,8 @@ def bind():
         "fclaw2d_advance_all_levels": ([C.POINTER(Global), d, d], d),
         "fclaw2d_time_sync": ([C.POINTER(Global), i, i], None),
         "fclaw2d_timeinterp": ([C.POINTER(Global), i, d], None),
+        "fclaw2d_initialize": ([C.POINTER(Global)], None),
+        "fclaw2d_regrid": ([C.POINTER(Global)], None),
         "fc2d_hipclaw_options_default": ([C.POINTER(Options), i], None),
         "fc2d_hipclaw_options_store": ([C.POINTER(Global), C.POINTER(Options)], None),
         "fc2d_hipclaw_check": ([C.POINTER(Options)], i),

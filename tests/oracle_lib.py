@@ -1,5 +1,6 @@
 """ctypes bindings to the CPU oracle (oracle/liboracle46.so) and to the
-reference's own Fortran built in place (oracle/_ref/libref46.so).
+reference's own Fortran built in place (oracle/_ref/libref46.so, and
+oracle/_ref/libref46_amr.so for the regrid transfer routines).
 
 Test infrastructure only: the product never imports this module.
 """
@@ -11,6 +12,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle46.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref46.so")
+REF_AMR_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_amr.so")
 
 RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
 
@@ -36,6 +38,7 @@ class OracleDomain(C.Structure):
 
 _oracle = None
 _ref = None
+_ref_amr = None
 
 
 def oracle():
@@ -62,6 +65,12 @@ def oracle():
         lib.oracle_advance_subcycle.argtypes = [C.POINTER(OracleDomain), C.POINTER(OracleClaw),
                                                 _dp, C.c_void_p, C.c_double, C.c_int]
         lib.oracle_advance_subcycle.restype = C.c_double
+        lib.oracle_patch_exceeds.argtypes = [C.c_int] * 3 + [_dp, C.c_int] + [C.c_double] * 3
+        lib.oracle_regrid_remap.argtypes = [C.c_int] * 5 + [_ip] * 3 + [_dp, C.c_int] + [_ip] * 3 + [_dp]
+        lib.oracle_compute_slopes.argtypes = [C.c_double, C.c_double]
+        lib.oracle_compute_slopes.restype = C.c_double
+        lib.oracle_interpolate2fine.argtypes = [C.c_int] * 4 + [_dp, _dp, C.c_int]
+        lib.oracle_average2coarse.argtypes = [C.c_int] * 4 + [_dp, _dp, C.c_int]
         lib.oracle_cons_flux.argtypes = [C.POINTER(OracleClaw), C.c_int, _dp, _dp, _dp]
         lib.oracle_mass.argtypes = [C.POINTER(OracleDomain), _dp, _dp]
         lib.oracle_bc2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip]
@@ -160,3 +169,49 @@ def domain_struct(level, gx, gy, nbx, nby, periodic, mx, my, mbc, meqn, maux, mt
     d.ax, d.bx, d.ay, d.by = extent
     d._keep = arrs
     return d
+
+
+def ref_amr_available():
+    return os.path.exists(REF_AMR_SO)
+
+
+def ref_amr():
+    """The reference's fclaw2d_clawpatch46_fort_interpolate2fine /
+    fclaw2d_clawpatch46_fort_average2coarse / fclaw2d_clawpatch_compute_slopes
+    (Fortran calling convention: every argument by reference)."""
+    global _ref_amr
+    if _ref_amr is None:
+        lib = C.CDLL(REF_AMR_SO)
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        for name in ("fclaw2d_clawpatch46_fort_interpolate2fine_",
+                     "fclaw2d_clawpatch46_fort_average2coarse_"):
+            getattr(lib, name).argtypes = [ip] * 4 + [_dp, _dp, _dp, _dp, ip, ip]
+        lib.fclaw2d_clawpatch_compute_slopes_.argtypes = [dp, dp, ip]
+        lib.fclaw2d_clawpatch_compute_slopes_.restype = C.c_double
+        _ref_amr = lib
+    return _ref_amr
+
+
+def _transfer_ref(name, mx, my, mbc, meqn, qcoarse, qfine, igrid):
+    area = np.zeros((my + 2 * mbc + 2) * (mx + 2 * mbc + 2))
+    i = lambda v: C.byref(C.c_int(v))
+    getattr(ref_amr(), name)(i(mx), i(my), i(mbc), i(meqn), qcoarse, qfine, area, area.copy(),
+                             i(igrid), i(0))
+
+
+def interpolate2fine_ref(mx, my, mbc, meqn, qcoarse, qfine, igrid):
+    """Reference interpolate2fine (manifold = 0) into qfine, in place."""
+    _transfer_ref("fclaw2d_clawpatch46_fort_interpolate2fine_", mx, my, mbc, meqn, qcoarse, qfine,
+                  igrid)
+
+
+def average2coarse_ref(mx, my, mbc, meqn, qcoarse, qfine, igrid):
+    """Reference average2coarse (manifold = 0) into qcoarse, in place."""
+    _transfer_ref("fclaw2d_clawpatch46_fort_average2coarse_", mx, my, mbc, meqn, qcoarse, qfine,
+                  igrid)
+
+
+def compute_slopes_ref(sl, sr, mth=5):
+    return ref_amr().fclaw2d_clawpatch_compute_slopes_(C.byref(C.c_double(sl)),
+                                                       C.byref(C.c_double(sr)),
+                                                       C.byref(C.c_int(mth)))

@@ -95,3 +95,104 @@ def test_ghost_program_phases_and_op_bounds():
     # op types live in their phase
     for ph, types in enumerate(({2, 3}, {0, 1}, {6}, {4, 5}, {6}, {6})):
         assert set(ops[ps[ph]:ps[ph + 1], 0].tolist()) <= types
+
+
+def _check_tiling_and_balance(forest):
+    lv, gx, gy = forest.leaves()
+    assert np.sum(np.ldexp(1.0, -2 * lv.astype(np.int64))) == forest.nbx * forest.nby
+    assert len(set(zip(lv.tolist(), gx.tolist(), gy.tolist()))) == len(lv)
+    for l, m in _neighbour_levels(forest):
+        assert abs(l - m) <= 1
+
+
+def _check_provenance(old, new, kind, src):
+    """Every new leaf is the same old leaf, a child of an old leaf or the
+    parent of four old leaves, and src names exactly those leaves."""
+    lo, xo, yo = old
+    ln, xn, yn = new
+    for p in range(len(ln)):
+        L, x, y = int(ln[p]), int(xn[p]), int(yn[p])
+        if kind[p] == fa.REGRID_SAME:
+            k = src[p, 0]
+            assert (lo[k], xo[k], yo[k]) == (L, x, y)
+        elif kind[p] == fa.REGRID_CHILD:
+            k, g = src[p, 0], src[p, 1]
+            assert (lo[k], xo[k], yo[k]) == (L - 1, x >> 1, y >> 1)
+            assert g == (x & 1) + 2 * (y & 1)
+        else:
+            assert kind[p] == fa.REGRID_PARENT
+            for g in range(4):
+                k = src[p, g]
+                assert (lo[k], xo[k], yo[k]) == (L + 1, 2 * x + (g & 1), 2 * y + (g >> 1))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_adapt_refines_coarsens_and_stays_balanced(seed):
+    rng = np.random.default_rng(seed)
+    f = fa.Forest(2, 1, (seed % 2 == 0, False), 2)
+    for _ in range(3):
+        f.refine((rng.random(f.npatch) < 0.35).astype(np.int32), 5)
+    minlevel, maxlevel = 1, 5
+    for _ in range(4):
+        old = [a.copy() for a in f.leaves()]
+        refine = (rng.random(f.npatch) < 0.2).astype(np.int32)
+        coarsen = (rng.random(f.npatch) < 0.8).astype(np.int32)
+        kind, src = f.adapt(refine, coarsen, minlevel, maxlevel)
+        new = f.leaves()
+        _check_tiling_and_balance(f)
+        _check_provenance(old, new, kind, src)
+        newset = set(zip(*(a.tolist() for a in new)))
+        for k in range(len(old[0])):
+            L, x, y = (int(a[k]) for a in old)
+            if refine[k] and L < maxlevel:
+                assert (L, x, y) not in newset          # split
+        for p in np.nonzero(kind == fa.REGRID_PARENT)[0]:
+            assert new[0][p] >= minlevel
+            # refine flags count only below maxlevel (cb_fclaw2d_regrid_tag4refinement)
+            assert all(coarsen[src[p, g]] and not (refine[src[p, g]] and new[0][p] + 1 < maxlevel)
+                       for g in range(4))
+
+
+def test_adapt_coarsens_one_level_per_pass_down_to_minlevel():
+    f = fa.Forest(1, 1, (False, False), 3)
+    ones = np.ones(f.npatch, np.int32)
+    kind, src = f.adapt(np.zeros_like(ones), ones, 1, 5)
+    assert np.all(f.leaves()[0] == 2) and np.all(kind == fa.REGRID_PARENT)
+    kind, src = f.adapt(np.zeros(f.npatch, np.int32), np.ones(f.npatch, np.int32), 1, 5)
+    assert np.all(f.leaves()[0] == 1)
+    kind, src = f.adapt(np.zeros(f.npatch, np.int32), np.ones(f.npatch, np.int32), 1, 5)
+    assert np.all(f.leaves()[0] == 1) and np.all(kind == fa.REGRID_SAME)
+    assert np.array_equal(src[:, 0], np.arange(f.npatch))
+
+
+def _split_one(level2_leaf):
+    f = fa.Forest(1, 1, (False, False), 2)
+    lv, gx, gy = f.leaves()
+    f.refine(((gx == level2_leaf[0]) & (gy == level2_leaf[1])).astype(np.int32), 5)
+    return f
+
+
+def test_adapt_coarsens_families_away_from_finer_leaves():
+    # 16 level-2 leaves, (3,3) split to level 3: the three families that do
+    # not hold it and do not touch level 3 coarsen
+    f = _split_one((3, 3))
+    lv, _, _ = f.leaves()
+    kind, src = f.adapt(np.zeros(f.npatch, np.int32), (lv == 2).astype(np.int32), 0, 5)
+    _check_tiling_and_balance(f)
+    leaves = set(zip(*(a.tolist() for a in f.leaves())))
+    assert {(1, 0, 0), (1, 1, 0), (1, 0, 1)} <= leaves
+    assert {(2, 2, 2), (2, 3, 2), (2, 2, 3)} <= leaves
+    assert np.count_nonzero(kind == fa.REGRID_PARENT) == 3
+
+
+def test_adapt_skips_a_coarsening_that_would_break_the_balance():
+    # (2,1) split: each intact family touches a level-3 leaf across a face or
+    # a corner, so coarsening any would put level 1 next to level 3
+    f = _split_one((2, 1))
+    before = [a.copy() for a in f.leaves()]
+    lv = before[0]
+    kind, src = f.adapt(np.zeros(f.npatch, np.int32), (lv == 2).astype(np.int32), 0, 5)
+    _check_tiling_and_balance(f)
+    for a, b in zip(before, f.leaves()):
+        np.testing.assert_array_equal(a, b)
+    assert np.all(kind == fa.REGRID_SAME)
