@@ -75,6 +75,40 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
                 graph=graph, total_steps=total_steps)
 
 
+def run_regrid(fa, prob, forest, q, repeats=3):
+    """Cost of one regrid of the benchmark forest from the state q (the FP64
+    form's state after the timed steps): device gradient tags, host forest
+    adapt, device transfer onto the new leaves (copy / interpolate2fine /
+    average2coarse) with the new ghost-fill program, and the ghost fill of
+    every level.  Median over `repeats` regrids, each from the same state."""
+    times = []
+    for _ in range(repeats):
+        f = fa.Forest(forest.nbx, forest.nby, forest.periodic, leaves=forest.leaves())
+        sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=fa.FP64, mthbc=prob.mthbc,
+                       grav=prob.grav, gamma=prob.gamma, max_patches=f.npatch)
+        sv.set_forest(f, prob.extent)
+        sv.set_q(q)
+        sv.sync()
+        t0 = time.perf_counter()
+        er, ec = sv.tag(prob.tag_field, prob.tag_threshold, 0.5 * prob.tag_threshold)
+        t1 = time.perf_counter()
+        kind, src = f.adapt(er, (1 - ec).astype(np.int32), prob.minlevel, prob.maxlevel)
+        t2 = time.perf_counter()
+        sv.regrid(f, kind, src, prob.extent)
+        sv.ghost_fill()
+        sv.sync()
+        t3 = time.perf_counter()
+        times.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2, f.npatch,
+                      int(np.sum(kind == fa.REGRID_CHILD)), int(np.sum(kind == fa.REGRID_PARENT))))
+        del sv
+    tot, tag, adapt, move, n, nchild, nparent = sorted(times)[len(times) // 2]
+    return {"ms": round(tot * 1e3, 3), "tag_ms": round(tag * 1e3, 3),
+            "adapt_ms": round(adapt * 1e3, 3), "transfer_and_ghost_fill_ms": round(move * 1e3, 3),
+            "patches_before": forest.npatch, "patches_after": n, "new_children": nchild,
+            "new_parents": nparent, "refine_threshold": prob.tag_threshold,
+            "coarsen_threshold": 0.5 * prob.tag_threshold}
+
+
 PROFILE_STEPS = 6
 COPY_BYTES = 256 << 20
 
@@ -218,6 +252,8 @@ def main():
     r32 = run_form(fa, prob, forest, q0, aux, fa.FP32_BULK, dt, args.steps, args.warmup,
                    args.tile32, not args.no_graph, not args.no_time_sync)
 
+    regrid = run_regrid(fa, prob, forest, r64["q"])
+
     B = algorithmic_bytes_per_cell(prob.meqn, prob.maux, prob.mx)
     value32 = cells * args.steps / r32["wall"]
     value64 = cells * args.steps / r64["wall"]
@@ -285,6 +321,7 @@ def main():
             "mass_change_fp32": mass_change(r32),
             "mass_change_fp64": mass_change(r64),
         },
+        "regrid": regrid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

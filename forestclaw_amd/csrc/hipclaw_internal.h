@@ -111,6 +111,15 @@ inline int launch_persistent(K kernel, int block, const StepArgs& a, hipStream_t
     return (int) hipGetLastError();
 }
 
+// Every wave family uses the MC limiter (mthlim 4), so the step may take the
+// instantiation with the limiter fixed at compile time.
+template <class RP>
+inline bool all_mc_limiter(const StepArgs& a) {
+    for (int mw = 0; mw < RP::MWAVES; mw++)
+        if (a.mthlim[mw] != 4) return false;
+    return true;
+}
+
 // Largest tile the FP64 / FP32 kernels are instantiated for, per solver.
 bool step_supported(int solver, int precision, int tile);
 

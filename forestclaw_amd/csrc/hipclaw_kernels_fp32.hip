@@ -9,11 +9,18 @@ namespace hipclaw {
 template <class RP, int T, int NW>
 static int launch_one(const StepArgs& a, hipStream_t st) {
     // kernels that keep time-sync registers are separate instantiations, so the
-    // plain update carries none of their code or register pressure
-    static int resident[2] = {0, 0};
+    // plain update carries none of their code or register pressure; the MC
+    // limiter on every wave family has its own compile-time instantiation
+    static int resident[2][2] = {{0, 0}, {0, 0}};
+    constexpr bool C = RP::CONSERVATIVE;
+    if (all_mc_limiter<RP>(a)) {
+        if (a.reg)
+            return launch_persistent(step_tiles<RP, float, T, NW, C, true, 4>, NW * 64, a, st, resident[1][1]);
+        return launch_persistent(step_tiles<RP, float, T, NW, C, false, 4>, NW * 64, a, st, resident[1][0]);
+    }
     if (a.reg)
-        return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE, true>, NW * 64, a, st, resident[1]);
-    return launch_persistent(step_tiles<RP, float, T, NW, RP::CONSERVATIVE, false>, NW * 64, a, st, resident[0]);
+        return launch_persistent(step_tiles<RP, float, T, NW, C, true>, NW * 64, a, st, resident[0][1]);
+    return launch_persistent(step_tiles<RP, float, T, NW, C, false>, NW * 64, a, st, resident[0][0]);
 }
 
 template <class RP, bool WITH32 = true>

@@ -511,14 +511,23 @@ int launch_time_sync(const SyncArgs& a, hipStream_t st) {
 template <class RP, int T, int NW, bool WITH_REGS = true>
 static int launch_one(const StepArgs& a, hipStream_t st) {
     // kernels that keep time-sync registers are separate instantiations, so the
-    // plain update carries none of their code or register pressure
-    static int resident[2] = {0, 0};
+    // plain update carries none of their code or register pressure; the MC
+    // limiter on every wave family has its own compile-time instantiation
+    static int resident[2][2] = {{0, 0}, {0, 0}};
+    if (all_mc_limiter<RP>(a)) {
+        if (a.reg) {
+            if constexpr (WITH_REGS)
+                return launch_persistent(step_tiles<RP, double, T, NW, false, true, 4>, NW * 64, a, st, resident[1][1]);
+            return (int) hipErrorInvalidValue;
+        }
+        return launch_persistent(step_tiles<RP, double, T, NW, false, false, 4>, NW * 64, a, st, resident[1][0]);
+    }
     if (a.reg) {
         if constexpr (WITH_REGS)
-            return launch_persistent(step_tiles<RP, double, T, NW, false, true>, NW * 64, a, st, resident[1]);
+            return launch_persistent(step_tiles<RP, double, T, NW, false, true>, NW * 64, a, st, resident[0][1]);
         return (int) hipErrorInvalidValue;
     }
-    return launch_persistent(step_tiles<RP, double, T, NW, false, false>, NW * 64, a, st, resident[0]);
+    return launch_persistent(step_tiles<RP, double, T, NW, false, false>, NW * 64, a, st, resident[0][0]);
 }
 
 template <class RP>

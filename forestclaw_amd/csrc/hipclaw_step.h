@@ -162,7 +162,14 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
 
 // At least 4 waves per SIMD: the FP64 kernels that keep time-sync registers
 // would otherwise settle at 3 (measured faster at 4 despite a few spills).
-template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS>
+//
+// LIM = 0 reads each wave family's limiter from a.mthlim at run time; LIM > 0
+// is that limiter for every family, fixed at compile time.  The run-time
+// switch keeps a per-family limiter code in scalar registers through both
+// sweeps and, with the kernel's other uniform values, spills scalar
+// registers into vector lanes; the fixed form has neither (about 40 % fewer
+// vector instructions in each sweep for the MC limiter).
+template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS, int LIM = 0>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
@@ -328,7 +335,7 @@ step_tiles(StepArgs a) {
             if (a.order2) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) {
-                    const int mth = a.mthlim[mw];
+                    const int mth = LIM > 0 ? LIM : a.mthlim[mw];
                     if (mth == 0) continue;
                     R wn = R(0), dotl = R(0);
 #pragma unroll
@@ -424,7 +431,7 @@ step_tiles(StepArgs a) {
             if (a.order2) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) {
-                    const int mth = a.mthlim[mw];
+                    const int mth = LIM > 0 ? LIM : a.mthlim[mw];
                     if (mth == 0) continue;
                     R wn = R(0), dotl = R(0);
 #pragma unroll
