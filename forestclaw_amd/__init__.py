@@ -90,6 +90,10 @@ def lib():
         "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
         "hipclaw_tag_patches": ([vp, i, d, d, _ip(), _ip()], i),
         "hipclaw_regrid": ([vp, i, _ip(), _ip(), _dp(), _dp()], i),
+        "hipclaw_ghost_pack_elems": ([vp, C.POINTER(C.c_longlong)], i),
+        "hipclaw_ghost_pack": ([vp, _ip(), i, i, vp], i),
+        "hipclaw_ghost_unpack": ([vp, _ip(), i, i, vp], i),
+        "hipclaw_get_registers": ([vp, i, i, _dp()], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -294,6 +298,32 @@ class Solver:
         self.npatch = len(kind)
         self.max_patches = max(self.max_patches, self.npatch)
         self.set_forest(forest, extent)
+
+    def ghost_pack_elems(self):
+        """Doubles in one patch's ghost-patch message."""
+        n = C.c_longlong()
+        _check(lib().hipclaw_ghost_pack_elems(self._h, C.byref(n)), "hipclaw_ghost_pack_elems")
+        return n.value
+
+    def ghost_pack(self, slots, device_ptr, time_interp=False):
+        """Pack the messages of `slots` into device memory at device_ptr."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        _check(lib().hipclaw_ghost_pack(self._h, sl, len(sl), int(time_interp), device_ptr),
+               "hipclaw_ghost_pack")
+
+    def ghost_unpack(self, slots, device_ptr, time_interp=False):
+        """Unpack messages from device memory at device_ptr into `slots`."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        _check(lib().hipclaw_ghost_unpack(self._h, sl, len(sl), int(time_interp), device_ptr),
+               "hipclaw_ghost_unpack")
+
+    def get_registers(self, first=0, count=None):
+        """Time-sync registers, shape (count, 4 faces, 4 kinds, meqn, mx)."""
+        count = self.npatch - first if count is None else count
+        out = np.zeros((count, 4, 4, self.meqn, self.mx))
+        _check(lib().hipclaw_get_registers(self._h, first, count, out.reshape(-1)),
+               "hipclaw_get_registers")
+        return out
 
     def set_geometry(self, dx, dy):
         dx = np.ascontiguousarray(dx, dtype=np.float64)

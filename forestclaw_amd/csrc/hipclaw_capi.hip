@@ -78,6 +78,9 @@ struct hipclaw_state {
     // subcycling: the state before each patch's last update, and time-interpolated copies
     double* qlast = nullptr;
     double* qti = nullptr;
+    // ghost-patch message: slab offsets of one field's band, in message order
+    int* band = nullptr;
+    int nband = 0;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -223,6 +226,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->cfl);
     hipFree(s->mass_buf);
     hipFree(s->reg);
+    hipFree(s->band);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
@@ -486,6 +490,91 @@ int hipclaw_ghost_fill_k(hipclaw_state_t* s, int k) {
 }
 
 int hipclaw_ghost_fill(hipclaw_state_t* s) { return hipclaw_ghost_fill_k(s, 0); }
+
+// ---------------------------------------------------------------- ghost-patch messages
+// refratio 2: the band keeps mint = 2 mbc interior layers
+// (clawpatch_ghost_pack_elems, src/patches/clawpatch/fclaw2d_clawpatch.cpp:1020-1060)
+static void message_shape(const hipclaw_state_t* s, long long* nband, long long* nreg) {
+    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc, mint = 2 * mbc;
+    *nband = (long long) (mx + 2 * mbc) * (my + 2 * mbc) - (long long) (mx - 2 * mint) * (my - 2 * mint);
+    *nreg = s->reg ? 2LL * (4 * s->cfg.meqn + 2) * (mx + my) : 0;
+}
+
+int hipclaw_ghost_pack_elems(const hipclaw_state_t* s, long long* elems) {
+    if (!s || !elems) return fail(HIPCLAW_EINVAL, "hipclaw_ghost_pack_elems: bad arguments%s");
+    long long nband, nreg;
+    message_shape(s, &nband, &nreg);
+    *elems = nband * s->cfg.meqn + nreg;
+    return HIPCLAW_OK;
+}
+
+static int ghost_message(hipclaw_state_t* s, const int* slots, int n, int time_interp,
+                         double* buf, int unpack) {
+    const char* what = unpack ? "hipclaw_ghost_unpack" : "hipclaw_ghost_pack";
+    if (!s || n < 0 || n > s->max_patches || (n > 0 && (!slots || !buf)))
+        return fail(HIPCLAW_EINVAL, "ghost message: bad arguments%s");
+    if (time_interp && !s->qti) return fail(HIPCLAW_EINVAL, "time_interp needs a state created with subcycle%s");
+    for (int k = 0; k < n; k++)
+        if (slots[k] < 0 || slots[k] >= s->npatch) return fail(HIPCLAW_EINVAL, "slot out of range%s");
+    if (n == 0) return HIPCLAW_OK;
+    long long nband, nreg;
+    message_shape(s, &nband, &nreg);
+    const long long elems = nband * s->cfg.meqn + nreg;
+    // the buffer must be device memory holding n messages
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, buf) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void) hipGetLastError();
+        return fail(HIPCLAW_EINVAL, "the message buffer is not device memory%s");
+    }
+    void* base = nullptr;
+    size_t bytes = 0;
+    HIP_TRY(hipMemGetAddressRange((hipDeviceptr_t*) &base, &bytes, buf));
+    if ((const char*) buf + sizeof(double) * (size_t) (elems * n) > (const char*) base + bytes)
+        return fail(HIPCLAW_EINVAL, "the message buffer is too small%s");
+    if (!s->band) {
+        // offsets of one field's band in the order of the reference's pack loops
+        const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc, ng = mbc, mint = 2 * mbc;
+        const int nxp = mx + 2 * mbc;
+        std::vector<int> off;
+        auto put = [&](int i, int j) { off.push_back((j + mbc - 1) * nxp + i + mbc - 1); };
+        for (int j = 1 - ng; j <= my - mint; j++) for (int i = 1 - ng; i <= mint; i++) put(i, j);
+        for (int j = 1 - ng; j <= mint; j++) for (int i = mint + 1; i <= mx + ng; i++) put(i, j);
+        for (int j = mint + 1; j <= my + ng; j++) for (int i = mx - mint + 1; i <= mx + ng; i++) put(i, j);
+        for (int j = my - mint + 1; j <= my + ng; j++) for (int i = 1 - ng; i <= mx - mint; i++) put(i, j);
+        if ((long long) off.size() != nband) return fail(HIPCLAW_EDEVICE, "ghost band size mismatch%s");
+        HIP_TRY(hipMalloc((void**) &s->band, sizeof(int) * off.size()));
+        HIP_TRY(hipMemcpy(s->band, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+        s->nband = (int) off.size();
+    }
+    HIP_TRY(hipMemcpyAsync(s->batch_slots, slots, sizeof(int) * n, hipMemcpyHostToDevice, s->st));
+    double* q = time_interp ? s->qti : s->q[s->cur];
+    int e = hipclaw::launch_ghost_message(q, s->reg, s->batch_slots, n, s->band, s->nband,
+                                          s->pstride, s->rstride, s->cfg.mx, s->cfg.my,
+                                          s->cfg.mbc, s->cfg.meqn, s->dxs, s->dys, buf, elems,
+                                          unpack, s->st);
+    if (e) return hipfail((hipError_t) e, what);
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_ghost_pack(hipclaw_state_t* s, const int* slots, int n, int time_interp, double* buf) {
+    return ghost_message(s, slots, n, time_interp, buf, 0);
+}
+
+int hipclaw_ghost_unpack(hipclaw_state_t* s, const int* slots, int n, int time_interp,
+                         const double* buf) {
+    return ghost_message(s, slots, n, time_interp, const_cast<double*>(buf), 1);
+}
+
+int hipclaw_get_registers(hipclaw_state_t* s, int first, int count, double* out) {
+    if (!s || first < 0 || count < 0 || first + count > s->npatch || (count > 0 && !out))
+        return fail(HIPCLAW_EINVAL, "hipclaw_get_registers: bad arguments%s");
+    if (!s->reg) return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    HIP_TRY(hipMemcpy(out, s->reg + (long long) first * s->rstride,
+                      sizeof(double) * s->rstride * (size_t) count, hipMemcpyDeviceToHost));
+    return HIPCLAW_OK;
+}
 
 int hipclaw_timeinterp(hipclaw_state_t* s, const int* slots, int nslots, double alpha) {
     if (!s || nslots < 0 || nslots > s->max_patches || (nslots > 0 && !slots))

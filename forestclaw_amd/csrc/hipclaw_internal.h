@@ -77,6 +77,10 @@ int launch_reset_scalar(unsigned long long* p, hipStream_t st);
 int launch_copy_patches(const double* src, double* dst, const int* slots, int n, long long pstride,
                         hipStream_t st);
 // qti = qlast + alpha (q - qlast) on the interior band of width mbc.
+int launch_ghost_message(double* q, double* reg, const int* slots, int n, const int* band,
+                         int nband, long long pstride, long long rstride, int mx, int my, int mbc,
+                         int meqn, const double* dxs, const double* dys, double* buf,
+                         long long elems, int unpack, hipStream_t st);
 int launch_timeinterp(const double* q, const double* qlast, double* qti, const int* slots, int n,
                       long long pstride, int mx, int my, int mbc, int meqn, double alpha,
                       hipStream_t st);

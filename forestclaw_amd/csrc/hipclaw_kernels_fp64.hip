@@ -4,6 +4,8 @@
 // reference's CPU Fortran does.
 #include "hipclaw_internal.h"
 
+#include <algorithm>
+
 namespace hipclaw {
 
 #define QIX(i, j, m) (((long long) (m) * nyp + ((j) + mbc - 1)) * nxp + ((i) + mbc - 1))
@@ -380,6 +382,74 @@ int launch_copy_patches(const double* src, double* dst, const int* slots, int n,
     if (n <= 0) return 0;
     dim3 grid((unsigned) ((pstride + 255) / 256), n);
     hipLaunchKernelGGL(copy_patches, grid, dim3(256), 0, st, src, dst, slots, n, pstride);
+    return (int) hipGetLastError();
+}
+
+// Ghost-patch message of each listed slot (the data a rank sends another for
+// a patch bordering it): the q band of mbc ghost and refratio*mbc interior
+// layers in the order of fclaw2d_clawpatch46_fort_local_ghost_pack
+// (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_local_ghost_pack.f:38-84;
+// band[] holds the slab offsets of one field in that order), then, with time
+// sync, the registers in the order of fclaw2d_clawpatch_time_sync_pack_registers
+// (src/patches/clawpatch/fclaw2d_clawpatch_conservation.c:112-195): per face
+// meqn*len quadruples (fm, fp, edge flux 0, edge flux 1), then len pairs
+// (edge length, area).  unpack = 1 writes q and the registers back (edge
+// lengths and areas follow from dx, dy and are not read).
+__global__ void __launch_bounds__(256)
+ghost_message(double* __restrict__ q, double* __restrict__ reg, const int* __restrict__ slots,
+              int n, const int* __restrict__ band, int nband, long long pstride, long long rstride,
+              int mx, int my, int mbc, int meqn, const double* __restrict__ dxs,
+              const double* __restrict__ dys, double* __restrict__ buf, long long elems,
+              int unpack) {
+    const int p = blockIdx.y;
+    if (p >= n) return;
+    const int slot = slots[p];
+    double* qp = q + (long long) slot * pstride;
+    double* b = buf + (long long) p * elems;
+    const long long plane = (long long) (mx + 2 * mbc) * (my + 2 * mbc);
+    const int nq = meqn * nband;
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += stride) {
+        const int m = k / nband, e = k - m * nband;
+        double* c = qp + m * plane + band[e];
+        if (unpack) *c = b[k];
+        else b[k] = *c;
+    }
+    if (!reg) return;
+    double* rp = reg + (long long) slot * rstride;
+    double* rb = b + nq;
+    const double dx = dxs[slot], dy = dys[slot];
+    const int nreg = 2 * (4 * meqn + 2) * (mx + my);
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nreg; r += stride) {
+        int f = 0, t = r;
+        for (; f < 3; f++) {
+            const int len = f < 2 ? my : mx, blk = (4 * meqn + 2) * len;
+            if (t < blk) break;
+            t -= blk;
+        }
+        const int len = f < 2 ? my : mx;
+        if (t < 4 * meqn * len) {
+            const int j = t >> 2, kind = t & 3;        // kinds FM, FP, EFIN, EFOUT in order
+            const int m = j / len, pos = j - m * len;
+            double* c = rp + ((f * 4 + kind) * meqn + m) * mx + pos;
+            if (unpack) *c = rb[r];
+            else rb[r] = *c;
+        } else if (!unpack) {
+            const int w = (t - 4 * meqn * len) & 1;
+            rb[r] = w == 0 ? (f < 2 ? dy : dx) : dx * dy;
+        }
+    }
+}
+
+int launch_ghost_message(double* q, double* reg, const int* slots, int n, const int* band,
+                         int nband, long long pstride, long long rstride, int mx, int my, int mbc,
+                         int meqn, const double* dxs, const double* dys, double* buf,
+                         long long elems, int unpack, hipStream_t st) {
+    if (n <= 0) return 0;
+    const long long per = elems;
+    dim3 grid((unsigned) std::min<long long>((per + 255) / 256, 64), n);
+    hipLaunchKernelGGL(ghost_message, grid, dim3(256), 0, st, q, reg, slots, n, band, nband,
+                       pstride, rstride, mx, my, mbc, meqn, dxs, dys, buf, elems, unpack);
     return (int) hipGetLastError();
 }
 

@@ -192,6 +192,36 @@ int hipclaw_tag_patches(hipclaw_state_t *s, int ivar, double refine_threshold,
 int hipclaw_regrid(hipclaw_state_t *s, int npatch_new, const int *kind, const int *src,
                    const double *dx, const double *dy);
 
+/* Ghost-patch messages: what one rank sends another for each patch that
+ * borders it, straight from and into the device slabs.  Replaces
+ * clawpatch_local_ghost_pack / clawpatch_remote_ghost_unpack
+ * (src/patches/clawpatch/fclaw2d_clawpatch.cpp:1019-1215) with
+ * fclaw2d_clawpatch46_fort_local_ghost_pack
+ * (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_local_ghost_pack.f) and,
+ * with time_sync, fclaw2d_clawpatch_time_sync_pack_registers
+ * (src/patches/clawpatch/fclaw2d_clawpatch_conservation.c:112-195), byte for
+ * byte the reference's message layout (refratio 2, no manifold area, no
+ * extra fields): *elems = doubles per patch (clawpatch_ghost_pack_elems,
+ * src/patches/clawpatch/fclaw2d_clawpatch.cpp:1020-1060).  buf is DEVICE
+ * memory holding n consecutive messages (checked against the allocation that
+ * contains it); slot k's message is
+ * buf[k * elems ...].  time_interp = 1 packs from / unpacks into the
+ * time-interpolated copies (config subcycle = 1).  Unpacking writes q and
+ * the registers; edge lengths and areas follow from dx, dy. */
+int hipclaw_ghost_pack_elems(const hipclaw_state_t *s, long long *elems);
+int hipclaw_ghost_pack(hipclaw_state_t *s, const int *slots, int n, int time_interp,
+                       double *buf);
+int hipclaw_ghost_unpack(hipclaw_state_t *s, const int *slots, int n, int time_interp,
+                         const double *buf);
+
+/* Copy the time-sync registers of slots first..first+count-1 to host memory,
+ * rstride = 16 mx meqn doubles per slot in the layout
+ * reg[((face * 4 + kind) * meqn + m) * mx + pos] (kind 0..3 = fm, fp, edge
+ * flux at the first interior cell, edge flux at the first ghost cell).  The
+ * counterpart of reading fclaw2d_clawpatch_get_registers
+ * (src/patches/clawpatch/fclaw2d_clawpatch.cpp) on the host. */
+int hipclaw_get_registers(hipclaw_state_t *s, int first, int count, double *out);
+
 /* nsteps of (update every patch at every level, ghost fill, and with
  * time_sync the conservation fix-up), all resident on the GPU; *maxcfl
  * receives the maximum over the steps.  Equivalent to

@@ -131,6 +131,14 @@ int oracle_regrid_remap(int mx, int my, int mbc, int meqn, int nold, const int *
                         const int *xold, const int *yold, const double *qold, int nnew,
                         const int *lnew, const int *xnew, const int *ynew, double *qnew);
 
+/* ---------------- ghost-patch message ---------------- */
+size_t oracle_ghost_pack_elems(int mx, int my, int mbc, int meqn, int refratio, int packarea,
+                               int packregisters);
+long oracle_local_ghost_pack(int mx, int my, int mbc, int meqn, int mint, double *q,
+                             double *area, double *pack, int packmode);
+long oracle_register_pack(int mx, int my, int meqn, double dx, double dy, double *reg,
+                          double *pack, int unpack);
+
 /* Kahan mass sum per field in patch order:
  * src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_diagnostics.f:13-55 */
 void oracle_mass(const oracle_domain_t *d, const double *q, double *sum);

@@ -67,6 +67,12 @@ def oracle():
         lib.oracle_advance_subcycle.restype = C.c_double
         lib.oracle_patch_exceeds.argtypes = [C.c_int] * 3 + [_dp, C.c_int] + [C.c_double] * 3
         lib.oracle_regrid_remap.argtypes = [C.c_int] * 5 + [_ip] * 3 + [_dp, C.c_int] + [_ip] * 3 + [_dp]
+        lib.oracle_ghost_pack_elems.argtypes = [C.c_int] * 7
+        lib.oracle_ghost_pack_elems.restype = C.c_size_t
+        lib.oracle_local_ghost_pack.argtypes = [C.c_int] * 5 + [_dp, C.c_void_p, _dp, C.c_int]
+        lib.oracle_local_ghost_pack.restype = C.c_long
+        lib.oracle_register_pack.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [_dp, _dp, C.c_int]
+        lib.oracle_register_pack.restype = C.c_long
         lib.oracle_compute_slopes.argtypes = [C.c_double, C.c_double]
         lib.oracle_compute_slopes.restype = C.c_double
         lib.oracle_interpolate2fine.argtypes = [C.c_int] * 4 + [_dp, _dp, C.c_int]
@@ -186,6 +192,7 @@ def ref_amr():
         for name in ("fclaw2d_clawpatch46_fort_interpolate2fine_",
                      "fclaw2d_clawpatch46_fort_average2coarse_"):
             getattr(lib, name).argtypes = [ip] * 4 + [_dp, _dp, _dp, _dp, ip, ip]
+        lib.fclaw2d_clawpatch46_fort_local_ghost_pack_.argtypes = [ip] * 5 + [_dp, _dp, _dp] + [ip] * 3
         lib.fclaw2d_clawpatch_compute_slopes_.argtypes = [dp, dp, ip]
         lib.fclaw2d_clawpatch_compute_slopes_.restype = C.c_double
         _ref_amr = lib
@@ -215,3 +222,13 @@ def compute_slopes_ref(sl, sr, mth=5):
     return ref_amr().fclaw2d_clawpatch_compute_slopes_(C.byref(C.c_double(sl)),
                                                        C.byref(C.c_double(sr)),
                                                        C.byref(C.c_int(mth)))
+
+
+def local_ghost_pack_ref(mx, my, mbc, meqn, mint, q, area, qpack, packmode):
+    """Reference fclaw2d_clawpatch46_fort_local_ghost_pack; returns ierror."""
+    i = lambda v: C.byref(C.c_int(v))
+    ierr = C.c_int(-1)
+    ref_amr().fclaw2d_clawpatch46_fort_local_ghost_pack_(i(mx), i(my), i(mbc), i(meqn), i(mint),
+                                                         q, area, qpack, i(len(qpack)),
+                                                         i(packmode), C.byref(ierr))
+    return ierr.value
