@@ -16,6 +16,7 @@
 #include "../../include/fc2d_hipclaw.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -979,4 +980,194 @@ void fclaw2d_regrid(fclaw2d_global_t *glob)
     }
     free(kind); free(src);
     glob->count_amr_regrid++;
+}
+
+/* ------------------------------------------------------------------ VTK output */
+
+/* The byte layout of a frame (UnstructuredGrid, raw appended data with UInt64
+ * byte counts, quads of type 9) is the reference's, so that ParaView/VisIt
+ * tooling written for ForestClaw reads these files unchanged. */
+
+void fclaw2d_output_vtk_coordinate_cb(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                      int patchno, char *a)
+{
+    int mx, my, mbc, i, j;
+    double xl, yl, dx, dy, *d = (double *) a;
+    (void) blockno; (void) patchno;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    for (j = 0; j <= my; j++)
+        for (i = 0; i <= mx; i++) {
+            *d++ = xl + i * dx;
+            *d++ = yl + j * dy;
+            *d++ = 0;
+        }
+}
+
+void fclaw2d_output_vtk_value_cb(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                 int patchno, char *a)
+{
+    int mx, my, mbc, meqn, i, j, k;
+    double xl, yl, dx, dy, *q;
+    float *f = (float *) a;
+    (void) blockno; (void) patchno;
+    fclaw2d_clawpatch_grid_data(glob, patch, &mx, &my, &mbc, &xl, &yl, &dx, &dy);
+    fclaw2d_clawpatch_soln_data(glob, patch, &q, &meqn);
+    for (j = 0; j < my; j++)
+        for (i = 0; i < mx; i++)
+            for (k = 0; k < meqn; k++)
+                *f++ = (float) q[((size_t) k * (my + 2 * mbc) + j + mbc) * (mx + 2 * mbc) + i + mbc];
+}
+
+/* values converted on the device, handed out patch by patch */
+static const float *vtk_device_values;
+
+static void vtk_device_value_cb(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                int patchno, char *a)
+{
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    const size_t per = (size_t) o->mx * o->my * o->meqn;
+    (void) blockno; (void) patchno;
+    memcpy(a, vtk_device_values + per * (size_t) patch->slot, per * sizeof(float));
+}
+
+typedef enum {
+    VTK_POSITION, VTK_CONNECTIVITY, VTK_OFFSETS, VTK_TYPES, VTK_MPIRANK, VTK_BLOCKNO,
+    VTK_PATCHNO, VTK_MEQN, VTK_NFIELDS
+} vtk_field_t;
+
+int fclaw2d_vtk_write_file(fclaw2d_global_t *glob, const char *basename, int mx, int my,
+                           int meqn, double vtkspace, int vtkwrite,
+                           fclaw2d_vtk_patch_data_t coordinate_cb,
+                           fclaw2d_vtk_patch_data_t value_cb)
+{
+    fclaw2d_domain_t *d = glob->domain;
+    const long long np = d->local_num_patches;
+    const long long ppp = (long long) (mx + 1) * (my + 1), cpp = (long long) mx * my;
+    const long long npoints = ppp * np, ncells = cpp * np, nconn = 4 * (ncells + 1);
+    const int fits32 = npoints <= INT32_MAX && nconn <= INT32_MAX;
+    const int isz = fits32 ? 4 : 8, ndsize = 8;
+    const char *itype = fits32 ? "Int32" : "Int64";
+    long long psize[VTK_NFIELDS], offset[VTK_NFIELDS + 1];
+    char fname[4096], *buf;
+    FILE *f;
+    int field, b, p, ok;
+    (void) vtkspace; (void) vtkwrite;
+    psize[VTK_POSITION] = ppp * 3 * 8;
+    psize[VTK_CONNECTIVITY] = cpp * 4 * isz;
+    psize[VTK_OFFSETS] = cpp * isz;
+    psize[VTK_TYPES] = cpp;
+    psize[VTK_MPIRANK] = cpp * 4;
+    psize[VTK_BLOCKNO] = cpp * 4;
+    psize[VTK_PATCHNO] = cpp * isz;
+    psize[VTK_MEQN] = cpp * meqn * 4;
+    offset[0] = 0;
+    for (field = 0; field < VTK_NFIELDS; field++)
+        offset[field + 1] = offset[field] + ndsize + psize[field] * np;
+    snprintf(fname, sizeof fname, "%s.vtu", basename);
+    f = fopen(fname, "wb");
+    if (!f) return -1;
+    ok = fprintf(f, "<?xml version=\"1.0\"?>\n"
+                    "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+                    "header_type=\"UInt64\" >\n"
+                    " <UnstructuredGrid>\n"
+                    "  <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n"
+                    "   <Points>\n"
+                    "    <DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" "
+                    "format=\"appended\" offset=\"%lld\">\n"
+                    "    </DataArray>\n"
+                    "   </Points>\n"
+                    "   <Cells>\n", npoints, ncells, offset[VTK_POSITION]) >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"%s\" Name=\"connectivity\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", itype, offset[VTK_CONNECTIVITY]) >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"%s\" Name=\"offsets\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", itype, offset[VTK_OFFSETS]) >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"UInt8\" Name=\"types\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", offset[VTK_TYPES]) >= 0;
+    ok = ok && fprintf(f, "   </Cells>\n   <CellData Scalars=\"mpirank,blockno,patchno\" "
+                          "Fields=\"meqn\">\n") >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"Int32\" Name=\"mpirank\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", offset[VTK_MPIRANK]) >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"Int32\" Name=\"blockno\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", offset[VTK_BLOCKNO]) >= 0;
+    ok = ok && fprintf(f, "    <DataArray type=\"%s\" Name=\"patchno\" format=\"appended\" "
+                          "offset=\"%lld\">\n    </DataArray>\n", itype, offset[VTK_PATCHNO]) >= 0;
+    if (meqn == 1)
+        ok = ok && fprintf(f, "    <DataArray type=\"Float32\" Name=\"meqn\" format=\"appended\" "
+                              "offset=\"%lld\">\n", offset[VTK_MEQN]) >= 0;
+    else
+        ok = ok && fprintf(f, "    <DataArray type=\"Float32\" Name=\"meqn\" NumberOfComponents=\"%d\" "
+                              "format=\"appended\" offset=\"%lld\">\n", meqn, offset[VTK_MEQN]) >= 0;
+    ok = ok && fprintf(f, "    </DataArray>\n   </CellData>\n   <PointData>\n   </PointData>\n"
+                          "  </Piece>\n </UnstructuredGrid>\n <AppendedData encoding=\"raw\">\n  _") >= 0;
+    buf = (char *) malloc((size_t) (psize[VTK_POSITION] > psize[VTK_CONNECTIVITY] ? psize[VTK_POSITION] : psize[VTK_CONNECTIVITY]) +
+                          (size_t) psize[VTK_MEQN] + 64);
+    if (!buf) { fclose(f); return -1; }
+    for (field = 0; ok && field < VTK_NFIELDS; field++) {
+        const unsigned long long bytes = (unsigned long long) (psize[field] * np);
+        ok = fwrite(&bytes, (size_t) ndsize, 1, f) == 1;
+        /* patches in global order: block, then patch within the block */
+        for (b = 0; ok && b < d->num_blocks; b++)
+            for (p = 0; ok && p < d->blocks[b].num_patches; p++) {
+                fclaw2d_patch_t *pt = &d->blocks[b].patches[p];
+                const long long gp = d->blocks[b].patchno_offset + p;
+                long long c, l;
+                int i, j;
+                switch (field) {
+                case VTK_POSITION: coordinate_cb(glob, pt, b, p, buf); break;
+                case VTK_CONNECTIVITY:
+                    for (j = 0, c = 0; j < my; j++)
+                        for (i = 0; i < mx; i++) {
+                            const long long base = ppp * gp + i + (long long) j * (mx + 1);
+                            const long long v[4] = {base, base + 1, base + mx + 2, base + mx + 1};
+                            for (l = 0; l < 4; l++, c++) {
+                                if (fits32) ((int32_t *) buf)[c] = (int32_t) v[l];
+                                else ((int64_t *) buf)[c] = v[l];
+                            }
+                        }
+                    break;
+                case VTK_OFFSETS:
+                    for (c = 0; c < cpp; c++) {
+                        const long long v = 4 * (cpp * gp + 1) + 4 * c;
+                        if (fits32) ((int32_t *) buf)[c] = (int32_t) v;
+                        else ((int64_t *) buf)[c] = v;
+                    }
+                    break;
+                case VTK_TYPES: memset(buf, 9, (size_t) cpp); break;   /* VTK_QUAD */
+                case VTK_MPIRANK: for (c = 0; c < cpp; c++) ((int32_t *) buf)[c] = 0; break;
+                case VTK_BLOCKNO: for (c = 0; c < cpp; c++) ((int32_t *) buf)[c] = b; break;
+                case VTK_PATCHNO:
+                    for (c = 0; c < cpp; c++) {
+                        if (fits32) ((int32_t *) buf)[c] = (int32_t) gp;
+                        else ((int64_t *) buf)[c] = gp;
+                    }
+                    break;
+                default: value_cb(glob, pt, b, p, buf); break;
+                }
+                ok = fwrite(buf, (size_t) psize[field], 1, f) == 1;
+            }
+    }
+    free(buf);
+    ok = ok && fprintf(f, "\n </AppendedData>\n</VTKFile>\n") >= 0;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+void fclaw2d_clawpatch_output_vtk(fclaw2d_global_t *glob, int iframe)
+{
+    const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
+    const int n = glob->domain->local_num_patches;
+    char basename[4096];
+    float *vals = (float *) malloc(sizeof(float) * (size_t) o->mx * o->my * o->meqn * (size_t) (n > 0 ? n : 1));
+    if (!vals) {
+        fprintf(stderr, "fclaw2d_clawpatch_output_vtk: out of memory\n");
+        exit(1);
+    }
+    device_check(hipclaw_vtk_values(glob->state, 0, n, vals), "hipclaw_vtk_values");
+    snprintf(basename, sizeof basename, "%s_frame_%04d", glob->prefix ? glob->prefix : "fort", iframe);
+    vtk_device_values = vals;
+    if (fclaw2d_vtk_write_file(glob, basename, o->mx, o->my, o->meqn, 0.0, 0,
+                               fclaw2d_output_vtk_coordinate_cb, vtk_device_value_cb))
+        fprintf(stderr, "fclaw2d_clawpatch_output_vtk: cannot write %s.vtu\n", basename);
+    vtk_device_values = NULL;
+    free(vals);
 }

@@ -566,6 +566,23 @@ int hipclaw_ghost_unpack(hipclaw_state_t* s, const int* slots, int n, int time_i
     return ghost_message(s, slots, n, time_interp, const_cast<double*>(buf), 1);
 }
 
+int hipclaw_vtk_values(hipclaw_state_t* s, int first, int count, float* out) {
+    if (!s || first < 0 || count < 0 || first + count > s->npatch || (count > 0 && !out))
+        return fail(HIPCLAW_EINVAL, "hipclaw_vtk_values: bad arguments%s");
+    if (count == 0) return HIPCLAW_OK;
+    const size_t n = (size_t) count * s->cfg.mx * s->cfg.my * s->cfg.meqn;
+    float* dev = nullptr;
+    HIP_TRY(hipMalloc((void**) &dev, sizeof(float) * n));
+    hipError_t e = (hipError_t) hipclaw::launch_vtk_cell_values(s->q[s->cur], first, count, s->pstride,
+                                                                s->cfg.mx, s->cfg.my, s->cfg.mbc,
+                                                                s->cfg.meqn, dev, s->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, sizeof(float) * n, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    hipFree(dev);
+    if (e != hipSuccess) return hipfail(e, "vtk_cell_values");
+    return HIPCLAW_OK;
+}
+
 int hipclaw_get_registers(hipclaw_state_t* s, int first, int count, double* out) {
     if (!s || first < 0 || count < 0 || first + count > s->npatch || (count > 0 && !out))
         return fail(HIPCLAW_EINVAL, "hipclaw_get_registers: bad arguments%s");

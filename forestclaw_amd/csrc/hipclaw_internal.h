@@ -81,6 +81,8 @@ int launch_ghost_message(double* q, double* reg, const int* slots, int n, const 
                          int nband, long long pstride, long long rstride, int mx, int my, int mbc,
                          int meqn, const double* dxs, const double* dys, double* buf,
                          long long elems, int unpack, hipStream_t st);
+int launch_vtk_cell_values(const double* q, int first, int count, long long pstride, int mx, int my,
+                           int mbc, int meqn, float* out, hipStream_t st);
 int launch_timeinterp(const double* q, const double* qlast, double* qti, const int* slots, int n,
                       long long pstride, int mx, int my, int mbc, int meqn, double alpha,
                       hipStream_t st);

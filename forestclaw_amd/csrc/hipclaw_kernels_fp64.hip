@@ -453,6 +453,33 @@ int launch_ghost_message(double* q, double* reg, const int* slots, int n, const 
     return (int) hipGetLastError();
 }
 
+// Cell values of the VTK "meqn" array: per patch, cells j = 1..my, i = 1..mx,
+// then the meqn fields of the cell, as float (fclaw2d_output_vtk_value_cb,
+// src/patches/clawpatch/fclaw2d_clawpatch_output_vtk.c:781-808).
+__global__ void __launch_bounds__(256)
+vtk_cell_values(const double* __restrict__ q, int first, long long pstride, int mx, int my,
+                int mbc, int meqn, float* __restrict__ out) {
+    const int p = blockIdx.y;
+    const int per = mx * my * meqn;
+    const double* qp = q + (long long) (first + p) * pstride;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < per; e += gridDim.x * blockDim.x) {
+        const int cell = e / meqn, k = e - cell * meqn;
+        const int j = cell / mx, i = cell - j * mx;
+        out[(long long) p * per + e] = (float) qp[((long long) k * nyp + j + mbc) * nxp + i + mbc];
+    }
+}
+
+int launch_vtk_cell_values(const double* q, int first, int count, long long pstride, int mx, int my,
+                           int mbc, int meqn, float* out, hipStream_t st) {
+    if (count <= 0) return 0;
+    const int per = mx * my * meqn;
+    dim3 grid((unsigned) std::min(64, (per + 255) / 256), count);
+    hipLaunchKernelGGL(vtk_cell_values, grid, dim3(256), 0, st, q, first, pstride, mx, my, mbc,
+                       meqn, out);
+    return (int) hipGetLastError();
+}
+
 int launch_timeinterp(const double* q, const double* qlast, double* qti, const int* slots, int n,
                       long long pstride, int mx, int my, int mbc, int meqn, double alpha,
                       hipStream_t st) {

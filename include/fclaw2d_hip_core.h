@@ -105,6 +105,7 @@ typedef struct fclaw2d_global {
     double refine_threshold, coarsen_threshold;
     int threshold_variable;
     int count_amr_regrid, count_amr_new_domain;   /* src/fclaw2d_regrid.c:321, 333 */
+    const char *prefix;            /* output file prefix, NULL = "fort" (src/fclaw_options.c:177) */
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -186,6 +187,27 @@ int fclaw2d_clawpatch_output_ascii_patch(const char *fname_q, int mx, int my, in
                                          double xlower, double ylower, double dx, double dy,
                                          const double *q, int patch_num, int level, int blockno,
                                          int mpirank);
+
+/* VTK frames as the reference writes them: <prefix>_frame_XXXX.vtu, an
+ * UnstructuredGrid with raw appended data (Position, connectivity, offsets,
+ * types, mpirank, blockno, patchno, meqn), fclaw2d_vtk_write_file and
+ * fclaw2d_clawpatch_output_vtk
+ * (src/patches/clawpatch/fclaw2d_clawpatch_output_vtk.c:597-900).  The
+ * callbacks fill one patch's points (3 doubles each, (mx+1)(my+1) of them)
+ * or cell values (mx my meqn floats) into a.  fclaw2d_clawpatch_output_vtk
+ * converts the values on the device (hipclaw_vtk_values); the callbacks below
+ * read the host views.  Returns 0 or -1 on a file error. */
+typedef void (*fclaw2d_vtk_patch_data_t)(fclaw2d_global_t *glob, fclaw2d_patch_t *this_patch,
+                                         int this_block_idx, int this_patch_idx, char *a);
+int fclaw2d_vtk_write_file(fclaw2d_global_t *glob, const char *basename, int mx, int my,
+                           int meqn, double vtkspace, int vtkwrite,
+                           fclaw2d_vtk_patch_data_t coordinate_cb,
+                           fclaw2d_vtk_patch_data_t value_cb);
+void fclaw2d_output_vtk_coordinate_cb(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                      int patchno, char *a);
+void fclaw2d_output_vtk_value_cb(fclaw2d_global_t *glob, fclaw2d_patch_t *patch, int blockno,
+                                 int patchno, char *a);
+void fclaw2d_clawpatch_output_vtk(fclaw2d_global_t *glob, int iframe);
 
 /* Conservation fix-up after all levels reached the same time
  * (src/fclaw2d_time_sync.c:161-202); needs glob->time_sync set before

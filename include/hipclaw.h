@@ -214,6 +214,13 @@ int hipclaw_ghost_pack(hipclaw_state_t *s, const int *slots, int n, int time_int
 int hipclaw_ghost_unpack(hipclaw_state_t *s, const int *slots, int n, int time_interp,
                          const double *buf);
 
+/* Cell values of slots first..first+count-1 as float, in the order of the
+ * VTK "meqn" cell array (per patch: j = 1..my, i = 1..mx, then the meqn
+ * fields), converted on the device so that half the bytes cross PCIe.
+ * Replaces fclaw2d_output_vtk_value_cb
+ * (src/patches/clawpatch/fclaw2d_clawpatch_output_vtk.c:781-808). */
+int hipclaw_vtk_values(hipclaw_state_t *s, int first, int count, float *out);
+
 /* Copy the time-sync registers of slots first..first+count-1 to host memory,
  * rstride = 16 mx meqn doubles per slot in the layout
  * reg[((face * 4 + kind) * meqn + m) * mx + pos] (kind 0..3 = fm, fp, edge

@@ -53,7 +53,8 @@ class Global(C.Structure):
                 ("time_sync", C.c_int), ("subcycle", C.c_int), ("advance_one_step", C.c_int),
                 ("minlevel", C.c_int), ("maxlevel", C.c_int), ("refine_threshold", C.c_double),
                 ("coarsen_threshold", C.c_double), ("threshold_variable", C.c_int),
-                ("count_amr_regrid", C.c_int), ("count_amr_new_domain", C.c_int)]
+                ("count_amr_regrid", C.c_int), ("count_amr_new_domain", C.c_int),
+                ("prefix", C.c_char_p)]
 
 
 class Options(C.Structure):
