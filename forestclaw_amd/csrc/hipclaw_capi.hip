@@ -52,6 +52,7 @@ struct SyncProgram {
     int* start = nullptr;     // [ngroups + 1]
     int* resets = nullptr;    // [nresets][2] (slot, face)
     int ngroups = 0, nresets = 0, ops_cap = 0, groups_cap = 0, resets_cap = 0;
+    int maxgroup = 0;         // most ops of one destination
 };
 
 struct hipclaw_state {
@@ -424,6 +425,8 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     HIP_TRY(hipStreamSynchronize(s->st));
     y.ngroups = ngroups;
     y.nresets = nresets;
+    y.maxgroup = 0;
+    for (int g = 0; g < ngroups; g++) y.maxgroup = std::max(y.maxgroup, start[g + 1] - start[g]);
     return HIPCLAW_OK;
 }
 
@@ -438,6 +441,7 @@ static int time_sync_async(hipclaw_state_t* s, int k) {
     a.ops = y.ops;
     a.start = y.start;
     a.ngroups = y.ngroups;
+    a.maxgroup = y.maxgroup;
     a.resets = y.resets;
     a.nresets = y.nresets;
     a.q = s->q[s->cur];

@@ -48,6 +48,7 @@ struct SyncArgs {
     const int* ops;      // [nops][8]: type, dst, src, face, ti, tj, -, -
     const int* start;    // [ngroups + 1]: ops of group g are start[g] .. start[g+1]-1
     int ngroups;
+    int maxgroup;        // most ops in one group
     double* q;
     const double* reg;   // flux registers, layout of StepArgs::reg
     const double* dxs;

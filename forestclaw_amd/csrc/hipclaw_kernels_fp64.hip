@@ -38,9 +38,10 @@ __device__ __forceinline__ void interp_cell(const double* qc, double* qf, int m,
 }
 
 // One wave per ghost op, OPW ops (waves) per workgroup.  Ops of one phase
-// write disjoint ghost cells and read only cells no op of the phase writes.
-// The physical-BC op sequences its faces with workgroup barriers, so phases
-// that hold such ops run with OPW = 1.
+// write disjoint ghost cells and read only cells no op of the phase writes,
+// so each lane issues the loads of up to U elements before their stores: the
+// source and destination slabs are the same allocation, and without this the
+// compiler must wait for every store before the next load.
 template <int OPW>
 __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
     const int o = blockIdx.x * OPW + (OPW > 1 ? (int) (threadIdx.x >> 6) : 0);
@@ -52,122 +53,187 @@ __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
     double* qd = ((op.dst & TI) ? g.qti : g.q) + (long long) (op.dst & ~TI) * g.pstride;
     const double* qs = ((op.src & TI) ? g.qti : g.q) + (long long) (op.src & ~TI) * g.pstride;
     const int t = threadIdx.x & 63;
+    constexpr int U = 4;
 
     switch (op.type) {
-    case GHOST_COPY_FACE: {  // fclaw2d_clawpatch46_copy.f:40-84
-        const int len = (op.which < 2) ? my : mx;
-        for (int e = t; e < meqn * mbc * len; e += 64) {
-            const int m = e / (mbc * len), r = e - m * mbc * len;
-            const int l = r / mbc, b = r - l * mbc;  // b: ghost layer 0..mbc-1
-            int i1, j1;
-            if (op.which == 0) { i1 = 1 - mbc + b; j1 = 1 + l; }
-            else if (op.which == 1) { i1 = mx + 1 + b; j1 = 1 + l; }
-            else if (op.which == 2) { i1 = 1 + l; j1 = 1 - mbc + b; }
-            else { i1 = 1 + l; j1 = my + 1 + b; }
-            qd[QIX(i1, j1, m)] = qs[QIX(i1 + op.ti, j1 + op.tj, m)];
-        }
-        break;
-    }
+    case GHOST_COPY_FACE:      // fclaw2d_clawpatch46_copy.f:40-84
     case GHOST_COPY_CORNER: {  // fclaw2d_clawpatch46_copy.f:95-133
-        for (int e = t; e < meqn * mbc * mbc; e += 64) {
-            const int m = e / (mbc * mbc), r = e - m * mbc * mbc;
-            int i1, j1;
-            corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, i1, j1);
-            qd[QIX(i1, j1, m)] = qs[QIX(i1 + op.ti, j1 + op.tj, m)];
-        }
-        break;
-    }
-    case GHOST_AVERAGE_FACE: {  // fclaw2d_clawpatch46_average.f:61-174 (manifold = 0)
+        const bool face = op.type == GHOST_COPY_FACE;
         const int len = (op.which < 2) ? my : mx;
-        for (int e = t; e < meqn * mbc * len; e += 64) {
-            const int m = e / (mbc * len), r = e - m * mbc * len;
-            const int l = r / mbc, b = r - l * mbc + 1;  // b = ibc/jbc 1..mbc
-            int ic, jc;
-            if (op.which < 2) { ic = (op.which == 0) ? 1 - b : mx + b; jc = 1 + l; }
-            else { ic = 1 + l; jc = (op.which == 2) ? 1 - b : my + b; }
-            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
-            // only cells whose four children are interior to this fine grid
-            if (fi < 1 || fi + 1 > mx || fj < 1 || fj + 1 > my) continue;
-            double sum = 0;
-            sum = sum + qs[QIX(fi, fj, m)];
-            sum = sum + qs[QIX(fi + 1, fj, m)];
-            sum = sum + qs[QIX(fi, fj + 1, m)];
-            sum = sum + qs[QIX(fi + 1, fj + 1, m)];
-            qd[QIX(ic, jc, m)] = sum / 4.0;
-        }
-        break;
-    }
-    case GHOST_AVERAGE_CORNER: {  // fclaw2d_clawpatch46_average.f:225-270
-        for (int e = t; e < meqn * mbc * mbc; e += 64) {
-            const int m = e / (mbc * mbc), r = e - m * mbc * mbc;
-            int ic, jc;
-            corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, ic, jc);
-            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
-            double sum = 0;
-            sum = sum + qs[QIX(fi, fj, m)];
-            sum = sum + qs[QIX(fi + 1, fj, m)];
-            sum = sum + qs[QIX(fi, fj + 1, m)];
-            sum = sum + qs[QIX(fi + 1, fj + 1, m)];
-            qd[QIX(ic, jc, m)] = sum / 4.0;
-        }
-        break;
-    }
-    case GHOST_INTERP_FACE: {  // fclaw2d_clawpatch46_interpolate.f:91-197; dst = fine, src = coarse
-        const int hb = mbc / 2;
-        for (int e = t; e < meqn * hb * mx; e += 64) {  // the reference bounds the face loop by mx
-            const int m = e / (hb * mx), r = e - m * hb * mx;
-            const int l = r / hb, b = r - l * hb + 1;
-            int ic, jc;
-            if (op.which < 2) { ic = (op.which == 0) ? b : mx - b + 1; jc = 1 + l; }
-            else { ic = 1 + l; jc = (op.which == 2) ? b : my - b + 1; }
-            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
-            if (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc) continue;
-            interp_cell(qs, qd, m, ic, jc, fi, fj, nxp, nyp, mbc);
-        }
-        break;
-    }
-    case GHOST_INTERP_CORNER: {  // fclaw2d_clawpatch46_interpolate.f:263-311
-        const int hb = mbc / 2;
-        for (int e = t; e < meqn * hb * hb; e += 64) {
-            const int m = e / (hb * hb), r = e - m * hb * hb;
-            const int ibc = r / hb + 1, jbc = r % hb + 1;
-            const int ic = (op.which & 1) ? mx - ibc + 1 : ibc;
-            const int jc = (op.which & 2) ? my - jbc + 1 : jbc;
-            const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
-            interp_cell(qs, qd, m, ic, jc, fi, fj, nxp, nyp, mbc);
-        }
-        break;
-    }
-    case GHOST_PHYSBC: {  // clawpack46_bc2_default.f, faces in order 0,1,2,3
-        if constexpr (OPW > 1) break;   // never launched this way (see launch_ghost_phase)
-        for (int face = 0; face < 4; face++) {
-            const int mth = (signed char) ((op.bc >> (8 * face)) & 0xff);
-            if (mth == 1 || mth == 3) {
-                const int len = (face < 2) ? my + 2 * mbc : mx + 2 * mbc;
-                for (int e = t; e < meqn * mbc * len; e += 64) {
-                    const int m = e / (mbc * len), r = e - m * mbc * len;
-                    const int l = r / mbc, b = r - l * mbc + 1;
-                    int id, jd, is, js;
-                    if (face < 2) {
-                        jd = js = 1 - mbc + l;
-                        if (face == 0) { id = 1 - b; is = (mth == 1) ? 1 : b; }
-                        else { id = mx + b; is = (mth == 1) ? mx : mx + 1 - b; }
-                    } else {
-                        id = is = 1 - mbc + l;
-                        if (face == 2) { jd = 1 - b; js = (mth == 1) ? 1 : b; }
-                        else { jd = my + b; js = (mth == 1) ? my : my + 1 - b; }
-                    }
-                    const int mn = (face < 2) ? 1 : 2;  // normal momentum negated on walls
-                    const double v = qd[QIX(is, js, m)];
-                    qd[QIX(id, jd, m)] = (mth == 3 && m == mn) ? -v : v;
+        const int n = face ? meqn * mbc * len : meqn * mbc * mbc;
+        for (int base = t; base < n; base += 64 * U) {
+            double v[U];
+            long long di[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = base + 64 * u;
+                di[u] = -1;
+                if (e >= n) continue;
+                int m, i1, j1;
+                if (face) {
+                    m = e / (mbc * len);
+                    const int r = e - m * mbc * len, l = r / mbc, bb = r - l * mbc;  // ghost layer 0..mbc-1
+                    if (op.which == 0) { i1 = 1 - mbc + bb; j1 = 1 + l; }
+                    else if (op.which == 1) { i1 = mx + 1 + bb; j1 = 1 + l; }
+                    else if (op.which == 2) { i1 = 1 + l; j1 = 1 - mbc + bb; }
+                    else { i1 = 1 + l; j1 = my + 1 + bb; }
+                } else {
+                    m = e / (mbc * mbc);
+                    const int r = e - m * mbc * mbc;
+                    corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, i1, j1);
                 }
+                v[u] = qs[QIX(i1 + op.ti, j1 + op.tj, m)];
+                di[u] = QIX(i1, j1, m);
             }
-            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (di[u] >= 0) qd[di[u]] = v[u];
+        }
+        break;
+    }
+    case GHOST_AVERAGE_FACE:      // fclaw2d_clawpatch46_average.f:61-174 (manifold = 0)
+    case GHOST_AVERAGE_CORNER: {  // fclaw2d_clawpatch46_average.f:225-270
+        const bool face = op.type == GHOST_AVERAGE_FACE;
+        const int len = (op.which < 2) ? my : mx;
+        const int n = face ? meqn * mbc * len : meqn * mbc * mbc;
+        for (int base = t; base < n; base += 64 * U) {
+            double v[U];
+            long long di[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = base + 64 * u;
+                di[u] = -1;
+                if (e >= n) continue;
+                int m, ic, jc;
+                if (face) {
+                    m = e / (mbc * len);
+                    const int r = e - m * mbc * len, l = r / mbc, bb = r - l * mbc + 1;  // ibc/jbc 1..mbc
+                    if (op.which < 2) { ic = (op.which == 0) ? 1 - bb : mx + bb; jc = 1 + l; }
+                    else { ic = 1 + l; jc = (op.which == 2) ? 1 - bb : my + bb; }
+                } else {
+                    m = e / (mbc * mbc);
+                    const int r = e - m * mbc * mbc;
+                    corner_cell(mx, my, op.which, r / mbc + 1, r % mbc + 1, ic, jc);
+                }
+                const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+                // faces: only cells whose four children are interior to this fine grid
+                if (face && (fi < 1 || fi + 1 > mx || fj < 1 || fj + 1 > my)) continue;
+                double sum = 0;
+                sum = sum + qs[QIX(fi, fj, m)];
+                sum = sum + qs[QIX(fi + 1, fj, m)];
+                sum = sum + qs[QIX(fi, fj + 1, m)];
+                sum = sum + qs[QIX(fi + 1, fj + 1, m)];
+                v[u] = sum / 4.0;
+                di[u] = QIX(ic, jc, m);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (di[u] >= 0) qd[di[u]] = v[u];
+        }
+        break;
+    }
+    case GHOST_INTERP_FACE:      // fclaw2d_clawpatch46_interpolate.f:91-197; dst = fine, src = coarse
+    case GHOST_INTERP_CORNER: {  // fclaw2d_clawpatch46_interpolate.f:263-311
+        const bool face = op.type == GHOST_INTERP_FACE;
+        const int hb = mbc / 2;
+        const int n = face ? meqn * hb * mx : meqn * hb * hb;  // the reference bounds the face loop by mx
+        constexpr int UI = 2;
+        for (int base = t; base < n; base += 64 * UI) {
+            double c[UI], gx[UI], gy[UI];
+            long long d0[UI];
+#pragma unroll
+            for (int u = 0; u < UI; u++) {
+                const int e = base + 64 * u;
+                d0[u] = -1;
+                if (e >= n) continue;
+                int m, ic, jc;
+                if (face) {
+                    m = e / (hb * mx);
+                    const int r = e - m * hb * mx, l = r / hb, bb = r - l * hb + 1;
+                    if (op.which < 2) { ic = (op.which == 0) ? bb : mx - bb + 1; jc = 1 + l; }
+                    else { ic = 1 + l; jc = (op.which == 2) ? bb : my - bb + 1; }
+                } else {
+                    m = e / (hb * hb);
+                    const int r = e - m * hb * hb, ibc = r / hb + 1, jbc = r % hb + 1;
+                    ic = (op.which & 1) ? mx - ibc + 1 : ibc;
+                    jc = (op.which & 2) ? my - jbc + 1 : jbc;
+                }
+                const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+                if (face && (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc)) continue;
+                const double cc = qs[QIX(ic, jc, m)];
+                gx[u] = amr_slope(cc - qs[QIX(ic - 1, jc, m)], qs[QIX(ic + 1, jc, m)] - cc);
+                gy[u] = amr_slope(cc - qs[QIX(ic, jc - 1, m)], qs[QIX(ic, jc + 1, m)] - cc);
+                c[u] = cc;
+                d0[u] = QIX(fi, fj, m);
+            }
+            // fine cells (fi..fi+1, fj..fj+1), fclaw2d_clawpatch46_interpolate.f:120-137
+#pragma unroll
+            for (int u = 0; u < UI; u++) {
+                if (d0[u] < 0) continue;
+                qd[d0[u]] = c[u] + gx[u] * -0.25 + gy[u] * -0.25;
+                qd[d0[u] + 1] = c[u] + gx[u] * 0.25 + gy[u] * -0.25;
+                qd[d0[u] + nxp] = c[u] + gx[u] * -0.25 + gy[u] * 0.25;
+                qd[d0[u] + nxp + 1] = c[u] + gx[u] * 0.25 + gy[u] * 0.25;
+            }
         }
         break;
     }
     default:
         break;
+    }
+}
+
+// Physical BC of one patch per workgroup (clawpack46_bc2_default.f): faces 0
+// and 1 together, then faces 2 and 3, whose strips include the corner cells
+// the first two wrote.  Within a pair no element reads a cell the pair
+// writes (mx, my >= 2 mbc), so all loads of a lane precede its stores.
+constexpr int PHYSBC_THREADS = 256;
+__global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
+    const int o = blockIdx.x;
+    if (o >= g.nops) return;
+    const GhostOp op = g.ops[o];
+    const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    const int TI = HIPCLAW_TIME_INTERP_SLOT;
+    double* qd = ((op.dst & TI) ? g.qti : g.q) + (long long) (op.dst & ~TI) * g.pstride;
+    constexpr int U = 4;
+    for (int pair = 0; pair < 2; pair++) {
+        const int len = pair == 0 ? my + 2 * mbc : mx + 2 * mbc;
+        const int per = meqn * mbc * len, n = 2 * per;
+        for (int base = threadIdx.x; base < n; base += PHYSBC_THREADS * U) {
+            double v[U];
+            long long di[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = base + PHYSBC_THREADS * u;
+                di[u] = -1;
+                if (e >= n) continue;
+                const int face = 2 * pair + (e >= per);
+                const int mth = (signed char) ((op.bc >> (8 * face)) & 0xff);
+                if (mth != 1 && mth != 3) continue;
+                const int f = e - (e >= per ? per : 0);
+                const int m = f / (mbc * len), r = f - m * mbc * len;
+                const int l = r / mbc, b = r - l * mbc + 1;
+                int id, jd, is, js;
+                if (face < 2) {
+                    jd = js = 1 - mbc + l;
+                    if (face == 0) { id = 1 - b; is = (mth == 1) ? 1 : b; }
+                    else { id = mx + b; is = (mth == 1) ? mx : mx + 1 - b; }
+                } else {
+                    id = is = 1 - mbc + l;
+                    if (face == 2) { jd = 1 - b; js = (mth == 1) ? 1 : b; }
+                    else { jd = my + b; js = (mth == 1) ? my : my + 1 - b; }
+                }
+                const int mn = (face < 2) ? 1 : 2;  // normal momentum negated on walls
+                const double x = qd[QIX(is, js, m)];
+                v[u] = (mth == 3 && m == mn) ? -x : x;
+                di[u] = QIX(id, jd, m);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (di[u] >= 0) qd[di[u]] = v[u];
+        }
+        __syncthreads();
     }
 }
 
@@ -370,8 +436,9 @@ patch_mass(const double* __restrict__ q, const int* __restrict__ slots, int n,
 
 int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st) {
     if (g.nops <= 0) return 0;
+    // a phase holds either physical-BC ops only or exchange ops only
     if (has_physbc)
-        hipLaunchKernelGGL(ghost_phase<1>, dim3(g.nops), dim3(64), 0, st, g);
+        hipLaunchKernelGGL(physbc_phase, dim3(g.nops), dim3(PHYSBC_THREADS), 0, st, g);
     else
         hipLaunchKernelGGL(ghost_phase<4>, dim3((g.nops + 3) / 4), dim3(256), 0, st, g);
     return (int) hipGetLastError();
@@ -590,9 +657,88 @@ __global__ void __launch_bounds__(128) time_sync_groups(SyncArgs a) {
     }
 }
 
+// The same corrections in two passes: every (op, field, position) of the
+// group computes its increment into LDS at once, then every edge cell adds
+// the increments of the ops on its face(s) in op order and is written once.
+// A corner cell belongs to the x-face thread, which also takes the y-face
+// ops' increments at the matching position; the sums are the one-op-at-a-time
+// sums, operation for operation.
+__global__ void __launch_bounds__(256) time_sync_cells(SyncArgs a) {
+    extern __shared__ double inc[];   // [maxgroup][meqn][mx], then the ops' faces
+    const int g = blockIdx.x;
+    if (g >= a.ngroups) return;
+    const int mx = a.mx, mbc = a.mbc, meqn = a.meqn, nx = mx + 2 * mbc;
+    const int per = meqn * mx;
+    const int o0 = a.start[g], nop = a.start[g + 1] - o0;
+    int* oface = (int*) (inc + (long long) a.maxgroup * per);
+    const int kout = a.fluxreg ? REG_EFIN : REG_EFOUT;
+    const int dst = a.ops[8 * o0 + 1];
+    const double area = a.dxs[dst] * a.dys[dst];
+    const double* __restrict__ rt = a.reg + (long long) dst * a.rstride;
+    for (int k = threadIdx.x; k < nop; k += blockDim.x) oface[k] = a.ops[8 * (o0 + k) + 3];
+    for (int e = threadIdx.x; e < nop * per; e += blockDim.x) {
+        const int k = e / per, r = e - k * per, m = r / mx, c = r - m * mx + 1;
+        const int* op = a.ops + 8 * (o0 + k);
+        const int type = op[0], src = op[2], face = op[3], ti = op[4], tj = op[5];
+        const double* __restrict__ rn = a.reg + (long long) src * a.rstride;
+        const int kn = face ^ 1;
+        const int kfl = (face == 0 || face == 2) ? REG_FP : REG_FM;
+        auto R = [&](const double* rr, int kk, int kind, int pos) -> double {
+            return rr[((kk * 4 + kind) * meqn + m) * mx + pos];
+        };
+        const double ein = R(rt, face, REG_EFIN, c - 1);
+        const double ft = a.fluxreg ? 0.0 : R(rt, face, kfl, c - 1);
+        double v;
+        if (type == SYNC_SAMESIZE) {
+            const double ef = R(rn, kn, kout, c - 1);
+            const double fn = a.fluxreg ? 0.0 : R(rn, kn, kfl, c - 1);
+            const double delta = (face == 0 || face == 2) ? (ef - ein) - (fn - ft) : (ein - ef) - (fn - ft);
+            v = 0.5 * delta / area;
+        } else {
+            const int f0 = 2 * c - 1 + (face < 2 ? tj : ti);
+            double deltac = 0.0;
+            if (f0 >= 1 && f0 + 1 <= mx) {
+                const double ef = R(rn, kn, kout, f0 - 1) + R(rn, kn, kout, f0);
+                const double flu = a.fluxreg ? 0.0 : R(rn, kn, kfl, f0 - 1) + R(rn, kn, kfl, f0);
+                deltac = (face == 0 || face == 2) ? (ef - ein) - (flu - ft) : (ein - ef) - (flu - ft);
+            }
+            v = deltac / area;
+        }
+        inc[e] = v;
+    }
+    __syncthreads();
+    double* __restrict__ q = a.q + (long long) dst * a.pstride;
+    for (int e = threadIdx.x; e < 4 * per; e += blockDim.x) {
+        const int f = e / per, r = e - f * per, m = r / mx, c = r - m * mx + 1;
+        if (f >= 2 && (c == 1 || c == mx)) continue;   // corners: the x-face thread
+        int i, j;
+        if (f == 0) { i = 1; j = c; }
+        else if (f == 1) { i = mx; j = c; }
+        else if (f == 2) { i = c; j = 1; }
+        else { i = c; j = mx; }
+        int f2 = -1, c2 = 0;
+        if (f < 2 && (c == 1 || c == mx)) { f2 = c == 1 ? 2 : 3; c2 = f == 0 ? 1 : mx; }
+        double* qc = q + ((long long) m * nx + (j + mbc - 1)) * nx + (i + mbc - 1);
+        double v = 0.0;
+        bool touched = false;
+        for (int k = 0; k < nop; k++) {
+            const int fk = oface[k];
+            if (fk != f && fk != f2) continue;
+            const double d = inc[(k * meqn + m) * mx + (fk == f ? c : c2) - 1];
+            v = touched ? v + d : *qc + d;
+            touched = true;
+        }
+        if (touched) *qc = v;
+    }
+}
+
 int launch_time_sync(const SyncArgs& a, hipStream_t st) {
     if (a.ngroups > 0) {
-        hipLaunchKernelGGL(time_sync_groups, dim3(a.ngroups), dim3(128), 0, st, a);
+        const size_t lds = sizeof(double) * (size_t) a.maxgroup * a.meqn * a.mx + sizeof(int) * (size_t) a.maxgroup;
+        if (lds <= 32768)
+            hipLaunchKernelGGL(time_sync_cells, dim3(a.ngroups), dim3(256), lds, st, a);
+        else
+            hipLaunchKernelGGL(time_sync_groups, dim3(a.ngroups), dim3(128), 0, st, a);
         if (hipError_t e = hipGetLastError()) return (int) e;
     }
     if (a.nresets > 0) {
