@@ -156,6 +156,15 @@ struct RPShallow {
         f[mv] = q[mv] * un;
     }
 
+    // f(q) and g(q) of one cell with one division (the flux form's stage)
+    __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
+                                                  const PhysParams& pp, double* f, double* g) {
+        const double h = q[0], rh = 1.0 / h, u = q[1] * rh, v = q[2] * rh;
+        const double p = 0.5 * pp.grav * h * h;
+        f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u;
+        g[0] = q[2]; g[1] = q[1] * v;     g[2] = q[2] * v + p;
+    }
+
     // conservative flux stored in the time-sync registers, operation for
     // operation as applications/clawpack/shallow/2d/radialdam/rpn2cons_update.f:27-46
     template <int IXY>
@@ -316,6 +325,15 @@ struct RPEuler {
         f[3] = (q[3] + p) * un;
     }
 
+    // f(q) and g(q) of one cell with one division (the flux form's stage)
+    __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
+                                                  const PhysParams& pp, double* f, double* g) {
+        const double rr = 1.0 / q[0], u = q[1] * rr, v = q[2] * rr;
+        const double p = (pp.gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+        f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u; f[3] = (q[3] + p) * u;
+        g[0] = q[2]; g[1] = q[1] * v;     g[2] = q[2] * v + p; g[3] = (q[3] + p) * v;
+    }
+
     // conservative flux stored in the time-sync registers (the reference has
     // no Euler rpn2_cons; this is the physical flux, evaluated as the oracle does)
     template <int IXY>
@@ -367,6 +385,12 @@ struct RPAdvection {
     __device__ __forceinline__ static void flux(const double* q, const double* aux,
                                                 const PhysParams&, double* f) {
         f[0] = aux[IXY - 1] * q[0];
+    }
+
+    __device__ __forceinline__ static void fluxes(const double* q, const double* aux,
+                                                  const PhysParams&, double* f, double* g) {
+        f[0] = aux[0] * q[0];
+        g[0] = aux[1] * q[0];
     }
 
     // conservative flux stored in the time-sync registers: u_n q with the

@@ -573,28 +573,35 @@ step_tiles(StepArgs a) {
                 }
             }
         } else {
-            // conservative FP64 interface fluxes, shared by both neighbours
-            for (int e = threadIdx.x; e < 2 * T * (T + 1); e += NT) {
-                double qcell[MEQN], fl[MEQN];
-                if (e < T * (T + 1)) {  // x-interface s at row b: F = f(q(i-1,j)) + fm(i,j)
-                    const int b = e / (T + 1), s = e - b * (T + 1);
+            // conservative FP64 interface fluxes, shared by both neighbours:
+            // F = f(q(i-1,j)) + fm(i,j) through x-interface s at row b and
+            // G = g(q(i,j-1)) + gm(i,j) through y-interface t at column c.
+            // One thread per cell of the (T+1)^2 block to the lower left of
+            // the interfaces evaluates f and g of its cell together.
+            for (int e = threadIdx.x; e < (T + 1) * (T + 1); e += NT) {
+                const int qj = 1 + e / (T + 1), qi = 1 + e % (T + 1);  // q-tile cell
+                const bool xf = qj >= 2, yf = qi >= 2;  // x-interface s = qi-1 at row b = qj-2; y t = qj-1 at c = qi-2
+                if (!xf && !yf) continue;
+                double qcell[MEQN], fl[MEQN], gl[MEQN];
+                double acell[MAUX > 0 ? MAUX : 1];
 #pragma unroll
-                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, s + 1, b + 2);
-                    RP::template flux<1>(qcell, nullptr, a.pp, fl);
+                for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
+#pragma unroll
+                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) acell[m] = (double) SA(m, qi, qj);
+                RP::fluxes(qcell, acell, a.pp, fl, gl);
+                if (xf) {
+                    const int s = qi - 1, b = qj - 2;
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
                         const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s]) + AB(1, m, s, b)) + AB(0, m, s + 1, b);
                         sm.fx[(m * T + b) * (T + 1) + s] = fl[m] + (double) fmv;
                     }
-                } else {                // y-interface t at column c: G = g(q(i,j-1)) + gm(i,j)
-                    const int f = e - T * (T + 1);
-                    const int t = f / T, c = f - t * T;
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, c + 2, t + 1);
-                    RP::template flux<2>(qcell, nullptr, a.pp, fl);
+                }
+                if (yf) {
+                    const int t = qj - 1, c = qi - 2;
 #pragma unroll
                     for (int m = 0; m < MEQN; m++)
-                        sm.fy[(m * (T + 1) + t) * T + c] = fl[m] + (double) GT(m, t, c);
+                        sm.fy[(m * (T + 1) + t) * T + c] = gl[m] + (double) GT(m, t, c);
                 }
             }
             __syncthreads();
