@@ -113,11 +113,12 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
     return HIPCLAW_OK;
 }
 
-// Tile sizes measured fastest on MI355X for the update kernel (the tile sweep of
-// the benchmark forest, mx = 16 and 32): FP64 16 x 16, FP32-bulk 8 x 8.
+// Tile size measured fastest on MI355X for the update kernel (the tile sweep of
+// the benchmark forest, mx = 16 and 32): 16 x 16 with four waves per workgroup,
+// one per SIMD, for both forms; smaller patches take 8 x 8.
 static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
-    int t = (c->precision == HIPCLAW_FP64) ? 16 : 8;
+    int t = 16;
     while (t > 8 && (t > m || !hipclaw::step_supported(c->solver, c->precision, t))) t >>= 1;
     return t;
 }

@@ -27,7 +27,7 @@ template <class RP, bool WITH32 = true>
 static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return launch_one<RP, 16, 6>(a, st);
+    case 16: return launch_one<RP, 16, 4>(a, st);
     case 32:
         if constexpr (WITH32) return launch_one<RP, 32, 16>(a, st);
         return (int) hipErrorInvalidValue;

@@ -777,7 +777,7 @@ template <class RP>
 static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return launch_one<RP, 16, 6>(a, st);
+    case 16: return launch_one<RP, 16, 4>(a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
