@@ -184,6 +184,45 @@ def pmc_summary(fetch_dir, write_dir, cells):
     return "\n".join(lines)
 
 
+VALU_COUNTERS = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
+                 "SQ_WAIT_ANY SQ_LDS_BANK_CONFLICT")
+CLOCK_GHZ = 2.4   # MI355X peak engine clock
+SIMDS = 256 * 4
+
+
+def valu_summary(pmc_dir, cells):
+    """Per-dispatch instruction counters of the step kernels from one
+    rocprofv3 --pmc run of --profile-run with VALU_COUNTERS: vector
+    instructions per cell update and the share of the SIMDs' issue cycles
+    they fill over the dispatch (wave64 VALU = 4 cycles of a 16-lane SIMD)."""
+    import csv
+    import glob as G
+    per = {}
+    span = {}
+    for f in G.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            k = r["Kernel_Name"]
+            if "step_tiles" not in k:
+                continue
+            per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            span.setdefault(k, {})[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    lines = ["counters: %s (one run); clock %.1f GHz, %d SIMDs" % (VALU_COUNTERS, CLOCK_GHZ, SIMDS)]
+    for k in sorted(per):
+        c = {n: sum(v) / len(v) for n, v in per[k].items()}
+        dur_ns = sum(span[k].values()) / len(span[k])
+        valu = c.get("SQ_INSTS_VALU", 0.0)
+        util = valu * 4 / (SIMDS * dur_ns * CLOCK_GHZ)
+        lines.append("kernel=%s dispatches=%d duration_us=%.1f waves=%.0f valu_per_cell=%.2f "
+                     "valu_lane_instr_per_cell=%.0f salu_per_cell=%.2f lds_per_cell=%.2f "
+                     "valu_issue_share=%.3f wait_any_share=%.3f lds_bank_conflict_per_cell=%.2f"
+                     % (k.split("(")[0][:120], len(span[k]), dur_ns / 1e3, c.get("SQ_WAVES", 0),
+                        valu / cells, 64 * valu / cells, c.get("SQ_INSTS_SALU", 0) / cells,
+                        c.get("SQ_INSTS_LDS", 0) / cells, util,
+                        c.get("SQ_WAIT_ANY", 0) / max(c.get("SQ_WAVE_CYCLES", 1), 1),
+                        c.get("SQ_LDS_BANK_CONFLICT", 0) / cells))
+    return "\n".join(lines)
+
+
 def committed_traffic(workload, kernel_tag):
     """HBM bytes per launch of the step kernel from the committed counter summary
     of this workload (None if there is none for this kernel)."""
@@ -215,6 +254,8 @@ def main():
                     help="skip the coarse/fine conservation fix-up (on by default)")
     ap.add_argument("--profile-run", action="store_true",
                     help="run the workload's step kernels a few times for rocprofv3 (no JSON)")
+    ap.add_argument("--valu-summary", metavar="PMC_DIR",
+                    help="summarise a rocprofv3 --pmc run of --profile-run with VALU_COUNTERS")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --profile-run")
     args = ap.parse_args()
@@ -240,6 +281,10 @@ def main():
     if args.pmc_summary:
         print("workload=%s cells=%d" % (workload, cells))
         print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], cells))
+        return
+    if args.valu_summary:
+        print("workload=%s cells=%d" % (workload, cells))
+        print(valu_summary(args.valu_summary, cells))
         return
 
     device = fa.initialize_device(0)

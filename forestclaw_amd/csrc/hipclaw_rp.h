@@ -156,9 +156,12 @@ struct RPShallow {
         f[mv] = q[mv] * un;
     }
 
-    // f(q) and g(q) of one cell with one division (the flux form's stage)
+    // f(q) and g(q) of one cell with one division (the flux form's update).
+    // Never contracted into FMAs: both cells beside an interface evaluate it
+    // and must get the same bits.
     __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
                                                   const PhysParams& pp, double* f, double* g) {
+#pragma clang fp contract(off)
         const double h = q[0], rh = 1.0 / h, u = q[1] * rh, v = q[2] * rh;
         const double p = 0.5 * pp.grav * h * h;
         f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u;
@@ -325,9 +328,12 @@ struct RPEuler {
         f[3] = (q[3] + p) * un;
     }
 
-    // f(q) and g(q) of one cell with one division (the flux form's stage)
+    // f(q) and g(q) of one cell with one division (the flux form's update).
+    // Never contracted into FMAs: both cells beside an interface evaluate it
+    // and must get the same bits.
     __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
                                                   const PhysParams& pp, double* f, double* g) {
+#pragma clang fp contract(off)
         const double rr = 1.0 / q[0], u = q[1] * rr, v = q[2] * rr;
         const double p = (pp.gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
         f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u; f[3] = (q[3] + p) * u;
@@ -389,6 +395,7 @@ struct RPAdvection {
 
     __device__ __forceinline__ static void fluxes(const double* q, const double* aux,
                                                   const PhysParams&, double* f, double* g) {
+#pragma clang fp contract(off)
         f[0] = aux[0] * q[0];
         g[0] = aux[1] * q[0];
     }

@@ -132,16 +132,20 @@ struct TileSmem {
     static constexpr int QS = QW | 1;           // padded row stride
     static constexpr int QP = QW * QS;          // one field
     static constexpr int GS = T + 1;            // padded row stride of gt and ab
+    static constexpr int NFM = MEQN * T * (T + 1);          // x-interface slot s = i - i0 in [0, T], row b
+    static constexpr int NFP = FLUXFORM ? 0 : NFM;
+    static constexpr int NGT = MEQN * (T + 1) * GS;         // y-interface slot t = j - j0 in [0, T], col a
+    static constexpr int NAB = 2 * MEQN * (T + 2) * GS;     // transverse corrections (x sweep, then y sweep)
+    static constexpr size_t WORK_BYTES = sizeof(R) * (size_t) (NFM + NFP + NGT + NAB);
     double q[MEQN * QP];
     R aux[MAUX > 0 ? MAUX * QP : 1];
-    R fm[MEQN * T * (T + 1)];       // x-interface slot s = i - i0 in [0, T], row b
-    R fp[FLUXFORM ? 1 : MEQN * T * (T + 1)];
-    R gt[MEQN * (T + 1) * GS];      // y-interface slot t = j - j0 in [0, T], col a
-    R ab[2 * MEQN * (T + 2) * GS];  // transverse corrections (x sweep, then y sweep)
-    double fx[FLUXFORM ? MEQN * T * (T + 1) : 1];
-    double fy[FLUXFORM ? MEQN * (T + 1) * T : 1];
+    double work[(WORK_BYTES + 7) / 8];
     // gm, gp at the patch's bottom and top edges, kept for the time-sync registers
     R eg[(REGS && !FLUXFORM) ? 2 * 2 * MEQN * T : 1];
+    __device__ R* fm() { return reinterpret_cast<R*>(work); }
+    __device__ R* fp() { return fm() + NFM; }
+    __device__ R* gt() { return fp() + NFP; }
+    __device__ R* ab() { return gt() + NGT; }
 };
 
 template <class RP, typename R, int T, bool FLUXFORM>
@@ -169,8 +173,16 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
 // sweeps and, with the kernel's other uniform values, spills scalar
 // registers into vector lanes; the fixed form has neither (about 40 % fewer
 // vector instructions in each sweep for the MC limiter).
+template <typename R, int T>
+constexpr int step_waves_per_eu() {
+#ifdef HIPCLAW_WPE_F32_T16
+    if (sizeof(R) == 4 && T == 16) return HIPCLAW_WPE_F32_T16;
+#endif
+    return 4;
+}
+
 template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS, int LIM = 0>
-__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<R, T>())))
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
     using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
@@ -184,6 +196,10 @@ step_tiles(StepArgs a) {
     constexpr int KA = MAUX > 0 ? (NA + NT - 1) / NT : 1;
     static_assert(SEG >= 1, "tile too wide for one wave");
     __shared__ Smem sm;
+    R* const s_fm = sm.fm();
+    R* const s_fp = sm.fp();
+    R* const s_gt = sm.gt();
+    R* const s_ab = sm.ab();
 
     const int ntiles = a.nwork * a.ntx * a.nty;
     const int mx = a.mx, my = a.my, mbc = a.mbc;
@@ -247,10 +263,10 @@ step_tiles(StepArgs a) {
     };
     // ab[w][m][r][a]
     auto AB = [&](int w, int m, int r, int c) -> R& {
-        return sm.ab[((w * MEQN + m) * (T + 2) + r) * GS + c];
+        return s_ab[((w * MEQN + m) * (T + 2) + r) * GS + c];
     };
     // gt[m][t][a]
-    auto GT = [&](int m, int t, int c) -> R& { return sm.gt[(m * (T + 1) + t) * GS + c]; };
+    auto GT = [&](int m, int t, int c) -> R& { return s_gt[(m * (T + 1) + t) * GS + c]; };
 
     // Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
     // that, in every round of the persistent loop, each XCD works on one
@@ -365,9 +381,9 @@ step_tiles(StepArgs a) {
             if (act && r >= 1 && r <= T && k >= 1 && k <= T + 1) {
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    sm.fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) + amdq[m]) + R(0.5) * cqxx[m];
+                    s_fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) + amdq[m]) + R(0.5) * cqxx[m];
                     if constexpr (!FLUXFORM)
-                        sm.fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) - apdq[m]) + R(0.5) * cqxx[m];
+                        s_fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) - apdq[m]) + R(0.5) * cqxx[m];
                 }
             }
             R tx1[MEQN], tx2[MEQN];
@@ -532,8 +548,8 @@ step_tiles(StepArgs a) {
             for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
                 const int m = e / (T * T), rem = e - m * T * T;
                 const int b = rem / T, c = rem - b * T;   // cell i = i0+c, j = j0+b
-                const R fmR = ((R(0) + sm.fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
-                const R fpL = ((R(0) + sm.fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
+                const R fmR = ((R(0) + s_fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
+                const R fpL = ((R(0) + s_fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
                 const R Dx = fmR - fpL;
                 const R Dy = GT(m, b, c);
                 const double qc = SQD(m, c + 2, b + 2);
@@ -554,8 +570,8 @@ step_tiles(StepArgs a) {
                     const int m = rem / T, b = rem - m * T;
                     if (side == 0 ? !first_x : !last_x) continue;
                     const int s0 = side ? T : 0;
-                    const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
-                    const R fpv = ((R(0) + sm.fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    const R fmv = ((R(0) + s_fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    const R fpv = ((R(0) + s_fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
                     reg_add(rg + ((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fmv, a.reg_accumulate);
                     reg_sub(rg + ((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fpv, a.reg_accumulate);
                 }
@@ -573,63 +589,62 @@ step_tiles(StepArgs a) {
                 }
             }
         } else {
-            // conservative FP64 interface fluxes, shared by both neighbours:
-            // F = f(q(i-1,j)) + fm(i,j) through x-interface s at row b and
-            // G = g(q(i,j-1)) + gm(i,j) through y-interface t at column c.
-            // One thread per cell of the (T+1)^2 block to the lower left of
-            // the interfaces evaluates f and g of its cell together.
-            for (int e = threadIdx.x; e < (T + 1) * (T + 1); e += NT) {
-                const int qj = 1 + e / (T + 1), qi = 1 + e % (T + 1);  // q-tile cell
-                const bool xf = qj >= 2, yf = qi >= 2;  // x-interface s = qi-1 at row b = qj-2; y t = qj-1 at c = qi-2
-                if (!xf && !yf) continue;
-                double qcell[MEQN], fl[MEQN], gl[MEQN];
-                double acell[MAUX > 0 ? MAUX : 1];
+            // conservative FP64 interface fluxes F = f(q(i-1,j)) + fm(i,j) and
+            // G = g(q(i,j-1)) + gm(i,j).  Each cell evaluates the four it needs
+            // (RP::fluxes is compiled without contraction), so the two cells on
+            // either side of an interface compute the same bits from the same
+            // LDS values and the FP64 update telescopes exactly; no flux array is
+            // kept in LDS.
+            auto xflux = [&](int sx, int b, int m, double f) -> double {  // x-interface sx at row b
+                const R fmv = ((R(0) + s_fm[(m * T + b) * (T + 1) + sx]) + AB(1, m, sx, b)) + AB(0, m, sx + 1, b);
+                return f + (double) fmv;
+            };
+            auto yflux = [&](int t, int c, int m, double g) -> double {   // y-interface t at column c
+                return g + (double) GT(m, t, c);
+            };
+            auto cell_fluxes = [&](int qi, int qj, double* f, double* g) {
+                double qv[MEQN], av[MAUX > 0 ? MAUX : 1];
 #pragma unroll
-                for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
+                for (int m = 0; m < MEQN; m++) qv[m] = SQD(m, qi, qj);
 #pragma unroll
-                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) acell[m] = (double) SA(m, qi, qj);
-                RP::fluxes(qcell, acell, a.pp, fl, gl);
-                if (xf) {
-                    const int s = qi - 1, b = qj - 2;
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++) {
-                        const R fmv = ((R(0) + sm.fm[(m * T + b) * (T + 1) + s]) + AB(1, m, s, b)) + AB(0, m, s + 1, b);
-                        sm.fx[(m * T + b) * (T + 1) + s] = fl[m] + (double) fmv;
-                    }
-                }
-                if (yf) {
-                    const int t = qj - 1, c = qi - 2;
-#pragma unroll
-                    for (int m = 0; m < MEQN; m++)
-                        sm.fy[(m * (T + 1) + t) * T + c] = gl[m] + (double) GT(m, t, c);
-                }
-            }
-            __syncthreads();
+                for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) av[m] = (double) SA(m, qi, qj);
+                RP::fluxes(qv, av, a.pp, f, g);
+            };
             if constexpr (REGS) {
                 // the shared FP64 flux through each patch edge, times dt * edge length
                 double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
                 const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
-                for (int e = threadIdx.x; e < 4 * MEQN * T; e += NT) {
-                    const int face = e / (MEQN * T), rem = e - face * (MEQN * T);
-                    const int m = rem / T, c = rem - m * T;
-                    double F;
-                    if (face == 0) { if (!first_x) continue; F = sm.fx[(m * T + c) * (T + 1)]; }
-                    else if (face == 1) { if (!last_x) continue; F = sm.fx[(m * T + c) * (T + 1) + T]; }
-                    else if (face == 2) { if (!first_y) continue; F = sm.fy[(m * (T + 1)) * T + c]; }
-                    else { if (!last_y) continue; F = sm.fy[(m * (T + 1) + T) * T + c]; }
+                for (int e = threadIdx.x; e < 4 * T; e += NT) {
+                    const int face = e / T, c = e - face * T;
+                    if (face == 0 ? !first_x : face == 1 ? !last_x : face == 2 ? !first_y : !last_y) continue;
+                    double f[MEQN], g[MEQN];
+                    if (face < 2) cell_fluxes(face == 0 ? 1 : T + 1, c + 2, f, g);
+                    else cell_fluxes(c + 2, face == 2 ? 1 : T + 1, f, g);
                     const int pos = (face < 2 ? j0 : i0) - 1 + c;
-                    reg_add(rg + ((face * 4 + REG_EFIN) * MEQN + m) * mx + pos, (face < 2 ? dtdy_el : dtdx_el) * F, a.reg_accumulate);
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) {
+                        const double F = face == 0 ? xflux(0, c, m, f[m]) : face == 1 ? xflux(T, c, m, f[m])
+                                       : face == 2 ? yflux(0, c, m, g[m]) : yflux(T, c, m, g[m]);
+                        reg_add(rg + ((face * 4 + REG_EFIN) * MEQN + m) * mx + pos,
+                                (face < 2 ? dtdy_el : dtdx_el) * F, a.reg_accumulate);
+                    }
                 }
             }
-            for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
-                const int m = e / (T * T), rem = e - m * T * T;
-                const int b = rem / T, c = rem - b * T;
+            for (int e = threadIdx.x; e < T * T; e += NT) {
+                const int b = e / T, c = e - b * T;   // cell i = i0+c, j = j0+b
+                double fc[MEQN], gc[MEQN], fl[MEQN], gl[MEQN], fb[MEQN], gb[MEQN];
+                cell_fluxes(c + 2, b + 2, fc, gc);    // this cell
+                cell_fluxes(c + 1, b + 2, fl, gl);    // its left neighbour
+                cell_fluxes(c + 2, b + 1, fb, gb);    // the one below
                 const int i = i0 + c, j = j0 + b;
-                const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
-                const double qc = SQD(m, c + 2, b + 2);
-                const double dfx = sm.fx[(m * T + b) * (T + 1) + c + 1] - sm.fx[(m * T + b) * (T + 1) + c];
-                const double dgy = sm.fy[(m * (T + 1) + b + 1) * T + c] - sm.fy[(m * (T + 1) + b) * T + c];
-                qo[off] = (qc - dtdx64 * dfx) - dtdy64 * dgy;
+#pragma unroll
+                for (int m = 0; m < MEQN; m++) {
+                    const long long off = ((long long) m * nyp + (j + mbc - 1)) * nxp + (i + mbc - 1);
+                    const double qc = SQD(m, c + 2, b + 2);
+                    const double dfx = xflux(c + 1, b, m, fc[m]) - xflux(c, b, m, fl[m]);
+                    const double dgy = yflux(b + 1, c, m, gc[m]) - yflux(b, c, m, gb[m]);
+                    qo[off] = (qc - dtdx64 * dfx) - dtdy64 * dgy;
+                }
             }
         }
         __syncthreads();  // LDS is rewritten by the next tile
