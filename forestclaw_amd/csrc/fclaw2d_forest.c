@@ -17,6 +17,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+enum { REL_BOUNDARY = 0, REL_SAME = 1, REL_HALF = 2, REL_DOUBLE = 3, REL_NONE = 4 };
+
+typedef struct {
+    int rel;
+    int nb[2];
+    int ti[2], tj[2];
+} nbinfo_t;
+
 struct fclaw2d_hip_forest {
     int nbx, nby, per_x, per_y;
     int n, cap;
@@ -27,6 +35,10 @@ struct fclaw2d_hip_forest {
     size_t hcap;
     uint64_t *hkey;
     int *hval;
+    /* face (4) and corner (4) relations of every leaf for one (mx, my), built
+     * on first use and dropped whenever the leaves change */
+    nbinfo_t *rel;
+    int rel_mx, rel_my, rel_valid;
 };
 
 /* ---------------------------------------------------------------- hashing */
@@ -45,6 +57,7 @@ static int rehash(fclaw2d_hip_forest_t *f)
 {
     size_t cap = 64, p;
     int i;
+    f->rel_valid = 0;
     while (cap < (size_t) f->n * 3) cap <<= 1;
     if (cap != f->hcap) {
         free(f->hkey);
@@ -191,28 +204,51 @@ static int split_marked(fclaw2d_hip_forest_t *f, const char *mark)
     return 0;
 }
 
+/* Restore the 2:1 face and corner balance by splitting leaves that are more
+ * than one level coarser than a neighbour.  The first pass checks every
+ * leaf; later passes check only the leaves a split created and the leaves
+ * that asked for it (a leaf no pass revisits saw no coarser neighbour, and
+ * splits only make its neighbours finer). */
 static int balance(fclaw2d_hip_forest_t *f)
 {
     static const int dx8[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     static const int dy8[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    int *work = NULL, nwork = f->n, first = 1, rc = 0;
     for (;;) {
-        char *mark;
-        int i, d, any = 0;
-        if (rehash(f)) return -1;
-        mark = (char *) calloc((size_t) f->n, 1);
-        if (!mark) return -1;
-        for (i = 0; i < f->n; i++) {
+        char *mark, *trig;
+        int *next, nnext = 0, w, d, any = 0, n0;
+        if (rehash(f)) { rc = -1; break; }
+        mark = (char *) calloc((size_t) f->n + 1, 1);
+        trig = (char *) calloc((size_t) f->n + 1, 1);
+        if (!mark || !trig) { free(mark); free(trig); rc = -1; break; }
+        for (w = 0; w < nwork; w++) {
+            const int i = first ? w : work[w];
             const int L = f->level[i];
             if (L < 2) continue;
             for (d = 0; d < 8; d++) {
                 const int q = covering(f, L, (int64_t) f->gx[i] + dx8[d], (int64_t) f->gy[i] + dy8[d]);
-                if (q >= 0 && f->level[q] < L - 1 && !mark[q]) { mark[q] = 1; any = 1; }
+                if (q >= 0 && f->level[q] < L - 1) {
+                    trig[i] = 1;
+                    if (!mark[q]) { mark[q] = 1; any = 1; }
+                }
             }
         }
-        if (any && split_marked(f, mark)) { free(mark); return -1; }
-        free(mark);
-        if (!any) break;
+        if (!any) { free(mark); free(trig); break; }
+        n0 = f->n;
+        if (split_marked(f, mark)) { free(mark); free(trig); rc = -1; break; }
+        /* next pass: the split leaves (their first child kept the index),
+         * the appended children and the leaves that triggered a split */
+        next = (int *) malloc(sizeof(int) * (size_t) (f->n + n0 + 1));
+        if (!next) { free(mark); free(trig); rc = -1; break; }
+        for (w = 0; w < n0; w++)
+            if (mark[w] || trig[w]) next[nnext++] = w;
+        for (w = n0; w < f->n; w++) next[nnext++] = w;
+        free(mark); free(trig);
+        free(work);
+        work = next; nwork = nnext; first = 0;
     }
+    free(work);
+    if (rc) return rc;
     return sort_leaves(f);
 }
 
@@ -281,7 +317,7 @@ void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
 {
     if (!f) return;
     free(f->level); free(f->gx); free(f->gy); free(f->hkey); free(f->hval);
-    free(f->pkind); free(f->psrc);
+    free(f->pkind); free(f->psrc); free(f->rel);
     free(f);
 }
 
@@ -443,14 +479,6 @@ int fclaw2d_hip_forest_provenance(const fclaw2d_hip_forest_t *f, int *kind, int 
 
 /* ---------------------------------------------------------------- neighbours */
 
-enum { REL_BOUNDARY = 0, REL_SAME = 1, REL_HALF = 2, REL_DOUBLE = 3, REL_NONE = 4 };
-
-typedef struct {
-    int rel;
-    int nb[2];
-    int ti[2], tj[2];
-} nbinfo_t;
-
 static const int fdx[4] = {-1, 1, 0, 0}, fdy[4] = {0, 0, -1, 1};
 static const int cdx[4] = {-1, 1, -1, 1}, cdy[4] = {-1, -1, 1, 1};
 
@@ -525,6 +553,25 @@ static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int
     }
 }
 
+/* Relations of leaf p for patches of mx x my cells: [0..3] faces, [4..7]
+ * corners; NULL if out of memory. */
+static const nbinfo_t *relations(const fclaw2d_hip_forest_t *cf, int p, int mx, int my)
+{
+    fclaw2d_hip_forest_t *f = (fclaw2d_hip_forest_t *) cf;   /* the cache is not part of the forest's value */
+    if (!f->rel_valid || f->rel_mx != mx || f->rel_my != my) {
+        int q, k;
+        free(f->rel);
+        f->rel = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * (size_t) (f->n > 0 ? f->n : 1));
+        if (!f->rel) return NULL;
+        for (q = 0; q < f->n; q++) {
+            for (k = 0; k < 4; k++) face_rel(f, q, k, mx, my, &f->rel[8 * (size_t) q + k]);
+            for (k = 0; k < 4; k++) corner_rel(f, q, k, mx, my, &f->rel[8 * (size_t) q + 4 + k]);
+        }
+        f->rel_mx = mx; f->rel_my = my; f->rel_valid = 1;
+    }
+    return f->rel + 8 * (size_t) p;
+}
+
 int fclaw2d_hip_forest_face_relation(const fclaw2d_hip_forest_t *f, int p, int iface, int nb[2])
 {
     nbinfo_t r;
@@ -584,12 +631,14 @@ static void emit_exchanges(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level,
     int p, k;
     for (p = 0; p < f->n; p++) {
         int phys[4];
+        const nbinfo_t *rp;
         if (f->level[p] != level) continue;
+        rp = relations(f, p, mx, my);
         physical_faces(f, p, phys);
         for (k = 0; k < 4; k++) {
             nbinfo_t r;
             if (phys[k]) continue;
-            face_rel(f, p, k, mx, my, &r);
+            r = rp[k];
             if (kind == OP_COPY_FACE && r.rel == REL_SAME)
                 emit(b, OP_COPY_FACE, p | tflag, r.nb[0] | tflag, k, r.ti[0], r.tj[0], 0);
             else if (kind == OP_AVG_FACE && r.rel == REL_HALF) {
@@ -603,7 +652,7 @@ static void emit_exchanges(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level,
         for (k = 0; k < 4; k++) {
             nbinfo_t r;
             if (!corner_interior(phys, k)) continue;
-            corner_rel(f, p, k, mx, my, &r);
+            r = rp[4 + k];
             if (kind == OP_COPY_FACE && r.rel == REL_SAME)
                 emit(b, OP_COPY_CORNER, p | tflag, r.nb[0] | tflag, k, r.ti[0], r.tj[0], 0);
             else if (kind == OP_AVG_FACE && r.rel == REL_HALF)
@@ -625,6 +674,7 @@ int fclaw2d_hip_forest_ghost_program_range(const fclaw2d_hip_forest_t *f, int mx
     (void) mbc;
     if (!f || f->n == 0 || !phase_start || !mthbc || minlevel > maxlevel) return -1;
     if (time_interp && ti < fclaw2d_hip_forest_min_level(f)) return -1;
+    if (!relations(f, 0, mx, my)) return -1;   /* builds the relation cache used below */
     b.ops = ops; b.cap = ops ? cap : 0; b.n = 0;
 
     /* phase 0: average fine -> coarse ghost cells (coarse levels, finest first) */

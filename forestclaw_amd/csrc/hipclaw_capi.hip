@@ -901,17 +901,24 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
         op[1] = p;
         for (int g = 0; g < 4; g++) op[2 + g] = sp[g];
     }
-    const int cap = std::max(s->max_patches, npatch_new);
+    // destination slab: the spare ping-pong slab when the new patches fit,
+    // otherwise a larger allocation (with room to grow) replacing both slabs
+    const bool reuse = !s->inplace && npatch_new <= s->max_patches;
+    const int cap = reuse ? s->max_patches
+                          : std::max(npatch_new, std::max(s->max_patches, npatch_new + npatch_new / 4));
     double* qnew = nullptr;
     int* dops = nullptr;
     HIP_TRY(hipStreamSynchronize(s->st));
-    if (hipMalloc((void**) &qnew, sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
+    if (reuse) qnew = s->q[1 - s->cur];
+    else if (hipMalloc((void**) &qnew, sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
         return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
     if (hipMalloc((void**) &dops, sizeof(int) * ops.size()) != hipSuccess) {
-        hipFree(qnew);
+        if (!reuse) hipFree(qnew);
         return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
     }
-    hipError_t he = hipMemsetAsync(qnew, 0, sizeof(double) * s->pstride * (size_t) cap, s->st);
+    // cells the transfer does not write (ghost cells of new children and
+    // parents) start at zero until the ghost fill
+    hipError_t he = hipMemsetAsync(qnew, 0, sizeof(double) * s->pstride * (size_t) npatch_new, s->st);
     if (he == hipSuccess) he = hipMemcpyAsync(dops, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, s->st);
     if (he == hipSuccess)
         he = (hipError_t) hipclaw::launch_regrid_remap(s->q[s->cur], qnew, dops, npatch_new, s->pstride,
@@ -919,18 +926,22 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     if (he == hipSuccess) he = hipStreamSynchronize(s->st);
     hipFree(dops);
     if (he != hipSuccess) {
-        hipFree(qnew);
+        if (!reuse) hipFree(qnew);
         return hipfail(he, "regrid_remap");
     }
-    hipFree(s->q[0]);
-    hipFree(s->q[1]);
-    s->q[0] = qnew;
-    s->q[1] = nullptr;
-    s->cur = 0;
-    if (!s->inplace) {
-        if (hipMalloc((void**) &s->q[1], sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
-        HIP_TRY(hipMemset(s->q[1], 0, sizeof(double) * s->pstride * (size_t) cap));
+    if (reuse) {
+        s->cur = 1 - s->cur;
+    } else {
+        hipFree(s->q[0]);
+        hipFree(s->q[1]);
+        s->q[0] = qnew;
+        s->q[1] = nullptr;
+        s->cur = 0;
+        if (!s->inplace) {
+            if (hipMalloc((void**) &s->q[1], sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
+                return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
+            HIP_TRY(hipMemset(s->q[1], 0, sizeof(double) * s->pstride * (size_t) cap));
+        }
     }
     if (cap > s->max_patches) {
         if (int rc = grow_patch_buffers(s, cap)) return rc;
