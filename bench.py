@@ -109,6 +109,64 @@ def run_regrid(fa, prob, forest, q, repeats=3):
             "coarsen_threshold": 0.5 * prob.tag_threshold}
 
 
+def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
+    """The workload as an application runs it (fclaw2d_run, src/fclaw2d_run.c:199-287):
+    initial refinement, then steps of all levels with dt set from the previous
+    step's Courant number (desired 0.9), the conservation fix-up, and a regrid
+    every regrid_interval steps, until t_final.  FP64 form; one sync per step
+    for the Courant number, as the reference's global reduction."""
+    precision = fa.FP64 if precision is None else precision
+    t0 = time.perf_counter()
+    forest = P.build_forest(prob)
+    q0, aux = P.initial_data(prob, forest)
+    sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
+                   grav=prob.grav, gamma=prob.gamma, max_patches=2 * forest.npatch, time_sync=True)
+    sv.set_forest(forest, prob.extent)
+    sv.set_q(q0)
+    if aux is not None:
+        sv.set_aux(aux)
+    sv.ghost_fill()
+    sv.sync()
+    t_init = time.perf_counter() - t0
+    dt = P.stable_dt(prob, forest, q0)
+    t, steps, regrids, updates, t_regrid, maxp = 0.0, 0, 0, 0, 0.0, forest.npatch
+    t1 = time.perf_counter()
+    while t < t_final * (1 - 1e-12):
+        step = min(dt, t_final - t)
+        cfl = sv.advance(step, 1)
+        t += step
+        steps += 1
+        updates += sv.npatch * prob.mx * prob.mx
+        if cfl > 0:
+            dt = step * prob.cfl / cfl
+        if regrid_interval > 0 and steps % regrid_interval == 0 and t < t_final:
+            r0 = time.perf_counter()
+            er, ec = sv.tag(prob.tag_field, prob.tag_threshold, 0.5 * prob.tag_threshold)
+            kind, src = forest.adapt(er, (1 - ec).astype(np.int32), prob.minlevel, prob.maxlevel)
+            if not (np.all(kind == fa.REGRID_SAME) and np.array_equal(src[:, 0], np.arange(len(kind)))):
+                sv.regrid(forest, kind, src, prob.extent)
+                if aux is not None:
+                    sv.set_aux(P.initial_data(prob, forest)[1])
+                sv.ghost_fill()
+            sv.sync()
+            t_regrid += time.perf_counter() - r0
+            regrids += 1
+            maxp = max(maxp, forest.npatch)
+    sv.sync()
+    wall = time.perf_counter() - t1
+    mass = sv.mass()
+    return {"metric": "time to t=%g, %s, regrid every %d steps" % (t_final, workload, regrid_interval),
+            "value": round(wall, 4), "unit": "s", "higher_is_better": False,
+            "steps": steps, "regrids": regrids, "final_patches": forest.npatch, "max_patches": maxp,
+            "cell_updates": updates, "cell_updates_per_s": round(updates / wall, 1),
+            "regrid_s": round(t_regrid, 4), "init_s": round(t_init, 4),
+            "final_time": t, "mass": [float(m) for m in mass[0]],
+            "dtype": "f64" if precision == fa.FP64 else "f32", "data": "synthetic",
+            "config": {"workload": workload, "cfl": prob.cfl, "time_sync": True,
+                       "refine_threshold": prob.tag_threshold,
+                       "coarsen_threshold": 0.5 * prob.tag_threshold}}
+
+
 PROFILE_STEPS = 6
 COPY_BYTES = 256 << 20
 
@@ -242,9 +300,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mx", type=int, default=32)
-    ap.add_argument("--minlevel", type=int, default=3)
-    ap.add_argument("--maxlevel", type=int, default=8)
+    ap.add_argument("--problem", choices=("dambreak", "advection", "shockbubble"), default="dambreak",
+                    help="workload: shallow-water dam break (default), periodic advection of a "
+                         "disk, Euler shock-bubble on a 4x1 brick")
+    ap.add_argument("--mx", type=int, default=None, help="patch size (default 32; 16 for "
+                                                          "advection and shock-bubble)")
+    ap.add_argument("--minlevel", type=int, default=None)
+    ap.add_argument("--maxlevel", type=int, default=None)
+    ap.add_argument("--run-to", type=float, default=None, metavar="T",
+                    help="instead of the step benchmark: run the workload to time T with the "
+                         "CFL-controlled dt and a regrid every --regrid-interval steps, and "
+                         "report the whole run (steps, regrids, time, cell updates/s)")
+    ap.add_argument("--regrid-interval", type=int, default=4)
     ap.add_argument("--tile64", type=int, default=0, help="tile for the FP64 form (0 = tuned)")
     ap.add_argument("--tile32", type=int, default=0, help="tile for the FP32 form (0 = tuned)")
     ap.add_argument("--no-graph", action="store_true")
@@ -270,14 +337,26 @@ def main():
     import forestclaw_amd as fa
     from forestclaw_amd import problems as P
 
-    prob = P.dambreak_problem(args.minlevel, args.maxlevel, mx=args.mx)
+    defaults = {"dambreak": (3, 8, 32), "advection": (4, 4, 16), "shockbubble": (2, 7, 16)}
+    dmin, dmax, dmx = defaults[args.problem]
+    minlevel = dmin if args.minlevel is None else args.minlevel
+    maxlevel = dmax if args.maxlevel is None else args.maxlevel
+    mx = dmx if args.mx is None else args.mx
+    make = {"dambreak": P.dambreak_problem, "advection": P.advection_problem,
+            "shockbubble": P.shockbubble_problem}[args.problem]
+    prob = make(minlevel, maxlevel, mx=mx)
+    workload = {"dambreak": "swe_dambreak_amr", "advection": "advection_disk",
+                "shockbubble": "euler_shockbubble_amr"}[args.problem] + "_L%d-%d_mx%d" % (minlevel, maxlevel, mx)
+    if args.run_to is not None:
+        fa.initialize_device(0)
+        print(json.dumps(run_to(fa, P, prob, workload, args.run_to, args.regrid_interval)))
+        return
     forest = P.build_forest(prob)
     q0, aux = P.initial_data(prob, forest)
     dt = P.stable_dt(prob, forest, q0)
     npatch = forest.npatch
     cells = npatch * prob.mx * prob.mx
     lv = forest.leaves()[0]
-    workload = "swe_dambreak_amr_L%d-%d_mx%d" % (args.minlevel, args.maxlevel, prob.mx)
     if args.pmc_summary:
         print("workload=%s cells=%d" % (workload, cells))
         print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], cells))
@@ -327,11 +406,12 @@ def main():
         import cpu_baseline
         idx = np.linspace(0, npatch - 1, min(64, npatch)).astype(int)
         xl, yl, dx, dy = forest.geometry(prob.extent, prob.mx, prob.mx)
-        cpu = cpu_baseline.measure(prob.solver, q0[idx], None, dx[idx], dt, prob.meqn,
+        cpu = cpu_baseline.measure(prob.solver, q0[idx], None if aux is None else aux[idx],
+                                   dx[idx], dt, prob.meqn,
                                    prob.maux, grav=prob.grav, target_seconds=args.cpu_seconds)
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.problem == "dambreak" else "cell-updates/s, %s, FP32-bulk vs FP64" % workload,
         "value": round(value32, 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
