@@ -40,7 +40,7 @@ def algorithmic_bytes_per_cell(meqn, maux, mx, mbc=2, s=8):
 
 
 def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_graph,
-             time_sync):
+             time_sync, dt_scale=None):
     sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
                    grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
                    time_sync=time_sync)
@@ -48,6 +48,8 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     sv.set_q(q0)
     if aux is not None:
         sv.set_aux(aux)
+    if dt_scale is not None:
+        sv.set_dt_scale(dt_scale)
     sv.ghost_fill()
     mass0 = sv.mass()
     if warmup > 0:
@@ -82,8 +84,10 @@ def run_regrid(fa, prob, forest, q, repeats=3):
     average2coarse) with the new ghost-fill program, and the ghost fill of
     every level.  Median over `repeats` regrids, each from the same state."""
     times = []
+    run_maxlevel = prob.params.get("maxlevel") if forest.isolated else None
     for _ in range(repeats):
-        f = fa.Forest(forest.nbx, forest.nby, forest.periodic, leaves=forest.leaves())
+        f = fa.Forest(forest.nbx, forest.nby, forest.periodic, leaves=forest.leaves(),
+                      isolated=forest.isolated)
         sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=fa.FP64, mthbc=prob.mthbc,
                        grav=prob.grav, gamma=prob.gamma, max_patches=f.npatch)
         sv.set_forest(f, prob.extent)
@@ -91,6 +95,10 @@ def run_regrid(fa, prob, forest, q, repeats=3):
         sv.sync()
         t0 = time.perf_counter()
         er, ec = sv.tag(prob.tag_field, prob.tag_threshold, 0.5 * prob.tag_threshold)
+        if run_maxlevel is not None:
+            # an ensemble: each run refines up to its own maxlevel
+            lv, gx, _ = f.leaves()
+            er = er & (lv < run_maxlevel[gx >> lv]).astype(er.dtype)
         t1 = time.perf_counter()
         kind, src = f.adapt(er, (1 - ec).astype(np.int32), prob.minlevel, prob.maxlevel)
         t2 = time.perf_counter()
@@ -300,9 +308,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--problem", choices=("dambreak", "advection", "shockbubble"), default="dambreak",
+    ap.add_argument("--problem", choices=("dambreak", "advection", "shockbubble", "ensemble"),
+                    default="dambreak",
                     help="workload: shallow-water dam break (default), periodic advection of a "
-                         "disk, Euler shock-bubble on a 4x1 brick")
+                         "disk, Euler shock-bubble on a 4x1 brick, or an ensemble of --runs "
+                         "independent dam breaks (one isolated block each, own dt)")
+    ap.add_argument("--runs", type=int, default=64, help="ensemble size (--problem ensemble)")
     ap.add_argument("--mx", type=int, default=None, help="patch size (default 32; 16 for "
                                                           "advection and shock-bubble)")
     ap.add_argument("--minlevel", type=int, default=None)
@@ -337,23 +348,42 @@ def main():
     import forestclaw_amd as fa
     from forestclaw_amd import problems as P
 
-    defaults = {"dambreak": (3, 8, 32), "advection": (4, 4, 16), "shockbubble": (2, 7, 16)}
+    defaults = {"dambreak": (3, 8, 32), "advection": (4, 4, 16), "shockbubble": (2, 7, 16),
+                "ensemble": (2, 7, 16)}
     dmin, dmax, dmx = defaults[args.problem]
     minlevel = dmin if args.minlevel is None else args.minlevel
     maxlevel = dmax if args.maxlevel is None else args.maxlevel
     mx = dmx if args.mx is None else args.mx
-    make = {"dambreak": P.dambreak_problem, "advection": P.advection_problem,
-            "shockbubble": P.shockbubble_problem}[args.problem]
-    prob = make(minlevel, maxlevel, mx=mx)
-    workload = {"dambreak": "swe_dambreak_amr", "advection": "advection_disk",
-                "shockbubble": "euler_shockbubble_amr"}[args.problem] + "_L%d-%d_mx%d" % (minlevel, maxlevel, mx)
+    ensemble = args.problem == "ensemble"
+    if ensemble:
+        # maxlevel of each run uniform in {4, ..., maxlevel}
+        prob = P.ensemble_problem(args.runs, mx=mx, minlevel=minlevel,
+                                  maxlevels=(min(4, maxlevel), maxlevel))
+        workload = "swe_dambreak_ensemble%d_L%d-%d_mx%d" % (args.runs, minlevel, maxlevel, mx)
+    else:
+        make = {"dambreak": P.dambreak_problem, "advection": P.advection_problem,
+                "shockbubble": P.shockbubble_problem}[args.problem]
+        prob = make(minlevel, maxlevel, mx=mx)
+        workload = {"dambreak": "swe_dambreak_amr", "advection": "advection_disk",
+                    "shockbubble": "euler_shockbubble_amr"}[args.problem] + \
+            "_L%d-%d_mx%d" % (minlevel, maxlevel, mx)
     if args.run_to is not None:
+        if ensemble:
+            sys.exit("bench.py: --run-to is for single runs, not --problem ensemble")
         fa.initialize_device(0)
         print(json.dumps(run_to(fa, P, prob, workload, args.run_to, args.regrid_interval)))
         return
-    forest = P.build_forest(prob)
-    q0, aux = P.initial_data(prob, forest)
-    dt = P.stable_dt(prob, forest, q0)
+    dt_scale = None
+    if ensemble:
+        forest = P.build_ensemble_forest(prob)
+        q0, aux = P.initial_data(prob, forest)
+        # each run at its own stable dt: step dt 1.0 times a per-slot scale
+        dt_scale, dt_run = P.ensemble_dt(prob, forest, q0)
+        dt = 1.0
+    else:
+        forest = P.build_forest(prob)
+        q0, aux = P.initial_data(prob, forest)
+        dt = P.stable_dt(prob, forest, q0)
     npatch = forest.npatch
     cells = npatch * prob.mx * prob.mx
     lv = forest.leaves()[0]
@@ -372,9 +402,9 @@ def main():
         return
 
     r64 = run_form(fa, prob, forest, q0, aux, fa.FP64, dt, args.steps, args.warmup,
-                   args.tile64, not args.no_graph, not args.no_time_sync)
+                   args.tile64, not args.no_graph, not args.no_time_sync, dt_scale)
     r32 = run_form(fa, prob, forest, q0, aux, fa.FP32_BULK, dt, args.steps, args.warmup,
-                   args.tile32, not args.no_graph, not args.no_time_sync)
+                   args.tile32, not args.no_graph, not args.no_time_sync, dt_scale)
 
     regrid = run_regrid(fa, prob, forest, r64["q"])
 
@@ -399,6 +429,24 @@ def main():
     l1 = (np.abs(i32 - i64) * area).sum(axis=(0, 2, 3)) / (np.abs(i64) * area).sum(axis=(0, 2, 3))
     maxerr = np.abs(i32 - i64).max(axis=(0, 2, 3)) / np.abs(i64).max(axis=(0, 2, 3))
     mass_change = lambda r: float((r["mass1"][0][0] - r["mass0"][0][0]) / r["mass0"][0][0])
+    runs = None
+    if ensemble:
+        # per-run accuracy of the FP32-bulk form (largest relative L1 error over the fields)
+        lvs, gxs, _ = forest.leaves()
+        blk = gxs >> lvs
+        l1_run = []
+        for b in range(prob.nbx):
+            sel = blk == b
+            num = (np.abs(i32[sel] - i64[sel]) * area[sel]).sum(axis=(0, 2, 3))
+            den = (np.abs(i64[sel]) * area[sel]).sum(axis=(0, 2, 3))
+            l1_run.append(float((num / den).max()))
+        per = np.bincount(blk, minlength=prob.nbx)
+        runs = {"runs": prob.nbx, "seed": prob.params["seed"],
+                "patches_per_run": {"min": int(per.min()), "max": int(per.max())},
+                "maxlevel_per_run": np.bincount(prob.params["maxlevel"]).tolist(),
+                "dt_per_run": {"min": float(dt_run.min()), "max": float(dt_run.max())},
+                "l1_rel_vs_fp64_per_run": {"max": float("%.3e" % max(l1_run)),
+                                           "median": float("%.3e" % np.median(l1_run))}}
 
     cpu = None
     if not args.no_cpu:
@@ -407,7 +455,7 @@ def main():
         idx = np.linspace(0, npatch - 1, min(64, npatch)).astype(int)
         xl, yl, dx, dy = forest.geometry(prob.extent, prob.mx, prob.mx)
         cpu = cpu_baseline.measure(prob.solver, q0[idx], None if aux is None else aux[idx],
-                                   dx[idx], dt, prob.meqn,
+                                   dx[idx], dt if dt_scale is None else float(dt_scale.min()), prob.meqn,
                                    prob.maux, grav=prob.grav, target_seconds=args.cpu_seconds)
 
     out = {
@@ -449,6 +497,9 @@ def main():
         "regrid": regrid,
         "cpu_baseline": cpu,
     }
+    if runs is not None:
+        out["ensemble"] = runs
+        out["config"]["dt"] = "per run (ensemble.dt_per_run)"
     print(json.dumps(out))
 
 

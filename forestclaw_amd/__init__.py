@@ -87,6 +87,8 @@ def lib():
         "fclaw2d_hip_forest_face_relation": ([vp, i, i, _ip()], i),
         "fclaw2d_hip_forest_sync_program": ([vp, i, i, vp, i], i),
         "fclaw2d_hip_forest_adapt": ([vp, _ip(), _ip(), i, i], i),
+        "fclaw2d_hip_forest_set_isolated": ([vp, i], i),
+        "hipclaw_set_dt_scale": ([vp, vp], i),
         "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
         "hipclaw_tag_patches": ([vp, i, d, d, _ip(), _ip()], i),
         "hipclaw_regrid": ([vp, i, _ip(), _ip(), _dp(), _dp()], i),
@@ -124,10 +126,11 @@ class Forest:
     """Leaves of a balanced forest of quadtrees over an nbx x nby brick."""
 
     def __init__(self, nbx=1, nby=1, periodic=(False, False), minlevel=0, leaves=None,
-                 balance=True):
+                 balance=True, isolated=False):
         L = lib()
         self.nbx, self.nby = nbx, nby
         self.periodic = (bool(periodic[0]), bool(periodic[1]))
+        self.isolated = bool(isolated)
         if leaves is None:
             self._h = L.fclaw2d_hip_forest_new(nbx, nby, int(periodic[0]), int(periodic[1]),
                                                minlevel)
@@ -138,6 +141,8 @@ class Forest:
                                                        int(balance))
         if not self._h:
             raise ValueError("invalid forest")
+        if isolated:
+            L.fclaw2d_hip_forest_set_isolated(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -298,6 +303,15 @@ class Solver:
         self.npatch = len(kind)
         self.max_patches = max(self.max_patches, self.npatch)
         self.set_forest(forest, extent)
+
+    def set_dt_scale(self, scale):
+        """Per-slot factor on dt (None: a common dt)."""
+        if scale is None:
+            _check(lib().hipclaw_set_dt_scale(self._h, None), "hipclaw_set_dt_scale")
+            return
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        assert len(sc) == self.npatch
+        _check(lib().hipclaw_set_dt_scale(self._h, sc.ctypes.data), "hipclaw_set_dt_scale")
 
     def ghost_pack_elems(self):
         """Doubles in one patch's ghost-patch message."""

@@ -27,6 +27,7 @@ typedef struct {
 
 struct fclaw2d_hip_forest {
     int nbx, nby, per_x, per_y;
+    int iso;   /* blocks are separate domains (an ensemble of runs) */
     int n, cap;
     int *level, *gx, *gy;
     /* provenance of the leaves after the last adapt: kind and 4 sources each */
@@ -96,10 +97,28 @@ static int lookup(const fclaw2d_hip_forest_t *f, int level, int64_t gx, int64_t 
 }
 
 /* Map a level-L patch coordinate into the brick; 0 if it falls outside a
- * non-periodic side. */
-static int wrap(const fclaw2d_hip_forest_t *f, int L, int64_t *x, int64_t *y)
+ * non-periodic side.  (sx, sy) is a level-L coordinate in the block the
+ * query starts from: when the blocks are separate domains (an ensemble of
+ * runs), a coordinate outside that block is outside the domain, or wraps
+ * around that block alone when periodic. */
+static int wrap(const fclaw2d_hip_forest_t *f, int L, int64_t *x, int64_t *y, int64_t sx, int64_t sy)
 {
     const int64_t nx = (int64_t) f->nbx << L, ny = (int64_t) f->nby << L;
+    if (f->iso) {
+        const int64_t w = (int64_t) 1 << L, bx0 = (sx >> L) * w, by0 = (sy >> L) * w;
+        int64_t lx = *x - bx0, ly = *y - by0;
+        if (lx < 0 || lx >= w) {
+            if (!f->per_x) return 0;
+            lx = ((lx % w) + w) % w;
+        }
+        if (ly < 0 || ly >= w) {
+            if (!f->per_y) return 0;
+            ly = ((ly % w) + w) % w;
+        }
+        *x = bx0 + lx;
+        *y = by0 + ly;
+        return 1;
+    }
     if (*x < 0 || *x >= nx) {
         if (!f->per_x) return 0;
         *x = ((*x % nx) + nx) % nx;
@@ -113,10 +132,11 @@ static int wrap(const fclaw2d_hip_forest_t *f, int L, int64_t *x, int64_t *y)
 
 /* Leaf covering level-L cell (x, y) at level <= L, or -1 if that region is
  * refined beyond L (or outside the domain). */
-static int covering(const fclaw2d_hip_forest_t *f, int L, int64_t x, int64_t y)
+static int covering(const fclaw2d_hip_forest_t *f, int L, int64_t x, int64_t y, int64_t sx,
+                    int64_t sy)
 {
     int l;
-    if (!wrap(f, L, &x, &y)) return -1;
+    if (!wrap(f, L, &x, &y, sx, sy)) return -1;
     for (l = L; l >= 0; l--) {
         int q = lookup(f, l, x >> (L - l), y >> (L - l));
         if (q >= 0) return q;
@@ -226,7 +246,8 @@ static int balance(fclaw2d_hip_forest_t *f)
             const int L = f->level[i];
             if (L < 2) continue;
             for (d = 0; d < 8; d++) {
-                const int q = covering(f, L, (int64_t) f->gx[i] + dx8[d], (int64_t) f->gy[i] + dy8[d]);
+                const int q = covering(f, L, (int64_t) f->gx[i] + dx8[d], (int64_t) f->gy[i] + dy8[d],
+                                       f->gx[i], f->gy[i]);
                 if (q >= 0 && f->level[q] < L - 1) {
                     trig[i] = 1;
                     if (!mark[q]) { mark[q] = 1; any = 1; }
@@ -313,6 +334,14 @@ fclaw2d_hip_forest_t *fclaw2d_hip_forest_from_leaves(int nbx, int nby, int perio
     return f;
 }
 
+int fclaw2d_hip_forest_set_isolated(fclaw2d_hip_forest_t *f, int isolated)
+{
+    if (!f) return -1;
+    f->iso = isolated != 0;
+    f->rel_valid = 0;
+    return 0;
+}
+
 void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
 {
     if (!f) return;
@@ -371,8 +400,8 @@ static int coarsen_keeps_balance(const fclaw2d_hip_forest_t *f, int L, int64_t X
         for (x = 2 * X - 1; x <= 2 * X + 2; x++) {
             int64_t wx = x, wy = y;
             if (x >= 2 * X && x <= 2 * X + 1 && y >= 2 * Y && y <= 2 * Y + 1) continue;
-            if (!wrap(f, L, &wx, &wy)) continue;          /* outside the domain */
-            if (covering(f, L, wx, wy) < 0) return 0;     /* refined beyond L */
+            if (!wrap(f, L, &wx, &wy, 2 * X, 2 * Y)) continue;          /* outside the domain */
+            if (covering(f, L, wx, wy, 2 * X, 2 * Y) < 0) return 0;     /* refined beyond L */
         }
     return 1;
 }
@@ -485,6 +514,14 @@ static const int cdx[4] = {-1, 1, -1, 1}, cdy[4] = {-1, -1, 1, 1};
 static void physical_faces(const fclaw2d_hip_forest_t *f, int p, int phys[4])
 {
     const int L = f->level[p];
+    if (f->iso) {   /* every block is a domain of its own */
+        const int w = 1 << L, lx = f->gx[p] & (w - 1), ly = f->gy[p] & (w - 1);
+        phys[0] = !f->per_x && lx == 0;
+        phys[1] = !f->per_x && lx == w - 1;
+        phys[2] = !f->per_y && ly == 0;
+        phys[3] = !f->per_y && ly == w - 1;
+        return;
+    }
     phys[0] = !f->per_x && f->gx[p] == 0;
     phys[1] = !f->per_x && f->gx[p] == (f->nbx << L) - 1;
     phys[2] = !f->per_y && f->gy[p] == 0;
@@ -498,7 +535,7 @@ static void face_rel(const fclaw2d_hip_forest_t *f, int p, int iface, int mx, in
     int64_t wx = x, wy = y;
     int q, k;
     r->rel = REL_BOUNDARY;
-    if (!wrap(f, L, &wx, &wy)) return;
+    if (!wrap(f, L, &wx, &wy, f->gx[p], f->gy[p])) return;
     if ((q = lookup(f, L, wx, wy)) >= 0) {
         r->rel = REL_SAME; r->nb[0] = q;
         r->ti[0] = -fdx[iface] * mx; r->tj[0] = -fdy[iface] * my;
@@ -517,7 +554,8 @@ static void face_rel(const fclaw2d_hip_forest_t *f, int p, int iface, int mx, in
         else { cx = 2 * (int64_t) f->gx[p] + k; cy = 2 * y; }
         r->ti[k] = (int) ((2 * (int64_t) f->gx[p] - cx) * mx);
         r->tj[k] = (int) ((2 * (int64_t) f->gy[p] - cy) * my);
-        if (!wrap(f, L + 1, &cx, &cy) || (q = lookup(f, L + 1, cx, cy)) < 0) { r->rel = REL_NONE; return; }
+        if (!wrap(f, L + 1, &cx, &cy, 2 * (int64_t) f->gx[p], 2 * (int64_t) f->gy[p]) ||
+            (q = lookup(f, L + 1, cx, cy)) < 0) { r->rel = REL_NONE; return; }
         r->nb[k] = q;
     }
 }
@@ -529,7 +567,7 @@ static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int
     int64_t wx = x, wy = y;
     int q;
     r->rel = REL_NONE;
-    if (!wrap(f, L, &wx, &wy)) return;
+    if (!wrap(f, L, &wx, &wy, f->gx[p], f->gy[p])) return;
     if ((q = lookup(f, L, wx, wy)) >= 0) {
         r->rel = REL_SAME; r->nb[0] = q;
         r->ti[0] = -cdx[ic] * mx; r->tj[0] = -cdy[ic] * my;
@@ -547,7 +585,8 @@ static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int
         int64_t cx = 2 * x + (cdx[ic] < 0), cy = 2 * y + (cdy[ic] < 0);
         r->ti[0] = (int) ((2 * (int64_t) f->gx[p] - cx) * mx);
         r->tj[0] = (int) ((2 * (int64_t) f->gy[p] - cy) * my);
-        if (wrap(f, L + 1, &cx, &cy) && (q = lookup(f, L + 1, cx, cy)) >= 0) {
+        if (wrap(f, L + 1, &cx, &cy, 2 * (int64_t) f->gx[p], 2 * (int64_t) f->gy[p]) &&
+            (q = lookup(f, L + 1, cx, cy)) >= 0) {
             r->rel = REL_HALF; r->nb[0] = q;
         }
     }

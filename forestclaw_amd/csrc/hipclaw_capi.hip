@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <vector>
 
 #include "../../include/hipclaw.h"
@@ -79,6 +80,7 @@ struct hipclaw_state {
     // subcycling: the state before each patch's last update, and time-interpolated copies
     double* qlast = nullptr;
     double* qti = nullptr;
+    double* dtscale = nullptr;    // per-slot factor on dt, or null
     // ghost-patch message: slab offsets of one field's band, in message order
     int* band = nullptr;
     int nband = 0;
@@ -229,6 +231,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->mass_buf);
     hipFree(s->reg);
     hipFree(s->band);
+    hipFree(s->dtscale);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
@@ -571,6 +574,29 @@ int hipclaw_ghost_unpack(hipclaw_state_t* s, const int* slots, int n, int time_i
     return ghost_message(s, slots, n, time_interp, const_cast<double*>(buf), 1);
 }
 
+int hipclaw_set_dt_scale(hipclaw_state_t* s, const double* scale) {
+    if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_set_dt_scale: bad arguments%s");
+    if (!scale) {
+        HIP_TRY(hipStreamSynchronize(s->st));
+        hipFree(s->dtscale);
+        s->dtscale = nullptr;
+    } else {
+        for (int p = 0; p < s->npatch; p++)
+            if (!(scale[p] > 0.0) || !std::isfinite(scale[p]))
+                return fail(HIPCLAW_EINVAL, "dt scale must be positive and finite%s");
+        if (!s->dtscale) HIP_TRY(hipMalloc((void**) &s->dtscale, sizeof(double) * (size_t) s->max_patches));
+        // ordered after the steps already queued on the (non-blocking) stream
+        HIP_TRY(hipMemcpyAsync(s->dtscale, scale, sizeof(double) * (size_t) s->npatch,
+                               hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
+    // a captured graph holds the old step arguments
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+    return HIPCLAW_OK;
+}
+
 int hipclaw_vtk_values(hipclaw_state_t* s, int first, int count, float* out) {
     if (!s || first < 0 || count < 0 || first + count > s->npatch || (count > 0 && !out))
         return fail(HIPCLAW_EINVAL, "hipclaw_vtk_values: bad arguments%s");
@@ -638,6 +664,7 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
     a.reg = s->reg;
     a.rstride = s->rstride;
     a.reg_accumulate = s->reg_accumulate;
+    a.dtscale = s->dtscale;
     return a;
 }
 
@@ -948,7 +975,9 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     } else if (s->reg) {
         HIP_TRY(hipMemset(s->reg, 0, sizeof(double) * s->rstride * (size_t) s->max_patches));
     }
-    // every program names old slots; the graph names old buffers
+    // every program and the dt scale name old slots; the graph names old buffers
+    hipFree(s->dtscale);
+    s->dtscale = nullptr;
     for (GhostProgram& g : s->gp) {
         g.nops = 0;
         for (int& v : g.phase_start) v = 0;

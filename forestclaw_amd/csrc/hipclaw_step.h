@@ -61,6 +61,7 @@ struct StepArgs {
     double* reg;
     long long rstride;
     int reg_accumulate;   // add to the registers (subcycling) instead of overwriting them
+    const double* dtscale;  // per-slot factor on dt (an ensemble of runs), or null
 };
 
 enum { REG_FM = 0, REG_FP = 1, REG_EFIN = 2, REG_EFOUT = 3 };
@@ -280,7 +281,8 @@ step_tiles(StepArgs a) {
         int tx, ty;
         const int slot = tile_slot(tile, tx, ty);
         const int i0 = 1 + tx * T, j0 = 1 + ty * T;  // first interior cell of the tile (1-based)
-        const double dtdx64 = a.dt / a.dxs[slot], dtdy64 = a.dt / a.dys[slot];
+        const double dt = a.dtscale ? a.dt * a.dtscale[slot] : a.dt;
+        const double dtdx64 = dt / a.dxs[slot], dtdy64 = dt / a.dys[slot];
         const R dtdx = (R) dtdx64, dtdy = (R) dtdy64;
         const bool first_x = tx == 0, last_x = tx == a.ntx - 1;
         const bool first_y = ty == 0, last_y = ty == a.nty - 1;
@@ -297,7 +299,7 @@ step_tiles(StepArgs a) {
         __syncthreads();
         if constexpr (REGS && !FLUXFORM) {
             double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
-            const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+            const double dtdy_el = dt * a.dys[slot], dtdx_el = dt * a.dxs[slot];
             // time-sync registers: conservative flux of the pre-step state at the first
             // interior and first ghost cell of every edge (clawpack46_time_sync.f:58-147)
             for (int e = threadIdx.x; e < 8 * T; e += NT) {
@@ -562,7 +564,7 @@ step_tiles(StepArgs a) {
             }
             if constexpr (REGS) {
                 double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
-                const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+                const double dtdy_el = dt * a.dys[slot], dtdx_el = dt * a.dxs[slot];
                 // fm, fp at the patch's left (i = 1) and right (i = mx+1) edges,
                 // summed as the update sums them (clawpack46_time_sync.f:192-199)
                 for (int e = threadIdx.x; e < 2 * MEQN * T; e += NT) {
@@ -613,7 +615,7 @@ step_tiles(StepArgs a) {
             if constexpr (REGS) {
                 // the shared FP64 flux through each patch edge, times dt * edge length
                 double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
-                const double dtdy_el = a.dt * a.dys[slot], dtdx_el = a.dt * a.dxs[slot];
+                const double dtdy_el = dt * a.dys[slot], dtdx_el = dt * a.dxs[slot];
                 for (int e = threadIdx.x; e < 4 * T; e += NT) {
                     const int face = e / T, c = e - face * T;
                     if (face == 0 ? !first_x : face == 1 ? !last_x : face == 2 ? !first_y : !last_y) continue;

@@ -93,6 +93,72 @@ def shockbubble_problem(minlevel=2, maxlevel=7, mx=16, gamma=1.4, mach=1.5):
                    params=dict(p2=p2, rho2=rho2, u2=u2))
 
 
+def ensemble_problem(nruns=64, mx=16, minlevel=2, maxlevels=(4, 7), seed=2026):
+    """An ensemble of radial dam breaks, one per block of an nruns x 1 brick of
+    isolated unit squares (block b is [b, b+1] x [0, 1]): r0 ~ U[0.1, 0.3],
+    h_in ~ U[1.5, 3.0], maxlevel ~ U{maxlevels}, drawn from numpy's PCG64
+    with the given seed.  Each run keeps its own stable dt (a per-slot dt
+    scale on the device)."""
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(0.1, 0.3, nruns)
+    hin = rng.uniform(1.5, 3.0, nruns)
+    mlev = rng.integers(maxlevels[0], maxlevels[1] + 1, nruns)
+
+    def q(x, y):
+        b = np.clip(np.floor(x).astype(np.int64), 0, nruns - 1)
+        xr = x - b
+        h = np.where((xr - 0.5) ** 2 + (y - 0.5) ** 2 < r0[b] ** 2, hin[b], 1.0)
+        return np.stack([h, np.zeros_like(h), np.zeros_like(h)])
+
+    return Problem("ensemble", RP_SHALLOW, (0.0, float(nruns), 0.0, 1.0), nruns, 1, (False, False),
+                   (1, 1, 1, 1), minlevel, int(mlev.max()), mx, 0.9, q, grav=1.0,
+                   params=dict(r0=r0, hin=hin, maxlevel=mlev, seed=seed))
+
+
+def member_problem(ens, b):
+    """Run b of an ensemble as a problem of its own on the unit square."""
+    pr = ens.params
+    return dambreak_problem(ens.minlevel, int(pr["maxlevel"][b]), mx=ens.mx, r0=float(pr["r0"][b]),
+                            hin=float(pr["hin"][b]))
+
+
+def build_ensemble_forest(ens, mbc=2):
+    """Initial refinement of every run of an ensemble up to its own maxlevel,
+    in one forest of isolated blocks."""
+    forest = Forest(ens.nbx, ens.nby, ens.periodic, ens.minlevel, isolated=True)
+    mlev = ens.params["maxlevel"]
+    for _ in range(ens.maxlevel - ens.minlevel):
+        q, _ = initial_data(ens, forest, mbc)
+        lv, gx, _ = forest.leaves()
+        flags = tag_gradient(ens, forest, q, mbc) & (lv < mlev[gx >> lv]).astype(np.int32)
+        if not flags.any():
+            break
+        forest.refine(flags, ens.maxlevel)
+    return forest
+
+
+def member_leaves(forest, b):
+    """Leaves of block b of a 1-row brick, moved to block 0."""
+    lv, gx, gy = forest.leaves()
+    sel = (gx >> lv) == b
+    return lv[sel], gx[sel] - (b << lv[sel]), gy[sel], np.nonzero(sel)[0]
+
+
+def ensemble_dt(ens, forest, q, mbc=2):
+    """Per-slot dt: each run's stable dt (Courant number 0.9 on its finest
+    patches for the initial data)."""
+    _, _, dx, _ = forest.geometry(ens.extent, ens.mx, ens.mx)
+    inner = q[:, :, mbc:-mbc, mbc:-mbc]
+    h = inner[:, 0]
+    s = np.maximum((np.abs(inner[:, 1] / h) + np.sqrt(ens.grav * h)).max(axis=(1, 2)),
+                   (np.abs(inner[:, 2] / h) + np.sqrt(ens.grav * h)).max(axis=(1, 2)))
+    lv, gx, _ = forest.leaves()
+    blk = gx >> lv
+    per_run = np.full(ens.nbx, np.inf)
+    np.minimum.at(per_run, blk, dx / s)
+    return ens.cfl * per_run[blk], ens.cfl * per_run
+
+
 def cell_centres(xlower, ylower, dx, dy, mx, my, mbc):
     """Cell-centre coordinates including ghost cells: arrays (npatch, ny, nx)."""
     i = np.arange(1 - mbc, mx + mbc + 1)

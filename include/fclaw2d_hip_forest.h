@@ -29,6 +29,12 @@ fclaw2d_hip_forest_t *fclaw2d_hip_forest_from_leaves(int nbx, int nby, int perio
                                                      int balance);
 void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f);
 
+/* Make every block of the brick a domain of its own (an ensemble of runs in
+ * one forest): no neighbour relations, ghost exchange or 2:1 balance cross a
+ * block boundary, and block faces are physical boundaries (or wrap around
+ * their own block when periodic).  Set it before refining.  Returns 0. */
+int fclaw2d_hip_forest_set_isolated(fclaw2d_hip_forest_t *f, int isolated);
+
 int fclaw2d_hip_forest_num_patches(const fclaw2d_hip_forest_t *f);
 int fclaw2d_hip_forest_min_level(const fclaw2d_hip_forest_t *f);
 int fclaw2d_hip_forest_max_level(const fclaw2d_hip_forest_t *f);

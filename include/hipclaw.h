@@ -214,6 +214,15 @@ int hipclaw_ghost_pack(hipclaw_state_t *s, const int *slots, int n, int time_int
 int hipclaw_ghost_unpack(hipclaw_state_t *s, const int *slots, int n, int time_interp,
                          const double *buf);
 
+/* Per-slot time step: every later update of slot p advances it by
+ * dt * scale[p] (the dt of hipclaw_advance / hipclaw_step_batch), so the
+ * patches of an ensemble of independent runs -- each with its own stable dt,
+ * in one forest of isolated blocks (fclaw2d_hip_forest_set_isolated) -- share
+ * one batched launch; the CFL number is then the maximum over all runs.
+ * scale = NULL restores a common dt.  A regrid drops the scale (slots
+ * change).  No counterpart in the reference, which runs one dt per level. */
+int hipclaw_set_dt_scale(hipclaw_state_t *s, const double *scale);
+
 /* Cell values of slots first..first+count-1 as float, in the order of the
  * VTK "meqn" cell array (per patch: j = 1..my, i = 1..mx, then the meqn
  * fields), converted on the device so that half the bytes cross PCIe.

@@ -196,3 +196,24 @@ def test_adapt_skips_a_coarsening_that_would_break_the_balance():
     for a, b in zip(before, f.leaves()):
         np.testing.assert_array_equal(a, b)
     assert np.all(kind == fa.REGRID_SAME)
+
+
+def test_isolated_blocks_hold_independent_runs():
+    """An ensemble forest of isolated blocks: each block's leaves are those of
+    the run built alone, and the product's ghost-fill program on the whole
+    ensemble fills every run exactly as the oracle fills that run alone."""
+    ens = P.ensemble_problem(6, mx=8, minlevel=2, maxlevels=(3, 5))
+    forest = P.build_ensemble_forest(ens)
+    rng = np.random.default_rng(4)
+    q = random_state(ens.solver, forest.npatch, 8, 8, rng)
+    ops, ps = forest.ghost_program(8, 8, 2, ens.mthbc)
+    qe = G.run(q.copy(), ops, ps, 8, 8, 2)
+    for b in range(ens.nbx):
+        mp = P.member_problem(ens, b)
+        lv, gx, gy, idx = P.member_leaves(forest, b)
+        alone = P.build_forest(mp)
+        assert sorted(zip(lv.tolist(), gx.tolist(), gy.tolist())) == \
+            sorted(zip(*(a.tolist() for a in alone.leaves())))
+        fb = fa.Forest(1, 1, (False, False), leaves=(lv, gx, gy))
+        qo = oracle_ghost_update(mp, fb, q[idx], 8)
+        np.testing.assert_array_equal(qe[idx], qo)
