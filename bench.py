@@ -179,7 +179,7 @@ PROFILE_STEPS = 6
 COPY_BYTES = 256 << 20
 
 
-def profile_run(fa, prob, forest, q0, aux, dt, tiles, time_sync):
+def profile_run(fa, prob, forest, q0, aux, dt, tiles, time_sync, dt_scale=None):
     """The program profiled by rocprofv3 (kernel trace, then one counter per
     pass): PROFILE_STEPS steps of each form at the benchmark tiles, then three
     device-to-device copies of COPY_BYTES (the HIP runtime's copy kernel) that
@@ -191,6 +191,8 @@ def profile_run(fa, prob, forest, q0, aux, dt, tiles, time_sync):
                        time_sync=time_sync)
         sv.set_forest(forest, prob.extent)
         sv.set_q(q0)
+        if dt_scale is not None:
+            sv.set_dt_scale(dt_scale)
         sv.ghost_fill()
         sv.advance_async(dt, PROFILE_STEPS)
         sv.sync()
@@ -398,7 +400,8 @@ def main():
 
     device = fa.initialize_device(0)
     if args.profile_run:
-        profile_run(fa, prob, forest, q0, aux, dt, (args.tile64, args.tile32), not args.no_time_sync)
+        profile_run(fa, prob, forest, q0, aux, dt, (args.tile64, args.tile32), not args.no_time_sync,
+                    dt_scale)
         return
 
     r64 = run_form(fa, prob, forest, q0, aux, fa.FP64, dt, args.steps, args.warmup,

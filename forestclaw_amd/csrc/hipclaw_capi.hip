@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hipclaw.h"
@@ -39,12 +40,19 @@ int hipfail(hipError_t e, const char* where) {
 
 }  // namespace
 
+// One kernel launch of a ghost-fill program: ops start .. start+nops-1 of the
+// program's device array, either exchange ops or physical-BC ops.
+struct GhostLaunch {
+    int start = 0, nops = 0;
+    bool physbc = false;
+};
+
 // One ghost-fill program (hipclaw_set_ghost_program_k).
 struct GhostProgram {
     GhostOp* ops = nullptr;
     int nops = 0, cap = 0;
-    int phase_start[7] = {0, 0, 0, 0, 0, 0, 0};
-    bool phase_physbc[6] = {false, false, false, false, false, false};
+    GhostLaunch launch[6];
+    int nlaunch = 0;
 };
 
 // One time-synchronisation program (hipclaw_set_sync_program_k).
@@ -355,7 +363,93 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         }
         // average/interpolate faces test every cell against the neighbour's bounds in the kernel
     }
+    // The launches: phases 0 and 1 (average, copy: disjoint ghost cells, interior
+    // reads only) share one, in the caller's order (neighbouring patches' ops
+    // stay close, which keeps the cache lines they share in cache).  Phases 4
+    // and 5 apply the physical BC to the fine levels, then to every level; a BC
+    // touches only its own patch, so when every patch of phase 4 is in phase 5
+    // with the same BC, one launch applies it twice to those patches.
     GhostProgram& g = s->gp[k];
+    std::vector<GhostOp> dev;
+    dev.reserve((size_t) nops);
+    int nl = 0;
+    auto to_op = [&](int o, int reps) {
+        const int* q = ops + 8 * o;
+        GhostOp op;
+        op.type = q[0]; op.dst = q[1]; op.src = q[2]; op.which = q[3];
+        op.ti = q[4]; op.tj = q[5]; op.bc = q[6]; op.reps = reps;
+        return op;
+    };
+    // Within an exchange launch, a copy op and its reverse (P <- Q beside
+    // Q <- P) sit next to each other, in one workgroup and so on one XCD: the
+    // row-end cache lines the pair reads and writes are then shared in that
+    // XCD's L2.  Other ops keep the caller's order.
+    // (buckets by the lower slot of a copy, the destination otherwise; a copy
+    // follows the op before it in its bucket that has the same slot pair)
+    const int TI = HIPCLAW_TIME_INTERP_SLOT;
+    auto add_exchange = [&](int a, int b) {
+        GhostLaunch L;
+        L.start = (int) dev.size();
+        const int nb = 2 * s->npatch;                 // slots, then time-interpolated slots
+        auto bucket = [&](int slot) { return (slot & TI) ? s->npatch + (slot & ~TI) : slot; };
+        auto lo_of = [&](int o) {
+            const int* q = ops + 8 * o;
+            const bool copy = q[0] == hipclaw::GHOST_COPY_FACE || q[0] == hipclaw::GHOST_COPY_CORNER;
+            return bucket(copy ? std::min(q[1], q[2]) : q[1]);
+        };
+        std::vector<int> head(nb + 1, 0), order((size_t) (b - a));
+        for (int o = a; o < b; o++) head[lo_of(o) + 1]++;
+        for (int i = 0; i < nb; i++) head[i + 1] += head[i];
+        for (int o = a; o < b; o++) order[head[lo_of(o)]++] = o;   // stable bucket pass
+        for (int i = nb; i > 0; i--) head[i] = head[i - 1];
+        head[0] = 0;
+        for (int i = 0; i < nb; i++) {
+            // within a bucket (a few ops), bring each copy's reverse next to it
+            for (int x = head[i]; x < head[i + 1]; x++) {
+                const int* q = ops + 8 * order[x];
+                if (q[0] != hipclaw::GHOST_COPY_FACE && q[0] != hipclaw::GHOST_COPY_CORNER) continue;
+                for (int y = x + 1; y < head[i + 1]; y++) {
+                    const int* r = ops + 8 * order[y];
+                    if (r[0] == q[0] && r[1] == q[2] && r[2] == q[1]) {
+                        const int v = order[y];
+                        for (int z = y; z > x + 1; z--) order[z] = order[z - 1];
+                        order[x + 1] = v;
+                        x++;
+                        break;
+                    }
+                }
+            }
+        }
+        for (int o : order) dev.push_back(to_op(o, 1));
+        L.nops = (int) dev.size() - L.start;
+        if (L.nops > 0) g.launch[nl++] = L;
+    };
+    auto add_physbc = [&](int a, int b, const std::unordered_map<int, int>* twice) {
+        GhostLaunch L;
+        L.start = (int) dev.size();
+        L.physbc = true;
+        for (int o = a; o < b; o++) dev.push_back(to_op(o, twice && twice->count(ops[8 * o + 1]) ? 2 : 1));
+        L.nops = (int) dev.size() - L.start;
+        if (L.nops > 0) g.launch[nl++] = L;
+    };
+    bool merge45 = true;
+    std::unordered_map<int, int> in5, in4;   // destination -> packed BC
+    for (int o = phase_start[5]; o < phase_start[6] && merge45; o++)
+        merge45 = in5.emplace(ops[8 * o + 1], ops[8 * o + 6]).second;
+    for (int o = phase_start[4]; o < phase_start[5] && merge45; o++) {
+        auto it = in5.find(ops[8 * o + 1]);
+        merge45 = it != in5.end() && it->second == ops[8 * o + 6] &&
+                  in4.emplace(ops[8 * o + 1], 1).second;
+    }
+    add_exchange(phase_start[0], phase_start[2]);
+    add_physbc(phase_start[2], phase_start[3], nullptr);
+    add_exchange(phase_start[3], phase_start[4]);
+    if (merge45) {
+        add_physbc(phase_start[5], phase_start[6], &in4);
+    } else {
+        add_physbc(phase_start[4], phase_start[5], nullptr);
+        add_physbc(phase_start[5], phase_start[6], nullptr);
+    }
     if (nops > g.cap) {
         hipFree(g.ops);
         g.ops = nullptr;
@@ -364,17 +458,12 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
             return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost ops failed%s");
         g.cap = nops;
     }
-    if (nops > 0) {
-        HIP_TRY(hipMemcpyAsync(g.ops, ops, sizeof(GhostOp) * nops, hipMemcpyHostToDevice, s->st));
+    if (!dev.empty()) {
+        HIP_TRY(hipMemcpyAsync(g.ops, dev.data(), sizeof(GhostOp) * dev.size(), hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
     }
-    g.nops = nops;
-    std::memcpy(g.phase_start, phase_start, sizeof(g.phase_start));
-    for (int ph = 0; ph < 6; ph++) {
-        g.phase_physbc[ph] = false;
-        for (int o = phase_start[ph]; o < phase_start[ph + 1]; o++)
-            if (ops[8 * o] == hipclaw::GHOST_PHYSBC) g.phase_physbc[ph] = true;
-    }
+    g.nops = (int) dev.size();
+    g.nlaunch = nl;
     return HIPCLAW_OK;
 }
 
@@ -476,16 +565,13 @@ static int ghost_fill_async(hipclaw_state_t* s, int k) {
     g.qti = s->qti;
     g.pstride = s->pstride;
     g.mx = s->cfg.mx; g.my = s->cfg.my; g.mbc = s->cfg.mbc; g.meqn = s->cfg.meqn;
-    // Phases 0 (average fine to coarse) and 1 (copy) write disjoint ghost cells
-    // and read interior cells only, so they share one launch.
-    for (int ph = 0; ph < 6; ph++) {
-        const int last = (ph == 0) ? 1 : ph;
-        g.ops = p.ops + p.phase_start[ph];
-        g.nops = p.phase_start[last + 1] - p.phase_start[ph];
-        const bool physbc = p.phase_physbc[ph] || p.phase_physbc[last];
-        if (int e = hipclaw::launch_ghost_phase(g, physbc, s->st))
-            return hipfail((hipError_t) e, "ghost_phase launch");
-        ph = last;
+    for (int l = 0; l < p.nlaunch; l++) {
+        const GhostLaunch& L = p.launch[l];
+        g.ops = p.ops + L.start;
+        g.nops = L.nops;
+        const int e = L.physbc ? hipclaw::launch_ghost_physbc(g, s->st)
+                               : hipclaw::launch_ghost_exchange(g, s->st);
+        if (e) return hipfail((hipError_t) e, L.physbc ? "ghost physbc launch" : "ghost exchange launch");
     }
     return HIPCLAW_OK;
 }
@@ -980,8 +1066,7 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     s->dtscale = nullptr;
     for (GhostProgram& g : s->gp) {
         g.nops = 0;
-        for (int& v : g.phase_start) v = 0;
-        for (bool& b : g.phase_physbc) b = false;
+        g.nlaunch = 0;
     }
     for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }

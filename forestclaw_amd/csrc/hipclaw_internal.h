@@ -19,7 +19,7 @@ struct GhostOp {
     int which;  // face or corner of `dst` (copy) / of the coarse patch (average, interpolate)
     int ti, tj; // index translation (see the op builder)
     int bc;     // physical BC: mthbc of faces 0..3 packed as 4 signed bytes
-    int pad;
+    int reps;   // physical BC: times the BC is applied in a row (the library sets it)
 };
 
 enum {
@@ -70,7 +70,10 @@ int launch_tag_patches(const double* q, const int* slots, int n, const double* d
 int launch_regrid_remap(const double* qold, double* qnew, const int* ops, int nops,
                         long long pstride, int mx, int my, int mbc, int meqn, hipStream_t st);
 int launch_time_sync(const SyncArgs& a, hipStream_t st);
-int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st);
+// Exchange ops (copy, average, interpolate) of one phase, one wave per op.
+int launch_ghost_exchange(const GhostArgs& g, hipStream_t st);
+// Physical-BC ops, one workgroup per patch.
+int launch_ghost_physbc(const GhostArgs& g, hipStream_t st);
 int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_reset_scalar(unsigned long long* p, hipStream_t st);

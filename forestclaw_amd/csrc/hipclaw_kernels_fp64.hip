@@ -186,7 +186,9 @@ __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
 // Physical BC of one patch per workgroup (clawpack46_bc2_default.f): faces 0
 // and 1 together, then faces 2 and 3, whose strips include the corner cells
 // the first two wrote.  Within a pair no element reads a cell the pair
-// writes (mx, my >= 2 mbc), so all loads of a lane precede its stores.
+// writes (mx, my >= 2 mbc), so all loads of a lane precede its stores.  The
+// BC reads and writes cells of its own patch only, so applying it op.reps
+// times in a row equals that many phases of it.
 constexpr int PHYSBC_THREADS = 256;
 __global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
     const int o = blockIdx.x;
@@ -197,7 +199,9 @@ __global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
     double* qd = ((op.dst & TI) ? g.qti : g.q) + (long long) (op.dst & ~TI) * g.pstride;
     constexpr int U = 4;
-    for (int pair = 0; pair < 2; pair++) {
+    const int npass = 2 * (op.reps > 1 ? op.reps : 1);
+    for (int pass = 0; pass < npass; pass++) {
+        const int pair = pass & 1;
         const int len = pair == 0 ? my + 2 * mbc : mx + 2 * mbc;
         const int per = meqn * mbc * len, n = 2 * per;
         for (int base = threadIdx.x; base < n; base += PHYSBC_THREADS * U) {
@@ -434,13 +438,15 @@ patch_mass(const double* __restrict__ q, const int* __restrict__ slots, int n,
     }
 }
 
-int launch_ghost_phase(const GhostArgs& g, bool has_physbc, hipStream_t st) {
+int launch_ghost_exchange(const GhostArgs& g, hipStream_t st) {
     if (g.nops <= 0) return 0;
-    // a phase holds either physical-BC ops only or exchange ops only
-    if (has_physbc)
-        hipLaunchKernelGGL(physbc_phase, dim3(g.nops), dim3(PHYSBC_THREADS), 0, st, g);
-    else
-        hipLaunchKernelGGL(ghost_phase<4>, dim3((g.nops + 3) / 4), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(ghost_phase<4>, dim3((g.nops + 3) / 4), dim3(256), 0, st, g);
+    return (int) hipGetLastError();
+}
+
+int launch_ghost_physbc(const GhostArgs& g, hipStream_t st) {
+    if (g.nops <= 0) return 0;
+    hipLaunchKernelGGL(physbc_phase, dim3(g.nops), dim3(PHYSBC_THREADS), 0, st, g);
     return (int) hipGetLastError();
 }
 
