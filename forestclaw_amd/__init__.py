@@ -89,6 +89,8 @@ def lib():
         "fclaw2d_hip_forest_adapt": ([vp, _ip(), _ip(), i, i], i),
         "fclaw2d_hip_forest_set_isolated": ([vp, i], i),
         "hipclaw_set_dt_scale": ([vp, vp], i),
+        "hipclaw_save_step": ([vp], i),
+        "hipclaw_restore_step": ([vp], i),
         "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
         "hipclaw_tag_patches": ([vp, i, d, d, _ip(), _ip()], i),
         "hipclaw_regrid": ([vp, i, _ip(), _ip(), _dp(), _dp()], i),
@@ -303,6 +305,14 @@ class Solver:
         self.npatch = len(kind)
         self.max_patches = max(self.max_patches, self.npatch)
         self.set_forest(forest, extent)
+
+    def save_step(self):
+        """Keep a device copy of every patch's state (to retake a step)."""
+        _check(lib().hipclaw_save_step(self._h), "hipclaw_save_step")
+
+    def restore_step(self):
+        """Put the state kept by save_step back."""
+        _check(lib().hipclaw_restore_step(self._h), "hipclaw_restore_step")
 
     def set_dt_scale(self, scale):
         """Per-slot factor on dt (None: a common dt)."""

@@ -43,7 +43,20 @@ void fclaw2d_hip_struct_sizes(size_t out[6])
 
 fclaw2d_global_t *fclaw2d_global_new(void)
 {
-    return (fclaw2d_global_t *) calloc(1, sizeof(fclaw2d_global_t));
+    fclaw2d_global_t *g = (fclaw2d_global_t *) calloc(1, sizeof(fclaw2d_global_t));
+    if (!g) return NULL;
+    /* time stepping defaults of src/fclaw_options.c:62-98, 196-198 */
+    g->initial_dt = 0.1;
+    g->max_cfl = 1.0;
+    g->desired_cfl = 0.9;
+    g->reduce_cfl = 1;
+    g->use_fixed_dt = 0;
+    g->outstyle = 1;
+    g->tfinal = 1.0;
+    g->nout = 10;
+    g->nstep = 1;
+    g->regrid_interval = 1;
+    return g;
 }
 
 void fclaw2d_global_destroy(fclaw2d_global_t *glob)
@@ -980,6 +993,143 @@ void fclaw2d_regrid(fclaw2d_global_t *glob)
     }
     free(kind); free(src);
     glob->count_amr_regrid++;
+}
+
+/* ------------------------------------------------------------------ run loop */
+
+void fclaw2d_save_time_step(fclaw2d_global_t *glob)
+{
+    device_check(hipclaw_save_step(glob->state), "hipclaw_save_step");
+}
+
+void fclaw2d_restore_time_step(fclaw2d_global_t *glob)
+{
+    device_check(hipclaw_restore_step(glob->state), "hipclaw_restore_step");
+}
+
+/* fclaw2d_output_frame: the frame files the solver's options ask for */
+static void output_frame(fclaw2d_global_t *glob, int iframe)
+{
+    const fc2d_hipclaw_options_t *opt = glob->hipclaw_opt;
+    if (!glob->output || !opt) return;
+    if (opt->ascii_out) fclaw2d_clawpatch_output_ascii(glob, iframe);
+    if (opt->vtk_out) fclaw2d_clawpatch_output_vtk(glob, iframe);
+}
+
+static int level_factor(const fclaw2d_global_t *glob)
+{
+    return 1 << (glob->maxlevel - glob->minlevel);
+}
+
+/* A step whose Courant number is above max_cfl is rejected: the saved state
+ * comes back and dt shrinks to give desired_cfl (src/fclaw2d_run.c:220-232,
+ * 379-395).  Returns 1 if the step stands. */
+static int accept_step(fclaw2d_global_t *glob, double maxcfl, double *dt_minlevel)
+{
+    if (glob->reduce_cfl && maxcfl > glob->max_cfl && !glob->use_fixed_dt) {
+        fclaw2d_restore_time_step(glob);
+        *dt_minlevel = *dt_minlevel * glob->desired_cfl / maxcfl;
+        glob->count_retake++;
+        return 0;
+    }
+    return 1;
+}
+
+/* Output times 0, T/nout, 2T/nout, ..., T (src/fclaw2d_run.c:120-300) */
+static void run_outstyle_1(fclaw2d_global_t *glob)
+{
+    const double dt_outer = glob->tfinal / (double) glob->nout;
+    double dt_minlevel = glob->initial_dt, t_curr = 0.0;
+    int n, n_inner = 0, iframe = 0;
+    output_frame(glob, iframe);
+    for (n = 0; n < glob->nout; n++) {
+        const double tend = t_curr + dt_outer;
+        glob->curr_time = t_curr;
+        while (t_curr < tend) {
+            double dt_step = dt_minlevel, maxcfl;
+            int took_small_step = 0;
+            if (!glob->use_fixed_dt) fclaw2d_save_time_step(glob);
+            if (glob->advance_one_step) dt_step /= level_factor(glob);
+            if (!glob->use_fixed_dt) {
+                /* a step at most 1 % longer to land on tend, rather than a tiny one after it */
+                const double small_step = tend - (t_curr + dt_step);
+                if (small_step < 1e-2 * dt_step) {
+                    dt_step = tend - t_curr;
+                    took_small_step = small_step < 0;
+                }
+            }
+            glob->curr_dt = dt_step;
+            maxcfl = fclaw2d_advance_all_levels(glob, t_curr, dt_step);
+            if (!accept_step(glob, maxcfl, &dt_minlevel)) continue;
+            n_inner++;
+            t_curr += dt_step;
+            /* a step shortened to land on tend keeps the dt it would have taken */
+            if (!glob->use_fixed_dt && !took_small_step)
+                dt_minlevel = dt_minlevel * glob->desired_cfl / maxcfl;
+            glob->curr_time = t_curr;
+            if (glob->regrid_interval > 0 && n_inner % glob->regrid_interval == 0) fclaw2d_regrid(glob);
+        }
+        glob->curr_time = t_curr;
+        output_frame(glob, ++iframe);
+    }
+}
+
+/* nout steps, a frame every nstep steps (src/fclaw2d_run.c:306-427) */
+static void run_outstyle_3(fclaw2d_global_t *glob)
+{
+    const int lf = level_factor(glob);
+    const int per_level = !glob->subcycle && glob->advance_one_step;
+    double dt_minlevel = glob->initial_dt, t_curr = 0.0;
+    int nstep_outer = glob->nout, nstep_inner = glob->nstep, n = 0, iframe = 0;
+    output_frame(glob, iframe);
+    glob->curr_time = t_curr;
+    if (per_level && !glob->outstyle_uses_maxlevel) {
+        nstep_outer *= lf;
+        nstep_inner *= lf;
+    }
+    while (n < nstep_outer) {
+        double dt_step = per_level ? dt_minlevel / lf : dt_minlevel, maxcfl;
+        if (!glob->use_fixed_dt) fclaw2d_save_time_step(glob);
+        glob->curr_dt = dt_step;
+        maxcfl = fclaw2d_advance_all_levels(glob, t_curr, dt_step);
+        if (!accept_step(glob, maxcfl, &dt_minlevel)) continue;
+        t_curr += dt_step;
+        glob->curr_time = t_curr;
+        if (!glob->use_fixed_dt) dt_minlevel = dt_minlevel * glob->desired_cfl / maxcfl;
+        n++;
+        if (glob->regrid_interval > 0 && n % glob->regrid_interval == 0) fclaw2d_regrid(glob);
+        if (n % nstep_inner == 0) output_frame(glob, ++iframe);
+    }
+}
+
+/* nout steps of the fixed dt initial_dt, no CFL check (src/fclaw2d_run.c:429-514) */
+static void run_outstyle_4(fclaw2d_global_t *glob)
+{
+    double t_curr = 0.0;
+    int n = 0, iframe = 0;
+    output_frame(glob, iframe);
+    glob->curr_time = t_curr;
+    while (n < glob->nout) {
+        fclaw2d_advance_all_levels(glob, t_curr, glob->initial_dt);
+        t_curr += glob->initial_dt;
+        n++;
+        glob->curr_time = t_curr;
+        if (glob->regrid_interval > 0 && n % glob->regrid_interval == 0) fclaw2d_regrid(glob);
+        if (n % glob->nstep == 0) output_frame(glob, ++iframe);
+    }
+}
+
+void fclaw2d_run(fclaw2d_global_t *glob)
+{
+    switch (glob->outstyle) {
+    case 0: output_frame(glob, 0); break;
+    case 1: run_outstyle_1(glob); break;
+    case 3: run_outstyle_3(glob); break;
+    case 4: run_outstyle_4(glob); break;
+    default:
+        fprintf(stderr, "Outstyle %d not implemented yet\n", glob->outstyle);
+        exit(0);
+    }
 }
 
 /* ------------------------------------------------------------------ VTK output */

@@ -89,6 +89,8 @@ struct hipclaw_state {
     double* qlast = nullptr;
     double* qti = nullptr;
     double* dtscale = nullptr;    // per-slot factor on dt, or null
+    double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
+    bool saved = false;           // qsave holds the current slots' state
     // ghost-patch message: slab offsets of one field's band, in message order
     int* band = nullptr;
     int nband = 0;
@@ -235,6 +237,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     }
     hipFree(s->qlast);
     hipFree(s->qti);
+    hipFree(s->qsave);
     hipFree(s->cfl);
     hipFree(s->mass_buf);
     hipFree(s->reg);
@@ -660,6 +663,27 @@ int hipclaw_ghost_unpack(hipclaw_state_t* s, const int* slots, int n, int time_i
     return ghost_message(s, slots, n, time_interp, const_cast<double*>(buf), 1);
 }
 
+int hipclaw_save_step(hipclaw_state_t* s) {
+    if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_save_step: bad arguments%s");
+    const size_t bytes = sizeof(double) * (size_t) s->pstride * (size_t) s->npatch;
+    if (!s->qsave) {
+        if (hipMalloc((void**) &s->qsave, sizeof(double) * (size_t) s->pstride * (size_t) s->max_patches) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of the saved state failed%s");
+    }
+    if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->qsave, s->q[s->cur], bytes, hipMemcpyDeviceToDevice, s->st));
+    s->saved = true;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_restore_step(hipclaw_state_t* s) {
+    if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_restore_step: bad arguments%s");
+    if (!s->saved) return fail(HIPCLAW_EINVAL, "no saved state for the current patches%s");
+    const size_t bytes = sizeof(double) * (size_t) s->pstride * (size_t) s->npatch;
+    if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->q[s->cur], s->qsave, bytes, hipMemcpyDeviceToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return HIPCLAW_OK;
+}
+
 int hipclaw_set_dt_scale(hipclaw_state_t* s, const double* scale) {
     if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_set_dt_scale: bad arguments%s");
     if (!scale) {
@@ -1056,7 +1080,11 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
             HIP_TRY(hipMemset(s->q[1], 0, sizeof(double) * s->pstride * (size_t) cap));
         }
     }
+    // a saved state names old slots (and is sized for the old capacity)
+    s->saved = false;
     if (cap > s->max_patches) {
+        hipFree(s->qsave);
+        s->qsave = nullptr;
         if (int rc = grow_patch_buffers(s, cap)) return rc;
     } else if (s->reg) {
         HIP_TRY(hipMemset(s->reg, 0, sizeof(double) * s->rstride * (size_t) s->max_patches));

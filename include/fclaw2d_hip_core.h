@@ -106,6 +106,13 @@ typedef struct fclaw2d_global {
     int threshold_variable;
     int count_amr_regrid, count_amr_new_domain;   /* src/fclaw2d_regrid.c:321, 333 */
     const char *prefix;            /* output file prefix, NULL = "fort" (src/fclaw_options.c:177) */
+    /* time stepping of fclaw2d_run (src/fclaw_options.c:62-98, 196-198); the
+     * defaults of fclaw2d_global_new are the reference's */
+    double initial_dt, max_cfl, desired_cfl, tfinal;
+    int reduce_cfl, use_fixed_dt, outstyle, nout, nstep, outstyle_uses_maxlevel;
+    int regrid_interval;
+    int output;                    /* fclaw_options output: write frames (src/fclaw_options.c:122) */
+    int count_retake;              /* steps rejected for max_cfl and retaken */
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -172,6 +179,21 @@ void fclaw2d_timeinterp(fclaw2d_global_t *glob, int level, double alpha);
  * Nothing moves when the leaves do not change. */
 void fclaw2d_initialize(fclaw2d_global_t *glob);
 void fclaw2d_regrid(fclaw2d_global_t *glob);
+
+/* ---- the run loop ----
+ * fclaw2d_run (src/fclaw2d_run.c:516-542) with output styles 0, 1, 3 and 4
+ * (src/fclaw2d_run.c:86-300, 306-514): steps of fclaw2d_advance_all_levels
+ * with dt set from the previous step's Courant number (dt * desired_cfl /
+ * maxcfl), a step whose Courant number exceeds max_cfl rejected and retaken
+ * from the saved state (unless use_fixed_dt), a regrid every
+ * regrid_interval steps, and a frame (fort.qXXXX and/or
+ * <prefix>_frame_XXXX.vtu, as the solver options ascii_out / vtk_out ask)
+ * at every output time when glob->output is set.  Other styles exit with
+ * the reference's message.  save / restore_time_step keep and put back the
+ * state of every patch on the device (src/fclaw2d_run.c:47-80). */
+void fclaw2d_run(fclaw2d_global_t *glob);
+void fclaw2d_save_time_step(fclaw2d_global_t *glob);
+void fclaw2d_restore_time_step(fclaw2d_global_t *glob);
 
 /* ---- output ----
  * ASCII frames as the reference writes them: fort.tXXXX (time header) and

@@ -223,6 +223,16 @@ int hipclaw_ghost_unpack(hipclaw_state_t *s, const int *slots, int n, int time_i
  * change).  No counterpart in the reference, which runs one dt per level. */
 int hipclaw_set_dt_scale(hipclaw_state_t *s, const double *scale);
 
+/* Copy the current solution of every patch (ghost cells included) to the
+ * saved state on the device, or the saved state back: clawpatch_save_step /
+ * clawpatch_restore_step (src/patches/clawpatch/fclaw2d_clawpatch.cpp:446-470)
+ * for all patches at once, as save_time_step / restore_time_step of the run
+ * loop use them to reject a step whose Courant number exceeds max_cfl
+ * (src/fclaw2d_run.c:47-80).  Restoring without a saved state, or after a
+ * regrid (the slots changed), is an error. */
+int hipclaw_save_step(hipclaw_state_t *s);
+int hipclaw_restore_step(hipclaw_state_t *s);
+
 /* Cell values of slots first..first+count-1 as float, in the order of the
  * VTK "meqn" cell array (per patch: j = 1..my, i = 1..mx, then the meqn
  * fields), converted on the device so that half the bytes cross PCIe.

@@ -54,7 +54,11 @@ class Global(C.Structure):
                 ("minlevel", C.c_int), ("maxlevel", C.c_int), ("refine_threshold", C.c_double),
                 ("coarsen_threshold", C.c_double), ("threshold_variable", C.c_int),
                 ("count_amr_regrid", C.c_int), ("count_amr_new_domain", C.c_int),
-                ("prefix", C.c_char_p)]
+                ("prefix", C.c_char_p), ("initial_dt", C.c_double), ("max_cfl", C.c_double),
+                ("desired_cfl", C.c_double), ("tfinal", C.c_double), ("reduce_cfl", C.c_int),
+                ("use_fixed_dt", C.c_int), ("outstyle", C.c_int), ("nout", C.c_int),
+                ("nstep", C.c_int), ("outstyle_uses_maxlevel", C.c_int),
+                ("regrid_interval", C.c_int), ("output", C.c_int), ("count_retake", C.c_int)]
 
 
 class Options(C.Structure):
@@ -91,6 +95,9 @@ def bind():
         "fclaw2d_timeinterp": ([C.POINTER(Global), i, d], None),
         "fclaw2d_initialize": ([C.POINTER(Global)], None),
         "fclaw2d_regrid": ([C.POINTER(Global)], None),
+        "fclaw2d_run": ([C.POINTER(Global)], None),
+        "fclaw2d_save_time_step": ([C.POINTER(Global)], None),
+        "fclaw2d_restore_time_step": ([C.POINTER(Global)], None),
         "fc2d_hipclaw_options_default": ([C.POINTER(Options), i], None),
         "fc2d_hipclaw_options_store": ([C.POINTER(Global), C.POINTER(Options)], None),
         "fc2d_hipclaw_check": ([C.POINTER(Options)], i),
