@@ -52,6 +52,9 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     if dt_scale is not None:
         sv.set_dt_scale(dt_scale)
     sv.ghost_fill()
+    tuned = None
+    if tile == 0:
+        tuned = {"%d/%d" % k: v for k, v in sv.autotune(dt, 8).items()}   # "tile/layout"
     mass0 = sv.mass()
     if warmup > 0:
         sv.advance_async(dt, warmup)
@@ -75,7 +78,7 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     mass1 = sv.mass()
     total_steps = warmup + 2 * steps
     return dict(wall=wall, kernel_ms=kernel_ms, q=q, mass0=mass0, mass1=mass1, tile=sv.tile,
-                graph=graph, total_steps=total_steps)
+                graph=graph, total_steps=total_steps, autotune=tuned, ghost_layout=sv.ghost_layout)
 
 
 def run_regrid(fa, prob, forest, q, repeats=3):
@@ -184,6 +187,39 @@ def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
             "config": {"workload": workload, "cfl": prob.cfl, "time_sync": True,
                        "refine_threshold": prob.tag_threshold,
                        "coarsen_threshold": 0.5 * prob.tag_threshold}}
+
+
+def sweep_one(fa, P, minlevel, maxlevel, mx, args):
+    """The dam break at one patch size: both forms with autotuned tiles, the
+    FP32-bulk form's error against the FP64 form and both mass changes."""
+    prob = P.dambreak_problem(minlevel, maxlevel, mx=mx)
+    forest = P.build_forest(prob)
+    q0, aux = P.initial_data(prob, forest)
+    dt = P.stable_dt(prob, forest, q0)
+    cells = forest.npatch * mx * mx
+    r = {}
+    for name, precision in (("fp64", fa.FP64), ("fp32", fa.FP32_BULK)):
+        r[name] = run_form(fa, prob, forest, q0, aux, precision, dt, args.steps, args.warmup, 0,
+                           not args.no_graph, not args.no_time_sync)
+    _, _, dx, dy = forest.geometry(prob.extent, mx, mx)
+    area = (dx * dy)[:, None, None, None]
+    a, b = r["fp32"]["q"][:, :, 2:-2, 2:-2], r["fp64"]["q"][:, :, 2:-2, 2:-2]
+    l1 = (np.abs(a - b) * area).sum(axis=(0, 2, 3)) / (np.abs(b) * area).sum(axis=(0, 2, 3))
+    mx_err = np.abs(a - b).max(axis=(0, 2, 3)) / np.abs(b).max(axis=(0, 2, 3))
+    B = algorithmic_bytes_per_cell(prob.meqn, prob.maux, mx)
+    out = {"workload": "swe_dambreak_amr_L%d-%d_mx%d" % (minlevel, maxlevel, mx),
+           "patches": forest.npatch, "cells": cells, "steps": args.steps}
+    for name, x in r.items():
+        out[name] = {"value": round(cells * args.steps / x["wall"], 1), "unit": "cell-updates/s",
+                     "ms_per_step": round(x["wall"] * 1e3 / args.steps, 5), "tile": x["tile"],
+                     "ghost_layout": x["ghost_layout"],
+                     "autotune_ms_per_step": x["autotune"],
+                     "kernel_avg_ms": round(x["kernel_ms"], 5),
+                     "hbm_frac": round(B * cells / (x["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "mass_change": float((x["mass1"][0][0] - x["mass0"][0][0]) / x["mass0"][0][0])}
+    out["fp32_l1_rel_vs_fp64"] = [float("%.3e" % v) for v in l1]
+    out["fp32_max_rel_vs_fp64"] = [float("%.3e" % v) for v in mx_err]
+    return out
 
 
 PROFILE_STEPS = 6
@@ -336,8 +372,11 @@ def main():
                          "CFL-controlled dt and a regrid every --regrid-interval steps, and "
                          "report the whole run (steps, regrids, time, cell updates/s)")
     ap.add_argument("--regrid-interval", type=int, default=4)
-    ap.add_argument("--tile64", type=int, default=0, help="tile for the FP64 form (0 = tuned)")
-    ap.add_argument("--tile32", type=int, default=0, help="tile for the FP32 form (0 = tuned)")
+    ap.add_argument("--tile64", type=int, default=0, help="tile for the FP64 form (0 = autotune)")
+    ap.add_argument("--tile32", type=int, default=0, help="tile for the FP32 form (0 = autotune)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="the dam break at levels --minlevel..--maxlevel for mx = 8, 16, 32, "
+                         "both forms autotuned: one JSON line per mx (no CPU baseline)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -380,6 +419,11 @@ def main():
         workload = {"dambreak": "swe_dambreak_amr", "advection": "advection_disk",
                     "shockbubble": "euler_shockbubble_amr"}[args.problem] + \
             "_L%d-%d_mx%d" % (minlevel, maxlevel, mx)
+    if args.sweep:
+        fa.initialize_device(0)
+        for smx in (8, 16, 32):
+            print(json.dumps(sweep_one(fa, P, minlevel, maxlevel, smx, args)), flush=True)
+        return
     if args.run_to is not None:
         if ensemble:
             sys.exit("bench.py: --run-to is for single runs, not --problem ensemble")
@@ -491,6 +535,8 @@ def main():
             "patches": npatch, "patches_per_level": np.bincount(lv).tolist(),
             "cells": cells, "mx": prob.mx, "mbc": 2, "meqn": prob.meqn,
             "dt": dt, "tile_fp32": r32["tile"], "tile_fp64": r64["tile"],
+            "ghost_layout_fp32": r32["ghost_layout"], "ghost_layout_fp64": r64["ghost_layout"],
+            "autotune_ms_per_step": {"fp32": r32["autotune"], "fp64": r64["autotune"]},
             "timed_with_hip_graph": bool(r32["graph"]), "time_sync": not args.no_time_sync,
             "device": device,
         },

@@ -91,6 +91,9 @@ def lib():
         "hipclaw_set_dt_scale": ([vp, vp], i),
         "hipclaw_save_step": ([vp], i),
         "hipclaw_restore_step": ([vp], i),
+        "hipclaw_autotune": ([vp, C.c_double, i, vp, vp, vp, vp], i),
+        "hipclaw_set_ghost_layout": ([vp, i], i),
+        "hipclaw_ghost_layout": ([vp], i),
         "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
         "hipclaw_tag_patches": ([vp, i, d, d, _ip(), _ip()], i),
         "hipclaw_regrid": ([vp, i, _ip(), _ip(), _dp(), _dp()], i),
@@ -230,7 +233,7 @@ class Solver:
 
     def __init__(self, solver, mx, my, mbc=2, precision=FP64, order=(2, 2), mthlim=None,
                  mthbc=(1, 1, 1, 1), grav=1.0, gamma=1.4, tile=0, max_patches=1,
-                 time_sync=False):
+                 time_sync=False, subcycle=False):
         meqn, mwaves, maux = SOLVER_SHAPES[solver]
         cfg = HipclawConfig()
         cfg.solver, cfg.precision = solver, precision
@@ -243,6 +246,7 @@ class Solver:
             cfg.mthbc[k] = mthbc[k]
         cfg.grav, cfg.gamma, cfg.tile = grav, gamma, tile
         cfg.time_sync = int(bool(time_sync))
+        cfg.subcycle = int(bool(subcycle))
         self.time_sync = bool(time_sync)
         self.cfg = cfg
         self.meqn, self.maux, self.mwaves = meqn, maux, mwaves
@@ -313,6 +317,27 @@ class Solver:
     def restore_step(self):
         """Put the state kept by save_step back."""
         _check(lib().hipclaw_restore_step(self._h), "hipclaw_restore_step")
+
+    def autotune(self, dt, nsteps=8):
+        """Time every allowed tile in both ghost-fill layouts over nsteps steps
+        of dt (the state is put back after each) and keep the fastest;
+        returns {(tile, layout): ms per step}."""
+        tiles = np.zeros(6, np.int32)
+        layouts = np.zeros(6, np.int32)
+        ms = np.zeros(6, np.float64)
+        n = C.c_int()
+        _check(lib().hipclaw_autotune(self._h, dt, nsteps, tiles.ctypes.data, layouts.ctypes.data,
+                                      ms.ctypes.data, C.byref(n)), "hipclaw_autotune")
+        return {(int(t), int(l)): float(m)
+                for t, l, m in zip(tiles[:n.value], layouts[:n.value], ms[:n.value])}
+
+    @property
+    def ghost_layout(self):
+        return lib().hipclaw_ghost_layout(self._h)
+
+    def set_ghost_layout(self, layout):
+        """0: one launch per ghost-fill phase; 1: per-destination groups."""
+        _check(lib().hipclaw_set_ghost_layout(self._h, layout), "hipclaw_set_ghost_layout")
 
     def set_dt_scale(self, scale):
         """Per-slot factor on dt (None: a common dt)."""

@@ -40,19 +40,26 @@ int hipfail(hipError_t e, const char* where) {
 
 }  // namespace
 
-// One kernel launch of a ghost-fill program: ops start .. start+nops-1 of the
-// program's device array, either exchange ops or physical-BC ops.
+// One kernel launch of a ghost-fill program: exchange ops or physical-BC ops
+// start .. start+nops-1 of the program's device array, or per-destination
+// groups group0 .. group0+ngroups-1 of its group array.
+enum { GHOST_LAUNCH_EXCHANGE, GHOST_LAUNCH_PHYSBC, GHOST_LAUNCH_GROUPS };
 struct GhostLaunch {
     int start = 0, nops = 0;
-    bool physbc = false;
+    int kind = GHOST_LAUNCH_EXCHANGE;
+    int group0 = 0, ngroups = 0;
 };
 
 // One ghost-fill program (hipclaw_set_ghost_program_k).
 struct GhostProgram {
     GhostOp* ops = nullptr;
     int nops = 0, cap = 0;
-    GhostLaunch launch[6];
-    int nlaunch = 0;
+    int4* groups = nullptr;
+    int groups_cap = 0;
+    // two layouts of the same program (hipclaw_set_ghost_layout): 0 = one
+    // launch per phase, 1 = per-destination groups
+    GhostLaunch launch[2][6];
+    int nlaunch[2] = {0, 0};
 };
 
 // One time-synchronisation program (hipclaw_set_sync_program_k).
@@ -89,6 +96,7 @@ struct hipclaw_state {
     double* qlast = nullptr;
     double* qti = nullptr;
     double* dtscale = nullptr;    // per-slot factor on dt, or null
+    int ghost_layout = 0;         // launch layout of the ghost fill (hipclaw_set_ghost_layout)
     double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
     bool saved = false;           // qsave holds the current slots' state
     // ghost-patch message: slab offsets of one field's band, in message order
@@ -166,6 +174,9 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     s->nty = cfg->my / tile;
     s->inplace = (s->ntx == 1 && s->nty == 1);
     s->max_patches = max_patches;
+    // measured on MI355X: per-destination groups pay off once a patch's ghost
+    // strips are long (mx >= 32); shorter ones keep one launch per phase
+    s->ghost_layout = cfg->mx >= 32 ? 1 : 0;
     const long long cells = (long long) (cfg->mx + 2 * cfg->mbc) * (cfg->my + 2 * cfg->mbc);
     s->pstride = cells * cfg->meqn;
     s->astride = cells * cfg->maux;
@@ -229,7 +240,10 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->dys);
     hipFree(s->all_slots);
     hipFree(s->batch_slots);
-    for (GhostProgram& g : s->gp) hipFree(g.ops);
+    for (GhostProgram& g : s->gp) {
+        hipFree(g.ops);
+        hipFree(g.groups);
+    }
     for (SyncProgram& y : s->sp) {
         hipFree(y.ops);
         hipFree(y.start);
@@ -375,7 +389,7 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
     GhostProgram& g = s->gp[k];
     std::vector<GhostOp> dev;
     dev.reserve((size_t) nops);
-    int nl = 0;
+    int layout = 0, nl[2] = {0, 0};
     auto to_op = [&](int o, int reps) {
         const int* q = ops + 8 * o;
         GhostOp op;
@@ -425,15 +439,15 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         }
         for (int o : order) dev.push_back(to_op(o, 1));
         L.nops = (int) dev.size() - L.start;
-        if (L.nops > 0) g.launch[nl++] = L;
+        if (L.nops > 0) g.launch[layout][nl[layout]++] = L;
     };
     auto add_physbc = [&](int a, int b, const std::unordered_map<int, int>* twice) {
         GhostLaunch L;
         L.start = (int) dev.size();
-        L.physbc = true;
+        L.kind = GHOST_LAUNCH_PHYSBC;
         for (int o = a; o < b; o++) dev.push_back(to_op(o, twice && twice->count(ops[8 * o + 1]) ? 2 : 1));
         L.nops = (int) dev.size() - L.start;
-        if (L.nops > 0) g.launch[nl++] = L;
+        if (L.nops > 0) g.launch[layout][nl[layout]++] = L;
     };
     bool merge45 = true;
     std::unordered_map<int, int> in5, in4;   // destination -> packed BC
@@ -444,6 +458,49 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         merge45 = it != in5.end() && it->second == ops[8 * o + 6] &&
                   in4.emplace(ops[8 * o + 1], 1).second;
     }
+    // Per-destination groups: the exchange ops of ops [a, b) by destination
+    // patch, each group followed by that patch's physical-BC op of [pa, pb)
+    // (applied `twice` where listed), in one launch.  Needs at most one BC op
+    // per patch; false (nothing added) otherwise.
+    std::vector<int4> groups;
+    auto add_groups = [&](int a, int b, int pa, int pb, const std::unordered_map<int, int>* twice) {
+        std::unordered_map<int, int> gi;   // destination -> group
+        std::vector<std::vector<int>> members;
+        std::vector<int> bc;
+        for (int o = a; o < b; o++) {
+            auto it = gi.emplace(ops[8 * o + 1], (int) members.size());
+            if (it.second) { members.emplace_back(); bc.push_back(-1); }
+            members[it.first->second].push_back(o);
+        }
+        for (int o = pa; o < pb; o++) {
+            auto it = gi.emplace(ops[8 * o + 1], (int) members.size());
+            if (it.second) { members.emplace_back(); bc.push_back(-1); }
+            if (bc[it.first->second] >= 0) return false;
+            bc[it.first->second] = o;
+        }
+        GhostLaunch L;
+        L.kind = GHOST_LAUNCH_GROUPS;
+        L.start = (int) dev.size();
+        L.group0 = (int) groups.size();
+        for (size_t k = 0; k < members.size(); k++) {
+            int4 gr;   // op indices relative to the launch's first op
+            gr.x = (int) dev.size() - L.start;
+            for (int o : members[k]) dev.push_back(to_op(o, 1));
+            gr.y = (int) dev.size() - L.start;
+            gr.z = -1;
+            gr.w = 0;
+            if (bc[k] >= 0) {
+                gr.z = (int) dev.size() - L.start;
+                dev.push_back(to_op(bc[k], twice && twice->count(ops[8 * bc[k] + 1]) ? 2 : 1));
+            }
+            groups.push_back(gr);
+        }
+        L.nops = (int) dev.size() - L.start;
+        L.ngroups = (int) groups.size() - L.group0;
+        if (L.ngroups > 0) g.launch[layout][nl[layout]++] = L;
+        return true;
+    };
+    // layout 0: one launch per phase group
     add_exchange(phase_start[0], phase_start[2]);
     add_physbc(phase_start[2], phase_start[3], nullptr);
     add_exchange(phase_start[3], phase_start[4]);
@@ -453,20 +510,50 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         add_physbc(phase_start[4], phase_start[5], nullptr);
         add_physbc(phase_start[5], phase_start[6], nullptr);
     }
-    if (nops > g.cap) {
+    // layout 1, phases 0-2: averages and copies into a patch, then its BC
+    // (coarse levels)
+    layout = 1;
+    if (!add_groups(phase_start[0], phase_start[2], phase_start[2], phase_start[3], nullptr)) {
+        add_exchange(phase_start[0], phase_start[2]);
+        add_physbc(phase_start[2], phase_start[3], nullptr);
+    }
+    // phases 3-5: interpolations into a patch, then its BC (fine levels twice)
+    if (!(merge45 && add_groups(phase_start[3], phase_start[4], phase_start[5], phase_start[6], &in4))) {
+        add_exchange(phase_start[3], phase_start[4]);
+        if (merge45) {
+            add_physbc(phase_start[5], phase_start[6], &in4);
+        } else {
+            add_physbc(phase_start[4], phase_start[5], nullptr);
+            add_physbc(phase_start[5], phase_start[6], nullptr);
+        }
+    }
+    if ((int) dev.size() > g.cap) {
         hipFree(g.ops);
         g.ops = nullptr;
         g.cap = 0;
-        if (hipMalloc((void**) &g.ops, sizeof(GhostOp) * nops) != hipSuccess)
+        if (hipMalloc((void**) &g.ops, sizeof(GhostOp) * dev.size()) != hipSuccess)
             return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost ops failed%s");
-        g.cap = nops;
+        g.cap = (int) dev.size();
     }
     if (!dev.empty()) {
         HIP_TRY(hipMemcpyAsync(g.ops, dev.data(), sizeof(GhostOp) * dev.size(), hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
     }
+    if ((int) groups.size() > g.groups_cap) {
+        hipFree(g.groups);
+        g.groups = nullptr;
+        g.groups_cap = 0;
+        if (hipMalloc((void**) &g.groups, sizeof(int4) * groups.size()) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost groups failed%s");
+        g.groups_cap = (int) groups.size();
+    }
+    if (!groups.empty()) {
+        HIP_TRY(hipMemcpyAsync(g.groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+    }
     g.nops = (int) dev.size();
-    g.nlaunch = nl;
+    g.nlaunch[0] = nl[0];
+    g.nlaunch[1] = nl[1];
     return HIPCLAW_OK;
 }
 
@@ -568,13 +655,16 @@ static int ghost_fill_async(hipclaw_state_t* s, int k) {
     g.qti = s->qti;
     g.pstride = s->pstride;
     g.mx = s->cfg.mx; g.my = s->cfg.my; g.mbc = s->cfg.mbc; g.meqn = s->cfg.meqn;
-    for (int l = 0; l < p.nlaunch; l++) {
-        const GhostLaunch& L = p.launch[l];
+    const int lay = s->ghost_layout;
+    for (int l = 0; l < p.nlaunch[lay]; l++) {
+        const GhostLaunch& L = p.launch[lay][l];
         g.ops = p.ops + L.start;
         g.nops = L.nops;
-        const int e = L.physbc ? hipclaw::launch_ghost_physbc(g, s->st)
-                               : hipclaw::launch_ghost_exchange(g, s->st);
-        if (e) return hipfail((hipError_t) e, L.physbc ? "ghost physbc launch" : "ghost exchange launch");
+        const int e = L.kind == GHOST_LAUNCH_PHYSBC ? hipclaw::launch_ghost_physbc(g, s->st)
+                      : L.kind == GHOST_LAUNCH_GROUPS
+                          ? hipclaw::launch_ghost_groups(g, p.groups + L.group0, L.ngroups, s->st)
+                          : hipclaw::launch_ghost_exchange(g, s->st);
+        if (e) return hipfail((hipError_t) e, "ghost fill launch");
     }
     return HIPCLAW_OK;
 }
@@ -682,6 +772,93 @@ int hipclaw_restore_step(hipclaw_state_t* s) {
     if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->q[s->cur], s->qsave, bytes, hipMemcpyDeviceToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     return HIPCLAW_OK;
+}
+
+static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool timed);
+
+// Tile sizes a state can switch to: supported by its solver and precision,
+// dividing the patch, and (FP64 with time sync) leaving LDS for the registers.
+static bool tile_allowed(const hipclaw_state_t* s, int t) {
+    return hipclaw::step_supported(s->cfg.solver, s->cfg.precision, t) && s->cfg.mx % t == 0 &&
+           s->cfg.my % t == 0 && !(s->cfg.time_sync && s->cfg.precision == HIPCLAW_FP64 && t == 32);
+}
+
+static int set_tile(hipclaw_state_t* s, int t) {
+    const int ntx = s->cfg.mx / t, nty = s->cfg.my / t;
+    const bool inplace = ntx == 1 && nty == 1;
+    if (!inplace && !s->q[1]) {   // a ping-pong tile needs the second slab
+        if (hipMalloc((void**) &s->q[1], sizeof(double) * s->pstride * (size_t) s->max_patches) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of the second slab failed%s");
+    }
+    s->tile = t;
+    s->ntx = ntx;
+    s->nty = nty;
+    s->inplace = inplace;
+    // a captured graph holds launches of the old tile
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_ghost_layout(hipclaw_state_t* s, int layout) {
+    if (!s || layout < 0 || layout > 1) return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_layout: bad arguments%s");
+    s->ghost_layout = layout;
+    // a captured graph holds the launches of the old layout
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_ghost_layout(const hipclaw_state_t* s) { return s ? s->ghost_layout : -1; }
+
+int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int* layouts, double* ms,
+                     int* ntrials) {
+    if (!s || nsteps <= 0 || !(dt > 0.0) || !tiles || !layouts || !ms || !ntrials)
+        return fail(HIPCLAW_EINVAL, "hipclaw_autotune: bad arguments%s");
+    if (s->reg_accumulate)
+        return fail(HIPCLAW_EINVAL, "hipclaw_autotune: tune before subcycling (registers accumulate)%s");
+    *ntrials = 0;
+    if (s->npatch == 0) return HIPCLAW_OK;
+    if (int rc = hipclaw_save_step(s)) return rc;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    int best_tile = s->tile, best_layout = s->ghost_layout, n = 0, rc = HIPCLAW_OK;
+    double best_ms = 0.0;
+    static const int candidates[3] = {8, 16, 32};
+    for (int t : candidates) {
+        if (!tile_allowed(s, t)) continue;
+        for (int lay = 0; lay < 2 && !rc; lay++) {
+            if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay))) break;
+            // one untimed step (first use of these kernels), then the timed ones
+            if ((rc = advance_launches(s, dt, 1, false))) break;
+            hipError_t e = hipEventRecord(ev[0], s->st);
+            if (e == hipSuccess && (rc = advance_launches(s, dt, nsteps, false))) break;
+            if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            float elapsed = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&elapsed, ev[0], ev[1]);
+            if (e != hipSuccess) { rc = hipfail(e, "hipclaw_autotune timing"); break; }
+            if ((rc = hipclaw_restore_step(s))) break;
+            tiles[n] = t;
+            layouts[n] = lay;
+            ms[n] = (double) elapsed / nsteps;
+            if (n == 0 || ms[n] < best_ms) { best_tile = t; best_layout = lay; best_ms = ms[n]; }
+            n++;
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (rc) {
+        hipclaw_restore_step(s);
+        return rc;
+    }
+    *ntrials = n;
+    if (int r = set_tile(s, best_tile)) return r;
+    return hipclaw_set_ghost_layout(s, best_layout);
 }
 
 int hipclaw_set_dt_scale(hipclaw_state_t* s, const double* scale) {
@@ -1094,7 +1271,7 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     s->dtscale = nullptr;
     for (GhostProgram& g : s->gp) {
         g.nops = 0;
-        g.nlaunch = 0;
+        g.nlaunch[0] = g.nlaunch[1] = 0;
     }
     for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }

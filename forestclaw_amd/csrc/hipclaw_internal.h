@@ -74,6 +74,9 @@ int launch_time_sync(const SyncArgs& a, hipStream_t st);
 int launch_ghost_exchange(const GhostArgs& g, hipStream_t st);
 // Physical-BC ops, one workgroup per patch.
 int launch_ghost_physbc(const GhostArgs& g, hipStream_t st);
+// Per-destination groups: a patch's exchange ops, then its physical BC;
+// groups[b] = (first op, end, physical-BC op or -1, -).
+int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hipStream_t st);
 int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);
 int launch_reset_scalar(unsigned long long* p, hipStream_t st);

@@ -42,17 +42,12 @@ __device__ __forceinline__ void interp_cell(const double* qc, double* qf, int m,
 // so each lane issues the loads of up to U elements before their stores: the
 // source and destination slabs are the same allocation, and without this the
 // compiler must wait for every store before the next load.
-template <int OPW>
-__global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
-    const int o = blockIdx.x * OPW + (OPW > 1 ? (int) (threadIdx.x >> 6) : 0);
-    if (o >= g.nops) return;
-    const GhostOp op = g.ops[o];
+__device__ __forceinline__ void ghost_exchange_op(const GhostArgs& g, const GhostOp& op, int t) {
     const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
     double* qd = ((op.dst & TI) ? g.qti : g.q) + (long long) (op.dst & ~TI) * g.pstride;
     const double* qs = ((op.src & TI) ? g.qti : g.q) + (long long) (op.src & ~TI) * g.pstride;
-    const int t = threadIdx.x & 63;
     constexpr int U = 4;
 
     switch (op.type) {
@@ -183,6 +178,13 @@ __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
     }
 }
 
+template <int OPW>
+__global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
+    const int o = blockIdx.x * OPW + (OPW > 1 ? (int) (threadIdx.x >> 6) : 0);
+    if (o >= g.nops) return;
+    ghost_exchange_op(g, g.ops[o], threadIdx.x & 63);
+}
+
 // Physical BC of one patch per workgroup (clawpack46_bc2_default.f): faces 0
 // and 1 together, then faces 2 and 3, whose strips include the corner cells
 // the first two wrote.  Within a pair no element reads a cell the pair
@@ -190,10 +192,8 @@ __global__ void __launch_bounds__(64 * OPW) ghost_phase(GhostArgs g) {
 // BC reads and writes cells of its own patch only, so applying it op.reps
 // times in a row equals that many phases of it.
 constexpr int PHYSBC_THREADS = 256;
-__global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
-    const int o = blockIdx.x;
-    if (o >= g.nops) return;
-    const GhostOp op = g.ops[o];
+// Every thread of the workgroup (PHYSBC_THREADS) calls this: it synchronises.
+__device__ __forceinline__ void physbc_op(const GhostArgs& g, const GhostOp& op) {
     const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
@@ -238,6 +238,34 @@ __global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
                 if (di[u] >= 0) qd[di[u]] = v[u];
         }
         __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(PHYSBC_THREADS) physbc_phase(GhostArgs g) {
+    const int o = blockIdx.x;
+    if (o >= g.nops) return;
+    physbc_op(g, g.ops[o]);
+}
+
+// One workgroup per destination patch of a phase group: the patch's
+// exchange ops (one wave each, the four waves taking them in turn), then,
+// once they are all written, its physical BC.  groups[b] = (first exchange
+// op, end of its exchange ops, physical-BC op or -1, unused), indices into
+// g.ops.  A group
+// writes ghost cells of its own patch only; exchange ops read interior cells
+// of other patches, or (interpolation) cells of a coarser patch that its own
+// physical BC may write again with the same values (the BC is a function of
+// the patch's interior and its non-physical ghost cells).
+__global__ void __launch_bounds__(PHYSBC_THREADS) ghost_groups(GhostArgs g, const int4* __restrict__ groups,
+                                                              int ngroups) {
+    const int b = blockIdx.x;
+    if (b >= ngroups) return;
+    const int4 gr = groups[b];
+    const int w = threadIdx.x >> 6;
+    for (int o = gr.x + w; o < gr.y; o += PHYSBC_THREADS / 64) ghost_exchange_op(g, g.ops[o], threadIdx.x & 63);
+    if (gr.z >= 0) {
+        __syncthreads();
+        physbc_op(g, g.ops[gr.z]);
     }
 }
 
@@ -441,6 +469,12 @@ patch_mass(const double* __restrict__ q, const int* __restrict__ slots, int n,
 int launch_ghost_exchange(const GhostArgs& g, hipStream_t st) {
     if (g.nops <= 0) return 0;
     hipLaunchKernelGGL(ghost_phase<4>, dim3((g.nops + 3) / 4), dim3(256), 0, st, g);
+    return (int) hipGetLastError();
+}
+
+int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hipStream_t st) {
+    if (ngroups <= 0) return 0;
+    hipLaunchKernelGGL(ghost_groups, dim3(ngroups), dim3(PHYSBC_THREADS), 0, st, g, groups, ngroups);
     return (int) hipGetLastError();
 }
 

@@ -233,6 +233,25 @@ int hipclaw_set_dt_scale(hipclaw_state_t *s, const double *scale);
 int hipclaw_save_step(hipclaw_state_t *s);
 int hipclaw_restore_step(hipclaw_state_t *s);
 
+/* Launch layout of the ghost fill, the same ops either way: 0 = one launch
+ * per phase (average + copy, physical BC, interpolate, physical BC), 1 = two
+ * launches of per-destination groups (a patch's exchange ops, then its
+ * physical BC).  The default is 1 for mx >= 32, else 0.  No counterpart in
+ * the reference, which fills ghost cells patch by patch on the host. */
+int hipclaw_set_ghost_layout(hipclaw_state_t *s, int layout);
+int hipclaw_ghost_layout(const hipclaw_state_t *s);
+
+/* Autotuner: time nsteps full steps (update, ghost fill, time sync) of dt on
+ * the current patches for every tile size the state's solver, precision and
+ * patch size allow, in both ghost-fill layouts, each trial from the state
+ * saved at entry and restored after it (the saved state of
+ * hipclaw_save_step is overwritten), and keep the fastest pair.  tiles[k],
+ * layouts[k], ms[k] (room for 6) receive each trial and its mean step time,
+ * *ntrials their count.  Not with subcycling.  No counterpart in the
+ * reference, whose CUDA kernels have a fixed block size. */
+int hipclaw_autotune(hipclaw_state_t *s, double dt, int nsteps, int *tiles, int *layouts,
+                     double *ms, int *ntrials);
+
 /* Cell values of slots first..first+count-1 as float, in the order of the
  * VTK "meqn" cell array (per patch: j = 1..my, i = 1..mx, then the meqn
  * fields), converted on the device so that half the bytes cross PCIe.

@@ -97,11 +97,14 @@ def _small_problems():
             P.shockbubble_problem(1, 3, mx=8)]
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("k", [0, 1, 2])
-def test_ghost_fill_fp64(gpu, k):
-    """Device ghost fill (copy/average/interpolate/physical BC) vs the oracle."""
+def test_ghost_fill_fp64(gpu, k, layout):
+    """Device ghost fill (copy/average/interpolate/physical BC) vs the oracle,
+    in both launch layouts."""
     prob = _small_problems()[k]
     forest, q0, aux, sv = setup_problem(prob)
+    sv.set_ghost_layout(layout)
     rng = np.random.default_rng(3 + k)
     q = random_state(prob.solver, forest.npatch, prob.mx, prob.mx, rng)
     sv.set_q(q)
@@ -111,11 +114,14 @@ def test_ghost_fill_fp64(gpu, k):
     np.testing.assert_array_equal(qg, qo)
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("k", [0, 1, 2])
-def test_advance_fp64_amr(gpu, k):
-    """Ten steps of (update all levels, ghost fill) on an AMR forest."""
+def test_advance_fp64_amr(gpu, k, layout):
+    """Ten steps of (update all levels, ghost fill) on an AMR forest, in both
+    ghost-fill layouts."""
     prob = _small_problems()[k]
     forest, q0, aux, sv = setup_problem(prob)
+    sv.set_ghost_layout(layout)
     dt = P.stable_dt(prob, forest, q0) * 0.9
     nsteps = 10
     cfl = sv.advance(dt, nsteps)

@@ -211,3 +211,38 @@ def test_run_outstyle_1_dambreak_amr_l3_6(gpu):
     assert abs(t - 0.006) <= 1e-15
     _compare(L, glob, rep, t)
     L.fclaw2d_global_destroy(glob)
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32_BULK])
+def test_autotune_keeps_the_state(gpu, precision):
+    """The autotuner times every allowed tile from the same state, puts it
+    back, and the chosen tile then steps as the oracle does."""
+    from helpers import oracle_advance
+    prob = P.dambreak_problem(2, 4, mx=32)
+    forest, q0, aux, sv = setup_problem(prob, precision=precision, time_sync=True)
+    sv.ghost_fill()
+    q_before = sv.get_q()
+    dt = P.stable_dt(prob, forest, q0)
+    timings = sv.autotune(dt, 4)
+    print("form %d: %s -> tile %d, layout %d" % (precision, timings, sv.tile, sv.ghost_layout), flush=True)
+    allowed = {8, 16} if precision == fa.FP64 else {8, 16, 32}
+    assert set(timings) == {(t, l) for t in allowed for l in (0, 1)}
+    assert all(v > 0 for v in timings.values())
+    assert (sv.tile, sv.ghost_layout) == min(timings, key=timings.get)
+    np.testing.assert_array_equal(sv.get_q(), q_before)
+    if precision == fa.FP64:
+        sv.advance(dt, 3)
+        qo, _ = oracle_advance_ts(prob, forest, q_before, None, prob.mx, dt, 3, time_sync=True)
+        assert rel_max_err(sv.get_q(), qo) <= TOL
+
+
+def test_autotune_refuses_subcycling(gpu):
+    prob = P.dambreak_problem(2, 3, mx=16)
+    forest = P.build_forest(prob)
+    sv = fa.Solver(prob.solver, 16, 16, mthbc=prob.mthbc, max_patches=forest.npatch,
+                   time_sync=True, subcycle=True)
+    sv.set_forest(forest, prob.extent)
+    q0, _ = P.initial_data(prob, forest)
+    sv.set_q(q0)
+    with pytest.raises(fa.HipclawError):
+        sv.autotune(1e-4, 2)

@@ -29,10 +29,11 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
-@pytest.mark.parametrize("tile", [0, 8])
-def test_fp64_time_sync_matches_oracle(gpu, case, tile):
+@pytest.mark.parametrize("tile,layout", [(0, 0), (8, 1)])
+def test_fp64_time_sync_matches_oracle(gpu, case, tile, layout):
     prob = CASES[case]()
     forest, q0, aux, sv = setup_problem(prob, tile=tile, time_sync=True)
+    sv.set_ghost_layout(layout)
     assert len(set(forest.leaves()[0].tolist())) >= 2
     dt = P.stable_dt(prob, forest, q0) * (0.5 if prob.solver == fa.RP_ADVECTION else 1.0)
     nsteps = 6
