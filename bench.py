@@ -287,7 +287,7 @@ def pmc_summary(fetch_dir, write_dir, cells):
     lines = ["calibration: 3 copies of %d bytes; FETCH_SIZE factor %.3f, WRITE_SIZE factor %.3f"
              % (COPY_BYTES, f_fetch, f_write)]
     for k in sorted(fetch):
-        if "step_tiles" not in k and "ghost_phase" not in k and "time_sync" not in k:
+        if not any(n in k for n in ("step_tiles", "ghost_", "physbc", "time_sync")):
             continue
         fk = sum(fetch[k]) / len(fetch[k])
         wk = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
