@@ -37,6 +37,18 @@ __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
 // stated tolerance.
 template <typename R>
 __device__ __forceinline__ R dv(R a, R b) { return a / b; }
+
+// FP64 reciprocal of the flux form's FP64 fluxes: the hardware estimate
+// (v_rcp_f64) refined by two Newton-Raphson steps r <- r + r (1 - x r), which
+// square the relative error each.  Not correctly rounded (within a few ulp),
+// which the flux form does not need: both cells beside an interface evaluate
+// its flux with this same code on the same values, so they get the same bits.
+__device__ __forceinline__ double flux_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return r;
+}
 template <>
 __device__ __forceinline__ float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R>
@@ -162,7 +174,7 @@ struct RPShallow {
     __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
                                                   const PhysParams& pp, double* f, double* g) {
 #pragma clang fp contract(off)
-        const double h = q[0], rh = 1.0 / h, u = q[1] * rh, v = q[2] * rh;
+        const double h = q[0], rh = flux_rcp(h), u = q[1] * rh, v = q[2] * rh;
         const double p = 0.5 * pp.grav * h * h;
         f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u;
         g[0] = q[2]; g[1] = q[1] * v;     g[2] = q[2] * v + p;
@@ -334,7 +346,7 @@ struct RPEuler {
     __device__ __forceinline__ static void fluxes(const double* q, const double* /*aux*/,
                                                   const PhysParams& pp, double* f, double* g) {
 #pragma clang fp contract(off)
-        const double rr = 1.0 / q[0], u = q[1] * rr, v = q[2] * rr;
+        const double rr = flux_rcp(q[0]), u = q[1] * rr, v = q[2] * rr;
         const double p = (pp.gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
         f[0] = q[1]; f[1] = q[1] * u + p; f[2] = q[2] * u; f[3] = (q[3] + p) * u;
         g[0] = q[2]; g[1] = q[1] * v;     g[2] = q[2] * v + p; g[3] = (q[3] + p) * v;

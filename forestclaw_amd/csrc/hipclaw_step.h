@@ -330,7 +330,10 @@ step_tiles(StepArgs a) {
         if (tile + G < ntiles) prefetch(tile + G);
 
         // ============================ x sweep ============================
-        // lane (seg, k): row j = j0-1+r, interface i = i0-1+k between cells i-1 and i
+        // lane (seg, k): row j = j0-1+r, interface i = i0-1+k between cells i-1 and i.
+        // The Courant number max |dtdx s| is dtdx * max |s| exactly (rounding is
+        // monotone), so a lane keeps max |s| and scales it once per sweep.
+        R smax = R(0);
         for (int rbase = wave * SEG; rbase < T + 2; rbase += NW * SEG) {
             const int r = rbase + seg;
             const bool act = (seg < SEG) && (r < T + 2);
@@ -345,7 +348,7 @@ step_tiles(StepArgs a) {
             RP::template rpn2<1>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, r, T, last_x, first_y, last_y)) {
 #pragma unroll
-                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdx64 * (double) s[mw]));
+                for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
             }
             R cqxx[MEQN];
 #pragma unroll
@@ -424,9 +427,11 @@ step_tiles(StepArgs a) {
             GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
         }
         __syncthreads();
+        cfl = fmax(cfl, dtdx64 * (double) smax);
 
         // ============================ y sweep ============================
         // lane (seg, k): column i = i0-1+c, interface j = j0-1+k between cells j-1 and j
+        smax = R(0);
         for (int cbase = wave * SEG; cbase < T + 2; cbase += NW * SEG) {
             const int c = cbase + seg;
             const bool act = (seg < SEG) && (c < T + 2);
@@ -441,7 +446,7 @@ step_tiles(StepArgs a) {
             RP::template rpn2<2>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, c, T, last_y, first_x, last_x)) {
 #pragma unroll
-                for (int mw = 0; mw < MW; mw++) cfl = fmax(cfl, fabs(dtdy64 * (double) s[mw]));
+                for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
             }
             R cqxx[MEQN];
 #pragma unroll
@@ -543,6 +548,7 @@ step_tiles(StepArgs a) {
             }
         }
         __syncthreads();
+        cfl = fmax(cfl, dtdy64 * (double) smax);
 
         double* __restrict__ qo = a.qdst + (long long) slot * a.pstride;
         // fm(i,j) = ((0 + faddm_x(i,j)) + ty2(i-1,j)) + ty1(i,j), same for fp
