@@ -134,8 +134,9 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
 }
 
 // Tile size measured fastest on MI355X for the update kernel (the tile sweep of
-// the benchmark forest, mx = 16 and 32): 16 x 16 with four waves per workgroup,
-// one per SIMD, for both forms; smaller patches take 8 x 8.
+// the benchmark forest, mx = 16 and 32): 16 x 16 for both forms (four waves
+// per workgroup for FP64, two for the FP32-bulk form with meqn <= 3); smaller
+// patches take 8 x 8.  hipclaw_autotune re-measures on the caller's patches.
 static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
     int t = 16;

@@ -23,11 +23,22 @@ static int launch_one(const StepArgs& a, hipStream_t st) {
     return launch_persistent(step_tiles<RP, float, T, NW, C, false>, NW * 64, a, st, resident[0][0]);
 }
 
+// Waves per 16 x 16 tile.  A sweep covers T + 2 = 18 rows, three per wave
+// (19 interfaces each), so two waves take three row groups each where four
+// take two, two, one and one and wait at the barrier; with 24 KB of LDS per
+// workgroup (meqn <= 3) six two-wave workgroups fit a CU, three waves per
+// SIMD.  Measured on MI355X: shallow water 119 -> 105 us per launch on the
+// benchmark forest, advection 21.5 -> 17.1 us.  The Euler tile (meqn = 4,
+// 32 KB) fits only five workgroups, and four waves stay faster (66 us
+// against 73 us).
+template <class RP>
+constexpr int waves_t16() { return RP::MEQN >= 4 ? 4 : 2; }
+
 template <class RP, bool WITH32 = true>
 static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return launch_one<RP, 16, 4>(a, st);
+    case 16: return launch_one<RP, 16, waves_t16<RP>()>(a, st);
     case 32:
         if constexpr (WITH32) return launch_one<RP, 32, 16>(a, st);
         return (int) hipErrorInvalidValue;
