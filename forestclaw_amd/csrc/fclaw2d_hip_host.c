@@ -1007,6 +1007,23 @@ void fclaw2d_restore_time_step(fclaw2d_global_t *glob)
     device_check(hipclaw_restore_step(glob->state), "hipclaw_restore_step");
 }
 
+void fclaw2d_diagnostics_gather(fclaw2d_global_t *glob, int init_flag)
+{
+    double hi[4], lo[4];
+    int m;
+    const int meqn = glob->clawpatch_opt.meqn;
+    if (!glob->conservation_check) return;
+    if (meqn > 4) { fprintf(stderr, "fclaw2d_diagnostics_gather: meqn > 4\n"); exit(1); }
+    device_check(hipclaw_mass(glob->state, hi, lo), "hipclaw_mass");
+    for (m = 0; m < meqn; m++) {
+        glob->cons_mass[m] = hi[m] + lo[m];
+        if (init_flag) glob->cons_mass0[m] = glob->cons_mass[m];
+        printf("sum[%d] =  %24.16e  %24.16e\n", m, glob->cons_mass[m],
+               fabs(glob->cons_mass[m] - glob->cons_mass0[m]));
+    }
+    fflush(stdout);
+}
+
 /* fclaw2d_output_frame: the frame files the solver's options ask for */
 static void output_frame(fclaw2d_global_t *glob, int iframe)
 {
@@ -1041,6 +1058,7 @@ static void run_outstyle_1(fclaw2d_global_t *glob)
     const double dt_outer = glob->tfinal / (double) glob->nout;
     double dt_minlevel = glob->initial_dt, t_curr = 0.0;
     int n, n_inner = 0, iframe = 0;
+    fclaw2d_diagnostics_gather(glob, 1);
     output_frame(glob, iframe);
     for (n = 0; n < glob->nout; n++) {
         const double tend = t_curr + dt_outer;
@@ -1067,8 +1085,10 @@ static void run_outstyle_1(fclaw2d_global_t *glob)
             if (!glob->use_fixed_dt && !took_small_step)
                 dt_minlevel = dt_minlevel * glob->desired_cfl / maxcfl;
             glob->curr_time = t_curr;
+            if (glob->advance_one_step) fclaw2d_diagnostics_gather(glob, 0);
             if (glob->regrid_interval > 0 && n_inner % glob->regrid_interval == 0) fclaw2d_regrid(glob);
         }
+        fclaw2d_diagnostics_gather(glob, 0);
         glob->curr_time = t_curr;
         output_frame(glob, ++iframe);
     }
@@ -1081,6 +1101,7 @@ static void run_outstyle_3(fclaw2d_global_t *glob)
     const int per_level = !glob->subcycle && glob->advance_one_step;
     double dt_minlevel = glob->initial_dt, t_curr = 0.0;
     int nstep_outer = glob->nout, nstep_inner = glob->nstep, n = 0, iframe = 0;
+    fclaw2d_diagnostics_gather(glob, 1);
     output_frame(glob, iframe);
     glob->curr_time = t_curr;
     if (per_level && !glob->outstyle_uses_maxlevel) {
@@ -1098,7 +1119,10 @@ static void run_outstyle_3(fclaw2d_global_t *glob)
         if (!glob->use_fixed_dt) dt_minlevel = dt_minlevel * glob->desired_cfl / maxcfl;
         n++;
         if (glob->regrid_interval > 0 && n % glob->regrid_interval == 0) fclaw2d_regrid(glob);
-        if (n % nstep_inner == 0) output_frame(glob, ++iframe);
+        if (n % nstep_inner == 0) {
+            fclaw2d_diagnostics_gather(glob, 0);
+            output_frame(glob, ++iframe);
+        }
     }
 }
 
@@ -1108,6 +1132,7 @@ static void run_outstyle_4(fclaw2d_global_t *glob)
     double t_curr = 0.0;
     int n = 0, iframe = 0;
     output_frame(glob, iframe);
+    fclaw2d_diagnostics_gather(glob, 1);
     glob->curr_time = t_curr;
     while (n < glob->nout) {
         fclaw2d_advance_all_levels(glob, t_curr, glob->initial_dt);
@@ -1115,14 +1140,20 @@ static void run_outstyle_4(fclaw2d_global_t *glob)
         n++;
         glob->curr_time = t_curr;
         if (glob->regrid_interval > 0 && n % glob->regrid_interval == 0) fclaw2d_regrid(glob);
-        if (n % glob->nstep == 0) output_frame(glob, ++iframe);
+        if (n % glob->nstep == 0) {
+            fclaw2d_diagnostics_gather(glob, 0);
+            output_frame(glob, ++iframe);
+        }
     }
 }
 
 void fclaw2d_run(fclaw2d_global_t *glob)
 {
     switch (glob->outstyle) {
-    case 0: output_frame(glob, 0); break;
+    case 0:
+        output_frame(glob, 0);
+        fclaw2d_diagnostics_gather(glob, 1);
+        break;
     case 1: run_outstyle_1(glob); break;
     case 3: run_outstyle_3(glob); break;
     case 4: run_outstyle_4(glob); break;

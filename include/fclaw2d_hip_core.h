@@ -113,6 +113,10 @@ typedef struct fclaw2d_global {
     int regrid_interval;
     int output;                    /* fclaw_options output: write frames (src/fclaw_options.c:122) */
     int count_retake;              /* steps rejected for max_cfl and retaken */
+    /* fclaw_options conservation-check (src/fclaw_options.c:233): the last
+     * gathered total of each field and the total at t = 0 */
+    int conservation_check;
+    double cons_mass[4], cons_mass0[4];
 } fclaw2d_global_t;
 
 /** Buffer contract between fclaw2d_update_single_step and single_step_update
@@ -192,6 +196,16 @@ void fclaw2d_regrid(fclaw2d_global_t *glob);
  * the reference's message.  save / restore_time_step keep and put back the
  * state of every patch on the device (src/fclaw2d_run.c:47-80). */
 void fclaw2d_run(fclaw2d_global_t *glob);
+
+/* fclaw2d_diagnostics_gather (src/fclaw2d_diagnostics.c:122-190) with the
+ * clawpatch conservation check
+ * (src/patches/clawpatch/fclaw2d_clawpatch_diagnostics.c:159-228): when
+ * glob->conservation_check is set, the compensated device sum of q dx dy of
+ * every field (hipclaw_mass) goes to glob->cons_mass (and cons_mass0 when
+ * init_flag), and "sum[m] = total |total - total at t=0|" is printed as the
+ * reference prints it.  The error norms against an exact solution are not
+ * provided. */
+void fclaw2d_diagnostics_gather(fclaw2d_global_t *glob, int init_flag);
 void fclaw2d_save_time_step(fclaw2d_global_t *glob);
 void fclaw2d_restore_time_step(fclaw2d_global_t *glob);
 

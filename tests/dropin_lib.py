@@ -58,7 +58,9 @@ class Global(C.Structure):
                 ("desired_cfl", C.c_double), ("tfinal", C.c_double), ("reduce_cfl", C.c_int),
                 ("use_fixed_dt", C.c_int), ("outstyle", C.c_int), ("nout", C.c_int),
                 ("nstep", C.c_int), ("outstyle_uses_maxlevel", C.c_int),
-                ("regrid_interval", C.c_int), ("output", C.c_int), ("count_retake", C.c_int)]
+                ("regrid_interval", C.c_int), ("output", C.c_int), ("count_retake", C.c_int),
+                ("conservation_check", C.c_int), ("cons_mass", C.c_double * 4),
+                ("cons_mass0", C.c_double * 4)]
 
 
 class Options(C.Structure):
@@ -96,6 +98,7 @@ def bind():
         "fclaw2d_initialize": ([C.POINTER(Global)], None),
         "fclaw2d_regrid": ([C.POINTER(Global)], None),
         "fclaw2d_run": ([C.POINTER(Global)], None),
+        "fclaw2d_diagnostics_gather": ([C.POINTER(Global), i], None),
         "fclaw2d_save_time_step": ([C.POINTER(Global)], None),
         "fclaw2d_restore_time_step": ([C.POINTER(Global)], None),
         "fc2d_hipclaw_options_default": ([C.POINTER(Options), i], None),

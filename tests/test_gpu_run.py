@@ -173,22 +173,39 @@ def _compare(L, glob, rep, t):
     assert err <= TOL
 
 
-def test_run_outstyle_3_retakes_and_regrids(gpu):
+def test_run_outstyle_3_retakes_and_regrids(gpu, capfd):
     """nout steps (times 2^(maxlevel-minlevel) with advance_one_step); the
-    initial dt is four stable coarse steps, so the first step is rejected."""
+    initial dt is four stable coarse steps, so the first step is rejected.
+    With conservation_check the run reports the total of each field at t = 0
+    and at every output (src/fclaw2d_run.c:306-427 gathers diagnostics there)."""
+    from helpers import oracle_mass
     L = D.bind()
     prob = P.dambreak_problem(2, 4, mx=16)
     thr_r, thr_c = 0.05, 0.02
     glob, keep = _dambreak_glob(L, prob, thr_r, thr_c)
     rep = _Replica(L, prob, glob, thr_r, thr_c)
+    mass0 = oracle_mass(prob, rep.forest, rep.q, prob.mx)
     lf = 2 ** (prob.maxlevel - prob.minlevel)
     initial_dt = 4 * lf * P.stable_dt(prob, rep.forest, rep.q)
     g = glob.contents
     g.outstyle, g.nout, g.nstep, g.regrid_interval = 3, 3, 3, 2
     g.initial_dt = initial_dt
+    g.conservation_check = 1
+    capfd.readouterr()
     L.fclaw2d_run(glob)
+    out = capfd.readouterr().out
     t = _replay_outstyle_3(rep, 3, initial_dt, lf, 2)
     _compare(L, glob, rep, t)
+    mass1 = oracle_mass(prob, rep.forest, rep.q, prob.mx)
+    for m in range(3):
+        scale = abs(mass0[0])
+        assert abs(g.cons_mass0[m] - mass0[m]) <= 1e-14 * scale
+        assert abs(g.cons_mass[m] - mass1[m]) <= 1e-14 * scale
+    lines = [l for l in out.splitlines() if l.startswith("sum[")]
+    # nout and nstep both scale by 2^(maxlevel-minlevel): the totals at t = 0
+    # and at the one output after nout * lf steps, three fields each
+    assert len(lines) == 3 * 2
+    assert lines[0].split()[0] == "sum[0]"
     L.fclaw2d_global_destroy(glob)
 
 
