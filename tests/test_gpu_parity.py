@@ -149,3 +149,23 @@ def test_step_batch_per_level(gpu):
     qo, cfl_o = oracle_advance(prob, forest, q0, aux, prob.mx, dt, 1)
     assert rel_max_err(qg, qo) <= TOL
     assert abs(cfl - cfl_o) <= TOL * cfl_o
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_advance_two_slabs(gpu, layout):
+    """Two slabs (16x16 patches, tile 8: the update writes the other slab)
+    with either ghost-fill layout, bit-identical to the oracle with and
+    without time sync."""
+    from helpers import oracle_advance_ts
+    prob = P.dambreak_problem(2, 5, mx=16)
+    for time_sync in (False, True):
+        forest, q0, aux, sv = setup_problem(prob, tile=8, time_sync=time_sync)
+        sv.set_ghost_layout(layout)
+        dt = P.stable_dt(prob, forest, q0) * 0.9
+        cfl = sv.advance(dt, 6)
+        qg = sv.get_q()
+        qo, cfl_o = oracle_advance_ts(prob, forest, q0, aux, prob.mx, dt, 6, time_sync=time_sync)
+        err = rel_max_err(qg, qo)
+        assert err <= TOL, "time_sync %d: %.3e (bit-exact %d of %d)" % (time_sync, err,
+                                                                        int(np.sum(qg == qo)), qo.size)
+        assert abs(cfl - cfl_o) <= TOL * cfl_o
