@@ -54,7 +54,7 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     sv.ghost_fill()
     tuned = None
     if tile == 0:
-        tuned = {"%d/%d" % k: v for k, v in sv.autotune(dt, 8).items()}   # "tile/layout"
+        tuned = {"%d/%d/%d" % k: v for k, v in sv.autotune(dt, 8).items()}   # "tile/waves/layout"
     mass0 = sv.mass()
     if warmup > 0:
         sv.advance_async(dt, warmup)
@@ -78,7 +78,8 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     mass1 = sv.mass()
     total_steps = warmup + 2 * steps
     return dict(wall=wall, kernel_ms=kernel_ms, q=q, mass0=mass0, mass1=mass1, tile=sv.tile,
-                graph=graph, total_steps=total_steps, autotune=tuned, ghost_layout=sv.ghost_layout)
+                graph=graph, total_steps=total_steps, autotune=tuned, ghost_layout=sv.ghost_layout,
+                waves=sv.waves)
 
 
 def run_regrid(fa, prob, forest, q, repeats=3):
@@ -212,7 +213,7 @@ def sweep_one(fa, P, minlevel, maxlevel, mx, args):
     for name, x in r.items():
         out[name] = {"value": round(cells * args.steps / x["wall"], 1), "unit": "cell-updates/s",
                      "ms_per_step": round(x["wall"] * 1e3 / args.steps, 5), "tile": x["tile"],
-                     "ghost_layout": x["ghost_layout"],
+                     "ghost_layout": x["ghost_layout"], "waves": x["waves"],
                      "autotune_ms_per_step": x["autotune"],
                      "kernel_avg_ms": round(x["kernel_ms"], 5),
                      "hbm_frac": round(B * cells / (x["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -536,6 +537,7 @@ def main():
             "cells": cells, "mx": prob.mx, "mbc": 2, "meqn": prob.meqn,
             "dt": dt, "tile_fp32": r32["tile"], "tile_fp64": r64["tile"],
             "ghost_layout_fp32": r32["ghost_layout"], "ghost_layout_fp64": r64["ghost_layout"],
+            "waves_fp32": r32["waves"], "waves_fp64": r64["waves"],
             "autotune_ms_per_step": {"fp32": r32["autotune"], "fp64": r64["autotune"]},
             "timed_with_hip_graph": bool(r32["graph"]), "time_sync": not args.no_time_sync,
             "device": device,

@@ -91,7 +91,9 @@ def lib():
         "hipclaw_set_dt_scale": ([vp, vp], i),
         "hipclaw_save_step": ([vp], i),
         "hipclaw_restore_step": ([vp], i),
-        "hipclaw_autotune": ([vp, C.c_double, i, vp, vp, vp, vp], i),
+        "hipclaw_autotune": ([vp, C.c_double, i, vp, vp, vp, vp, vp], i),
+        "hipclaw_set_waves": ([vp, i], i),
+        "hipclaw_waves": ([vp], i),
         "hipclaw_set_ghost_layout": ([vp, i], i),
         "hipclaw_ghost_layout": ([vp], i),
         "fclaw2d_hip_forest_provenance": ([vp, _ip(), _ip()], i),
@@ -319,17 +321,29 @@ class Solver:
         _check(lib().hipclaw_restore_step(self._h), "hipclaw_restore_step")
 
     def autotune(self, dt, nsteps=8):
-        """Time every allowed tile in both ghost-fill layouts over nsteps steps
-        of dt (the state is put back after each) and keep the fastest;
-        returns {(tile, layout): ms per step}."""
-        tiles = np.zeros(6, np.int32)
-        layouts = np.zeros(6, np.int32)
-        ms = np.zeros(6, np.float64)
+        """Time every allowed tile (16 x 16 with 2 and 4 waves) in both
+        ghost-fill layouts over nsteps steps of dt (the state is put back
+        after each) and keep the fastest; returns
+        {(tile, waves, layout): ms per step}."""
+        tiles = np.zeros(8, np.int32)
+        waves = np.zeros(8, np.int32)
+        layouts = np.zeros(8, np.int32)
+        ms = np.zeros(8, np.float64)
         n = C.c_int()
-        _check(lib().hipclaw_autotune(self._h, dt, nsteps, tiles.ctypes.data, layouts.ctypes.data,
-                                      ms.ctypes.data, C.byref(n)), "hipclaw_autotune")
-        return {(int(t), int(l)): float(m)
-                for t, l, m in zip(tiles[:n.value], layouts[:n.value], ms[:n.value])}
+        _check(lib().hipclaw_autotune(self._h, dt, nsteps, tiles.ctypes.data, waves.ctypes.data,
+                                      layouts.ctypes.data, ms.ctypes.data, C.byref(n)),
+               "hipclaw_autotune")
+        k = n.value
+        return {(int(t), int(w), int(l)): float(m)
+                for t, w, l, m in zip(tiles[:k], waves[:k], layouts[:k], ms[:k])}
+
+    @property
+    def waves(self):
+        return lib().hipclaw_waves(self._h)
+
+    def set_waves(self, waves):
+        """Waves per workgroup of 16 x 16 tiles: 2, 4 or 0 (the default)."""
+        _check(lib().hipclaw_set_waves(self._h, waves), "hipclaw_set_waves")
 
     @property
     def ghost_layout(self):

@@ -97,6 +97,7 @@ struct hipclaw_state {
     double* qti = nullptr;
     double* dtscale = nullptr;    // per-slot factor on dt, or null
     int ghost_layout = 0;         // launch layout of the ghost fill (hipclaw_set_ghost_layout)
+    int waves = 0;                // waves per workgroup of 16 x 16 tiles, 0 = the default
     double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
     bool saved = false;           // qsave holds the current slots' state
     // ghost-patch message: slab offsets of one field's band, in message order
@@ -135,8 +136,8 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
 
 // Tile size measured fastest on MI355X for the update kernel (the tile sweep of
 // the benchmark forest, mx = 16 and 32): 16 x 16 for both forms (four waves
-// per workgroup for FP64, two for the FP32-bulk form with meqn <= 3); smaller
-// patches take 8 x 8.  hipclaw_autotune re-measures on the caller's patches.
+// per workgroup for FP64, two for the FP32-bulk form); smaller patches take
+// 8 x 8.  hipclaw_autotune re-measures on the caller's patches.
 static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
     int t = 16;
@@ -814,9 +815,28 @@ int hipclaw_set_ghost_layout(hipclaw_state_t* s, int layout) {
 
 int hipclaw_ghost_layout(const hipclaw_state_t* s) { return s ? s->ghost_layout : -1; }
 
-int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int* layouts, double* ms,
-                     int* ntrials) {
-    if (!s || nsteps <= 0 || !(dt > 0.0) || !tiles || !layouts || !ms || !ntrials)
+static int effective_waves(const hipclaw_state_t* s) {
+    if (s->waves) return s->waves;
+    return s->cfg.precision == HIPCLAW_FP64 ? hipclaw::default_waves_fp64(s->cfg.solver, s->tile)
+                                            : hipclaw::default_waves_fp32(s->cfg.solver, s->tile);
+}
+
+int hipclaw_set_waves(hipclaw_state_t* s, int waves) {
+    if (!s || (waves != 0 && waves != 2 && waves != 4))
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_waves: waves must be 0, 2 or 4%s");
+    s->waves = waves;
+    // a captured graph holds the launches of the old kernel
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_waves(const hipclaw_state_t* s) { return s ? effective_waves(s) : -1; }
+
+int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int* waves, int* layouts,
+                     double* ms, int* ntrials) {
+    if (!s || nsteps <= 0 || !(dt > 0.0) || !tiles || !waves || !layouts || !ms || !ntrials)
         return fail(HIPCLAW_EINVAL, "hipclaw_autotune: bad arguments%s");
     if (s->reg_accumulate)
         return fail(HIPCLAW_EINVAL, "hipclaw_autotune: tune before subcycling (registers accumulate)%s");
@@ -826,13 +846,15 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
     hipEvent_t ev[2] = {nullptr, nullptr};
     HIP_TRY(hipEventCreate(&ev[0]));
     HIP_TRY(hipEventCreate(&ev[1]));
-    int best_tile = s->tile, best_layout = s->ghost_layout, n = 0, rc = HIPCLAW_OK;
+    int best_tile = s->tile, best_layout = s->ghost_layout, best_waves = s->waves, n = 0, rc = HIPCLAW_OK;
     double best_ms = 0.0;
     static const int candidates[3] = {8, 16, 32};
     for (int t : candidates) {
         if (!tile_allowed(s, t)) continue;
-        for (int lay = 0; lay < 2 && !rc; lay++) {
-            if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay))) break;
+        for (int trial = 0; trial < (t == 16 ? 4 : 2) && !rc; trial++) {
+            const int lay = trial & 1, w = t == 16 ? (trial < 2 ? 2 : 4) : 0;
+            if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay)) || (rc = hipclaw_set_waves(s, w)))
+                break;
             // one untimed step (first use of these kernels), then the timed ones
             if ((rc = advance_launches(s, dt, 1, false))) break;
             hipError_t e = hipEventRecord(ev[0], s->st);
@@ -844,9 +866,10 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
             if (e != hipSuccess) { rc = hipfail(e, "hipclaw_autotune timing"); break; }
             if ((rc = hipclaw_restore_step(s))) break;
             tiles[n] = t;
+            waves[n] = effective_waves(s);
             layouts[n] = lay;
             ms[n] = (double) elapsed / nsteps;
-            if (n == 0 || ms[n] < best_ms) { best_tile = t; best_layout = lay; best_ms = ms[n]; }
+            if (n == 0 || ms[n] < best_ms) { best_tile = t; best_layout = lay; best_waves = w; best_ms = ms[n]; }
             n++;
         }
         if (rc) break;
@@ -859,6 +882,7 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
     }
     *ntrials = n;
     if (int r = set_tile(s, best_tile)) return r;
+    if (int r = hipclaw_set_waves(s, best_waves)) return r;
     return hipclaw_set_ghost_layout(s, best_layout);
 }
 
@@ -958,8 +982,8 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
 
 static int launch_step(hipclaw_state_t* s, const hipclaw::StepArgs& a) {
     int e = (s->cfg.precision == HIPCLAW_FP64)
-                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, a, s->st)
-                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, a, s->st);
+                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, s->waves, a, s->st)
+                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, s->waves, a, s->st);
     if (e) return hipfail((hipError_t) e, "step_tiles launch");
     return HIPCLAW_OK;
 }

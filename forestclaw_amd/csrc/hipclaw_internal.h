@@ -77,8 +77,12 @@ int launch_ghost_physbc(const GhostArgs& g, hipStream_t st);
 // Per-destination groups: a patch's exchange ops, then its physical BC;
 // groups[b] = (first op, end, physical-BC op or -1, -).
 int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hipStream_t st);
-int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st);
-int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st);
+// waves: waves per workgroup of a 16 x 16 tile (2 or 4), 0 = the default.
+int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, hipStream_t st);
+int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, hipStream_t st);
+// The default waves per workgroup of a tile (0 where there is no choice).
+int default_waves_fp64(int solver, int tile);
+int default_waves_fp32(int solver, int tile);
 int launch_reset_scalar(unsigned long long* p, hipStream_t st);
 // Whole patch slabs src[slot] -> dst[slot].
 int launch_copy_patches(const double* src, double* dst, const int* slots, int n, long long pstride,

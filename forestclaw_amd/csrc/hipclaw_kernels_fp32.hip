@@ -23,22 +23,24 @@ static int launch_one(const StepArgs& a, hipStream_t st) {
     return launch_persistent(step_tiles<RP, float, T, NW, C, false>, NW * 64, a, st, resident[0][0]);
 }
 
-// Waves per 16 x 16 tile.  A sweep covers T + 2 = 18 rows, three per wave
+// Waves per 16 x 16 tile by default (hipclaw_set_waves and the autotuner
+// choose between 2 and 4).  A sweep covers T + 2 = 18 rows, three per wave
 // (19 interfaces each), so two waves take three row groups each where four
-// take two, two, one and one and wait at the barrier; with 24 KB of LDS per
-// workgroup (meqn <= 3) six two-wave workgroups fit a CU, three waves per
-// SIMD.  Measured on MI355X: shallow water 119 -> 105 us per launch on the
-// benchmark forest, advection 21.5 -> 17.1 us.  The Euler tile (meqn = 4,
-// 32 KB) fits only five workgroups, and four waves stay faster (66 us
-// against 73 us).
+// take two, two, one and one and wait at the barrier, and the smaller
+// workgroups pack more densely on a CU.  Measured on MI355X (autotuner
+// trials, ms per step): shallow water on the benchmark forest 0.133 with
+// two waves against 0.150 with four, the Euler shock-bubble 0.076 against
+// 0.094, the 64-run ensemble 0.476 against 0.494.
 template <class RP>
-constexpr int waves_t16() { return RP::MEQN >= 4 ? 4 : 2; }
+constexpr int waves_t16() { return 2; }
 
+// waves = 0: waves_t16 for 16 x 16 tiles; 2 or 4 select the variant.
 template <class RP, bool WITH32 = true>
-static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
+static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
+    if (tile == 16 && waves == 0) waves = waves_t16<RP>();
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return launch_one<RP, 16, waves_t16<RP>()>(a, st);
+    case 16: return waves == 2 ? launch_one<RP, 16, 2>(a, st) : launch_one<RP, 16, 4>(a, st);
     case 32:
         if constexpr (WITH32) return launch_one<RP, 32, 16>(a, st);
         return (int) hipErrorInvalidValue;
@@ -46,12 +48,12 @@ static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
     }
 }
 
-int launch_step_fp32(int solver, int tile, const StepArgs& a, hipStream_t st) {
+int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, hipStream_t st) {
     switch (solver) {
-    case 0: return launch_tile<RPAdvection<float>>(tile, a, st);
-    case 1: return launch_tile<RPShallow<float>>(tile, a, st);
+    case 0: return launch_tile<RPAdvection<float>>(tile, waves, a, st);
+    case 1: return launch_tile<RPShallow<float>>(tile, waves, a, st);
     case 2:  // meqn = 4 at T = 32 needs more LDS than a workgroup may hold
-        return launch_tile<RPEuler<float>, false>(tile, a, st);
+        return launch_tile<RPEuler<float>, false>(tile, waves, a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
@@ -61,6 +63,15 @@ bool step_supported(int solver, int precision, int tile) {
     // T = 32 fits the 160 KB of LDS for meqn = 3 (FP64) and meqn <= 3 (FP32)
     if (tile == 32) return precision == 0 ? solver == 1 : solver != 2;
     return solver >= 0 && solver <= 2;
+}
+
+int default_waves_fp32(int solver, int tile) {
+    if (tile != 16) return 0;
+    switch (solver) {
+    case 0: return waves_t16<RPAdvection<float>>();
+    case 1: return waves_t16<RPShallow<float>>();
+    default: return waves_t16<RPEuler<float>>();
+    }
 }
 
 }  // namespace hipclaw

@@ -814,24 +814,28 @@ static int launch_one(const StepArgs& a, hipStream_t st) {
 }
 
 template <class RP>
-static int launch_tile(int tile, const StepArgs& a, hipStream_t st) {
+// waves = 0: the default per tile (16 x 16: four waves per workgroup,
+// measured fastest for FP64); 2 or 4 select the 16 x 16 variant.
+static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return launch_one<RP, 16, 4>(a, st);
+    case 16: return waves == 2 ? launch_one<RP, 16, 2>(a, st) : launch_one<RP, 16, 4>(a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
 
-int launch_step_fp64(int solver, int tile, const StepArgs& a, hipStream_t st) {
+int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, hipStream_t st) {
     switch (solver) {
-    case 0: return launch_tile<RPAdvection<double>>(tile, a, st);
+    case 0: return launch_tile<RPAdvection<double>>(tile, waves, a, st);
     case 1:
         // T = 32 fits the LDS only without the time-sync edge buffers
         if (tile == 32) return launch_one<RPShallow<double>, 32, 16, false>(a, st);
-        return launch_tile<RPShallow<double>>(tile, a, st);
-    case 2: return launch_tile<RPEuler<double>>(tile, a, st);
+        return launch_tile<RPShallow<double>>(tile, waves, a, st);
+    case 2: return launch_tile<RPEuler<double>>(tile, waves, a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
+
+int default_waves_fp64(int /*solver*/, int tile) { return tile == 16 ? 4 : 0; }
 
 }  // namespace hipclaw

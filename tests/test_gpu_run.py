@@ -241,11 +241,12 @@ def test_autotune_keeps_the_state(gpu, precision):
     q_before = sv.get_q()
     dt = P.stable_dt(prob, forest, q0)
     timings = sv.autotune(dt, 4)
-    print("form %d: %s -> tile %d, layout %d" % (precision, timings, sv.tile, sv.ghost_layout), flush=True)
-    allowed = {8, 16} if precision == fa.FP64 else {8, 16, 32}
-    assert set(timings) == {(t, l) for t in allowed for l in (0, 1)}
+    print("form %d: %s -> tile %d, waves %d, layout %d" % (precision, timings, sv.tile, sv.waves,
+                                                           sv.ghost_layout), flush=True)
+    allowed = [(8, 0), (16, 2), (16, 4)] + ([] if precision == fa.FP64 else [(32, 0)])
+    assert set(timings) == {(t, w, l) for t, w in allowed for l in (0, 1)}
     assert all(v > 0 for v in timings.values())
-    assert (sv.tile, sv.ghost_layout) == min(timings, key=timings.get)
+    assert (sv.tile, sv.waves, sv.ghost_layout) == min(timings, key=timings.get)
     np.testing.assert_array_equal(sv.get_q(), q_before)
     if precision == fa.FP64:
         sv.advance(dt, 3)
