@@ -109,7 +109,7 @@ struct hipclaw_state {
     int graph_steps = 0;
     // timing of the update kernel inside advance_async
     bool time_kernels = true;
-    std::vector<hipEvent_t> kev;  // pairs (before, after) per step
+    std::vector<hipEvent_t> kev;  // pairs (kernel start, kernel end) per step
     int timed_steps = 0;
     hipEvent_t region[2] = {nullptr, nullptr};
 };
@@ -980,10 +980,13 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
     return a;
 }
 
-static int launch_step(hipclaw_state_t* s, const hipclaw::StepArgs& a) {
+// ev0 / ev1: optional events bound to the kernel's own start and end.
+static int launch_step(hipclaw_state_t* s, const hipclaw::StepArgs& a, hipEvent_t ev0 = nullptr,
+                       hipEvent_t ev1 = nullptr) {
+    const hipclaw::StepLaunch L(s->st, ev0, ev1);
     int e = (s->cfg.precision == HIPCLAW_FP64)
-                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, s->waves, a, s->st)
-                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, s->waves, a, s->st);
+                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, s->waves, a, L)
+                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, s->waves, a, L);
     if (e) return hipfail((hipError_t) e, "step_tiles launch");
     return HIPCLAW_OK;
 }
@@ -1043,9 +1046,8 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
     for (int n = 0; n < nsteps; n++) {
         double* src = s->q[s->cur];
         double* dst = s->inplace ? src : s->q[1 - s->cur];
-        if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n], s->st));
-        if (int rc = launch_step(s, step_args(s, src, dst, s->all_slots, s->npatch, dt))) return rc;
-        if (timed) HIP_TRY(hipEventRecord(s->kev[2 * n + 1], s->st));
+        const hipclaw::StepArgs a = step_args(s, src, dst, s->all_slots, s->npatch, dt);
+        if (int rc = timed ? launch_step(s, a, s->kev[2 * n], s->kev[2 * n + 1]) : launch_step(s, a)) return rc;
         if (!s->inplace) s->cur = 1 - s->cur;
         if (int rc = ghost_fill_async(s, 0)) return rc;
         if (int rc = time_sync_async(s, 0)) return rc;

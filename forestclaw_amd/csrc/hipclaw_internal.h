@@ -2,6 +2,7 @@
 // kernel translation units (FP64 parity form, FP32-bulk form).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "../../include/hipclaw.h"
@@ -77,9 +78,17 @@ int launch_ghost_physbc(const GhostArgs& g, hipStream_t st);
 // Per-destination groups: a patch's exchange ops, then its physical BC;
 // groups[b] = (first op, end, physical-BC op or -1, -).
 int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hipStream_t st);
+// The stream of a step launch and, for a timed launch, two events that
+// hipExtLaunchKernelGGL binds to the kernel's own start and end, so their
+// elapsed time is the kernel's execution without the gaps around it.
+struct StepLaunch {
+    hipStream_t st;
+    hipEvent_t start, stop;
+    StepLaunch(hipStream_t s, hipEvent_t a = nullptr, hipEvent_t b = nullptr) : st(s), start(a), stop(b) {}
+};
 // waves: waves per workgroup of a 16 x 16 tile (2 or 4), 0 = the default.
-int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, hipStream_t st);
-int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, hipStream_t st);
+int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
+int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
 // The default waves per workgroup of a tile (0 where there is no choice).
 int default_waves_fp64(int solver, int tile);
 int default_waves_fp32(int solver, int tile);
@@ -118,13 +127,16 @@ inline int resident_blocks(const void* kernel, int block) {
 // Launch a persistent step kernel: min(tiles, resident workgroups) of them.
 // `resident` caches the occupancy query; each kernel instantiation owns one.
 template <typename K>
-inline int launch_persistent(K kernel, int block, const StepArgs& a, hipStream_t st, int& resident) {
+inline int launch_persistent(K kernel, int block, const StepArgs& a, const StepLaunch& L, int& resident) {
     const int ntiles = a.nwork * a.ntx * a.nty;
     if (ntiles <= 0) return 0;
     if (resident <= 0) resident = resident_blocks((const void*) kernel, block);
     if (resident <= 0) return (int) hipErrorInvalidDeviceFunction;
     const int nblocks = ntiles < resident ? ntiles : resident;
-    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, st, a);
+    if (L.start || L.stop)
+        hipExtLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, L.st, L.start, L.stop, 0, a);
+    else
+        hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, L.st, a);
     return (int) hipGetLastError();
 }
 

@@ -7,7 +7,7 @@
 namespace hipclaw {
 
 template <class RP, int T, int NW>
-static int launch_one(const StepArgs& a, hipStream_t st) {
+static int launch_one(const StepArgs& a, const StepLaunch& st) {
     // kernels that keep time-sync registers are separate instantiations, so the
     // plain update carries none of their code or register pressure; the MC
     // limiter on every wave family has its own compile-time instantiation
@@ -36,7 +36,7 @@ constexpr int waves_t16() { return 2; }
 
 // waves = 0: waves_t16 for 16 x 16 tiles; 2 or 4 select the variant.
 template <class RP, bool WITH32 = true>
-static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
+static int launch_tile(int tile, int waves, const StepArgs& a, const StepLaunch& st) {
     if (tile == 16 && waves == 0) waves = waves_t16<RP>();
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
@@ -48,7 +48,7 @@ static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
     }
 }
 
-int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, hipStream_t st) {
+int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st) {
     switch (solver) {
     case 0: return launch_tile<RPAdvection<float>>(tile, waves, a, st);
     case 1: return launch_tile<RPShallow<float>>(tile, waves, a, st);

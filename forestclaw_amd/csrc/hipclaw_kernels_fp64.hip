@@ -792,7 +792,7 @@ int launch_time_sync(const SyncArgs& a, hipStream_t st) {
 
 // ---------------------------------------------------------------- FP64 step
 template <class RP, int T, int NW, bool WITH_REGS = true>
-static int launch_one(const StepArgs& a, hipStream_t st) {
+static int launch_one(const StepArgs& a, const StepLaunch& st) {
     // kernels that keep time-sync registers are separate instantiations, so the
     // plain update carries none of their code or register pressure; the MC
     // limiter on every wave family has its own compile-time instantiation
@@ -816,7 +816,7 @@ static int launch_one(const StepArgs& a, hipStream_t st) {
 template <class RP>
 // waves = 0: the default per tile (16 x 16: four waves per workgroup,
 // measured fastest for FP64); 2 or 4 select the 16 x 16 variant.
-static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
+static int launch_tile(int tile, int waves, const StepArgs& a, const StepLaunch& st) {
     switch (tile) {
     case 8: return launch_one<RP, 8, 2>(a, st);
     case 16: return waves == 2 ? launch_one<RP, 16, 2>(a, st) : launch_one<RP, 16, 4>(a, st);
@@ -824,7 +824,7 @@ static int launch_tile(int tile, int waves, const StepArgs& a, hipStream_t st) {
     }
 }
 
-int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, hipStream_t st) {
+int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st) {
     switch (solver) {
     case 0: return launch_tile<RPAdvection<double>>(tile, waves, a, st);
     case 1:

@@ -302,7 +302,8 @@ int hipclaw_mass(hipclaw_state_t *s, double *sum_hi, double *sum_lo);
 
 /* Timing of the last hipclaw_advance_async/replay region, measured with HIP
  * events on the state's stream: total milliseconds and milliseconds spent in
- * the patch-update kernel alone (per launch average). */
+ * the patch-update kernel alone (per launch average, from events bound to each
+ * update kernel's own start and end, as a kernel trace reports it). */
 int hipclaw_timing(hipclaw_state_t *s, int nsteps, double *total_ms, double *step_kernel_ms);
 
 #ifdef __cplusplus
