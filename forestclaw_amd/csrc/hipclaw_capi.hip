@@ -141,7 +141,9 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
 static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
     int t = 16;
-    while (t > 8 && (t > m || !hipclaw::step_supported(c->solver, c->precision, t))) t >>= 1;
+    while (t > 8 && (t > m || c->mx % t != 0 || c->my % t != 0 ||
+                     !hipclaw::step_supported(c->solver, c->precision, t)))
+        t >>= 1;
     return t;
 }
 

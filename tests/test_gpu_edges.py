@@ -1,0 +1,64 @@
+"""Edges of the batched step that the configurations above leave out.
+
+  * a patch width that is a multiple of 8 but not of 16 (mx = 24: three 8x8
+    tiles per row) on an AMR forest, FP64 with time sync, against the oracle;
+  * a width the tiles cannot cover (mx = 20) is refused with a message;
+  * a forest of one patch (level 0: every ghost cell comes from the physical
+    boundary conditions), FP64 against the oracle and FP32-bulk within 1e-5;
+  * an advance of zero steps leaves q unchanged and reports a Courant
+    number of 0.
+"""
+import numpy as np
+import pytest
+
+import forestclaw_amd as fa
+from forestclaw_amd import problems as P
+from helpers import oracle_advance, oracle_advance_ts, rel_max_err, setup_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mx24_amr_fp64_time_sync(gpu):
+    prob = P.dambreak_problem(2, 4, mx=24)
+    forest, q0, aux, sv = setup_problem(prob, time_sync=True)
+    assert sv.tile == 8 and len(set(forest.leaves()[0].tolist())) > 1
+    dt = P.stable_dt(prob, forest, q0)
+    cfl = sv.advance(dt, 5)
+    qo, cfl_o = oracle_advance_ts(prob, forest, q0, aux, prob.mx, dt, 5, time_sync=True)
+    err = rel_max_err(sv.get_q(), qo)
+    print("mx=24, %d patches: err %.3e, cfl %.6f" % (forest.npatch, err, cfl), flush=True)
+    assert err <= 1e-12
+    assert abs(cfl - cfl_o) <= 1e-12 * cfl_o
+
+
+def test_mx20_is_refused(gpu):
+    with pytest.raises(fa.HipclawError, match="multiples of the tile size"):
+        fa.Solver(fa.RP_SHALLOW, 20, 20, precision=fa.FP64, mthbc=(1, 1, 1, 1), max_patches=4)
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32_BULK])
+def test_one_patch_forest(gpu, precision):
+    prob = P.dambreak_problem(0, 0, mx=32)
+    forest, q0, aux, sv = setup_problem(prob, precision=precision)
+    assert forest.npatch == 1
+    dt = P.stable_dt(prob, forest, q0)
+    sv.advance(dt, 10)
+    qo, _ = oracle_advance(prob, forest, q0, aux, prob.mx, dt, 10)
+    a, b = sv.get_q()[..., 2:-2, 2:-2], qo[..., 2:-2, 2:-2]
+    if precision == fa.FP64:
+        err = rel_max_err(a, b)
+        print("one patch FP64: err %.3e" % err, flush=True)
+        assert err <= 1e-12
+    else:
+        l1 = np.abs(a - b).sum(axis=(0, 2, 3)) / np.maximum(np.abs(b).sum(axis=(0, 2, 3)),
+                                                               np.abs(b[:, 0]).sum())
+        print("one patch FP32-bulk: L1 per field", l1, flush=True)
+        assert np.all(l1 <= 1e-5)
+
+
+def test_zero_steps(gpu):
+    prob = P.advection_problem(2, 2, mx=16)
+    forest, q0, aux, sv = setup_problem(prob)
+    before = sv.get_q()
+    assert sv.advance(0.01, 0) == 0.0
+    assert np.array_equal(sv.get_q(), before)
