@@ -393,8 +393,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (the batched patch update runs on "
-              "a single MI355X; multi-GPU sharding is described in DESIGN.md, not run)",
+        print("bench.py: measures one GPU only (the batched patch update runs on a single "
+              "MI355X; sharding the forest over several GPUs is not implemented)",
               file=sys.stderr)
         sys.exit(2)
 
