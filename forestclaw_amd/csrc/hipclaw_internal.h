@@ -2,6 +2,8 @@
 // kernel translation units (FP64 parity form, FP32-bulk form).
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -126,13 +128,20 @@ inline int resident_blocks(const void* kernel, int block) {
 
 // Launch a persistent step kernel: min(tiles, resident workgroups) of them.
 // `resident` caches the occupancy query; each kernel instantiation owns one.
+// Host threads may race to fill it: they compute the same value, and the
+// atomic makes the store and the loads well defined.
 template <typename K>
-inline int launch_persistent(K kernel, int block, const StepArgs& a, const StepLaunch& L, int& resident) {
+inline int launch_persistent(K kernel, int block, const StepArgs& a, const StepLaunch& L,
+                             std::atomic<int>& resident) {
     const int ntiles = a.nwork * a.ntx * a.nty;
     if (ntiles <= 0) return 0;
-    if (resident <= 0) resident = resident_blocks((const void*) kernel, block);
-    if (resident <= 0) return (int) hipErrorInvalidDeviceFunction;
-    const int nblocks = ntiles < resident ? ntiles : resident;
+    int nres = resident.load(std::memory_order_relaxed);
+    if (nres <= 0) {
+        nres = resident_blocks((const void*) kernel, block);
+        resident.store(nres, std::memory_order_relaxed);
+    }
+    if (nres <= 0) return (int) hipErrorInvalidDeviceFunction;
+    const int nblocks = ntiles < nres ? ntiles : nres;
     if (L.start || L.stop)
         hipExtLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, L.st, L.start, L.stop, 0, a);
     else

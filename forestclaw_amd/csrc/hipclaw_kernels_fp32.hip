@@ -11,7 +11,7 @@ static int launch_one(const StepArgs& a, const StepLaunch& st) {
     // kernels that keep time-sync registers are separate instantiations, so the
     // plain update carries none of their code or register pressure; the MC
     // limiter on every wave family has its own compile-time instantiation
-    static int resident[2][2] = {{0, 0}, {0, 0}};
+    static std::atomic<int> resident[2][2];   // zero-initialised (static storage)
     constexpr bool C = RP::CONSERVATIVE;
     if (all_mc_limiter<RP>(a)) {
         if (a.reg)
