@@ -117,9 +117,15 @@ def measure(solver, q, aux, dx, dt, mwaves, maux, grav=1.0, gamma=1.4, nproc=Non
         times = pool.map(wfn, jobs)
     wall = max(times)
     cells = npatch * mx * mx * reps
+    # the same sample on one process, about 2 s of it
+    reps1 = max(1, int(2.0 / (per_patch * npatch)))
+    t1 = wfn((solver, q.copy(), None if aux is None else aux.copy(), dx, dt, mwaves, maux, reps1,
+              grav, gamma, _NoBarrier()))
+    single = npatch * mx * mx * reps1 / t1
     return dict(value=cells / wall, unit="cell-updates/s", cores=nproc, kind=kind,
+                single_process_value=single,
                 sample="%d patches of %dx%d x %d repetitions of one Clawpack-4.6 step "
-                       "(%s), %d processes, %.1f s wall" % (npatch, mx, mx, reps,
-                                                           "reference Fortran via oracle/_ref"
-                                                           if kind == "reference" else
-                                                           "oracle C port", nproc, wall))
+                       "(%s), %d processes, %.1f s wall; one process: %d repetitions, %.1f s"
+                       % (npatch, mx, mx, reps,
+                          "reference Fortran via oracle/_ref" if kind == "reference" else
+                          "oracle C port", nproc, wall, reps1, t1))
