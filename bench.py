@@ -59,7 +59,8 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
     if warmup > 0:
         sv.advance_async(dt, warmup)
     sv.sync()
-    # kernel timing: an eager pass with HIP events around every update launch
+    # kernel timing: an eager pass whose update launches carry HIP events bound
+    # to each kernel's own start and end
     sv.advance_async(dt, steps)
     sv.sync()
     _, kernel_ms = sv.timing(steps)
@@ -339,6 +340,31 @@ def valu_summary(pmc_dir, cells):
     return "\n".join(lines)
 
 
+def counter_summary(pmc_dir, cells):
+    """Every counter of one rocprofv3 --pmc run of --profile-run, for each step
+    kernel: the average per dispatch and per cell update; SQ_WAIT_* and
+    SQ_ACTIVE_* also as a share of SQ_WAVE_CYCLES when the run holds it."""
+    import csv
+    import glob as G
+    per = {}
+    for f in G.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            k = r["Kernel_Name"]
+            if "step_tiles" in k:
+                per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    lines = []
+    for k in sorted(per):
+        c = {n: sum(v) / len(v) for n, v in per[k].items()}
+        wc = c.get("SQ_WAVE_CYCLES")
+        lines.append("kernel=%s" % k.split("(")[0][:120])
+        for n in sorted(c):
+            share = ""
+            if wc and n != "SQ_WAVE_CYCLES" and (n.startswith("SQ_WAIT") or n.startswith("SQ_ACTIVE")):
+                share = " share_of_wave_cycles=%.3f" % (c[n] / wc)
+            lines.append("  %-26s per_dispatch=%.4g per_cell=%.3f%s" % (n, c[n], c[n] / cells, share))
+    return "\n".join(lines)
+
+
 def committed_traffic(workload, kernel_tag):
     """HBM bytes per launch of the step kernel from the committed counter summary
     of this workload (None if there is none for this kernel)."""
@@ -387,6 +413,8 @@ def main():
                     help="run the workload's step kernels a few times for rocprofv3 (no JSON)")
     ap.add_argument("--valu-summary", metavar="PMC_DIR",
                     help="summarise a rocprofv3 --pmc run of --profile-run with VALU_COUNTERS")
+    ap.add_argument("--counter-summary", metavar="PMC_DIR",
+                    help="every counter of a rocprofv3 --pmc run of --profile-run, per step kernel")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --profile-run")
     args = ap.parse_args()
@@ -452,6 +480,10 @@ def main():
     if args.valu_summary:
         print("workload=%s cells=%d" % (workload, cells))
         print(valu_summary(args.valu_summary, cells))
+        return
+    if args.counter_summary:
+        print("workload=%s cells=%d" % (workload, cells))
+        print(counter_summary(args.counter_summary, cells))
         return
 
     device = fa.initialize_device(0)

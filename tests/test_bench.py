@@ -78,3 +78,16 @@ def test_valu_summary_issue_share(tmp_path):
     assert "valu_issue_share=0.500" in line
     assert "wait_any_share=0.250" in line
     assert "valu_per_cell=%.2f" % (valu / 2048) in line
+
+
+def test_counter_summary_shares(tmp_path):
+    """Generic counters per cell, and wait counters as a share of wave cycles."""
+    step = "void hipclaw::step_tiles<hipclaw::RPShallow<float>, float, 16, 2, true, true, 4>(hipclaw::StepArgs)"
+    rows = [dict(Dispatch_Id=d, Kernel_Name=step, Counter_Name=n, Counter_Value=v, Start_Timestamp=0,
+                 End_Timestamp=1)
+            for d in (1, 2) for n, v in (("SQ_WAVE_CYCLES", 1000.0), ("SQ_WAIT_INST_LDS", 200.0),
+                                         ("SQ_INSTS_VALU_FMA_F32", 4096.0))]
+    _write_counters(str(tmp_path / "c" / "c_counter_collection.csv"), rows)
+    out = bench.counter_summary(str(tmp_path / "c"), cells=1024)
+    assert "SQ_WAIT_INST_LDS           per_dispatch=200 per_cell=0.195 share_of_wave_cycles=0.200" in out
+    assert "SQ_INSTS_VALU_FMA_F32      per_dispatch=4096 per_cell=4.000" in out
