@@ -365,6 +365,37 @@ def counter_summary(pmc_dir, cells):
     return "\n".join(lines)
 
 
+def copy_peak_gbs(nbytes=1 << 30, reps=10):
+    """Measured stream-copy peak: read + write bytes per second of the HIP
+    runtime's device-to-device copy of nbytes, reps times, timed with HIP
+    events on the null stream."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 0, n) or hip.hipEventCreate(C.byref(e0)) or hip.hipEventCreate(C.byref(e1)):
+            raise RuntimeError("hipMemset / hipEventCreate failed")
+        hip.hipMemcpyAsync(b, a, n, 3, None)      # warm-up, device to device
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            if hip.hipMemcpyAsync(b, a, n, 3, None):
+                raise RuntimeError("hipMemcpyAsync failed")
+        hip.hipEventRecord(e1, None)
+        if hip.hipEventSynchronize(e1):
+            raise RuntimeError("hipEventSynchronize failed")
+        ms = C.c_float(0.0)
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
 def committed_traffic(workload, kernel_tag):
     """HBM bytes per launch of the step kernel from the committed counter summary
     of this workload (None if there is none for this kernel)."""
@@ -503,6 +534,8 @@ def main():
     value32 = cells * args.steps / r32["wall"]
     value64 = cells * args.steps / r64["wall"]
 
+    copy_peak = copy_peak_gbs()
+
     def roofline(r, rtype):
         achieved = B * cells / (r["kernel_ms"] * 1e-3) / 1e9
         traffic = committed_traffic(workload, "%s, %d," % (rtype, r["tile"]))
@@ -510,7 +543,8 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else round(traffic),
                 "kernel": "step_tiles", "kernel_avg_ms": round(r["kernel_ms"], 5),
-                "bytes_per_cell_update": round(B, 4), "cell_updates_per_launch": cells}
+                "bytes_per_cell_update": round(B, 4), "cell_updates_per_launch": cells,
+                "copy_peak": round(copy_peak, 1), "frac_of_copy_peak": round(achieved / copy_peak, 4)}
 
     # accuracy of the FP32-bulk form against the FP64 form after the same steps
     _, _, dxs, dys = forest.geometry(prob.extent, prob.mx, prob.mx)
