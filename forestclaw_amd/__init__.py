@@ -402,6 +402,9 @@ class Solver:
 
     def set_q(self, q, first=0):
         q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.ndim != 4 or q.shape[1:] != self.patch_shape:
+            raise ValueError("q must have shape (npatch,) + %s (ghost cells included), got %s"
+                             % (self.patch_shape, q.shape))
         _check(lib().hipclaw_set_q(self._h, q.reshape(-1), first, q.shape[0]), "hipclaw_set_q")
 
     def get_q(self, first=0, count=None):
@@ -412,6 +415,10 @@ class Solver:
 
     def set_aux(self, aux, first=0):
         aux = np.ascontiguousarray(aux, dtype=np.float64)
+        ashape = (self.maux, self.my + 2 * self.mbc, self.mx + 2 * self.mbc)
+        if aux.ndim != 4 or aux.shape[1:] != ashape:
+            raise ValueError("aux must have shape (npatch,) + %s (ghost cells included), got %s"
+                             % (ashape, aux.shape))
         _check(lib().hipclaw_set_aux(self._h, aux.reshape(-1), first, aux.shape[0]),
                "hipclaw_set_aux")
 

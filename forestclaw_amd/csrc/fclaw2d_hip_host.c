@@ -56,6 +56,8 @@ fclaw2d_global_t *fclaw2d_global_new(void)
     g->nout = 10;
     g->nstep = 1;
     g->regrid_interval = 1;
+    g->threshold_variable = 1;
+    g->refinement_criteria = FCLAW_REFINE_CRITERIA_MINMAX;
     return g;
 }
 
@@ -206,6 +208,19 @@ int fc2d_hipclaw_check(const fc2d_hipclaw_options_t *opt)
     if (opt->mcapa != 0 || opt->use_fwaves != 0) {
         fprintf(stderr, "fc2d_hipclaw : mcapa and f-waves are not supported\n");
         return 1;
+    }
+    /* mthbc 1 (extrapolation) and 3 (wall) are applied on physical faces; 2
+     * (periodic) is left to the domain's topology.  The reference stops on 0
+     * and on 2 at a physical face (clawpack46_bc2_default.f:22-25, 41-43); the
+     * ghost program refuses 2 there. */
+    {
+        int k;
+        for (k = 0; k < 4; k++)
+            if (opt->mthbc[k] != 1 && opt->mthbc[k] != 2 && opt->mthbc[k] != 3) {
+                fprintf(stderr, "fc2d_hipclaw : mthbc(%d) = %d is not 1, 2 or 3\n", k + 1,
+                        opt->mthbc[k]);
+                return 1;
+            }
     }
     return 0;
 }
@@ -884,6 +899,12 @@ static void tag_leaves(fclaw2d_global_t *glob, int domain_init, int *refine, int
 {
     const int n = glob->domain->local_num_patches;
     int p, *ec = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    if (glob->refinement_criteria != FCLAW_REFINE_CRITERIA_GRADIENT) {
+        fprintf(stderr, "fclaw2d_regrid: refinement-criteria %d is not supported; the device tags "
+                        "with the gradient criterion only (refinement_criteria = %d)\n",
+                glob->refinement_criteria, FCLAW_REFINE_CRITERIA_GRADIENT);
+        exit(1);
+    }
     if (glob->threshold_variable < 1 || glob->threshold_variable > glob->clawpatch_opt.meqn) {
         fprintf(stderr, "fclaw2d_regrid: threshold_variable %d is not in 1..meqn\n",
                 glob->threshold_variable);

@@ -346,7 +346,19 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
             return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
         if (!(phase_types[ph] & (1 << type)))
             return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
-        if (type == hipclaw::GHOST_PHYSBC) continue;
+        if (type == hipclaw::GHOST_PHYSBC) {
+            // per face: -1 (not a physical face), 1 (extrapolation) or 3 (wall).  The
+            // reference stops on mthbc 0 and 2 and leaves other codes' ghost cells
+            // unwritten (clawpack46_bc2_default.f:17-25, 41-43), which with two slabs
+            // would leave them to whatever the spare slab held.
+            for (int f = 0; f < 4; f++) {
+                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
+                if (b != -1 && b != 1 && b != 3)
+                    return fail(HIPCLAW_EINVAL,
+                                "physical boundary condition code must be 1 (extrapolation) or 3 (wall)%s");
+            }
+            continue;
+        }
         if (!op_slot_ok(s, src) || which < 0 || which > 3)
             return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
         // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the

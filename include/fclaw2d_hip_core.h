@@ -85,6 +85,13 @@ typedef struct fclaw2d_clawpatch_options {
     int mx, my, mbc, meqn, maux;
 } fclaw2d_clawpatch_options_t;
 
+/* refinement criteria codes (src/patches/clawpatch/fclaw2d_clawpatch_options.h:48-56) */
+#define FCLAW_REFINE_CRITERIA_VALUE 0
+#define FCLAW_REFINE_CRITERIA_DIFFERENCE 1
+#define FCLAW_REFINE_CRITERIA_MINMAX 2
+#define FCLAW_REFINE_CRITERIA_GRADIENT 3
+#define FCLAW_REFINE_CRITERIA_USER 4
+
 typedef struct fclaw2d_global {
     fclaw2d_domain_t *domain;
     int count_single_step;         /* patch updates so far (src/fclaw_timer.c:184) */
@@ -104,6 +111,11 @@ typedef struct fclaw2d_global {
     int minlevel, maxlevel;
     double refine_threshold, coarsen_threshold;
     int threshold_variable;
+    /* clawpatch refinement-criteria, FCLAW_REFINE_CRITERIA_* below (default
+     * minmax, src/patches/clawpatch/fclaw2d_clawpatch_options.c:107-110);
+     * fclaw2d_initialize and fclaw2d_regrid tag with the gradient criterion
+     * only and stop with a message for any other value */
+    int refinement_criteria;
     int count_amr_regrid, count_amr_new_domain;   /* src/fclaw2d_regrid.c:321, 333 */
     const char *prefix;            /* output file prefix, NULL = "fort" (src/fclaw_options.c:177) */
     /* time stepping of fclaw2d_run (src/fclaw_options.c:62-98, 196-198); the

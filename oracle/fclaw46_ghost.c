@@ -788,30 +788,39 @@ typedef struct sub_ctx {
     double dt_step[32], current_time[32];
 } sub_ctx_t;
 
+/* One patch: qinterp = qlast + alpha (qcurr - qlast) on the interior band of
+ * width mint = mbc, in the four loops of fclaw2d_clawpatch46_fort_timeinterp
+ * (fclaw2d_clawpatch46_timeinterp.f:28-62). */
+void oracle_timeinterp_patch(int mx, int my, int mbc, int meqn, const double *qc,
+                             const double *ql, double *qi, double alpha)
+{
+    const int mint = mbc;
+    int i, j, m;
+    for (m = 0; m < meqn; m++) {
+        for (j = 1; j <= my - mint; j++)
+            for (i = 1; i <= mint; i++)
+                Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+        for (j = 1; j <= mint; j++)
+            for (i = mint + 1; i <= mx; i++)
+                Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+        for (j = mint + 1; j <= my; j++)
+            for (i = mx - mint + 1; i <= mx; i++)
+                Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+        for (j = my - mint + 1; j <= my; j++)
+            for (i = 1; i <= mx - mint; i++)
+                Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
+    }
+}
+
 static void timeinterp_level(sub_ctx_t *c, int level, double alpha)
 {
     const oracle_domain_t *d = c->d;
-    const int mx = d->mx, my = d->my, mbc = d->mbc, meqn = d->meqn, mint = mbc;
-    const size_t psize = (size_t) meqn * (mx + 2 * mbc) * (my + 2 * mbc);
-    int p, i, j, m;
+    const size_t psize = (size_t) d->meqn * (d->mx + 2 * d->mbc) * (d->my + 2 * d->mbc);
+    int p;
     for (p = 0; p < d->npatch; p++) {
-        const double *qc = PATCH(c->q, p), *ql = PATCH(c->qlast, p);
-        double *qi = PATCH(c->qti, p);
         if (d->level[p] != level) continue;
-        for (m = 0; m < meqn; m++) {
-            for (j = 1; j <= my - mint; j++)
-                for (i = 1; i <= mint; i++)
-                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
-            for (j = 1; j <= mint; j++)
-                for (i = mint + 1; i <= mx; i++)
-                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
-            for (j = mint + 1; j <= my; j++)
-                for (i = mx - mint + 1; i <= mx; i++)
-                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
-            for (j = my - mint + 1; j <= my; j++)
-                for (i = 1; i <= mx - mint; i++)
-                    Q(qi, i, j, m) = Q(ql, i, j, m) + alpha * (Q(qc, i, j, m) - Q(ql, i, j, m));
-        }
+        oracle_timeinterp_patch(d->mx, d->my, d->mbc, d->meqn, PATCH(c->q, p), PATCH(c->qlast, p),
+                                PATCH(c->qti, p), alpha);
     }
 }
 

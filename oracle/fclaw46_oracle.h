@@ -159,6 +159,9 @@ void oracle_average_corner(int mx, int my, int mbc, int meqn, double *qcoarse,
 void oracle_interpolate_face(int mx, int my, int mbc, int meqn,
                              const double *qcoarse, double *qfine, int idir,
                              int iface_coarse, int ti, int tj);
+/* fclaw2d_clawpatch46_fort_timeinterp on one patch (the interior band of width mbc) */
+void oracle_timeinterp_patch(int mx, int my, int mbc, int meqn, const double *qcurr,
+                             const double *qlast, double *qinterp, double alpha);
 void oracle_interpolate_corner(int mx, int my, int mbc, int meqn,
                                const double *qcoarse, double *qfine,
                                int icorner, int ti, int tj);

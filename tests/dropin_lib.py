@@ -53,6 +53,7 @@ class Global(C.Structure):
                 ("time_sync", C.c_int), ("subcycle", C.c_int), ("advance_one_step", C.c_int),
                 ("minlevel", C.c_int), ("maxlevel", C.c_int), ("refine_threshold", C.c_double),
                 ("coarsen_threshold", C.c_double), ("threshold_variable", C.c_int),
+                ("refinement_criteria", C.c_int),
                 ("count_amr_regrid", C.c_int), ("count_amr_new_domain", C.c_int),
                 ("prefix", C.c_char_p), ("initial_dt", C.c_double), ("max_cfl", C.c_double),
                 ("desired_cfl", C.c_double), ("tfinal", C.c_double), ("reduce_cfl", C.c_int),

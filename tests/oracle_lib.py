@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle46.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref46.so")
 REF_AMR_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_amr.so")
+REF_SYNC_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_sync.so")
 
 RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
 
@@ -39,6 +40,7 @@ class OracleDomain(C.Structure):
 _oracle = None
 _ref = None
 _ref_amr = None
+_ref_sync = None
 
 
 def oracle():
@@ -87,6 +89,10 @@ def oracle():
             getattr(lib, name).argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * nint
         lib.oracle_interpolate_face.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 4
         lib.oracle_interpolate_corner.argtypes = [C.c_int] * 4 + [_dp, _dp] + [C.c_int] * 3
+        lib.oracle_timeinterp_patch.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, C.c_double]
+        lib.oracle_store_flux.argtypes = [C.POINTER(OracleClaw), C.c_int, C.c_int, C.c_int, _dp,
+                                          C.c_void_p, C.c_double, C.c_double, C.c_double, _dp]
+        lib.oracle_accumulate_waves.argtypes = [C.c_int] * 4 + [C.c_double] * 3 + [_dp] * 5
         _oracle = lib
     return _oracle
 
@@ -193,6 +199,7 @@ def ref_amr():
                      "fclaw2d_clawpatch46_fort_average2coarse_"):
             getattr(lib, name).argtypes = [ip] * 4 + [_dp, _dp, _dp, _dp, ip, ip]
         lib.fclaw2d_clawpatch46_fort_local_ghost_pack_.argtypes = [ip] * 5 + [_dp, _dp, _dp] + [ip] * 3
+        lib.fclaw2d_clawpatch46_fort_timeinterp_.argtypes = [ip] * 5 + [_dp, _dp, _dp, dp, ip]
         lib.fclaw2d_clawpatch_compute_slopes_.argtypes = [dp, dp, ip]
         lib.fclaw2d_clawpatch_compute_slopes_.restype = C.c_double
         _ref_amr = lib
@@ -232,3 +239,32 @@ def local_ghost_pack_ref(mx, my, mbc, meqn, mint, q, area, qpack, packmode):
                                                          q, area, qpack, i(len(qpack)),
                                                          i(packmode), C.byref(ierr))
     return ierr.value
+
+
+def timeinterp_ref(mx, my, mbc, meqn, qcurr, qlast, qinterp, alpha, psize):
+    """Reference fclaw2d_clawpatch46_fort_timeinterp into qinterp, in place;
+    returns its ierror (2 when psize is not the band's size)."""
+    i = lambda v: C.byref(C.c_int(v))
+    ierr = C.c_int(-1)
+    ref_amr().fclaw2d_clawpatch46_fort_timeinterp_(i(mx), i(my), i(mbc), i(meqn), i(psize), qcurr,
+                                                   qlast, qinterp, C.byref(C.c_double(alpha)),
+                                                   C.byref(ierr))
+    return ierr.value
+
+
+def ref_sync_available():
+    return os.path.exists(REF_SYNC_SO) and os.path.exists(REF_SO)
+
+
+def ref_sync():
+    """The reference's clawpack46_time_sync_store_flux / _accumulate_waves."""
+    global _ref_sync
+    if _ref_sync is None:
+        lib = C.CDLL(REF_SYNC_SO)
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        lib.clawpack46_time_sync_store_flux_.argtypes = [ip] * 7 + [dp] + [_dp] * 4 + \
+            [_dp, C.c_void_p] + [_dp] * 4 + [C.c_void_p] + [_dp] * 4
+        lib.clawpack46_time_sync_accumulate_waves_.argtypes = [ip] * 4 + [dp] * 3 + [ip] + \
+            [_dp] * 4 + [_dp] * 4 + [_dp] * 8
+        _ref_sync = lib
+    return _ref_sync

@@ -166,6 +166,7 @@ def test_dropin_initialize_and_regrid_cycle(gpu):
     g.advance_one_step = 1
     g.minlevel, g.maxlevel = prob.minlevel, prob.maxlevel
     g.refine_threshold, g.coarsen_threshold, g.threshold_variable = thr_r, thr_c, 1
+    g.refinement_criteria = 3   # FCLAW_REFINE_CRITERIA_GRADIENT
     cp = g.clawpatch_opt
     cp.mx, cp.my, cp.mbc, cp.meqn, cp.maux = 16, 16, 2, 3, 0
     opt = D.Options()

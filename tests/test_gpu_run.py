@@ -65,6 +65,7 @@ def _dambreak_glob(L, prob, thr_r, thr_c):
     g.time_sync = 1
     g.minlevel, g.maxlevel = prob.minlevel, prob.maxlevel
     g.refine_threshold, g.coarsen_threshold, g.threshold_variable = thr_r, thr_c, 1
+    g.refinement_criteria = 3   # FCLAW_REFINE_CRITERIA_GRADIENT
     cp = g.clawpatch_opt
     cp.mx, cp.my, cp.mbc, cp.meqn, cp.maux = prob.mx, prob.mx, 2, 3, 0
     opt = D.Options()

@@ -79,3 +79,52 @@ def test_struct_layouts_match_the_build():
     L.fclaw2d_hip_struct_sizes(sizes)
     assert list(sizes) == [C.sizeof(D.Global), C.sizeof(D.Domain), C.sizeof(D.Block),
                            C.sizeof(D.Patch), C.sizeof(D.Options), C.sizeof(D.Vtable)]
+
+
+def test_options_check_rejects_unknown_boundary_codes():
+    """mthbc 1 (extrapolation), 2 (periodic, left to the topology) and 3 (wall)
+    pass the options check; 0 (user BC: the reference stops,
+    clawpack46_bc2_default.f:22-25) and codes the BC routine would skip are
+    refused."""
+    import dropin_lib as D
+    L = D.bind()
+    opt = D.Options()
+    L.fc2d_hipclaw_options_default(C.byref(opt), 3)
+    for code, ok in ((1, True), (2, True), (3, True), (0, False), (4, False), (-1, False)):
+        for face in range(4):
+            opt.mthbc[face] = code
+            assert (L.fc2d_hipclaw_check(C.byref(opt)) == 0) == ok, (face, code)
+            opt.mthbc[face] = 1
+
+
+def test_global_defaults_follow_the_reference():
+    """fclaw2d_global_new: time-stepping defaults of src/fclaw_options.c and the
+    clawpatch tagging defaults (threshold-variable 1, refinement-criteria
+    minmax: fclaw2d_clawpatch_options.c:107-114)."""
+    import dropin_lib as D
+    L = D.bind()
+    g = L.fclaw2d_global_new()
+    try:
+        c = g.contents
+        assert (c.initial_dt, c.max_cfl, c.desired_cfl, c.regrid_interval) == (0.1, 1.0, 0.9, 1)
+        assert (c.threshold_variable, c.refinement_criteria) == (1, 2)
+    finally:
+        L.fclaw2d_global_destroy(g)
+
+
+def test_set_q_and_aux_check_the_patch_shape():
+    """Arrays without ghost cells (or of another solver) are refused on the host
+    before any copy: the C side copies count * patch-size doubles."""
+    import numpy as np
+    import pytest
+    sv = fa.Solver.__new__(fa.Solver)   # no device state: the check runs first
+    sv.meqn, sv.maux, sv.mx, sv.my, sv.mbc, sv._h = 3, 0, 8, 8, 2, None
+    with pytest.raises(ValueError):
+        sv.set_q(np.zeros((2, 3, 8, 8)))          # interior only
+    with pytest.raises(ValueError):
+        sv.set_q(np.zeros((2, 4, 12, 12)))        # meqn of another solver
+    with pytest.raises(ValueError):
+        sv.set_q(np.zeros((3, 12, 12)))           # one patch without the patch axis
+    sv.maux = 2
+    with pytest.raises(ValueError):
+        sv.set_aux(np.zeros((2, 2, 8, 8)))

@@ -9,8 +9,11 @@ Pins, each chosen so that a plausible slip fails one of them:
     (a wrong shift sign, a transposed gradient or a wrong child offset fails);
   * averaging the interpolation of a patch gives the patch back (the four
     shifts cancel), and averaging conserves the sum;
-  * the gradient tag agrees with a direct numpy evaluation of
-    max |(dq/dx, dq/dy)| over interior cells, including threshold edge cases;
+  * the gradient tag against closed forms: a linear field has |grad q| =
+    sqrt(b^2 + c^2) in every interior cell; a one-cell spike is seen by its
+    interior neighbours at s/(2 dx) or s/(2 dy), by one cell from the first
+    ghost layer and by none from the second (dx != dy: an axis swap or a
+    missing factor 2 moves the switch);
   * repopulation on a small brick equals the per-patch routines.
 """
 import numpy as np
@@ -127,25 +130,60 @@ def test_average_of_interpolation_is_identity_and_sums_are_kept(igrid):
                                rtol=1e-13, atol=1e-12)
 
 
-def _numpy_max_gradient(q, ivar, dx, dy, mx=MX, mbc=MBC):
-    v = q[ivar]
-    I = slice(mbc, mbc + mx)
-    dqx = (v[I, mbc + 1:mbc + mx + 1] - v[I, mbc - 1:mbc + mx - 1]) / (2 * dx)
-    dqy = (v[mbc + 1:mbc + mx + 1, I] - v[mbc - 1:mbc + mx - 1, I]) / (2 * dy)
-    return np.sqrt(dqx * dqx + dqy * dqy).max()
-
-
-def test_gradient_tag_matches_numpy():
+def test_gradient_tag_linear_field_closed_form():
+    """q = a + b x + c y: every centred difference is exact, so every interior
+    cell has |grad q| = sqrt(b^2 + c^2) in physical units
+    (fclaw2d_clawpatch_gradient_exceeds_th.f90: (q(i+1)-q(i-1))/(2 dx));
+    the tag flips between thresholds just below and just above it.  dx != dy
+    and b != c, so a swapped axis or a missing factor 2 moves the switch."""
     o = oracle()
-    rng = np.random.default_rng(5)
-    for trial in range(40):
-        q = patch(rng)
-        ivar = trial % MEQN
-        dx, dy = 0.1 * (1 + rng.random()), 0.1 * (1 + rng.random())
-        g = _numpy_max_gradient(q, ivar, dx, dy)
-        for thr in (0.5 * g, 0.999 * g, 1.001 * g, 2 * g):
-            assert o.oracle_patch_exceeds(MX, MX, MBC, q, ivar, dx, dy, thr) == int(g > thr)
-        assert o.oracle_patch_exceeds(MX, MX, MBC, q, ivar, dx, dy, -1.0) == 1
+    idx = np.arange(1 - MBC, MX + MBC + 1)
+    for (b, c, dx, dy) in ((3.0, -1.5, 0.1, 0.05), (0.0, 2.0, 0.02, 0.2), (-4.0, 0.0, 0.3, 0.1)):
+        x = (idx - 0.5) * dx
+        y = (idx - 0.5) * dy
+        q = np.zeros((MEQN, N, N))
+        q[1] = 0.7 + b * x[None, :] + c * y[:, None]
+        g = float(np.hypot(b, c))
+        assert o.oracle_patch_exceeds(MX, MX, MBC, q, 1, dx, dy, g * (1 - 1e-9)) == 1
+        assert o.oracle_patch_exceeds(MX, MX, MBC, q, 1, dx, dy, g * (1 + 1e-9)) == 0
+        # a constant field in the other variables never tags
+        assert o.oracle_patch_exceeds(MX, MX, MBC, q, 0, dx, dy, 1e-300) == 0
+        assert o.oracle_patch_exceeds(MX, MX, MBC, q, 1, dx, dy, -1.0) == 1   # always refine
+
+
+def test_gradient_tag_spike_closed_form():
+    """One cell of height s in a zero field: its four neighbours see
+    |grad q| = s / (2 dx) (left/right) or s / (2 dy) (below/above); the tag is
+    the largest of those that are interior cells.  A spike in the first ghost
+    layer is seen by one interior cell only, one in the second ghost layer or
+    a ghost corner by none."""
+    o = oracle()
+    dx, dy, s = 0.1, 0.04, 1.0
+    gx, gy = s / (2 * dx), s / (2 * dy)      # 5 and 12.5
+
+    def tag_threshold_switch(i, j):
+        q = np.zeros((MEQN, N, N))
+        q[2, j + MBC - 1, i + MBC - 1] = s   # cell (i, j), 1-based interior indices
+        lo = o.oracle_patch_exceeds(MX, MX, MBC, q, 2, dx, dy, 0.0)
+        cases = {}
+        for name, g in (("x", gx), ("y", gy)):
+            cases[name] = (o.oracle_patch_exceeds(MX, MX, MBC, q, 2, dx, dy, g * (1 - 1e-12)),
+                           o.oracle_patch_exceeds(MX, MX, MBC, q, 2, dx, dy, g * (1 + 1e-12)))
+        return lo, cases
+
+    # interior spike: both neighbour pairs are interior, the y one is larger
+    lo, c = tag_threshold_switch(4, 5)
+    assert lo == 1 and c["y"] == (1, 0) and c["x"] == (1, 1)
+    # first ghost column i = 0: only cell (1, j) sees it, in x
+    lo, c = tag_threshold_switch(0, 5)
+    assert lo == 1 and c["x"] == (1, 0) and c["y"] == (0, 0)
+    # first ghost row j = my + 1: only cell (i, my) sees it, in y
+    lo, c = tag_threshold_switch(3, MX + 1)
+    assert lo == 1 and c["y"] == (1, 0)
+    # second ghost layer and a ghost corner: no interior cell is adjacent
+    for (i, j) in ((-1, 4), (4, MX + 2), (0, 0), (MX + 1, MX + 1)):
+        lo, _ = tag_threshold_switch(i, j)
+        assert lo == 0, (i, j)
 
 
 def test_gradient_tag_ignores_ghost_only_gradients():

@@ -1,8 +1,10 @@
 """CPU checks that pin the oracle's subcycled advance (fclaw2d_advance_all_levels
 with subcycle = T: local time steps per level, time-interpolated coarse data
 for the fine levels' ghost cells, synchronisation with accumulated fine
-registers).  The reference's time-interpolation Fortran is a three-line
-formula restated directly; the orchestration is pinned by
+registers).  The time interpolation of one patch is bit-equal to the
+reference's fclaw2d_clawpatch46_fort_timeinterp compiled in place (band
+cells written, every other cell untouched, the band size the Fortran checks);
+the orchestration is pinned by
 
 1. a single-level forest: subcycling is one step of dt, identical to the
    plain advance;
@@ -91,3 +93,28 @@ def test_subcycled_close_to_global_fine_step():
     area = (dx * dy)[:, None, None]
     l1 = float((np.abs(ia[:, 0] - ib[:, 0]) * area).sum() / (np.abs(ib[:, 0]) * area).sum())
     assert 0.0 < l1 <= 2e-3, l1
+
+
+import pytest
+from oracle_lib import oracle, ref_amr_available, timeinterp_ref
+
+
+@pytest.mark.skipif(not ref_amr_available(), reason="reference Fortran not built (make -C oracle ref)")
+@pytest.mark.parametrize("mx,my,meqn", [(8, 8, 3), (16, 16, 1), (8, 12, 4)])
+def test_timeinterp_patch_equals_reference_fortran(mx, my, meqn):
+    mbc = 2
+    rng = np.random.default_rng(mx + 10 * my + meqn)
+    shape = (meqn, my + 2 * mbc, mx + 2 * mbc)
+    qc, ql = rng.normal(size=shape), rng.normal(size=shape)
+    fill = rng.normal(size=shape)   # cells outside the band keep what they held
+    band = mx * my - (mx - 2 * mbc) * (my - 2 * mbc)
+    for alpha in (0.0, 0.25, 0.5, 0.75, 1.0, 0.3333333333333333):
+        a, b = fill.copy(), fill.copy()
+        oracle().oracle_timeinterp_patch(mx, my, mbc, meqn, qc.reshape(-1), ql.reshape(-1),
+                                         a.reshape(-1), alpha)
+        assert timeinterp_ref(mx, my, mbc, meqn, qc, ql, b, alpha, meqn * band) == 0
+        np.testing.assert_array_equal(a, b)
+        changed = a != fill
+        assert changed.sum() <= meqn * band
+    # the Fortran's own size check: a wrong psize is reported
+    assert timeinterp_ref(mx, my, mbc, meqn, qc, ql, fill.copy(), 0.5, meqn * band + 1) == 2
