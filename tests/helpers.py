@@ -69,10 +69,10 @@ def oracle_advance(prob, forest, q, aux, mx, dt, nsteps):
     return out, cfl
 
 
-def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True):
+def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True, order=(2, 2)):
     """nsteps of (update all levels, ghost update, time sync) on the oracle."""
     d = oracle_domain(prob, forest, mx)
-    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma)
+    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma, order=order)
     out = np.ascontiguousarray(q, dtype=np.float64).copy()
     auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
     cfl = oracle().oracle_advance_ts(C.byref(d), C.byref(claw), out.reshape(-1),

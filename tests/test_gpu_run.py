@@ -92,7 +92,7 @@ class _Replica:
         shape = (3, prob.mx + 4, prob.mx + 4)
         L.fc2d_hipclaw_download(glob)
         self.q = D.host_q(glob, self.forest.npatch, shape).copy()
-        self.retakes = self.regrids = 0
+        self.retakes = self.regrids = self.accepted = 0
 
     def step(self, dt):
         return oracle_advance_ts(self.prob, self.forest, self.q, None, self.prob.mx, dt, 1, time_sync=True)
@@ -114,6 +114,7 @@ class _Replica:
         if cfl > MAX_CFL:
             self.retakes += 1
             return False, None, cfl, dt_minlevel * DESIRED_CFL / cfl
+        self.accepted += 1
         return True, qn, cfl, dt_minlevel
 
 
@@ -227,6 +228,30 @@ def test_run_outstyle_1_dambreak_amr_l3_6(gpu):
     L.fclaw2d_run(glob)
     t = _replay_outstyle_1(rep, 0.006, 2, 0.01, lf, 4)
     assert abs(t - 0.006) <= 1e-15
+    _compare(L, glob, rep, t)
+    L.fclaw2d_global_destroy(glob)
+
+
+def test_run_outstyle_1_dambreak_amr_l3_6_long(gpu):
+    """The same run over many regrids: to t = 0.1 in four output intervals,
+    at least 50 accepted steps and 10 regrids (the forest follows the
+    spreading wave), ending bit-identical to the replay."""
+    L = D.bind()
+    prob = P.dambreak_problem(3, 6, mx=32)
+    thr_r, thr_c = 0.01, 0.005
+    glob, keep = _dambreak_glob(L, prob, thr_r, thr_c)
+    rep = _Replica(L, prob, glob, thr_r, thr_c)
+    npatch0 = rep.forest.npatch
+    lf = 2 ** (prob.maxlevel - prob.minlevel)
+    g = glob.contents
+    g.outstyle, g.tfinal, g.nout, g.regrid_interval = 1, 0.1, 4, 4
+    g.initial_dt = 0.01
+    L.fclaw2d_run(glob)
+    t = _replay_outstyle_1(rep, 0.1, 4, 0.01, lf, 4)
+    assert abs(t - 0.1) <= 1e-15
+    print("accepted steps %d, regrids %d, patches %d -> %d" % (
+        rep.accepted, rep.regrids, npatch0, rep.forest.npatch), flush=True)
+    assert rep.accepted >= 50 and rep.regrids >= 10
     _compare(L, glob, rep, t)
     L.fclaw2d_global_destroy(glob)
 
