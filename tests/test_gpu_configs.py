@@ -170,7 +170,9 @@ def test_method_orders_amr_with_time_sync(gpu, order):
     forest = P.build_forest(prob)
     q0, aux = P.initial_data(prob, forest)
     q0 = oracle_ghost_update(prob, forest, q0, prob.mx)
-    dt = P.stable_dt(prob, forest, q0)
+    # without transverse terms the unsplit method is stable only to Courant
+    # number 1/2 (at 0.9 the oracle itself reaches negative depths by step 6)
+    dt = P.stable_dt(prob, forest, q0) * (1.0 if order[1] > 0 else 0.5)
     out = {}
     for precision in (fa.FP64, fa.FP32_BULK):
         sv = fa.Solver(prob.solver, 16, 16, precision=precision, order=order, mthbc=prob.mthbc,
@@ -180,6 +182,7 @@ def test_method_orders_amr_with_time_sync(gpu, order):
         sv.advance(dt, 6)
         out[precision] = sv.get_q()
     qo, _ = oracle_advance_ts(prob, forest, q0, aux, 16, dt, 6, time_sync=True, order=order)
+    assert np.isfinite(qo).all()
     assert rel_max_err(out[fa.FP64], qo) <= TOL
     _, _, dx, dy = forest.geometry(prob.extent, 16, 16)
     l1 = _l1_per_field(out[fa.FP32_BULK], qo, dx * dy)
