@@ -103,6 +103,8 @@ def lib():
         "hipclaw_ghost_pack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_ghost_unpack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_get_registers": ([vp, i, i, _dp()], i),
+        "hipclaw_step_batch_noalloc": ([C.POINTER(HipclawConfig), i, vp, vp, vp, vp, vp, _ip(), i,
+                                        d, vp, C.POINTER(d)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

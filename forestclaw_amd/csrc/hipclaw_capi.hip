@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -114,6 +115,46 @@ struct hipclaw_state {
     hipEvent_t region[2] = {nullptr, nullptr};
 };
 
+// ---------------------------------------------------------------- allocation-free step
+// Device scratch of hipclaw_step_batch_noalloc: the CFL word, then the staged
+// slot list.  Every GPU holds its own copy of this array for the life of the
+// process; its device address is looked up once per device (scratch_ptr) and
+// only that address is passed to kernels, copies and fills.
+namespace {
+constexpr size_t NOALLOC_CFL_BYTES = 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    NOALLOC_CFL_BYTES + sizeof(int) * (size_t) HIPCLAW_NOALLOC_MAX_SLOTS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// The device address of `scratch` on the current device, looked up on the
+// first call per device; nullptr (and the error text set) on failure.
+static void* scratch_ptr(int* rc) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { *rc = hipfail(e, "hipGetDevice"); return nullptr; }
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        *rc = fail(HIPCLAW_ENODEV, "device index beyond the scratch table%s");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc = hipfail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *rc = HIPCLAW_OK;
+    return g_scratch_ptr[dev];
+}
+
 extern "C" {
 
 const char* hipclaw_last_error(void) { return g_err; }
@@ -126,6 +167,8 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
     if (e != hipSuccess || count == 0) return fail(HIPCLAW_ENODEV, "no HIP device visible%s");
     if (device < 0 || device >= count) return fail(HIPCLAW_EINVAL, "device index out of range%s");
     HIP_TRY(hipSetDevice(device));
+    int rc;
+    if (!scratch_ptr(&rc)) return rc;
     if (name && name_len > 0) {
         hipDeviceProp_t p;
         HIP_TRY(hipGetDeviceProperties(&p, device));
@@ -147,9 +190,8 @@ static int default_tile(const hipclaw_config_t* c) {
     return t;
 }
 
-int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t** out) {
-    if (!cfg || !out || max_patches <= 0) return fail(HIPCLAW_EINVAL, "hipclaw_create: bad arguments%s");
-    *out = nullptr;
+// Checks a configuration for the step kernels; *tile_out = the tile to use.
+static int check_config(const hipclaw_config_t* cfg, int* tile_out) {
     const int solver = cfg->solver;
     static const int meqn_of[3] = {1, 3, 4}, mw_of[3] = {1, 3, 4}, maux_of[3] = {2, 0, 0};
     if (solver < 0 || solver > 2) return fail(HIPCLAW_EINVAL, "unknown Riemann solver%s");
@@ -170,6 +212,15 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     for (int k = 0; k < cfg->mwaves; k++)
         if (cfg->mthlim[k] < 0 || cfg->mthlim[k] > 6) return fail(HIPCLAW_EINVAL, "mthlim out of range%s");
     if (cfg->mx != cfg->my) return fail(HIPCLAW_EINVAL, "the batched step requires mx = my%s");
+    *tile_out = tile;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t** out) {
+    if (!cfg || !out || max_patches <= 0) return fail(HIPCLAW_EINVAL, "hipclaw_create: bad arguments%s");
+    *out = nullptr;
+    int tile = 0;
+    if (int rc = check_config(cfg, &tile)) return rc;
 
     hipclaw_state* s = new hipclaw_state();
     s->cfg = *cfg;
@@ -1044,6 +1095,73 @@ int hipclaw_step_batch(hipclaw_state_t* s, const int* slots, int nslots, double 
     const double c = read_cfl(s, &rc);
     if (rc) return rc;
     if (maxcfl) *maxcfl = c;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_step_batch_noalloc(const hipclaw_config_t* cfg, int npatch, const double* q_in,
+                               double* q_out, const double* aux, const double* dx,
+                               const double* dy, const int* slots, int nslots, double dt,
+                               void* stream, double* maxcfl) {
+    if (!cfg || npatch < 0 || nslots < 0 || (nslots > 0 && (!slots || !q_in || !q_out || !dx || !dy)) ||
+        !(dt > 0.0))
+        return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: bad arguments%s");
+    if (nslots > HIPCLAW_NOALLOC_MAX_SLOTS)
+        return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: more slots than the scratch holds "
+                                    "(HIPCLAW_NOALLOC_MAX_SLOTS)%s");
+    int tile = 0;
+    if (int rc = check_config(cfg, &tile)) return rc;
+    if ((cfg->maux > 0) != (aux != nullptr))
+        return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: aux must be given exactly when maux > 0%s");
+    const int ntx = cfg->mx / tile, nty = cfg->my / tile;
+    if (q_in == q_out && (ntx > 1 || nty > 1))
+        return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: q_in and q_out must differ when a "
+                                    "tile does not cover a patch%s");
+    for (int k = 0; k < nslots; k++)
+        if (slots[k] < 0 || slots[k] >= npatch) return fail(HIPCLAW_EINVAL, "slot out of range%s");
+    if (maxcfl) *maxcfl = 0.0;
+    if (nslots == 0) return HIPCLAW_OK;
+    int rc;
+    unsigned char* base = static_cast<unsigned char*>(scratch_ptr(&rc));
+    if (!base) return rc;
+    unsigned long long* dcfl = reinterpret_cast<unsigned long long*>(base);
+    int* dslots = reinterpret_cast<int*>(base + NOALLOC_CFL_BYTES);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(dslots, slots, sizeof(int) * (size_t) nslots, hipMemcpyHostToDevice, st));
+    if (int e = hipclaw::launch_reset_scalar(dcfl, st)) return hipfail((hipError_t) e, "reset");
+    hipclaw::StepArgs a;
+    a.qsrc = q_in;
+    a.qdst = q_out;
+    a.aux = aux;
+    a.work = dslots;
+    a.dxs = dx;
+    a.dys = dy;
+    a.cfl = dcfl;
+    const long long cells = (long long) (cfg->mx + 2 * cfg->mbc) * (cfg->my + 2 * cfg->mbc);
+    a.pstride = cells * cfg->meqn;
+    a.astride = cells * cfg->maux;
+    a.nwork = nslots;
+    a.mx = cfg->mx; a.my = cfg->my; a.mbc = cfg->mbc;
+    a.ntx = ntx; a.nty = nty;
+    a.order2 = cfg->order[0] == 2;
+    a.transverse = cfg->order[1];
+    for (int k = 0; k < 4; k++) a.mthlim[k] = k < cfg->mwaves ? cfg->mthlim[k] : 0;
+    a.dt = dt;
+    a.pp.grav = cfg->grav;
+    a.pp.gamma = cfg->gamma;
+    a.reg = nullptr;
+    a.rstride = 0;
+    a.reg_accumulate = 0;
+    a.dtscale = nullptr;
+    const hipclaw::StepLaunch L(st);
+    int e = (cfg->precision == HIPCLAW_FP64) ? hipclaw::launch_step_fp64(cfg->solver, tile, 0, a, L)
+                                             : hipclaw::launch_step_fp32(cfg->solver, tile, 0, a, L);
+    if (e) return hipfail((hipError_t) e, "step_tiles launch");
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, dcfl, sizeof(bits), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    if (maxcfl) *maxcfl = v;
     return HIPCLAW_OK;
 }
 

@@ -129,6 +129,34 @@ int hipclaw_ghost_fill(hipclaw_state_t *s);
 int hipclaw_step_batch(hipclaw_state_t *s, const int *slots, int nslots, double t,
                        double dt, double *maxcfl);
 
+/* Most slots one hipclaw_step_batch_noalloc call takes. */
+#define HIPCLAW_NOALLOC_MAX_SLOTS 65536
+
+/* The batched update of hipclaw_step_batch on caller-owned device memory,
+ * allocating nothing: no hipMalloc, hipFree or allocating library call, so
+ * callers that cannot allocate (code under graph capture, tight serving loops,
+ * harnesses that forbid it) can drive it.  No counterpart in the reference.
+ *   q_in, q_out : device, npatch slabs in the layout above; the interior cells
+ *                 of the listed slots of q_out receive the updated state (their
+ *                 ghost cells are not written).  q_in == q_out only when one
+ *                 tile covers a patch (cfg->tile or its default).
+ *   aux         : device, npatch aux slabs, exactly when cfg->maux > 0.
+ *   dx, dy      : device, npatch doubles.
+ *   slots       : host, nslots <= HIPCLAW_NOALLOC_MAX_SLOTS indices < npatch.
+ *   stream      : the hipStream_t to run on (NULL: the default stream); the
+ *                 call returns after the stream has finished its work.
+ *   maxcfl      : the largest Courant number of the batch (may be NULL).
+ * The slot list and the CFL word are staged in one fixed device scratch array
+ * of the library (about 256 KiB per GPU, held until the process ends), cleared
+ * by every call: calls that overlap on one GPU race on it.  No time-sync
+ * registers and no per-slot dt scale.  Make the device current first (the
+ * scratch address is looked up once per device, at hipclaw_initialize_device
+ * or on the first call). */
+int hipclaw_step_batch_noalloc(const hipclaw_config_t *cfg, int npatch, const double *q_in,
+                               double *q_out, const double *aux, const double *dx,
+                               const double *dy, const int *slots, int nslots, double dt,
+                               void *stream, double *maxcfl);
+
 /* Ghost-fill program number k (0 <= k < HIPCLAW_MAX_PROGRAMS); program 0 is
  * hipclaw_set_ghost_program's.  Programs for fclaw2d_ghost_update over a
  * partial level range with time interpolation (subcycling) reference the
