@@ -119,3 +119,124 @@ def run(q, ops, phase_start, mx, my, mbc):
                                 v = Q(dst, is_, js, m)
                                 S(dst, idd, jd, m, -v if (mth == 3 and m == mn) else v)
     return q
+
+
+def _bc_code(bc, face):
+    mth = (bc >> (8 * face)) & 0xFF
+    return mth - 256 if mth > 127 else mth
+
+
+def face_table(ops, npatch):
+    """What fills each face ghost strip, from the ops (the fused launch's face
+    table): ('same', nb, ti, tj) / ('half', [(nb, ti, tj), ...]) /
+    ('double',) / ('phys', mth) / None."""
+    rel = [[None] * 4 for _ in range(npatch)]
+    for op in ops:
+        t, dst, src, which, ti, tj, bc, _ = (int(v) for v in op)
+        if t == COPY_FACE:
+            rel[dst][which] = ('same', src, ti, tj)
+        elif t == AVG_FACE:
+            if rel[dst][which] is None or rel[dst][which][0] != 'half':
+                rel[dst][which] = ('half', [])
+            rel[dst][which][1].append((src, ti, tj))
+        elif t == INTERP_FACE:
+            rel[dst][which ^ 1] = ('double',)
+        elif t == PHYSBC:
+            for f in range(4):
+                mth = _bc_code(bc, f)
+                if mth in (1, 3):
+                    rel[dst][f] = ('phys', mth)
+    return rel
+
+
+def run_fused(q, ops, phase_start, mx, my, mbc, order=None):
+    """The fused launch's semantics: every patch on its own, in any order
+    (`order`, default reversed): its averages and copies, then its
+    interpolations reading the coarse patch's face ghost cells as recomputed
+    from the face table (never the array, which other patches' groups are
+    writing), then its physical BC once.  Reads of interior cells see the
+    values before the fill (no group writes an interior cell)."""
+    npatch, meqn = q.shape[0], q.shape[1]
+    src_q = q.copy()        # interiors (and nothing else) are read from here
+    o = mbc - 1
+    rel = face_table(ops, npatch)
+
+    def Qs(p, i, j, m):
+        return src_q[p, m, j + o, i + o]
+
+    def cell(c, i, j, m):
+        k = 0 if i < 1 else 1 if i > mx else 2 if j < 1 else 3 if j > my else -1
+        if k < 0:
+            return Qs(c, i, j, m)
+        r = rel[c][k]
+        assert r is not None and r[0] in ('same', 'half', 'phys'), (c, k, r)
+        if r[0] == 'same':
+            return Qs(r[1], i + r[2], j + r[3], m)
+        if r[0] == 'half':
+            for nb, ti, tj in r[1]:
+                fi, fj = 2 * i - 1 + ti, 2 * j - 1 + tj
+                if 1 <= fi and fi + 1 <= mx and 1 <= fj and fj + 1 <= my:
+                    s = 0.0
+                    for a, b in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                        s = s + Qs(nb, fi + a, fj + b, m)
+                    return s / 4.0
+            raise AssertionError("no child covers the cell")
+        mth = r[1]
+        i2, j2, mn = i, j, (1 if k < 2 else 2)
+        if k == 0:
+            i2 = 1 if mth == 1 else 1 - i
+        elif k == 1:
+            i2 = mx if mth == 1 else 2 * mx + 1 - i
+        elif k == 2:
+            j2 = 1 if mth == 1 else 1 - j
+        else:
+            j2 = my if mth == 1 else 2 * my + 1 - j
+        v = Qs(c, i2, j2, m)
+        return -v if (mth == 3 and m == mn) else v
+
+    groups = {}
+    for ph in range(6):
+        for k in range(phase_start[ph], phase_start[ph + 1]):
+            dst = int(ops[k][1])
+            g = groups.setdefault(dst, ([], [], []))
+            (g[0] if ph <= 1 else g[1] if ph == 3 else g[2]).append(k)
+    dsts = sorted(groups, reverse=True) if order is None else order
+    for dst in dsts:
+        a_ops, i_ops, bc_ops = groups[dst]
+        # averages and copies (interior reads only) -- the sequential emulator per op
+        sub = np.array([ops[k] for k in a_ops], dtype=np.int32).reshape(-1, 8)
+        tmp = src_q.copy()
+        run(tmp, sub, [0, len(sub), len(sub), len(sub), len(sub), len(sub), len(sub)], mx, my, mbc)
+        q[dst] = tmp[dst]
+        for k in i_ops:
+            t, _, c, which, ti, tj, _, _ = (int(v) for v in ops[k])
+            hb = mbc // 2
+            cells = []
+            if t == INTERP_FACE:
+                for l in range(mx):
+                    for b in range(1, hb + 1):
+                        if which < 2:
+                            cells.append(((b if which == 0 else mx - b + 1), 1 + l))
+                        else:
+                            cells.append((1 + l, (b if which == 2 else my - b + 1)))
+            else:
+                for ibc in range(1, hb + 1):
+                    for jbc in range(1, hb + 1):
+                        cells.append((mx - ibc + 1 if which & 1 else ibc,
+                                      my - jbc + 1 if which & 2 else jbc))
+            for m in range(meqn):
+                for ic, jc in cells:
+                    fi, fj = 2 * ic - 1 + ti, 2 * jc - 1 + tj
+                    if fi < 1 - mbc or fi + 1 > mx + mbc or fj < 1 - mbc or fj + 1 > my + mbc:
+                        continue
+                    cc = cell(c, ic, jc, m)
+                    gx = _slope(cc - cell(c, ic - 1, jc, m), cell(c, ic + 1, jc, m) - cc)
+                    gy = _slope(cc - cell(c, ic, jc - 1, m), cell(c, ic, jc + 1, m) - cc)
+                    for a, b, sx, sy in ((0, 0, -0.25, -0.25), (1, 0, 0.25, -0.25),
+                                         (0, 1, -0.25, 0.25), (1, 1, 0.25, 0.25)):
+                        q[dst, m, fj + b + o, fi + a + o] = cc + gx * sx + gy * sy
+        if bc_ops:
+            last = ops[bc_ops[-1]]
+            sub = np.array([last], dtype=np.int32)
+            run(q, sub, [0, 0, 0, 0, 0, 0, 1], mx, my, mbc)
+    return q

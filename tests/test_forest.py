@@ -66,6 +66,44 @@ def test_forest_from_leaves_rejects_gaps_and_overlaps():
         fa.Forest(1, 1, (False, False), leaves=([0, 1, 1, 1, 1], [0, 0, 1, 0, 1], [0, 0, 0, 1, 1]))
 
 
+def _random_forests():
+    """Random 2:1 forests: a brick with a periodic direction, a periodic
+    square, and the three workloads' initial forests."""
+    out = []
+    for seed, (nbx, nby, per) in enumerate([(2, 1, (True, False)), (1, 1, (True, True)),
+                                            (1, 2, (False, False))]):
+        rng = np.random.default_rng(40 + seed)
+        f = fa.Forest(nbx, nby, per, 1)
+        for _ in range(3):
+            f.refine((rng.random(f.npatch) < 0.35).astype(np.int32), 4)
+        mthbc = tuple(2 if per[k // 2] else (1, 3)[(seed + k) % 2] for k in range(4))
+        out.append((f, mthbc))
+    return out
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_fused_ghost_fill_matches_oracle(k):
+    """The fused launch's semantics (each patch's ghost cells on their own, in
+    any order, interpolation recomputing the coarse patch's face ghost cells
+    from their sources, the physical BC once) reproduce the oracle's
+    sequential ghost update bit for bit on random 2:1 forests."""
+    import dataclasses
+    forest, mthbc = _random_forests()[k]
+    prob = dataclasses.replace(P.dambreak_problem(1, 4, mx=8), nbx=forest.nbx, nby=forest.nby,
+                               periodic=forest.periodic, mthbc=mthbc,
+                               extent=(0.0, float(forest.nbx), 0.0, float(forest.nby)))
+    assert len(set(forest.leaves()[0].tolist())) >= 3
+    rng = np.random.default_rng(7 + k)
+    q = random_state(prob.solver, forest.npatch, 8, 8, rng)
+    ops, ps = forest.ghost_program(8, 8, 2, mthbc)
+    qo = oracle_ghost_update(prob, forest, q, 8)
+    for order in (None, list(rng.permutation(forest.npatch))):
+        qf = G.run_fused(q.copy(), ops, ps, 8, 8, 2,
+                         order=None if order is None else [d for d in order
+                                                           if d in set(ops[:, 1].tolist())])
+        np.testing.assert_array_equal(qf, qo)
+
+
 @pytest.mark.parametrize("prob", [P.dambreak_problem(2, 4, mx=8), P.advection_problem(2, 4, mx=8),
                                   P.shockbubble_problem(1, 3, mx=8)], ids=lambda p: p.name)
 def test_ghost_program_matches_oracle(prob):
