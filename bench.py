@@ -123,19 +123,20 @@ def run_regrid(fa, prob, forest, q, repeats=3):
             "coarsen_threshold": 0.5 * prob.tag_threshold}
 
 
-def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
-    """The workload as an application runs it (fclaw2d_run, src/fclaw2d_run.c:199-287):
-    initial refinement, then steps of all levels with dt set from the previous
-    step's Courant number (desired 0.9), the conservation fix-up, and a regrid
-    every regrid_interval steps, until t_final.  Each step starts from a saved
-    state (a device copy of every patch), and a step whose Courant number
-    exceeds 1 is put back and retaken with the smaller dt.  FP64 form; one
-    sync per step for the Courant number, as the reference's global
-    reduction."""
-    precision = fa.FP64 if precision is None else precision
-    t0 = time.perf_counter()
-    forest = P.build_forest(prob)
-    q0, aux = P.initial_data(prob, forest)
+def run_loop(fa, P, prob, forest, q0, aux, precision, regrid_interval, t_final=None, nsteps=None,
+             stop_patches=None, t0=0.0, time_limit=None):
+    """The workload as an application runs it (fclaw2d_run, src/fclaw2d_run.c:199-287)
+    from the state (forest, q0): steps of all levels with dt set from the
+    previous step's Courant number (desired prob.cfl), the conservation
+    fix-up, and a regrid (device tags, host adapt, device transfer, ghost fill)
+    every regrid_interval steps; until time t_final, nsteps accepted steps,
+    or the first regrid after which the forest has stop_patches leaves.  Each
+    step starts from a saved state (a device copy of every patch), and a step
+    whose Courant number exceeds 1 is put back and retaken with the smaller
+    dt.  One sync per step for the Courant number, as the reference's global
+    reduction.  Returns the solver, the forest and the run's counters."""
+    forest = fa.Forest(forest.nbx, forest.nby, forest.periodic, leaves=forest.leaves(),
+                       isolated=forest.isolated)
     sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
                    grav=prob.grav, gamma=prob.gamma, max_patches=2 * forest.npatch, time_sync=True)
     sv.set_forest(forest, prob.extent)
@@ -144,12 +145,17 @@ def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
         sv.set_aux(aux)
     sv.ghost_fill()
     sv.sync()
-    t_init = time.perf_counter() - t0
     dt = P.stable_dt(prob, forest, q0)
-    t, steps, regrids, updates, t_regrid, maxp, retakes = 0.0, 0, 0, 0, 0.0, forest.npatch, 0
-    t1 = time.perf_counter()
-    while t < t_final * (1 - 1e-12):
-        step = min(dt, t_final - t)
+    t, steps, regrids, updates, t_regrid, maxp, retakes = t0, 0, 0, 0, 0.0, forest.npatch, 0
+    w0 = time.perf_counter()
+    while True:
+        if t_final is not None and t >= t_final * (1 - 1e-12):
+            break
+        if nsteps is not None and steps >= nsteps:
+            break
+        if time_limit is not None and time.perf_counter() - w0 > time_limit:
+            break
+        step = dt if t_final is None else min(dt, t_final - t)
         sv.save_step()
         cfl = sv.advance(step, 1)
         if cfl > MAX_CFL:
@@ -162,7 +168,7 @@ def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
         updates += sv.npatch * prob.mx * prob.mx
         if cfl > 0:
             dt = step * prob.cfl / cfl
-        if regrid_interval > 0 and steps % regrid_interval == 0 and t < t_final:
+        if regrid_interval > 0 and steps % regrid_interval == 0 and (t_final is None or t < t_final):
             r0 = time.perf_counter()
             er, ec = sv.tag(prob.tag_field, prob.tag_threshold, 0.5 * prob.tag_threshold)
             kind, src = forest.adapt(er, (1 - ec).astype(np.int32), prob.minlevel, prob.maxlevel)
@@ -175,20 +181,69 @@ def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
             t_regrid += time.perf_counter() - r0
             regrids += 1
             maxp = max(maxp, forest.npatch)
+            if stop_patches is not None and forest.npatch >= stop_patches:
+                break
     sv.sync()
-    wall = time.perf_counter() - t1
+    wall = time.perf_counter() - w0
+    return sv, forest, dict(wall=wall, t=t, steps=steps, retakes=retakes, regrids=regrids,
+                            updates=updates, regrid_s=t_regrid, max_patches=maxp)
+
+
+def run_to(fa, P, prob, workload, t_final, regrid_interval, precision=None):
+    """The whole run from t = 0 to t_final (run_loop), reported as its own line."""
+    precision = fa.FP64 if precision is None else precision
+    t0 = time.perf_counter()
+    forest = P.build_forest(prob)
+    q0, aux = P.initial_data(prob, forest)
+    t_init = time.perf_counter() - t0
+    sv, forest, r = run_loop(fa, P, prob, forest, q0, aux, precision, regrid_interval, t_final=t_final)
     mass = sv.mass()
     return {"metric": "time to t=%g, %s, regrid every %d steps" % (t_final, workload, regrid_interval),
-            "value": round(wall, 4), "unit": "s", "higher_is_better": False,
-            "steps": steps, "retaken_steps": retakes, "regrids": regrids,
-            "final_patches": forest.npatch, "max_patches": maxp,
-            "cell_updates": updates, "cell_updates_per_s": round(updates / wall, 1),
-            "regrid_s": round(t_regrid, 4), "init_s": round(t_init, 4),
-            "final_time": t, "mass": [float(m) for m in mass[0]],
+            "value": round(r["wall"], 4), "unit": "s", "higher_is_better": False,
+            "steps": r["steps"], "retaken_steps": r["retakes"], "regrids": r["regrids"],
+            "final_patches": forest.npatch, "max_patches": r["max_patches"],
+            "cell_updates": r["updates"], "cell_updates_per_s": round(r["updates"] / r["wall"], 1),
+            "regrid_s": round(r["regrid_s"], 4), "init_s": round(t_init, 4),
+            "final_time": r["t"], "mass": [float(m) for m in mass[0]],
             "dtype": "f64" if precision == fa.FP64 else "f32", "data": "synthetic",
             "config": {"workload": workload, "cfl": prob.cfl, "time_sync": True,
                        "refine_threshold": prob.tag_threshold,
                        "coarsen_threshold": 0.5 * prob.tag_threshold}}
+
+
+GROW_PATCHES = 10000
+
+
+def grow_forest(fa, P, prob, target=GROW_PATCHES, t_max=0.5, time_limit=120.0):
+    """The benchmark state: the FP64 run (run_loop, regrid every 4 steps) from
+    t = 0 until the first regrid after which the forest has `target` leaves
+    (or t_max).  Returns the forest, its state with ghost cells filled, aux and
+    a description of how it was reached."""
+    forest = P.build_forest(prob)
+    q0, aux = P.initial_data(prob, forest)
+    n0 = forest.npatch
+    sv, forest, r = run_loop(fa, P, prob, forest, q0, aux, fa.FP64, 4, t_final=t_max,
+                             stop_patches=target, time_limit=time_limit)
+    q = sv.get_q()
+    del sv
+    if aux is not None:
+        aux = P.initial_data(prob, forest)[1]
+    return forest, q, aux, {"grown_from_patches": n0, "patches": forest.npatch, "t": r["t"],
+                            "steps": r["steps"], "regrids": r["regrids"],
+                            "wall_s": round(r["wall"], 2), "target_patches": target}
+
+
+def configured_run(fa, P, prob, forest, q0, aux, precision, nsteps=100, regrid_interval=4):
+    """Throughput of the run as configured (regrid every regrid_interval steps)
+    over nsteps accepted steps from the benchmark state: cell updates over the
+    whole wall time, regrids included."""
+    sv, f, r = run_loop(fa, P, prob, forest, q0, aux, precision, regrid_interval, nsteps=nsteps)
+    del sv
+    return {"value": round(r["updates"] / r["wall"], 1), "unit": "cell-updates/s",
+            "steps": r["steps"], "retaken_steps": r["retakes"], "regrids": r["regrids"],
+            "regrid_interval": regrid_interval, "wall_s": round(r["wall"], 4),
+            "regrid_s": round(r["regrid_s"], 4), "patches_start": forest.npatch,
+            "patches_end": f.npatch, "max_patches": r["max_patches"]}
 
 
 def sweep_one(fa, P, minlevel, maxlevel, mx, args):
@@ -396,17 +451,18 @@ def copy_peak_gbs(nbytes=1 << 30, reps=10):
         hip.hipFree(b)
 
 
-def committed_traffic(workload, kernel_tag):
-    """HBM bytes per launch of the step kernel from the committed counter summary
-    of this workload (None if there is none for this kernel)."""
+def committed_traffic(workload, kernel_tag, cells):
+    """HBM bytes per launch of the step kernel: the measured bytes per cell
+    update of the committed counter summary of this workload, times the cells
+    of this launch (None if there is no summary for this kernel)."""
     path = os.path.join(ROOT, "profiles", "step_tiles_%s_pmc.txt" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("kernel=") and kernel_tag in line:
             for tok in line.split():
-                if tok.startswith("traffic_bytes="):
-                    return float(tok.split("=")[1])
+                if tok.startswith("bytes_per_cell="):
+                    return float(tok.split("=")[1]) * cells
     return None
 
 
@@ -430,6 +486,11 @@ def main():
                          "CFL-controlled dt and a regrid every --regrid-interval steps, and "
                          "report the whole run (steps, regrids, time, cell updates/s)")
     ap.add_argument("--regrid-interval", type=int, default=4)
+    ap.add_argument("--forest", choices=("grown", "initial"), default=None,
+                    help="dam break: time the forest the run reaches when it first has %d "
+                         "leaves (grown, the default) or the t = 0 forest (initial)" % GROW_PATCHES)
+    ap.add_argument("--no-configured-run", action="store_true",
+                    help="skip the run with its regrid cadence (configured_run)")
     ap.add_argument("--tile64", type=int, default=0, help="tile for the FP64 form (0 = autotune)")
     ap.add_argument("--tile32", type=int, default=0, help="tile for the FP32 form (0 = autotune)")
     ap.add_argument("--sweep", action="store_true",
@@ -491,12 +552,22 @@ def main():
         print(json.dumps(run_to(fa, P, prob, workload, args.run_to, args.regrid_interval)))
         return
     dt_scale = None
+    grown = None
+    forest_kind = args.forest or ("grown" if args.problem == "dambreak" else "initial")
     if ensemble:
         forest = P.build_ensemble_forest(prob)
         q0, aux = P.initial_data(prob, forest)
         # each run at its own stable dt: step dt 1.0 times a per-slot scale
         dt_scale, dt_run = P.ensemble_dt(prob, forest, q0)
         dt = 1.0
+    elif forest_kind == "grown":
+        if args.pmc_summary or args.valu_summary or args.counter_summary:
+            sys.exit("bench.py: the summaries need the forest of their run: pass --forest initial "
+                     "to both, or summarise from the kernel counts of a grown run")
+        fa.initialize_device(0)
+        forest, q0, aux, grown = grow_forest(fa, P, prob)
+        print("grown forest: %s" % json.dumps(grown), file=sys.stderr, flush=True)
+        dt = P.stable_dt(prob, forest, q0)
     else:
         forest = P.build_forest(prob)
         q0, aux = P.initial_data(prob, forest)
@@ -529,6 +600,10 @@ def main():
                    args.tile32, not args.no_graph, not args.no_time_sync, dt_scale)
 
     regrid = run_regrid(fa, prob, forest, r64["q"])
+    configured = None
+    if not args.no_configured_run and not ensemble:
+        configured = {name: configured_run(fa, P, prob, forest, q0, aux, precision)
+                      for name, precision in (("fp32", fa.FP32_BULK), ("fp64", fa.FP64))}
 
     B = algorithmic_bytes_per_cell(prob.meqn, prob.maux, prob.mx)
     value32 = cells * args.steps / r32["wall"]
@@ -538,7 +613,7 @@ def main():
 
     def roofline(r, rtype):
         achieved = B * cells / (r["kernel_ms"] * 1e-3) / 1e9
-        traffic = committed_traffic(workload, "%s, %d," % (rtype, r["tile"]))
+        traffic = committed_traffic(workload, "%s, %d," % (rtype, r["tile"]), cells)
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else round(traffic),
@@ -607,6 +682,7 @@ def main():
             "autotune_ms_per_step": {"fp32": r32["autotune"], "fp64": r64["autotune"]},
             "timed_with_hip_graph": bool(r32["graph"]), "time_sync": not args.no_time_sync,
             "device": device,
+            "forest": grown if grown is not None else {"kind": "initial refinement at t = 0"},
         },
         "roofline": roofline(r32, "RPShallow<float>, float"),
         "fp64": {
@@ -623,6 +699,7 @@ def main():
             "mass_change_fp64": mass_change(r64),
         },
         "regrid": regrid,
+        "configured_run": configured,
         "cpu_baseline": cpu,
     }
     if runs is not None:

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libforestclaw_amd.so")
 
 RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
 FP64, FP32_BULK = 0, 1
+AUTOTUNE_MAX_TRIALS = 12   # HIPCLAW_AUTOTUNE_MAX_TRIALS
 # provenance of a leaf after a forest adapt (fclaw2d_hip_forest_provenance)
 REGRID_SAME, REGRID_CHILD, REGRID_PARENT = 0, 1, 2
 SOLVER_SHAPES = {RP_ADVECTION: (1, 1, 2), RP_SHALLOW: (3, 3, 0), RP_EULER: (4, 4, 0)}
@@ -323,14 +324,14 @@ class Solver:
         _check(lib().hipclaw_restore_step(self._h), "hipclaw_restore_step")
 
     def autotune(self, dt, nsteps=8):
-        """Time every allowed tile (16 x 16 with 2 and 4 waves) in both
-        ghost-fill layouts over nsteps steps of dt (the state is put back
+        """Time every allowed tile (16 x 16 with 2 and 4 waves) in every
+        ghost-fill layout over nsteps steps of dt (the state is put back
         after each) and keep the fastest; returns
         {(tile, waves, layout): ms per step}."""
-        tiles = np.zeros(8, np.int32)
-        waves = np.zeros(8, np.int32)
-        layouts = np.zeros(8, np.int32)
-        ms = np.zeros(8, np.float64)
+        tiles = np.zeros(AUTOTUNE_MAX_TRIALS, np.int32)
+        waves = np.zeros(AUTOTUNE_MAX_TRIALS, np.int32)
+        layouts = np.zeros(AUTOTUNE_MAX_TRIALS, np.int32)
+        ms = np.zeros(AUTOTUNE_MAX_TRIALS, np.float64)
         n = C.c_int()
         _check(lib().hipclaw_autotune(self._h, dt, nsteps, tiles.ctypes.data, waves.ctypes.data,
                                       layouts.ctypes.data, ms.ctypes.data, C.byref(n)),

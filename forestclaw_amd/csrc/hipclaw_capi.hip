@@ -113,6 +113,28 @@ struct hipclaw_state {
     std::vector<hipEvent_t> kev;  // pairs (kernel start, kernel end) per step
     int timed_steps = 0;
     hipEvent_t region[2] = {nullptr, nullptr};
+    // host copies of program 0 (ghost fill with its phases, time sync): the
+    // fused step is built from them
+    std::vector<int> gp0_ops, sp0_ops;
+    int gp0_phase[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool gp0_set = false;
+    // fused step (ghost layout 2): the update pulls same-size ghost strips, one
+    // ghost_flat launch fills the rest and computes the time-sync increments,
+    // which the next update adds (deferred through qcorr)
+    bool fused_ok = false;
+    GhostOp* fops = nullptr;      // ops [0, nlite): not copies; [nlite, nall): copies
+    int fops_cap = 0, nlite = 0, nall = 0;
+    int* pull = nullptr;          // [slot][8] offsets of same-size ghost strips (StepArgs::pull)
+    int pull_cap = 0;
+    bool pull_ok = false;
+    hipclaw::FaceRel* frel = nullptr;
+    int frel_cap = 0;
+    int* cmask = nullptr;
+    int cmask_cap = 0;
+    double* qcorr = nullptr;
+    long long cstride = 0;
+    int qcorr_cap = 0;            // patches qcorr holds
+    bool fsync = false;           // the fused step applies time-sync corrections
 };
 
 // ---------------------------------------------------------------- allocation-free step
@@ -154,6 +176,8 @@ static void* scratch_ptr(int* rc) {
     *rc = HIPCLAW_OK;
     return g_scratch_ptr[dev];
 }
+
+static int build_fused(hipclaw_state_t* s);
 
 extern "C" {
 
@@ -231,7 +255,7 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     s->max_patches = max_patches;
     // measured on MI355X: per-destination groups pay off once a patch's ghost
     // strips are long (mx >= 32); shorter ones keep one launch per phase
-    s->ghost_layout = cfg->mx >= 32 ? 1 : 0;
+    s->ghost_layout = 2;
     const long long cells = (long long) (cfg->mx + 2 * cfg->mbc) * (cfg->my + 2 * cfg->mbc);
     s->pstride = cells * cfg->meqn;
     s->astride = cells * cfg->maux;
@@ -312,6 +336,11 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->reg);
     hipFree(s->band);
     hipFree(s->dtscale);
+    hipFree(s->fops);
+    hipFree(s->pull);
+    hipFree(s->frel);
+    hipFree(s->cmask);
+    hipFree(s->qcorr);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
 }
@@ -621,6 +650,12 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
     g.nops = (int) dev.size();
     g.nlaunch[0] = nl[0];
     g.nlaunch[1] = nl[1];
+    if (k == 0) {
+        s->gp0_ops.assign(ops, ops + 8 * (size_t) nops);
+        for (int p = 0; p < 7; p++) s->gp0_phase[p] = phase_start[p];
+        s->gp0_set = true;
+        return build_fused(s);
+    }
     return HIPCLAW_OK;
 }
 
@@ -677,8 +712,178 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     y.nresets = nresets;
     y.maxgroup = 0;
     for (int g = 0; g < ngroups; g++) y.maxgroup = std::max(y.maxgroup, start[g + 1] - start[g]);
+    if (k == 0) {
+        s->sp0_ops.assign(ops, ops + 8 * (size_t) nops);
+        return build_fused(s);
+    }
     return HIPCLAW_OK;
 }
+
+}  // extern "C"
+
+// Upload n elements of `host` into a device array grown as needed.
+template <typename T>
+static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, int n, const char* what) {
+    if (n > cap) {
+        hipFree(dev);
+        dev = nullptr;
+        cap = 0;
+        if (hipMalloc((void**) &dev, sizeof(T) * (size_t) (n > 0 ? n : 1)) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
+        cap = n;
+    }
+    if (n > 0) HIP_TRY(hipMemcpyAsync(dev, host, sizeof(T) * (size_t) n, hipMemcpyHostToDevice, s->st));
+    return HIPCLAW_OK;
+}
+
+// The fused step of program 0 (ghost layout 2).  Every ghost op becomes one
+// wave of ghost_flat that reads interior cells only: copies and averages do
+// already; interpolation and the physical BC recompute the ghost cells they
+// read from the face table (same-size neighbour, fine children, physical BC,
+// or for a physical BC beside a face filled from a coarser patch, that
+// interpolation).  The program is fused only if every such cell has one of
+// those sources (which the 2:1 balance of the forest guarantees) and it has no
+// time-interpolated slots; otherwise fused_ok stays false and the advance
+// keeps the two-launch layouts.  With two slabs the update also stages the
+// same-size ghost strips and corners straight from the neighbours (pull), and
+// the copy ops run only in the last step of an advance, so that the slab's
+// ghost cells are complete for readers.
+static int build_fused(hipclaw_state_t* s) {
+    using hipclaw::FaceRel;
+    s->fused_ok = false;
+    s->fsync = false;
+    s->pull_ok = false;
+    if (!s->gp0_set || s->npatch <= 0) return HIPCLAW_OK;
+    const int np = s->npatch, mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    const int nxp = mx + 2 * mbc;
+    const int nops = (int) s->gp0_ops.size() / 8;
+    const int* ops = s->gp0_ops.data();
+    const int TI = HIPCLAW_TIME_INTERP_SLOT;
+    std::vector<FaceRel> frel((size_t) 4 * np);
+    for (auto& r : frel) r = FaceRel{hipclaw::FREL_NONE, -1, 0, 0, -1, 0, 0, 0};
+    std::vector<int> bcop(np, -1);
+    for (int o = 0; o < nops; o++) {
+        const int* op = ops + 8 * o;
+        const int type = op[0], dst = op[1], src = op[2], which = op[3];
+        if ((dst & TI) || (type != hipclaw::GHOST_PHYSBC && (src & TI))) return HIPCLAW_OK;
+        if (dst < 0 || dst >= np) return HIPCLAW_OK;
+        if (type == hipclaw::GHOST_COPY_FACE) {
+            frel[4 * dst + which] = FaceRel{hipclaw::FREL_SAME, src, op[4], op[5], -1, 0, 0, 0};
+        } else if (type == hipclaw::GHOST_AVERAGE_FACE) {
+            FaceRel& r = frel[4 * dst + which];
+            if (r.rel != hipclaw::FREL_HALF) r = FaceRel{hipclaw::FREL_HALF, src, op[4], op[5], -1, 0, 0, 0};
+            else { r.nb1 = src; r.ti1 = op[4]; r.tj1 = op[5]; }
+        } else if (type == hipclaw::GHOST_INTERP_FACE) {
+            frel[4 * dst + (which ^ 1)] = FaceRel{hipclaw::FREL_DOUBLE, src, op[4], op[5], -1, 0, 0, which};
+        } else if (type == hipclaw::GHOST_PHYSBC) {
+            if (bcop[dst] >= 0 && ops[8 * bcop[dst] + 6] != op[6]) return HIPCLAW_OK;
+            bcop[dst] = o;
+            for (int f = 0; f < 4; f++) {
+                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
+                if (b == 1 || b == 3) frel[4 * dst + f] = FaceRel{hipclaw::FREL_PHYS, -1, 0, 0, -1, 0, 0, b};
+            }
+        }
+    }
+    // cells read from a coarse patch's face strips (interpolation) need a source
+    // that is not itself an interpolation; a physical BC may read any face strip
+    auto simple = [&](int c, int f) {
+        const FaceRel& r = frel[4 * c + f];
+        return r.rel == hipclaw::FREL_SAME || r.rel == hipclaw::FREL_PHYS ||
+               (r.rel == hipclaw::FREL_HALF && r.nb1 >= 0);
+    };
+    for (int o = 0; o < nops; o++) {
+        const int* op = ops + 8 * o;
+        const int type = op[0], c = op[2], which = op[3], ti = op[4], tj = op[5];
+        if (type == hipclaw::GHOST_INTERP_FACE) {
+            if (c < 0 || c >= np || !(frel[4 * c + which].rel == hipclaw::FREL_HALF && frel[4 * c + which].nb1 >= 0))
+                return HIPCLAW_OK;
+            bool lo = false, hi = false;
+            for (int l = 1; l <= mx; l++) {   // the kernel bounds the loop by mx (mx == my)
+                const int ic = which < 2 ? (which == 0 ? 1 : mx) : l, jc = which < 2 ? l : (which == 2 ? 1 : my);
+                const int fi = 2 * ic - 1 + ti, fj = 2 * jc - 1 + tj;
+                if (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc) continue;
+                if (l == 1) lo = true;
+                if (l == mx) hi = true;
+            }
+            const int flo = which < 2 ? 2 : 0, fhi = which < 2 ? 3 : 1;
+            if ((lo && !simple(c, flo)) || (hi && !simple(c, fhi))) return HIPCLAW_OK;
+        } else if (type == hipclaw::GHOST_INTERP_CORNER) {
+            if (c < 0 || c >= np || !simple(c, (which & 1) ? 1 : 0) || !simple(c, (which & 2) ? 3 : 2))
+                return HIPCLAW_OK;
+        } else if (type == hipclaw::GHOST_PHYSBC) {
+            for (int f = 0; f < 4; f++)
+                if (frel[4 * op[1] + f].rel == hipclaw::FREL_NONE) return HIPCLAW_OK;
+        }
+    }
+    // flat op list: everything but the copies, then the copies
+    std::vector<GhostOp> fops;
+    fops.reserve((size_t) nops);
+    auto to_op = [&](int o) {
+        const int* q = ops + 8 * o;
+        GhostOp op;
+        op.type = q[0]; op.dst = q[1]; op.src = q[2]; op.which = q[3];
+        op.ti = q[4]; op.tj = q[5]; op.bc = q[6]; op.reps = 1;
+        return op;
+    };
+    for (int o = 0; o < nops; o++) {
+        const int type = ops[8 * o];
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) continue;
+        if (type == hipclaw::GHOST_PHYSBC && bcop[ops[8 * o + 1]] != o) continue;   // once per patch
+        fops.push_back(to_op(o));
+    }
+    const int nlite = (int) fops.size();
+    for (int o = 0; o < nops; o++) {
+        const int type = ops[8 * o];
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) fops.push_back(to_op(o));
+    }
+    // same-size strips and corners the update stages from the neighbours
+    // (offsets in doubles must fit an int)
+    std::vector<int> pull((size_t) 8 * np, 0);
+    bool pull_ok = (double) s->pstride * s->max_patches < 2.0e9;
+    for (int o = 0; o < nops && pull_ok; o++) {
+        const int* op = ops + 8 * o;
+        if (op[0] != hipclaw::GHOST_COPY_FACE && op[0] != hipclaw::GHOST_COPY_CORNER) continue;
+        const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
+        pull[8 * (size_t) op[1] + (op[0] == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
+    }
+    // time sync (contiguous per destination): the faces each patch corrects
+    std::vector<int> cm(np, 0);
+    const bool sync = s->reg && !s->reg_accumulate && !s->sp0_ops.empty();
+    if (sync) {
+        const int nso = (int) s->sp0_ops.size() / 8;
+        for (int o = 0; o < nso; o++) {
+            const int* op = s->sp0_ops.data() + 8 * o;
+            if (op[1] < 0 || op[1] >= np || op[2] < 0 || op[2] >= np) return HIPCLAW_OK;
+            cm[op[1]] |= (1 << op[3]) | (op[0] == hipclaw::SYNC_F2C ? 1 << (4 + op[3]) : 0);
+        }
+    }
+    if (int rc = upload_array(s, s->fops, s->fops_cap, fops.data(), (int) fops.size(), "fused ghost ops")) return rc;
+    if (int rc = upload_array(s, s->frel, s->frel_cap, frel.data(), (int) frel.size(), "face table")) return rc;
+    if (pull_ok)
+        if (int rc = upload_array(s, s->pull, s->pull_cap, pull.data(), (int) pull.size(), "pull offsets")) return rc;
+    if (sync) {
+        if (int rc = upload_array(s, s->cmask, s->cmask_cap, cm.data(), np, "sync face masks")) return rc;
+        const long long cstride = 4LL * s->cfg.meqn * mx;
+        if (!s->qcorr || s->cstride != cstride || s->qcorr_cap < s->max_patches) {
+            hipFree(s->qcorr);
+            s->qcorr = nullptr;
+            s->qcorr_cap = 0;
+            if (hipMalloc((void**) &s->qcorr, sizeof(double) * (size_t) cstride * (size_t) s->max_patches) != hipSuccess)
+                return fail(HIPCLAW_ENOMEM, "hipMalloc of the time-sync increments failed%s");
+            s->cstride = cstride;
+            s->qcorr_cap = s->max_patches;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->nlite = nlite;
+    s->nall = (int) fops.size();
+    s->pull_ok = pull_ok;
+    s->fsync = sync;
+    s->fused_ok = true;
+    return HIPCLAW_OK;
+}
+
+extern "C" {
 
 int hipclaw_set_sync_program(hipclaw_state_t* s, const int* ops, int nops) {
     return hipclaw_set_sync_program_k(s, 0, ops, nops, nullptr, 0);
@@ -715,14 +920,41 @@ int hipclaw_time_sync_k(hipclaw_state_t* s, int k) {
 
 int hipclaw_time_sync(hipclaw_state_t* s) { return hipclaw_time_sync_k(s, 0); }
 
+// The fused launch of program 0 on the current slab: the ghost fill (`full`:
+// with the same-size copies, else without the ghost cells the update pulls),
+// and with `sync` the time-sync increments into qcorr.
+static int ghost_flat_async(hipclaw_state_t* s, bool full, bool sync) {
+    hipclaw::FlatArgs f;
+    f.g.ops = s->fops;
+    f.g.nops = 0;
+    f.g.q = s->q[s->cur];
+    f.g.qti = nullptr;
+    f.g.pstride = s->pstride;
+    f.g.mx = s->cfg.mx; f.g.my = s->cfg.my; f.g.mbc = s->cfg.mbc; f.g.meqn = s->cfg.meqn;
+    f.nghost = full ? s->nall : s->nlite;
+    f.frel = s->frel;
+    f.sops = sync ? s->sp[0].ops : nullptr;
+    f.nsync = sync ? (int) s->sp0_ops.size() / 8 : 0;
+    f.reg = s->reg;
+    f.dxs = s->dxs;
+    f.dys = s->dys;
+    f.rstride = s->rstride;
+    f.qinc = s->qcorr;
+    f.cstride = s->cstride;
+    f.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->cfg.solver != HIPCLAW_RP_ADVECTION;
+    if (int e = hipclaw::launch_ghost_flat(f, s->st)) return hipfail((hipError_t) e, "ghost_flat launch");
+    return HIPCLAW_OK;
+}
+
 static int ghost_fill_async(hipclaw_state_t* s, int k) {
     const GhostProgram& p = s->gp[k];
+    if (k == 0 && s->ghost_layout == 2 && s->fused_ok) return ghost_flat_async(s, true, false);
     hipclaw::GhostArgs g;
     g.q = s->q[s->cur];
     g.qti = s->qti;
     g.pstride = s->pstride;
     g.mx = s->cfg.mx; g.my = s->cfg.my; g.mbc = s->cfg.mbc; g.meqn = s->cfg.meqn;
-    const int lay = s->ghost_layout;
+    const int lay = s->ghost_layout == 2 ? (s->cfg.mx >= 32 ? 1 : 0) : s->ghost_layout;
     for (int l = 0; l < p.nlaunch[lay]; l++) {
         const GhostLaunch& L = p.launch[lay][l];
         g.ops = p.ops + L.start;
@@ -869,7 +1101,7 @@ static int set_tile(hipclaw_state_t* s, int t) {
 }
 
 int hipclaw_set_ghost_layout(hipclaw_state_t* s, int layout) {
-    if (!s || layout < 0 || layout > 1) return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_layout: bad arguments%s");
+    if (!s || layout < 0 || layout > 2) return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_layout: bad arguments%s");
     s->ghost_layout = layout;
     // a captured graph holds the launches of the old layout
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
@@ -878,7 +1110,11 @@ int hipclaw_set_ghost_layout(hipclaw_state_t* s, int layout) {
     return HIPCLAW_OK;
 }
 
-int hipclaw_ghost_layout(const hipclaw_state_t* s) { return s ? s->ghost_layout : -1; }
+int hipclaw_ghost_layout(const hipclaw_state_t* s) {
+    if (!s) return -1;
+    if (s->ghost_layout == 2 && !s->fused_ok) return s->cfg.mx >= 32 ? 1 : 0;
+    return s->ghost_layout;
+}
 
 static int effective_waves(const hipclaw_state_t* s) {
     if (s->waves) return s->waves;
@@ -916,8 +1152,9 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
     static const int candidates[3] = {8, 16, 32};
     for (int t : candidates) {
         if (!tile_allowed(s, t)) continue;
-        for (int trial = 0; trial < (t == 16 ? 4 : 2) && !rc; trial++) {
-            const int lay = trial & 1, w = t == 16 ? (trial < 2 ? 2 : 4) : 0;
+        const int nlay = s->fused_ok ? 3 : 2;
+        for (int trial = 0; trial < (t == 16 ? 2 * nlay : nlay) && !rc; trial++) {
+            const int lay = trial % nlay, w = t == 16 ? (trial < nlay ? 2 : 4) : 0;
             if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay)) || (rc = hipclaw_set_waves(s, w)))
                 break;
             // one untimed step (first use of these kernels), then the timed ones
@@ -1175,6 +1412,37 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
         }
     }
     if (int e = hipclaw::launch_reset_scalar(s->cfl, s->st)) return hipfail((hipError_t) e, "reset");
+    if (s->ghost_layout == 2 && s->fused_ok && !s->reg_accumulate) {
+        // two launches per step: the update (staging the corrected edge cells of
+        // the last synchronisation) and the fused ghost fill + time sync; the
+        // corrections of the last step are written into the slab at the end
+        for (int n = 0; n < nsteps; n++) {
+            double* src = s->q[s->cur];
+            double* dst = s->inplace ? src : s->q[1 - s->cur];
+            hipclaw::StepArgs a = step_args(s, src, dst, s->all_slots, s->npatch, dt);
+            if (s->fsync && n > 0) {
+                a.qcorr = s->qcorr;
+                a.cmask = s->cmask;
+                a.cstride = s->cstride;
+            }
+            // Pulling reads the neighbours' cells of the slab the update reads, so
+            // only when it writes the other one, and not in the first step: the
+            // slab then holds the last advance's synchronised interior cells, while
+            // its ghost cells are the copies made before that synchronisation.
+            const bool pull = s->pull_ok && !s->inplace;
+            if (pull && n > 0) a.pull = s->pull;
+            if (int rc = timed ? launch_step(s, a, s->kev[2 * n], s->kev[2 * n + 1]) : launch_step(s, a)) return rc;
+            if (!s->inplace) s->cur = 1 - s->cur;
+            if (int rc = ghost_flat_async(s, !pull || n == nsteps - 1, s->fsync)) return rc;
+        }
+        if (s->fsync && nsteps > 0) {
+            int e = hipclaw::launch_apply_qcorr(s->q[s->cur], s->qcorr, s->cmask, s->npatch, s->pstride,
+                                                s->cstride, s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn,
+                                                s->st);
+            if (e) return hipfail((hipError_t) e, "apply_qcorr launch");
+        }
+        return HIPCLAW_OK;
+    }
     for (int n = 0; n < nsteps; n++) {
         double* src = s->q[s->cur];
         double* dst = s->inplace ? src : s->q[1 - s->cur];
@@ -1433,6 +1701,12 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
         g.nlaunch[0] = g.nlaunch[1] = 0;
     }
     for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
+    s->gp0_set = false;
+    s->gp0_ops.clear();
+    s->sp0_ops.clear();
+    s->fused_ok = false;
+    s->fsync = false;
+    s->pull_ok = false;
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
     s->graph_steps = 0;

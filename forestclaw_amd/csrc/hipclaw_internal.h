@@ -63,6 +63,37 @@ struct SyncArgs {
     int fluxreg;         // registers hold one shared edge flux (FP32 flux form)
 };
 
+// Fused step (ghost layout 2).  What fills the ghost strip of each face of a
+// patch, taken from its ghost ops: every ghost cell the fused launch needs is
+// recomputed from interior cells through this table, never read from a ghost
+// cell another wave writes.
+enum { FREL_NONE = 0, FREL_SAME = 1, FREL_HALF = 2, FREL_DOUBLE = 3, FREL_PHYS = 4 };
+struct FaceRel {
+    int rel;
+    int nb0, ti0, tj0;   // same size: the neighbour; half size: the first child;
+                         // double size: the coarse patch and its interpolation's (ti, tj)
+    int nb1, ti1, tj1;   // half size: the second child
+    int bc;              // physical: mthbc of this face (1 or 3); double size: the
+                         // coarse patch's face toward this one
+};
+
+// One launch of the fused ghost fill and time-sync increments, one wave per
+// op: ghost ops [0, nghost) of g.ops, then sync ops [0, nsync) of sops.
+struct FlatArgs {
+    GhostArgs g;
+    int nghost;
+    const FaceRel* frel;       // [slot][4]
+    const int* sops;           // [nsync][8] as SyncArgs::ops
+    int nsync;
+    const double* reg;
+    const double* dxs;
+    const double* dys;
+    long long rstride;
+    double* qinc;              // [slot][face][m][pos] time-sync increments
+    long long cstride;
+    int fluxreg;
+};
+
 // Regrid op kinds (the forest's provenance kinds).
 enum { REGRID_SAME = 0, REGRID_CHILD = 1, REGRID_PARENT = 2 };
 
@@ -80,6 +111,11 @@ int launch_ghost_physbc(const GhostArgs& g, hipStream_t st);
 // Per-destination groups: a patch's exchange ops, then its physical BC;
 // groups[b] = (first op, end, physical-BC op or -1, -).
 int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hipStream_t st);
+// The fused ghost fill (+ time-sync increments into qinc) of one step.
+int launch_ghost_flat(const FlatArgs& f, hipStream_t st);
+// qcorr -> the interior edge cells of q (slots 0..n-1, faces in cmask).
+int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, int n, long long pstride,
+                       long long cstride, int mx, int my, int mbc, int meqn, hipStream_t st);
 // The stream of a step launch and, for a timed launch, two events that
 // hipExtLaunchKernelGGL binds to the kernel's own start and end, so their
 // elapsed time is the kernel's execution without the gaps around it.

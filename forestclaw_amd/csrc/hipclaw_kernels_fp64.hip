@@ -269,6 +269,272 @@ __global__ void __launch_bounds__(PHYSBC_THREADS) ghost_groups(GhostArgs g, cons
     }
 }
 
+// ---------------------------------------------------------------- fused ghost fill + sync
+// Cell (i, j) of patch `slot` as the ghost fill of this step leaves it, for an
+// interior cell or a face ghost cell that is not a corner and not filled by
+// interpolation: the neighbour's interior cell (same size), the average of the
+// fine child's four cells (fclaw2d_clawpatch46_average.f:61-174, the child
+// whose cells lie in its interior), or the physical BC of the patch's own
+// interior (clawpack46_bc2_default.f).  All of it reads interior cells only,
+// which no wave of the fused launch writes.
+__device__ double fused_cell_simple(const GhostArgs& g, const FaceRel* __restrict__ frel, int slot, int i,
+                                    int j, int m) {
+    const int mx = g.mx, my = g.my, mbc = g.mbc;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    const double* qc = g.q + (long long) slot * g.pstride;
+    const int k = i < 1 ? 0 : i > mx ? 1 : j < 1 ? 2 : j > my ? 3 : -1;
+    if (k < 0) return qc[QIX(i, j, m)];
+    const FaceRel r = frel[4 * slot + k];
+    if (r.rel == FREL_SAME) return g.q[(long long) r.nb0 * g.pstride + QIX(i + r.ti0, j + r.tj0, m)];
+    if (r.rel == FREL_HALF) {
+        int fi = 2 * i - 1 + r.ti0, fj = 2 * j - 1 + r.tj0, nb = r.nb0;
+        if (fi < 1 || fi + 1 > mx || fj < 1 || fj + 1 > my) {
+            fi = 2 * i - 1 + r.ti1;
+            fj = 2 * j - 1 + r.tj1;
+            nb = r.nb1;
+            // one of the two children covers every cell of the face strip
+            if (fi < 1 || fi + 1 > mx || fj < 1 || fj + 1 > my) return 0.0;
+        }
+        const double* qs = g.q + (long long) nb * g.pstride;
+        double sum = 0;
+        sum = sum + qs[QIX(fi, fj, m)];
+        sum = sum + qs[QIX(fi + 1, fj, m)];
+        sum = sum + qs[QIX(fi, fj + 1, m)];
+        sum = sum + qs[QIX(fi + 1, fj + 1, m)];
+        return sum / 4.0;
+    }
+    // FREL_PHYS (the host admits nothing else where this is called): extrapolation
+    // copies the first interior cell, a wall mirrors the strip and negates the
+    // normal momentum
+    const int mth = r.bc;
+    int is = i, js = j, mn;
+    if (k == 0) { is = mth == 1 ? 1 : 1 - i; mn = 1; }
+    else if (k == 1) { is = mth == 1 ? mx : 2 * mx + 1 - i; mn = 1; }
+    else if (k == 2) { js = mth == 1 ? 1 : 1 - j; mn = 2; }
+    else { js = mth == 1 ? my : 2 * my + 1 - j; mn = 2; }
+    const double x = qc[QIX(is, js, m)];
+    return (mth == 3 && m == mn) ? -x : x;
+}
+
+// Interpolated value of coarse cell (ic, jc) of patch c at child (a, b) of its
+// 2 x 2 fine cells (fclaw2d_clawpatch46_interpolate.f:120-137), the coarse
+// neighbours read through fused_cell_simple.
+__device__ __forceinline__ double interp_child(const GhostArgs& g, const FaceRel* __restrict__ frel, int c,
+                                               int ic, int jc, int m, int a, int b) {
+    const double cc = fused_cell_simple(g, frel, c, ic, jc, m);
+    const double gx = amr_slope(cc - fused_cell_simple(g, frel, c, ic - 1, jc, m),
+                                fused_cell_simple(g, frel, c, ic + 1, jc, m) - cc);
+    const double gy = amr_slope(cc - fused_cell_simple(g, frel, c, ic, jc - 1, m),
+                                fused_cell_simple(g, frel, c, ic, jc + 1, m) - cc);
+    return cc + gx * (a ? 0.25 : -0.25) + gy * (b ? 0.25 : -0.25);
+}
+
+// fused_cell_simple, plus the face strips filled by interpolation from a coarser
+// patch (recomputed from the coarse patch's cells).
+__device__ double fused_cell(const GhostArgs& g, const FaceRel* __restrict__ frel, int slot, int i, int j,
+                             int m) {
+    const int mx = g.mx, my = g.my;
+    const int k = i < 1 ? 0 : i > mx ? 1 : j < 1 ? 2 : j > my ? 3 : -1;
+    if (k >= 0) {
+        const FaceRel r = frel[4 * slot + k];
+        if (r.rel == FREL_DOUBLE) {
+            // fine cell (i, j) = child (a, b) of coarse cell (ic, jc):
+            // i = 2 ic - 1 + ti + a
+            const int u = i - r.ti0 + 1, v = j - r.tj0 + 1;
+            return interp_child(g, frel, r.nb0, u >> 1, v >> 1, m, u & 1, v & 1);
+        }
+    }
+    return fused_cell_simple(g, frel, slot, i, j, m);
+}
+
+// Interpolation op (dst = fine, src = coarse) as in ghost_exchange_op, its
+// coarse-side reads through fused_cell_simple.
+__device__ void fused_interp_op(const GhostArgs& g, const FaceRel* __restrict__ frel, const GhostOp& op,
+                                int t) {
+    const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    double* qd = g.q + (long long) op.dst * g.pstride;
+    const bool face = op.type == GHOST_INTERP_FACE;
+    const int hb = mbc / 2;
+    const int n = face ? meqn * hb * mx : meqn * hb * hb;
+    for (int e = t; e < n; e += 64) {
+        int m, ic, jc;
+        if (face) {
+            m = e / (hb * mx);
+            const int r = e - m * hb * mx, l = r / hb, bb = r - l * hb + 1;
+            if (op.which < 2) { ic = (op.which == 0) ? bb : mx - bb + 1; jc = 1 + l; }
+            else { ic = 1 + l; jc = (op.which == 2) ? bb : my - bb + 1; }
+        } else {
+            m = e / (hb * hb);
+            const int r = e - m * hb * hb, ibc = r / hb + 1, jbc = r % hb + 1;
+            ic = (op.which & 1) ? mx - ibc + 1 : ibc;
+            jc = (op.which & 2) ? my - jbc + 1 : jbc;
+        }
+        const int fi = 2 * ic - 1 + op.ti, fj = 2 * jc - 1 + op.tj;
+        if (face && (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc)) continue;
+        const int c = op.src;
+        const double cc = fused_cell_simple(g, frel, c, ic, jc, m);
+        const double gx = amr_slope(cc - fused_cell_simple(g, frel, c, ic - 1, jc, m),
+                                    fused_cell_simple(g, frel, c, ic + 1, jc, m) - cc);
+        const double gy = amr_slope(cc - fused_cell_simple(g, frel, c, ic, jc - 1, m),
+                                    fused_cell_simple(g, frel, c, ic, jc + 1, m) - cc);
+        const long long d0 = QIX(fi, fj, m);
+        qd[d0] = cc + gx * -0.25 + gy * -0.25;
+        qd[d0 + 1] = cc + gx * 0.25 + gy * -0.25;
+        qd[d0 + nxp] = cc + gx * -0.25 + gy * 0.25;
+        qd[d0 + nxp + 1] = cc + gx * 0.25 + gy * 0.25;
+    }
+}
+
+// Physical BC of one patch by one wave (clawpack46_bc2_default.f), each ghost
+// cell written once with its final value: the sources are interior cells or
+// ghost cells of a neighbouring face strip, recomputed through fused_cell, and
+// a corner both of whose faces are physical takes the y-face's value (the
+// reference applies faces 2 and 3 after faces 0 and 1).
+__device__ void fused_physbc_op(const GhostArgs& g, const FaceRel* __restrict__ frel, const GhostOp& op, int t) {
+    const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    double* qd = g.q + (long long) op.dst * g.pstride;
+    auto code = [&](int f) { return (int) (signed char) ((op.bc >> (8 * f)) & 0xff); };
+    const bool phys[4] = {code(0) == 1 || code(0) == 3, code(1) == 1 || code(1) == 3,
+                          code(2) == 1 || code(2) == 3, code(3) == 1 || code(3) == 3};
+    for (int face = 0; face < 4; face++) {
+        if (!phys[face]) continue;
+        const int mth = code(face);
+        const int len = face < 2 ? my + 2 * mbc : mx + 2 * mbc;
+        const int n = meqn * mbc * len;
+        for (int e = t; e < n; e += 64) {
+            const int m = e / (mbc * len), r = e - m * mbc * len;
+            const int l = r / mbc, b = r - l * mbc + 1;
+            int id, jd, is, js;
+            if (face < 2) {
+                jd = js = 1 - mbc + l;
+                // the corners of two physical faces are the y-face's
+                if ((jd < 1 && phys[2]) || (jd > my && phys[3])) continue;
+                if (face == 0) { id = 1 - b; is = (mth == 1) ? 1 : b; }
+                else { id = mx + b; is = (mth == 1) ? mx : mx + 1 - b; }
+            } else {
+                id = is = 1 - mbc + l;
+                if (face == 2) { jd = 1 - b; js = (mth == 1) ? 1 : b; }
+                else { jd = my + b; js = (mth == 1) ? my : my + 1 - b; }
+            }
+            const int mn = (face < 2) ? 1 : 2;  // normal momentum negated on walls
+            const double x = fused_cell(g, frel, op.dst, is, js, m);
+            qd[QIX(id, jd, m)] = (mth == 3 && m == mn) ? -x : x;
+        }
+    }
+}
+
+// Time-sync increments of one op (as time_sync_cells): a same-size op writes
+// every position of its face, a coarse/fine op the positions its fine patch
+// covers (the other child's op writes the rest; the reference adds zero for
+// the positions a child does not cover).
+__device__ void fused_sync_op(const FlatArgs& f, const int* __restrict__ op, int t) {
+    const int mx = f.g.mx, meqn = f.g.meqn;
+    const int type = op[0], dst = op[1], src = op[2], face = op[3], ti = op[4], tj = op[5];
+    const double area = f.dxs[dst] * f.dys[dst];
+    const double* __restrict__ rt = f.reg + (long long) dst * f.rstride;
+    const double* __restrict__ rn = f.reg + (long long) src * f.rstride;
+    double* __restrict__ qo = f.qinc + (long long) dst * f.cstride;
+    const int kout = f.fluxreg ? REG_EFIN : REG_EFOUT;
+    const int kn = face ^ 1;
+    const int kfl = (face == 0 || face == 2) ? REG_FP : REG_FM;
+    for (int e = t; e < meqn * mx; e += 64) {
+        const int m = e / mx, c = e - m * mx + 1;
+        auto R = [&](const double* rr, int kk, int kind, int pos) -> double {
+            return rr[((kk * 4 + kind) * meqn + m) * mx + pos];
+        };
+        const double ein = R(rt, face, REG_EFIN, c - 1);
+        const double ft = f.fluxreg ? 0.0 : R(rt, face, kfl, c - 1);
+        double v;
+        if (type == SYNC_SAMESIZE) {
+            const double ef = R(rn, kn, kout, c - 1);
+            const double fn = f.fluxreg ? 0.0 : R(rn, kn, kfl, c - 1);
+            const double delta = (face == 0 || face == 2) ? (ef - ein) - (fn - ft) : (ein - ef) - (fn - ft);
+            v = 0.5 * delta / area;
+        } else {
+            const int f0 = 2 * c - 1 + (face < 2 ? tj : ti);
+            if (f0 < 1 || f0 + 1 > mx) continue;
+            const double ef = R(rn, kn, kout, f0 - 1) + R(rn, kn, kout, f0);
+            const double flu = f.fluxreg ? 0.0 : R(rn, kn, kfl, f0 - 1) + R(rn, kn, kfl, f0);
+            const double deltac = (face == 0 || face == 2) ? (ef - ein) - (flu - ft) : (ein - ef) - (flu - ft);
+            v = deltac / area;
+        }
+        qo[(face * meqn + m) * mx + c - 1] = 0.0 + v;
+    }
+}
+
+// The fused launch, one wave per op, no op depending on another: ghost ops
+// (copies and averages read interior cells; interpolations and physical BCs
+// recompute the ghost cells they read), then time-sync increments.  The
+// interior cells keep their values before the time sync, which every ghost op
+// reads (the reference fills ghost cells before it synchronises,
+// src/fclaw2d_advance.c:330-341); the next update adds the increments to the
+// cells it stages.
+constexpr int FLAT_OPW = 4;
+__global__ void __launch_bounds__(64 * FLAT_OPW) ghost_flat(FlatArgs f) {
+    const int o = blockIdx.x * FLAT_OPW + (int) (threadIdx.x >> 6);
+    const int t = threadIdx.x & 63;
+    if (o < f.nghost) {
+        const GhostOp op = f.g.ops[o];
+        if (op.type == GHOST_INTERP_FACE || op.type == GHOST_INTERP_CORNER) fused_interp_op(f.g, f.frel, op, t);
+        else if (op.type == GHOST_PHYSBC) fused_physbc_op(f.g, f.frel, op, t);
+        else ghost_exchange_op(f.g, op, t);
+    } else if (o - f.nghost < f.nsync) {
+        fused_sync_op(f, f.sops + 8 * (o - f.nghost), t);
+    }
+}
+
+// The pending increments added into the slab (after the last step of an
+// advance, so the state is the synchronised one), one thread per edge cell in
+// the update's order: a corner cell belongs to its x-face when that face is
+// corrected, and takes both faces' increments in the reference's order.
+__global__ void apply_qcorr(double* __restrict__ q, const double* __restrict__ qcorr,
+                            const int* __restrict__ cmask, int n, long long pstride, long long cstride,
+                            int mx, int my, int mbc, int meqn) {
+    const int per = 4 * meqn * mx;
+    const long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long) n * per) return;
+    const int slot = (int) (e / per), r = (int) (e - (long long) slot * per);
+    const int fc = r / (meqn * mx), r2 = r - fc * meqn * mx, m = r2 / mx, c = r2 - m * mx + 1;
+    const int mask = cmask[slot];
+    if (!((mask >> fc) & 1)) return;
+    const int len = fc < 2 ? my : mx;
+    const int other = (c == 1 || c == len) ? (fc < 2 ? (c == 1 ? 2 : 3) : (c == 1 ? 0 : 1)) : -1;
+    const bool other_on = other >= 0 && ((mask >> other) & 1);
+    if (fc >= 2 && other_on) return;
+    const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
+    const int i = fc == 0 ? 1 : fc == 1 ? mx : c, j = fc == 2 ? 1 : fc == 3 ? my : c;
+    const double* qc = qcorr + (long long) slot * cstride;
+    const double v = qc[(fc * meqn + m) * len + c - 1];
+    double* cell = q + (long long) slot * pstride + QIX(i, j, m);
+    double x = *cell;
+    if (other_on) {
+        const double w = qc[(other * meqn + m) * mx + (fc == 0 ? 0 : mx - 1)];
+        if (sync_y_first(mask, fc, other)) { x = x + w; x = x + v; }
+        else { x = x + v; x = x + w; }
+    } else {
+        x = x + v;
+    }
+    *cell = x;
+}
+
+int launch_ghost_flat(const FlatArgs& f, hipStream_t st) {
+    const int n = f.nghost + f.nsync;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(ghost_flat, dim3((n + FLAT_OPW - 1) / FLAT_OPW), dim3(64 * FLAT_OPW), 0, st, f);
+    return (int) hipGetLastError();
+}
+
+int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, int n, long long pstride,
+                       long long cstride, int mx, int my, int mbc, int meqn, hipStream_t st) {
+    const long long total = (long long) n * 4 * meqn * mx;
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(apply_qcorr, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, st, q, qcorr, cmask,
+                       n, pstride, cstride, mx, my, mbc, meqn);
+    return (int) hipGetLastError();
+}
+
 __global__ void reset_scalar(unsigned long long* p) { *p = 0ull; }
 
 // ---------------------------------------------------------------- regrid

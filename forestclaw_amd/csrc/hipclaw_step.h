@@ -62,7 +62,29 @@ struct StepArgs {
     long long rstride;
     int reg_accumulate;   // add to the registers (subcycling) instead of overwriting them
     const double* dtscale;  // per-slot factor on dt (an ensemble of runs), or null
+    // Deferred conservation fix-up (null: none pending).  qcorr[slot][face][m][pos]
+    // holds the time-sync increment of interior edge cell pos of `face` for the
+    // faces set in bits 0-3 of cmask[slot] (bits 4-7: that face's increment
+    // comes from a coarse/fine op); qsrc still holds the values before the
+    // fix-up, and staging adds the increments (sync_corner_order).
+    const double* qcorr = nullptr;
+    const int* cmask = nullptr;
+    long long cstride = 0;
+    // Ghost strips staged straight from a same-size neighbour (null: every halo
+    // cell from the patch's own slab).  pull[slot][8]: for faces 0..3, then
+    // corners 0..3, the offset in doubles from a ghost cell of `slot` to the
+    // neighbour's interior cell it copies (fclaw2d_clawpatch46_copy.f), 0 where
+    // the ghost cells come from the patch's own slab.  Only with two slabs.
+    const int* pull = nullptr;
 };
+
+// The reference applies a patch's same-size corrections (faces 0..3) before
+// its coarse/fine ones (src/fclaw2d_time_sync.c:126-151), so a corner cell
+// takes its y-face increment first only when its x-face is coarse/fine and its
+// y-face same-size.
+__host__ __device__ __forceinline__ bool sync_y_first(int mask, int fx, int fy) {
+    return ((mask >> (4 + fx)) & 1) && !((mask >> (4 + fy)) & 1);
+}
 
 enum { REG_FM = 0, REG_FP = 1, REG_EFIN = 2, REG_EFOUT = 3 };
 
@@ -165,8 +187,9 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
     return kn && rt;
 }
 
-// At least 4 waves per SIMD: the FP64 kernels that keep time-sync registers
-// would otherwise settle at 3 (measured faster at 4 despite a few spills).
+// Waves per SIMD the register allocation aims for: 4, except FP64 16 x 16
+// tiles, which need more than the 128 VGPRs of 4 waves and are held to the 168
+// of 3 (without the target the allocator let them grow to 2 waves).
 //
 // LIM = 0 reads each wave family's limiter from a.mthlim at run time; LIM > 0
 // is that limiter for every family, fixed at compile time.  The run-time
@@ -179,7 +202,7 @@ constexpr int step_waves_per_eu() {
 #ifdef HIPCLAW_WPE_F32_T16
     if (sizeof(R) == 4 && T == 16) return HIPCLAW_WPE_F32_T16;
 #endif
-    return 4;
+    return (sizeof(R) == 8 && T == 16) ? 3 : 4;
 }
 
 template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS, int LIM = 0>
@@ -229,6 +252,9 @@ step_tiles(StepArgs a) {
         goff[kk] = e < NQ ? (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2 : -1;
         loff[kk] = e < NQ ? m * QP + qj * QS + qi : -1;
     }
+    // Deferred fix-up: entries (face, field m, position 0..QW-1 along the strip)
+    // of the tile's four interior edge strips.
+    constexpr int NEDGE = 4 * MEQN * QW;
     int gaoff[KA], laoff[KA];
 #pragma unroll
     for (int kk = 0; kk < KA; kk++) {
@@ -239,6 +265,22 @@ step_tiles(StepArgs a) {
         laoff[kk] = (MAUX > 0 && e < NA) ? m * QP + qj * QS + qi : -1;
     }
 
+    // Side of the tile's halo each staged element lies on: code = sx + 3 sy,
+    // sx = 1 (left of the tile) / 2 (right) / 0, sy = 1 (below) / 2 (above) / 0,
+    // four bits per element; s_pull[code] is the tile's offset for that side
+    // (a patch face or corner the tile touches), 0 for the patch's own cells.
+    static_assert(KQ <= 16, "staging codes hold 16 elements");
+    unsigned long long pcode = 0;
+#pragma unroll
+    for (int kk = 0; kk < KQ; kk++) {
+        const int e = threadIdx.x + kk * NT;
+        const int m = e / (QW * QW), rem = e - m * QW * QW;
+        const int qj = rem / QW, qi = rem - qj * QW;
+        const int sx = qi < 2 ? 1 : qi > T + 1 ? 2 : 0, sy = qj < 2 ? 1 : qj > T + 1 ? 2 : 0;
+        if (e < NQ) pcode |= (unsigned long long) (sx + 3 * sy) << (4 * kk);
+    }
+    __shared__ int s_pull[9];
+
     // prefetch registers for the next tile
     double pq[KQ];
     R pa[KA];
@@ -247,8 +289,14 @@ step_tiles(StepArgs a) {
         const int slot = tile_slot(tile, tx, ty);
         const long long shift = (long long) (ty * T) * nxp + tx * T;  // tile origin in the slab
         const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride + shift;
+        if (a.pull) {
 #pragma unroll
-        for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
+            for (int kk = 0; kk < KQ; kk++)
+                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode >> (4 * kk)) & 15]] : 0.0;
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
+        }
         if constexpr (MAUX > 0) {
             const double* __restrict__ ap = a.aux + (long long) slot * a.astride + shift;
 #pragma unroll
@@ -273,8 +321,31 @@ step_tiles(StepArgs a) {
     // that, in every round of the persistent loop, each XCD works on one
     // contiguous run of tiles: neighbouring tiles of a patch share halo cells,
     // which then come from the same L2.
+    // The offsets of tile t for s_pull (threads 0..8, one side each): a side
+    // on a patch face takes that face's offset, one on a patch corner the
+    // corner's, a side inside the patch 0.
+    auto pull_table = [&](int t) {
+        if (threadIdx.x < 9) {
+            int tx, ty;
+            const int slot = tile_slot(t, tx, ty);
+            const int* __restrict__ pt = a.pull + 8 * (long long) slot;
+            const int sx = threadIdx.x % 3, sy = threadIdx.x / 3;
+            const bool ex = (sx == 1 && tx == 0) || (sx == 2 && tx == a.ntx - 1);
+            const bool ey = (sy == 1 && ty == 0) || (sy == 2 && ty == a.nty - 1);
+            int d = 0;
+            if (ex && ey) d = pt[4 + (sx == 2 ? 1 : 0) + (sy == 2 ? 2 : 0)];
+            else if (ex) d = pt[sx == 2 ? 1 : 0];
+            else if (ey) d = pt[sy == 2 ? 3 : 2];
+            s_pull[threadIdx.x] = d;
+        }
+    };
+
     const int G = gridDim.x;
     int tile = (G % 8 == 0) ? (int) (blockIdx.x % 8) * (G / 8) + (int) (blockIdx.x / 8) : (int) blockIdx.x;
+    if (a.pull && tile < ntiles) {
+        pull_table(tile);
+        __syncthreads();
+    }
     if (tile < ntiles) prefetch(tile);
 
     for (; tile < ntiles; tile += G) {
@@ -287,6 +358,8 @@ step_tiles(StepArgs a) {
         const bool first_x = tx == 0, last_x = tx == a.ntx - 1;
         const bool first_y = ty == 0, last_y = ty == a.nty - 1;
 
+        // the next tile's offsets, read by its prefetch after the staging barrier
+        if (a.pull && tile + G < ntiles) pull_table(tile + G);
         // ---- stage 0: the prefetched tile + 2-cell halo into LDS ----
 #pragma unroll
         for (int kk = 0; kk < KQ; kk++)
@@ -297,6 +370,44 @@ step_tiles(StepArgs a) {
                 if (laoff[kk] >= 0) sm.aux[laoff[kk]] = pa[kk];
         }
         __syncthreads();
+        if (a.qcorr) {
+            // the pending fix-up of the last synchronisation: each corrected edge
+            // cell of the staged tile plus its increments, in the reference's
+            // order (one thread per cell; a corner cell belongs to its x-face
+            // entry when that face is corrected)
+            const int mask = a.cmask[slot];
+            const int tb = ((int) first_x | ((int) last_x << 1) | ((int) first_y << 2) | ((int) last_y << 3)) & mask;
+            if (tb) {
+                const double* __restrict__ qc = a.qcorr + (long long) slot * a.cstride;
+#pragma unroll 1
+                for (int x = threadIdx.x; x < NEDGE; x += NT) {
+                    const int face = x / (MEQN * QW), r = x - face * (MEQN * QW);
+                    const int m = r / QW, q = r - m * QW;   // tile index along the strip
+                    if (!((tb >> face) & 1)) continue;
+                    // patch cell along the strip: j (faces 0, 1) or i (faces 2, 3), 1-based
+                    const int len = face < 2 ? my : mx;
+                    const int c = (face < 2 ? ty : tx) * T + q - 1;
+                    if (c < 1 || c > len) continue;
+                    const int other = (c == 1 || c == len) ? (face < 2 ? (c == 1 ? 2 : 3) : (c == 1 ? 0 : 1)) : -1;
+                    const bool other_on = other >= 0 && ((mask >> other) & 1);
+                    if (face >= 2 && other_on) continue;
+                    const int qi = face == 0 ? 2 : face == 1 ? T + 1 : q;
+                    const int qj = face == 2 ? 2 : face == 3 ? T + 1 : q;
+                    double& cell = sm.q[m * QP + qj * QS + qi];
+                    const double v = qc[(face * MEQN + m) * len + c - 1];
+                    double y = cell;
+                    if (other_on) {
+                        const double w = qc[(other * MEQN + m) * mx + (face == 0 ? 0 : mx - 1)];
+                        if (sync_y_first(mask, face, other)) { y = y + w; y = y + v; }
+                        else { y = y + v; y = y + w; }
+                    } else {
+                        y = y + v;
+                    }
+                    cell = y;
+                }
+            }
+            __syncthreads();
+        }
         if constexpr (REGS && !FLUXFORM) {
             double* __restrict__ rg = a.reg + (long long) slot * a.rstride;
             const double dtdy_el = dt * a.dys[slot], dtdx_el = dt * a.dxs[slot];

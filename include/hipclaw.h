@@ -264,8 +264,17 @@ int hipclaw_restore_step(hipclaw_state_t *s);
 /* Launch layout of the ghost fill, the same ops either way: 0 = one launch
  * per phase (average + copy, physical BC, interpolate, physical BC), 1 = two
  * launches of per-destination groups (a patch's exchange ops, then its
- * physical BC).  The default is 1 for mx >= 32, else 0.  No counterpart in
- * the reference, which fills ghost cells patch by patch on the host. */
+ * physical BC), 2 = fused: one launch per step of per-patch groups in which
+ * interpolation recomputes the coarse patch's face ghost cells it reads from
+ * their sources (same-size neighbour, fine children, physical BC), and, with
+ * time sync, the corrected edge values go to a side buffer that the next
+ * update stages (the last step's are written into q at the end of
+ * hipclaw_advance), so a step is two launches.  The default is 2; where the
+ * program cannot be fused (time-interpolated slots, subcycling)
+ * hipclaw_ghost_layout reports and uses 1 for mx >= 32, else 0.  No
+ * counterpart in the reference, which fills ghost cells patch by patch on
+ * the host (src/fclaw2d_ghost_fill.c:611-830) and then synchronises
+ * (src/fclaw2d_time_sync.c:157-202). */
 int hipclaw_set_ghost_layout(hipclaw_state_t *s, int layout);
 int hipclaw_ghost_layout(const hipclaw_state_t *s);
 
@@ -278,12 +287,16 @@ int hipclaw_ghost_layout(const hipclaw_state_t *s);
 int hipclaw_set_waves(hipclaw_state_t *s, int waves);
 int hipclaw_waves(const hipclaw_state_t *s);
 
+/* Most trials of one hipclaw_autotune call. */
+#define HIPCLAW_AUTOTUNE_MAX_TRIALS 12
+
 /* Autotuner: time nsteps full steps (update, ghost fill, time sync) of dt on
  * the current patches for every tile size the state's solver, precision and
  * patch size allow, with 2 and 4 waves per workgroup for 16 x 16 tiles, in
- * both ghost-fill layouts, each trial from the state saved at entry and
- * restored after it (the saved state of hipclaw_save_step is overwritten),
- * and keep the fastest.  tiles[k], waves[k], layouts[k], ms[k] (room for 8)
+ * every ghost-fill layout the program allows, each trial from the state saved
+ * at entry and restored after it (the saved state of hipclaw_save_step is
+ * overwritten), and keep the fastest.  tiles[k], waves[k], layouts[k], ms[k]
+ * (room for HIPCLAW_AUTOTUNE_MAX_TRIALS)
  * receive each trial and its mean step time, *ntrials their count.  Not
  * with subcycling.  No counterpart in the reference, whose CUDA kernels have
  * a fixed block size. */

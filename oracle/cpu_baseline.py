@@ -89,11 +89,28 @@ def _worker_port(args):
     return time.perf_counter() - t0
 
 
+# Host cores a GPU box allots to one GPU (its worker pools are sized to this
+# share, whatever os.cpu_count() reports there).
+HOST_CORES_PER_GPU = 16
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def measure(solver, q, aux, dx, dt, mwaves, maux, grav=1.0, gamma=1.4, nproc=None,
             target_seconds=10.0):
-    """Returns dict(value=cell updates/s, cores=P, kind, sample=str)."""
+    """Returns dict(value=cell updates/s, cores=P, kind, sample=str): the patch
+    step only (clawpack46_step2 with its Riemann solvers), no ghost fill and no
+    time synchronisation."""
     _, kind = _load()
-    nproc = nproc or max(1, min(16, os.cpu_count() or 1))
+    nproc = nproc or max(1, min(HOST_CORES_PER_GPU, os.cpu_count() or 1))
     npatch = q.shape[0]
     mx = q.shape[3] - 4
     # calibrate: one pass over a few patches in this process
@@ -123,7 +140,10 @@ def measure(solver, q, aux, dx, dt, mwaves, maux, grav=1.0, gamma=1.4, nproc=Non
               grav, gamma, _NoBarrier()))
     single = npatch * mx * mx * reps1 / t1
     return dict(value=cells / wall, unit="cell-updates/s", cores=nproc, kind=kind,
-                single_process_value=single,
+                single_process_value=single, cpu_model=cpu_model(),
+                host_cpus_visible=os.cpu_count(),
+                covers="the patch update only (clawpack46_step2_wrap with the Roe solvers); "
+                       "no ghost fill, no time synchronisation",
                 sample="%d patches of %dx%d x %d repetitions of one Clawpack-4.6 step "
                        "(%s), %d processes, %.1f s wall; one process: %d repetitions, %.1f s"
                        % (npatch, mx, mx, reps,

@@ -149,22 +149,27 @@ def face_table(ops, npatch):
     return rel
 
 
-def run_fused(q, ops, phase_start, mx, my, mbc, order=None):
-    """The fused launch's semantics: every patch on its own, in any order
-    (`order`, default reversed): its averages and copies, then its
-    interpolations reading the coarse patch's face ghost cells as recomputed
-    from the face table (never the array, which other patches' groups are
-    writing), then its physical BC once.  Reads of interior cells see the
-    values before the fill (no group writes an interior cell)."""
+def run_flat(q, ops, phase_start, mx, my, mbc):
+    """The semantics of the fused launch (ghost_flat) with every op: each ghost
+    cell computed from interior cells only -- copies and averages read interior
+    cells; interpolation reads the coarse patch's face strips recomputed from the
+    face table; the physical BC (once per patch) reads its sources the same way,
+    an interpolated strip included, and writes each cell once (a corner of two
+    physical faces takes the y-face's value)."""
     npatch, meqn = q.shape[0], q.shape[1]
-    src_q = q.copy()        # interiors (and nothing else) are read from here
+    src_q = q.copy()
     o = mbc - 1
     rel = face_table(ops, npatch)
+    interp_src = {}
+    for op in ops:
+        t, dst, src, which, ti, tj, bc, _ = (int(v) for v in op)
+        if t == INTERP_FACE:
+            interp_src[(dst, which ^ 1)] = (src, ti, tj)
 
     def Qs(p, i, j, m):
         return src_q[p, m, j + o, i + o]
 
-    def cell(c, i, j, m):
+    def simple(c, i, j, m):
         k = 0 if i < 1 else 1 if i > mx else 2 if j < 1 else 3 if j > my else -1
         if k < 0:
             return Qs(c, i, j, m)
@@ -194,22 +199,38 @@ def run_fused(q, ops, phase_start, mx, my, mbc, order=None):
         v = Qs(c, i2, j2, m)
         return -v if (mth == 3 and m == mn) else v
 
-    groups = {}
+    def interp_child(c, ic, jc, m, a, b):
+        cc = simple(c, ic, jc, m)
+        gx = _slope(cc - simple(c, ic - 1, jc, m), simple(c, ic + 1, jc, m) - cc)
+        gy = _slope(cc - simple(c, ic, jc - 1, m), simple(c, ic, jc + 1, m) - cc)
+        return cc + gx * (0.25 if a else -0.25) + gy * (0.25 if b else -0.25)
+
+    def cell(p, i, j, m):
+        k = 0 if i < 1 else 1 if i > mx else 2 if j < 1 else 3 if j > my else -1
+        if k >= 0 and rel[p][k] is not None and rel[p][k][0] == 'double':
+            c, ti, tj = interp_src[(p, k)]
+            u, v = i - ti + 1, j - tj + 1
+            return interp_child(c, u >> 1, v >> 1, m, u & 1, v & 1)
+        return simple(p, i, j, m)
+
+    bc_last = {}
     for ph in range(6):
         for k in range(phase_start[ph], phase_start[ph + 1]):
-            dst = int(ops[k][1])
-            g = groups.setdefault(dst, ([], [], []))
-            (g[0] if ph <= 1 else g[1] if ph == 3 else g[2]).append(k)
-    dsts = sorted(groups, reverse=True) if order is None else order
-    for dst in dsts:
-        a_ops, i_ops, bc_ops = groups[dst]
-        # averages and copies (interior reads only) -- the sequential emulator per op
-        sub = np.array([ops[k] for k in a_ops], dtype=np.int32).reshape(-1, 8)
-        tmp = src_q.copy()
-        run(tmp, sub, [0, len(sub), len(sub), len(sub), len(sub), len(sub), len(sub)], mx, my, mbc)
-        q[dst] = tmp[dst]
-        for k in i_ops:
-            t, _, c, which, ti, tj, _, _ = (int(v) for v in ops[k])
+            if int(ops[k][0]) == PHYSBC:
+                bc_last[int(ops[k][1])] = k
+    for k, op in enumerate(ops):
+        t, dst, src, which, ti, tj, bc, _ = (int(v) for v in op)
+        if t in (COPY_FACE, COPY_CORNER, AVG_FACE, AVG_CORNER):
+            tmp = src_q.copy()
+            run(tmp, np.array([op], dtype=np.int32), [0, 1, 1, 1, 1, 1, 1], mx, my, mbc)
+            changed = tmp[dst] != src_q[dst]
+            # the op's cells: wherever the sequential op wrote (compare with NaN marks)
+            mark = src_q.copy()
+            mark[dst] = np.nan
+            run(mark, np.array([op], dtype=np.int32), [0, 1, 1, 1, 1, 1, 1], mx, my, mbc)
+            w = ~np.isnan(mark[dst])
+            q[dst][w] = tmp[dst][w]
+        elif t in (INTERP_FACE, INTERP_CORNER):
             hb = mbc // 2
             cells = []
             if t == INTERP_FACE:
@@ -229,14 +250,34 @@ def run_fused(q, ops, phase_start, mx, my, mbc, order=None):
                     fi, fj = 2 * ic - 1 + ti, 2 * jc - 1 + tj
                     if fi < 1 - mbc or fi + 1 > mx + mbc or fj < 1 - mbc or fj + 1 > my + mbc:
                         continue
-                    cc = cell(c, ic, jc, m)
-                    gx = _slope(cc - cell(c, ic - 1, jc, m), cell(c, ic + 1, jc, m) - cc)
-                    gy = _slope(cc - cell(c, ic, jc - 1, m), cell(c, ic, jc + 1, m) - cc)
-                    for a, b, sx, sy in ((0, 0, -0.25, -0.25), (1, 0, 0.25, -0.25),
-                                         (0, 1, -0.25, 0.25), (1, 1, 0.25, 0.25)):
-                        q[dst, m, fj + b + o, fi + a + o] = cc + gx * sx + gy * sy
-        if bc_ops:
-            last = ops[bc_ops[-1]]
-            sub = np.array([last], dtype=np.int32)
-            run(q, sub, [0, 0, 0, 0, 0, 0, 1], mx, my, mbc)
+                    for a in (0, 1):
+                        for b in (0, 1):
+                            q[dst, m, fj + b + o, fi + a + o] = interp_child(src, ic, jc, m, a, b)
+        elif t == PHYSBC and bc_last.get(dst) == k:
+            phys = [_bc_code(bc, f) in (1, 3) for f in range(4)]
+            for face in range(4):
+                if not phys[face]:
+                    continue
+                mth = _bc_code(bc, face)
+                for m in range(meqn):
+                    for b in range(1, mbc + 1):
+                        n_l = my + 2 * mbc if face < 2 else mx + 2 * mbc
+                        for l in range(n_l):
+                            if face < 2:
+                                jd = js = 1 - mbc + l
+                                if (jd < 1 and phys[2]) or (jd > my and phys[3]):
+                                    continue
+                                if face == 0:
+                                    idd, is_ = 1 - b, (1 if mth == 1 else b)
+                                else:
+                                    idd, is_ = mx + b, (mx if mth == 1 else mx + 1 - b)
+                            else:
+                                idd = is_ = 1 - mbc + l
+                                if face == 2:
+                                    jd, js = 1 - b, (1 if mth == 1 else b)
+                                else:
+                                    jd, js = my + b, (my if mth == 1 else my + 1 - b)
+                            mn = 1 if face < 2 else 2
+                            v = cell(dst, is_, js, m)
+                            q[dst, m, jd + o, idd + o] = -v if (mth == 3 and m == mn) else v
     return q

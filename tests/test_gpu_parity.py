@@ -97,7 +97,7 @@ def _small_problems():
             P.shockbubble_problem(1, 3, mx=8)]
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_ghost_fill_fp64(gpu, k, layout):
     """Device ghost fill (copy/average/interpolate/physical BC) vs the oracle,
@@ -114,7 +114,7 @@ def test_ghost_fill_fp64(gpu, k, layout):
     np.testing.assert_array_equal(qg, qo)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_advance_fp64_amr(gpu, k, layout):
     """Ten steps of (update all levels, ghost fill) on an AMR forest, in both
@@ -151,7 +151,7 @@ def test_step_batch_per_level(gpu):
     assert abs(cfl - cfl_o) <= TOL * cfl_o
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_advance_two_slabs(gpu, layout):
     """Two slabs (16x16 patches, tile 8: the update writes the other slab)
     with either ghost-fill layout, bit-identical to the oracle with and

@@ -29,7 +29,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
-@pytest.mark.parametrize("tile,layout", [(0, 0), (8, 1)])
+@pytest.mark.parametrize("tile,layout", [(0, 0), (8, 1), (0, 2), (8, 2)])
 def test_fp64_time_sync_matches_oracle(gpu, case, tile, layout):
     prob = CASES[case]()
     forest, q0, aux, sv = setup_problem(prob, tile=tile, time_sync=True)
