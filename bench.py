@@ -317,6 +317,35 @@ def profile_run(fa, prob, forest, q0, aux, dt, tiles, time_sync, dt_scale=None):
     print("calibration copies: 3 x %d bytes" % COPY_BYTES, flush=True)
 
 
+def _profiled_rows(rows):
+    """The dispatches of the profiled steps: those that start after the last
+    regrid of the run that grew the forest (all of them for the t = 0 forest)."""
+    cut = max((int(r["End_Timestamp"]) for r in rows if "regrid_remap" in r["Kernel_Name"]), default=-1)
+    return [r for r in rows if int(r["Start_Timestamp"]) > cut]
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob as G
+    rows = []
+    for f in G.glob(os.path.join(d, "**", "*" + suffix), recursive=True):
+        rows.extend(csv.DictReader(open(f)))
+    return rows
+
+
+def trace_summary(trace_dir):
+    """Per-kernel dispatch count and average duration of the profiled steps of
+    a rocprofv3 --kernel-trace run of --profile-run."""
+    per = {}
+    for r in _profiled_rows(_csv_rows(trace_dir, "kernel_trace.csv")):
+        per.setdefault(r["Kernel_Name"].split("(")[0][:120], []).append(
+            int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    lines = ["%-100s %6s %10s" % ("kernel", "calls", "avg_us")]
+    for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        lines.append("%-100s %6d %10.2f" % (k, len(v), sum(v) / len(v) / 1e3))
+    return "\n".join(lines)
+
+
 def pmc_summary(fetch_dir, write_dir, cells):
     """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of every kernel in two
     rocprofv3 --pmc runs of --profile-run, calibrated against the copy kernel:
@@ -326,19 +355,19 @@ def pmc_summary(fetch_dir, write_dir, cells):
 
     def load(d, counter):
         out = {}
-        for f in G.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            for r in csv.DictReader(open(f)):
-                if r["Counter_Name"] != counter:
-                    continue
-                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        for r in _profiled_rows(_csv_rows(d, "counter_collection.csv")):
+            if r["Counter_Name"] != counter:
+                continue
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
         return out
 
     fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
     copy = [k for k in fetch if "copyBuffer" in k]
     if not copy:
         raise RuntimeError("no copy kernel in the counter runs")
-    nf = sum(fetch[copy[0]]) * 1024.0   # bytes reported over all copy dispatches
-    nw = sum(write[copy[0]]) * 1024.0
+    # the three calibration copies are the largest copy dispatches
+    nf = sum(sorted(fetch[copy[0]])[-3:]) * 1024.0
+    nw = sum(sorted(write[copy[0]])[-3:]) * 1024.0
     f_fetch = 3 * COPY_BYTES / nf
     f_write = 3 * COPY_BYTES / nw
     lines = ["calibration: 3 copies of %d bytes; FETCH_SIZE factor %.3f, WRITE_SIZE factor %.3f"
@@ -371,13 +400,12 @@ def valu_summary(pmc_dir, cells):
     import glob as G
     per = {}
     span = {}
-    for f in G.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
-        for r in csv.DictReader(open(f)):
-            k = r["Kernel_Name"]
-            if "step_tiles" not in k:
-                continue
-            per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-            span.setdefault(k, {})[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    for r in _profiled_rows(_csv_rows(pmc_dir, "counter_collection.csv")):
+        k = r["Kernel_Name"]
+        if "step_tiles" not in k:
+            continue
+        per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        span.setdefault(k, {})[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
     lines = ["counters: %s (one run); clock %.1f GHz, %d SIMDs" % (VALU_COUNTERS, CLOCK_GHZ, SIMDS)]
     for k in sorted(per):
         c = {n: sum(v) / len(v) for n, v in per[k].items()}
@@ -402,11 +430,10 @@ def counter_summary(pmc_dir, cells):
     import csv
     import glob as G
     per = {}
-    for f in G.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
-        for r in csv.DictReader(open(f)):
-            k = r["Kernel_Name"]
-            if "step_tiles" in k:
-                per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    for r in _profiled_rows(_csv_rows(pmc_dir, "counter_collection.csv")):
+        k = r["Kernel_Name"]
+        if "step_tiles" in k:
+            per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
     lines = []
     for k in sorted(per):
         c = {n: sum(v) / len(v) for n, v in per[k].items()}
@@ -509,6 +536,12 @@ def main():
                     help="every counter of a rocprofv3 --pmc run of --profile-run, per step kernel")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs of --profile-run")
+    ap.add_argument("--trace-summary", metavar="TRACE_DIR",
+                    help="per-kernel averages of the profiled steps of a rocprofv3 --kernel-trace "
+                         "run of --profile-run")
+    ap.add_argument("--cells", type=int, default=None,
+                    help="cells of the profiled forest, for the summaries (the --profile-run "
+                         "output prints it)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -560,10 +593,13 @@ def main():
         # each run at its own stable dt: step dt 1.0 times a per-slot scale
         dt_scale, dt_run = P.ensemble_dt(prob, forest, q0)
         dt = 1.0
+    elif forest_kind == "grown" and (args.pmc_summary or args.valu_summary or args.counter_summary
+                                     or args.trace_summary):
+        if args.cells is None:
+            sys.exit("bench.py: a summary of the grown forest needs --cells (printed by --profile-run)")
+        forest = q0 = aux = None
+        dt = 0.0
     elif forest_kind == "grown":
-        if args.pmc_summary or args.valu_summary or args.counter_summary:
-            sys.exit("bench.py: the summaries need the forest of their run: pass --forest initial "
-                     "to both, or summarise from the kernel counts of a grown run")
         fa.initialize_device(0)
         forest, q0, aux, grown = grow_forest(fa, P, prob)
         print("grown forest: %s" % json.dumps(grown), file=sys.stderr, flush=True)
@@ -572,9 +608,13 @@ def main():
         forest = P.build_forest(prob)
         q0, aux = P.initial_data(prob, forest)
         dt = P.stable_dt(prob, forest, q0)
-    npatch = forest.npatch
-    cells = npatch * prob.mx * prob.mx
-    lv = forest.leaves()[0]
+    npatch = forest.npatch if forest is not None else args.cells // (prob.mx * prob.mx)
+    cells = args.cells if args.cells is not None else npatch * prob.mx * prob.mx
+    lv = forest.leaves()[0] if forest is not None else None
+    if args.trace_summary:
+        print("workload=%s cells=%d" % (workload, cells))
+        print(trace_summary(args.trace_summary))
+        return
     if args.pmc_summary:
         print("workload=%s cells=%d" % (workload, cells))
         print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], cells))
@@ -590,6 +630,7 @@ def main():
 
     device = fa.initialize_device(0)
     if args.profile_run:
+        print("workload=%s patches=%d cells=%d" % (workload, npatch, cells), flush=True)
         profile_run(fa, prob, forest, q0, aux, dt, (args.tile64, args.tile32), not args.no_time_sync,
                     dt_scale)
         return
