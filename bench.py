@@ -108,16 +108,21 @@ def run_regrid(fa, prob, forest, q, repeats=3):
         t1 = time.perf_counter()
         kind, src = f.adapt(er, (1 - ec).astype(np.int32), prob.minlevel, prob.maxlevel)
         t2 = time.perf_counter()
-        sv.regrid(f, kind, src, prob.extent)
+        parts = {}
+        sv.regrid(f, kind, src, prob.extent, times=parts)
+        t3 = time.perf_counter()
         sv.ghost_fill()
         sv.sync()
-        t3 = time.perf_counter()
-        times.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2, f.npatch,
-                      int(np.sum(kind == fa.REGRID_CHILD)), int(np.sum(kind == fa.REGRID_PARENT))))
+        t4 = time.perf_counter()
+        times.append((t4 - t0, t1 - t0, t2 - t1, t4 - t2, f.npatch,
+                      int(np.sum(kind == fa.REGRID_CHILD)), int(np.sum(kind == fa.REGRID_PARENT)),
+                      parts["transfer"], parts["programs"], t4 - t3))
         del sv
-    tot, tag, adapt, move, n, nchild, nparent = sorted(times)[len(times) // 2]
+    tot, tag, adapt, move, n, nchild, nparent, transfer, programs, fill = sorted(times)[len(times) // 2]
     return {"ms": round(tot * 1e3, 3), "tag_ms": round(tag * 1e3, 3),
             "adapt_ms": round(adapt * 1e3, 3), "transfer_and_ghost_fill_ms": round(move * 1e3, 3),
+            "transfer_ms": round(transfer * 1e3, 3), "programs_ms": round(programs * 1e3, 3),
+            "ghost_fill_ms": round(fill * 1e3, 3),
             "patches_before": forest.npatch, "patches_after": n, "new_children": nchild,
             "new_parents": nparent, "refine_threshold": prob.tag_threshold,
             "coarsen_threshold": 0.5 * prob.tag_threshold}

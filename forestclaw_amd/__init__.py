@@ -188,13 +188,16 @@ class Forest:
     def ghost_program(self, mx, my, mbc, mthbc):
         mth = np.ascontiguousarray(mthbc, dtype=np.int32)
         ps = np.zeros(7, np.int32)
-        n = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth, None, 0, ps)
+        ops = np.empty((16 * self.npatch + 16, 8), np.int32)   # room for the usual forest
+        n = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth, ops.ctypes.data,
+                                                   len(ops), ps)
         if n < 0:
             raise RuntimeError("ghost program failed")
-        ops = np.zeros((max(n, 1), 8), np.int32)
-        n2 = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth,
-                                                    ops.ctypes.data, n, ps)
-        assert n2 == n
+        if n > len(ops):
+            ops = np.zeros((n, 8), np.int32)
+            n2 = lib().fclaw2d_hip_forest_ghost_program(self._h, mx, my, mbc, mth,
+                                                        ops.ctypes.data, n, ps)
+            assert n2 == n
         return ops[:n], ps
 
     def adapt(self, refine, coarsen, minlevel, maxlevel):
@@ -214,12 +217,14 @@ class Forest:
 
     def sync_program(self, mx, my):
         """Time-synchronisation ops (hipclaw_set_sync_program)."""
-        n = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, None, 0)
+        ops = np.empty((8 * self.npatch + 8, 8), np.int32)   # at most two ops per face
+        n = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, ops.ctypes.data, len(ops))
         if n < 0:
             raise RuntimeError("sync program failed")
-        ops = np.zeros((max(n, 1), 8), np.int32)
-        n2 = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, ops.ctypes.data, n)
-        assert n2 == n
+        if n > len(ops):
+            ops = np.zeros((n, 8), np.int32)
+            n2 = lib().fclaw2d_hip_forest_sync_program(self._h, mx, my, ops.ctypes.data, n)
+            assert n2 == n
         return ops[:n]
 
     def geometry(self, extent, mx, my):
@@ -302,9 +307,13 @@ class Solver:
                "hipclaw_tag_patches")
         return er, ec
 
-    def regrid(self, forest, kind, src, extent):
+    def regrid(self, forest, kind, src, extent, times=None):
         """Move the solution onto the adapted `forest` (provenance kind, src),
-        then install its geometry and programs (aux must be set again)."""
+        then install its geometry and programs (aux must be set again).
+        `times`, a dict, receives the seconds of the transfer and of the
+        programs."""
+        import time
+        t0 = time.perf_counter()
         _, _, dx, dy = forest.geometry(extent, self.mx, self.my)
         kind = np.ascontiguousarray(kind, dtype=np.int32)
         src = np.ascontiguousarray(src, dtype=np.int32).reshape(-1)
@@ -313,7 +322,11 @@ class Solver:
                                     np.ascontiguousarray(dy, dtype=np.float64)), "hipclaw_regrid")
         self.npatch = len(kind)
         self.max_patches = max(self.max_patches, self.npatch)
+        t1 = time.perf_counter()
         self.set_forest(forest, extent)
+        if times is not None:
+            times["transfer"] = t1 - t0
+            times["programs"] = time.perf_counter() - t1
 
     def save_step(self):
         """Keep a device copy of every patch's state (to retake a step)."""

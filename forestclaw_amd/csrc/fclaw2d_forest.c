@@ -768,28 +768,28 @@ int fclaw2d_hip_forest_sync_program_range(const fclaw2d_hip_forest_t *f, int mx,
     opbuf_t b;
     int p, k, g, nr = 0;
     if (!f || mx != my || cap < 0 || rcap < 0) return -1;
+    if (f->n > 0 && !relations(f, 0, mx, my)) return -1;   /* builds the relation cache used below */
     b.ops = ops;
     b.cap = ops ? cap : 0;
     b.n = 0;
     for (p = 0; p < f->n; p++) {
-        nbinfo_t r;
+        const nbinfo_t *rp;
         if (f->level[p] < minlevel) continue;
+        rp = relations(f, p, mx, my);
         for (k = 0; k < 4; k++) {
-            face_rel(f, p, k, mx, my, &r);
-            if (r.rel == REL_NONE) return -1;
-            if (r.rel == REL_SAME) emit(&b, 0, p, r.nb[0], k, 0, 0, 0);
+            if (rp[k].rel == REL_NONE) return -1;
+            if (rp[k].rel == REL_SAME) emit(&b, 0, p, rp[k].nb[0], k, 0, 0, 0);
         }
         for (k = 0; k < 4; k++) {
-            face_rel(f, p, k, mx, my, &r);
-            if (r.rel != REL_HALF) continue;
-            for (g = 0; g < 2; g++) emit(&b, 1, p, r.nb[g], k, r.ti[g], r.tj[g], 0);
+            if (rp[k].rel != REL_HALF) continue;
+            for (g = 0; g < 2; g++) emit(&b, 1, p, rp[k].nb[g], k, rp[k].ti[g], rp[k].tj[g], 0);
         }
     }
     for (p = 0; p < f->n; p++) {
-        nbinfo_t r;
+        const nbinfo_t *rp = relations(f, p, mx, my);
         for (k = 0; k < 4; k++) {
             int reset;
-            face_rel(f, p, k, mx, my, &r);
+            const nbinfo_t r = rp[k];
             if (f->level[p] >= minlevel)
                 reset = r.rel == REL_SAME || r.rel == REL_HALF || r.rel == REL_BOUNDARY ||
                         (r.rel == REL_DOUBLE && f->level[p] - 1 >= minlevel);

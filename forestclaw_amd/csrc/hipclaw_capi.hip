@@ -61,6 +61,7 @@ struct GhostProgram {
     // launch per phase, 1 = per-destination groups
     GhostLaunch launch[2][6];
     int nlaunch[2] = {0, 0};
+    bool layouts_built = false;
 };
 
 // One time-synchronisation program (hipclaw_set_sync_program_k).
@@ -135,6 +136,10 @@ struct hipclaw_state {
     long long cstride = 0;
     int qcorr_cap = 0;            // patches qcorr holds
     bool fsync = false;           // the fused step applies time-sync corrections
+    // device ints for the regrid's tags and transfer ops, grown as needed and
+    // kept (a regrid allocates nothing when the forest fits)
+    int* iwork = nullptr;
+    size_t iwork_cap = 0;
 };
 
 // ---------------------------------------------------------------- allocation-free step
@@ -178,6 +183,7 @@ static void* scratch_ptr(int* rc) {
 }
 
 static int build_fused(hipclaw_state_t* s);
+static int build_fused_sync(hipclaw_state_t* s);
 
 extern "C" {
 
@@ -338,6 +344,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->dtscale);
     hipFree(s->fops);
     hipFree(s->pull);
+    hipFree(s->iwork);
     hipFree(s->frel);
     hipFree(s->cmask);
     hipFree(s->qcorr);
@@ -399,83 +406,9 @@ static bool op_slot_ok(const hipclaw_state_t* s, int slot) {
     return slot >= 0 && p < s->npatch;
 }
 
-int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int nops,
-                                const int* phase_start) {
-    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) || !phase_start)
-        return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
-    if (phase_start[0] != 0 || phase_start[6] != nops)
-        return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
-    for (int k = 0; k < 6; k++)
-        if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
-    // validate every op on the host: a bad slot or translation would fault the GPU
-    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
-    // the op types each phase may hold (bit mask over GHOST_*)
-    static const int phase_types[6] = {
-        (1 << hipclaw::GHOST_AVERAGE_FACE) | (1 << hipclaw::GHOST_AVERAGE_CORNER),
-        (1 << hipclaw::GHOST_COPY_FACE) | (1 << hipclaw::GHOST_COPY_CORNER),
-        1 << hipclaw::GHOST_PHYSBC,
-        (1 << hipclaw::GHOST_INTERP_FACE) | (1 << hipclaw::GHOST_INTERP_CORNER),
-        1 << hipclaw::GHOST_PHYSBC,
-        1 << hipclaw::GHOST_PHYSBC};
-    int ph = 0;
-    for (int o = 0; o < nops; o++) {
-        const int* op = ops + 8 * o;
-        const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
-        while (o >= phase_start[ph + 1]) ph++;
-        if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
-            return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
-        if (!(phase_types[ph] & (1 << type)))
-            return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
-        if (type == hipclaw::GHOST_PHYSBC) {
-            // per face: -1 (not a physical face), 1 (extrapolation) or 3 (wall).  The
-            // reference stops on mthbc 0 and 2 and leaves other codes' ghost cells
-            // unwritten (clawpack46_bc2_default.f:17-25, 41-43), which with two slabs
-            // would leave them to whatever the spare slab held.
-            for (int f = 0; f < 4; f++) {
-                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
-                if (b != -1 && b != 1 && b != 3)
-                    return fail(HIPCLAW_EINVAL,
-                                "physical boundary condition code must be 1 (extrapolation) or 3 (wall)%s");
-            }
-            continue;
-        }
-        if (!op_slot_ok(s, src) || which < 0 || which > 3)
-            return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
-        // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
-        // neighbour's array; check the extreme cells the op can touch
-        int lo_i, hi_i, lo_j, hi_j;  // range of the translated index
-        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
-            lo_i = 1 - mbc + ti; hi_i = mx + mbc + ti; lo_j = 1 - mbc + tj; hi_j = my + mbc + tj;
-            // copy reads the neighbour's interior band only
-            int ci0, ci1, cj0, cj1;
-            if (type == hipclaw::GHOST_COPY_FACE) {
-                if (which == 0) { ci0 = 1 - mbc; ci1 = 0; cj0 = 1; cj1 = my; }
-                else if (which == 1) { ci0 = mx + 1; ci1 = mx + mbc; cj0 = 1; cj1 = my; }
-                else if (which == 2) { ci0 = 1; ci1 = mx; cj0 = 1 - mbc; cj1 = 0; }
-                else { ci0 = 1; ci1 = mx; cj0 = my + 1; cj1 = my + mbc; }
-            } else {
-                ci0 = (which & 1) ? mx + 1 : 1 - mbc; ci1 = ci0 + mbc - 1;
-                cj0 = (which & 2) ? my + 1 : 1 - mbc; cj1 = cj0 + mbc - 1;
-            }
-            lo_i = ci0 + ti; hi_i = ci1 + ti; lo_j = cj0 + tj; hi_j = cj1 + tj;
-            if (lo_i < 1 || hi_i > mx || lo_j < 1 || hi_j > my)
-                return fail(HIPCLAW_EINVAL, "copy op reads outside the neighbour's interior%s");
-        } else if (type == hipclaw::GHOST_AVERAGE_CORNER) {
-            const int ic0 = (which & 1) ? mx + 1 : 1 - mbc, jc0 = (which & 2) ? my + 1 : 1 - mbc;
-            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + mbc - 1) + ti;
-            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + mbc - 1) + tj;
-            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
-                return fail(HIPCLAW_EINVAL, "average-corner op reads outside the fine patch%s");
-        } else if (type == hipclaw::GHOST_INTERP_CORNER) {
-            const int hb = mbc / 2;
-            const int ic0 = (which & 1) ? mx - hb + 1 : 1, jc0 = (which & 2) ? my - hb + 1 : 1;
-            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + hb - 1) + ti;
-            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + hb - 1) + tj;
-            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
-                return fail(HIPCLAW_EINVAL, "interpolate-corner op writes outside the fine patch%s");
-        }
-        // average/interpolate faces test every cell against the neighbour's bounds in the kernel
-    }
+// The two launch layouts of ghost program k (hipclaw_set_ghost_layout 0 and 1)
+// from its validated ops.
+static int build_layouts(hipclaw_state_t* s, int k, const int* ops, int nops, const int* phase_start) {
     // The launches: phases 0 and 1 (average, copy: disjoint ghost cells, interior
     // reads only) share one, in the caller's order (neighbouring patches' ops
     // stay close, which keeps the cache lines they share in cache).  Phases 4
@@ -650,13 +583,109 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
     g.nops = (int) dev.size();
     g.nlaunch[0] = nl[0];
     g.nlaunch[1] = nl[1];
+    g.layouts_built = true;
+    return HIPCLAW_OK;
+}
+
+// Layouts 0 and 1 of program 0, built on first use (never inside a graph
+// capture: the callers that may capture call this first).
+static int ensure_layouts(hipclaw_state_t* s) {
+    GhostProgram& g = s->gp[0];
+    if (g.layouts_built || !s->gp0_set) return HIPCLAW_OK;
+    return build_layouts(s, 0, s->gp0_ops.data(), (int) s->gp0_ops.size() / 8, s->gp0_phase);
+}
+
+int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int nops,
+                                const int* phase_start) {
+    if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) || !phase_start)
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
+    if (phase_start[0] != 0 || phase_start[6] != nops)
+        return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
+    for (int k = 0; k < 6; k++)
+        if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
+    // validate every op on the host: a bad slot or translation would fault the GPU
+    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    // the op types each phase may hold (bit mask over GHOST_*)
+    static const int phase_types[6] = {
+        (1 << hipclaw::GHOST_AVERAGE_FACE) | (1 << hipclaw::GHOST_AVERAGE_CORNER),
+        (1 << hipclaw::GHOST_COPY_FACE) | (1 << hipclaw::GHOST_COPY_CORNER),
+        1 << hipclaw::GHOST_PHYSBC,
+        (1 << hipclaw::GHOST_INTERP_FACE) | (1 << hipclaw::GHOST_INTERP_CORNER),
+        1 << hipclaw::GHOST_PHYSBC,
+        1 << hipclaw::GHOST_PHYSBC};
+    int ph = 0;
+    for (int o = 0; o < nops; o++) {
+        const int* op = ops + 8 * o;
+        const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
+        while (o >= phase_start[ph + 1]) ph++;
+        if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
+            return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
+        if (!(phase_types[ph] & (1 << type)))
+            return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
+        if (type == hipclaw::GHOST_PHYSBC) {
+            // per face: -1 (not a physical face), 1 (extrapolation) or 3 (wall).  The
+            // reference stops on mthbc 0 and 2 and leaves other codes' ghost cells
+            // unwritten (clawpack46_bc2_default.f:17-25, 41-43), which with two slabs
+            // would leave them to whatever the spare slab held.
+            for (int f = 0; f < 4; f++) {
+                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
+                if (b != -1 && b != 1 && b != 3)
+                    return fail(HIPCLAW_EINVAL,
+                                "physical boundary condition code must be 1 (extrapolation) or 3 (wall)%s");
+            }
+            continue;
+        }
+        if (!op_slot_ok(s, src) || which < 0 || which > 3)
+            return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
+        // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
+        // neighbour's array; check the extreme cells the op can touch
+        int lo_i, hi_i, lo_j, hi_j;  // range of the translated index
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
+            lo_i = 1 - mbc + ti; hi_i = mx + mbc + ti; lo_j = 1 - mbc + tj; hi_j = my + mbc + tj;
+            // copy reads the neighbour's interior band only
+            int ci0, ci1, cj0, cj1;
+            if (type == hipclaw::GHOST_COPY_FACE) {
+                if (which == 0) { ci0 = 1 - mbc; ci1 = 0; cj0 = 1; cj1 = my; }
+                else if (which == 1) { ci0 = mx + 1; ci1 = mx + mbc; cj0 = 1; cj1 = my; }
+                else if (which == 2) { ci0 = 1; ci1 = mx; cj0 = 1 - mbc; cj1 = 0; }
+                else { ci0 = 1; ci1 = mx; cj0 = my + 1; cj1 = my + mbc; }
+            } else {
+                ci0 = (which & 1) ? mx + 1 : 1 - mbc; ci1 = ci0 + mbc - 1;
+                cj0 = (which & 2) ? my + 1 : 1 - mbc; cj1 = cj0 + mbc - 1;
+            }
+            lo_i = ci0 + ti; hi_i = ci1 + ti; lo_j = cj0 + tj; hi_j = cj1 + tj;
+            if (lo_i < 1 || hi_i > mx || lo_j < 1 || hi_j > my)
+                return fail(HIPCLAW_EINVAL, "copy op reads outside the neighbour's interior%s");
+        } else if (type == hipclaw::GHOST_AVERAGE_CORNER) {
+            const int ic0 = (which & 1) ? mx + 1 : 1 - mbc, jc0 = (which & 2) ? my + 1 : 1 - mbc;
+            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + mbc - 1) + ti;
+            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + mbc - 1) + tj;
+            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+                return fail(HIPCLAW_EINVAL, "average-corner op reads outside the fine patch%s");
+        } else if (type == hipclaw::GHOST_INTERP_CORNER) {
+            const int hb = mbc / 2;
+            const int ic0 = (which & 1) ? mx - hb + 1 : 1, jc0 = (which & 2) ? my - hb + 1 : 1;
+            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + hb - 1) + ti;
+            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + hb - 1) + tj;
+            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+                return fail(HIPCLAW_EINVAL, "interpolate-corner op writes outside the fine patch%s");
+        }
+        // average/interpolate faces test every cell against the neighbour's bounds in the kernel
+    }
+    GhostProgram& g = s->gp[k];
+    g.layouts_built = false;
+    g.nops = 0;
+    g.nlaunch[0] = g.nlaunch[1] = 0;
     if (k == 0) {
         s->gp0_ops.assign(ops, ops + 8 * (size_t) nops);
         for (int p = 0; p < 7; p++) s->gp0_phase[p] = phase_start[p];
         s->gp0_set = true;
-        return build_fused(s);
+        if (int rc = build_fused(s)) return rc;
+        // the fused step needs no launch layout of its own; the two-launch
+        // layouts are built when one is asked for (ensure_layouts)
+        if (s->ghost_layout == 2 && s->fused_ok) return HIPCLAW_OK;
     }
-    return HIPCLAW_OK;
+    return build_layouts(s, k, ops, nops, phase_start);
 }
 
 int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start) {
@@ -714,7 +743,7 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     for (int g = 0; g < ngroups; g++) y.maxgroup = std::max(y.maxgroup, start[g + 1] - start[g]);
     if (k == 0) {
         s->sp0_ops.assign(ops, ops + 8 * (size_t) nops);
-        return build_fused(s);
+        return build_fused_sync(s);
     }
     return HIPCLAW_OK;
 }
@@ -846,7 +875,24 @@ static int build_fused(hipclaw_state_t* s) {
         const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
         pull[8 * (size_t) op[1] + (op[0] == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
     }
-    // time sync (contiguous per destination): the faces each patch corrects
+    if (int rc = upload_array(s, s->fops, s->fops_cap, fops.data(), (int) fops.size(), "fused ghost ops")) return rc;
+    if (int rc = upload_array(s, s->frel, s->frel_cap, frel.data(), (int) frel.size(), "face table")) return rc;
+    if (pull_ok)
+        if (int rc = upload_array(s, s->pull, s->pull_cap, pull.data(), (int) pull.size(), "pull offsets")) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->nlite = nlite;
+    s->nall = (int) fops.size();
+    s->pull_ok = pull_ok;
+    s->fused_ok = true;
+    return build_fused_sync(s);
+}
+
+// The time-sync part of the fused step: the faces each patch corrects (from
+// sync program 0, contiguous per destination) and room for the increments.
+static int build_fused_sync(hipclaw_state_t* s) {
+    s->fsync = false;
+    if (!s->fused_ok) return HIPCLAW_OK;
+    const int np = s->npatch, mx = s->cfg.mx;
     std::vector<int> cm(np, 0);
     const bool sync = s->reg && !s->reg_accumulate && !s->sp0_ops.empty();
     if (sync) {
@@ -857,10 +903,6 @@ static int build_fused(hipclaw_state_t* s) {
             cm[op[1]] |= (1 << op[3]) | (op[0] == hipclaw::SYNC_F2C ? 1 << (4 + op[3]) : 0);
         }
     }
-    if (int rc = upload_array(s, s->fops, s->fops_cap, fops.data(), (int) fops.size(), "fused ghost ops")) return rc;
-    if (int rc = upload_array(s, s->frel, s->frel_cap, frel.data(), (int) frel.size(), "face table")) return rc;
-    if (pull_ok)
-        if (int rc = upload_array(s, s->pull, s->pull_cap, pull.data(), (int) pull.size(), "pull offsets")) return rc;
     if (sync) {
         if (int rc = upload_array(s, s->cmask, s->cmask_cap, cm.data(), np, "sync face masks")) return rc;
         const long long cstride = 4LL * s->cfg.meqn * mx;
@@ -875,11 +917,7 @@ static int build_fused(hipclaw_state_t* s) {
         }
     }
     HIP_TRY(hipStreamSynchronize(s->st));
-    s->nlite = nlite;
-    s->nall = (int) fops.size();
-    s->pull_ok = pull_ok;
     s->fsync = sync;
-    s->fused_ok = true;
     return HIPCLAW_OK;
 }
 
@@ -949,6 +987,13 @@ static int ghost_flat_async(hipclaw_state_t* s, bool full, bool sync) {
 static int ghost_fill_async(hipclaw_state_t* s, int k) {
     const GhostProgram& p = s->gp[k];
     if (k == 0 && s->ghost_layout == 2 && s->fused_ok) return ghost_flat_async(s, true, false);
+    if (k == 0 && !p.layouts_built) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(s->st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(HIPCLAW_EINVAL, "ghost-fill layouts not built before the capture%s");
+        if (int rc = ensure_layouts(s)) return rc;
+    }
     hipclaw::GhostArgs g;
     g.q = s->q[s->cur];
     g.qti = s->qti;
@@ -1102,6 +1147,8 @@ static int set_tile(hipclaw_state_t* s, int t) {
 
 int hipclaw_set_ghost_layout(hipclaw_state_t* s, int layout) {
     if (!s || layout < 0 || layout > 2) return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_layout: bad arguments%s");
+    if (layout != 2 || !s->fused_ok)
+        if (int rc = ensure_layouts(s)) return rc;
     s->ghost_layout = layout;
     // a captured graph holds the launches of the old layout
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
@@ -1459,6 +1506,8 @@ int hipclaw_advance_async(hipclaw_state_t* s, double dt, int nsteps) {
     if (!s || nsteps < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance: bad arguments%s");
     if (s->reg_accumulate && s->sp[0].nresets == 0 && s->sp[0].ngroups > 0)
         return fail(HIPCLAW_EINVAL, "with subcycle, the full sync program needs its register resets%s");
+    if (!(s->ghost_layout == 2 && s->fused_ok && !s->reg_accumulate))
+        if (int rc = ensure_layouts(s)) return rc;
     HIP_TRY(hipEventRecord(s->region[0], s->st));
     if (int rc = advance_launches(s, dt, nsteps, s->time_kernels)) return rc;
     HIP_TRY(hipEventRecord(s->region[1], s->st));
@@ -1488,6 +1537,8 @@ int hipclaw_capture_graph(hipclaw_state_t* s, double dt, int nsteps) {
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
     HIP_TRY(hipStreamSynchronize(s->st));
+    if (!(s->ghost_layout == 2 && s->fused_ok && !s->reg_accumulate))
+        if (int rc = ensure_layouts(s)) return rc;
     const int cur0 = s->cur;
     HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     int rc = advance_launches(s, dt, nsteps, false);
@@ -1564,14 +1615,28 @@ int hipclaw_mass(hipclaw_state_t* s, double* sum_hi, double* sum_lo) {
     return HIPCLAW_OK;
 }
 
+// s->iwork with room for n ints.
+static int ensure_iwork(hipclaw_state_t* s, size_t n) {
+    if (n <= s->iwork_cap) return HIPCLAW_OK;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    hipFree(s->iwork);
+    s->iwork = nullptr;
+    s->iwork_cap = 0;
+    const size_t cap = n + n / 4;
+    if (hipMalloc((void**) &s->iwork, sizeof(int) * cap) != hipSuccess)
+        return fail(HIPCLAW_ENOMEM, "hipMalloc of the regrid workspace failed%s");
+    s->iwork_cap = cap;
+    return HIPCLAW_OK;
+}
+
 int hipclaw_tag_patches(hipclaw_state_t* s, int ivar, double refine_threshold,
                         double coarsen_threshold, int* exceeds_refine, int* exceeds_coarsen) {
     if (!s || ivar < 0 || ivar >= s->cfg.meqn || !exceeds_refine || !exceeds_coarsen)
         return fail(HIPCLAW_EINVAL, "hipclaw_tag_patches: bad arguments%s");
     const int n = s->npatch;
     if (n == 0) return HIPCLAW_OK;
-    int* dev = nullptr;
-    HIP_TRY(hipMalloc((void**) &dev, sizeof(int) * 2 * (size_t) n));
+    if (int rc = ensure_iwork(s, 2 * (size_t) n)) return rc;
+    int* dev = s->iwork;
     int e = hipclaw::launch_tag_patches(s->q[s->cur], s->all_slots, n, s->dxs, s->dys, s->pstride,
                                         s->cfg.mx, s->cfg.my, s->cfg.mbc, ivar, refine_threshold,
                                         coarsen_threshold, dev, s->st);
@@ -1579,7 +1644,6 @@ int hipclaw_tag_patches(hipclaw_state_t* s, int ivar, double refine_threshold,
     hipError_t he = e ? (hipError_t) e
                       : hipMemcpyAsync(out.data(), dev, sizeof(int) * 2 * (size_t) n, hipMemcpyDeviceToHost, s->st);
     if (he == hipSuccess) he = hipStreamSynchronize(s->st);
-    hipFree(dev);
     if (he != hipSuccess) return hipfail(he, "tag_patches");
     for (int p = 0; p < n; p++) {
         exceeds_refine[p] = out[2 * p];
@@ -1648,15 +1712,12 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     const int cap = reuse ? s->max_patches
                           : std::max(npatch_new, std::max(s->max_patches, npatch_new + npatch_new / 4));
     double* qnew = nullptr;
-    int* dops = nullptr;
+    if (int rc = ensure_iwork(s, ops.size())) return rc;
+    int* dops = s->iwork;
     HIP_TRY(hipStreamSynchronize(s->st));
     if (reuse) qnew = s->q[1 - s->cur];
     else if (hipMalloc((void**) &qnew, sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
         return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
-    if (hipMalloc((void**) &dops, sizeof(int) * ops.size()) != hipSuccess) {
-        if (!reuse) hipFree(qnew);
-        return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
-    }
     // cells the transfer does not write (ghost cells of new children and
     // parents) start at zero until the ghost fill
     hipError_t he = hipMemsetAsync(qnew, 0, sizeof(double) * s->pstride * (size_t) npatch_new, s->st);
@@ -1665,7 +1726,6 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
         he = (hipError_t) hipclaw::launch_regrid_remap(s->q[s->cur], qnew, dops, npatch_new, s->pstride,
                                                        s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn, s->st);
     if (he == hipSuccess) he = hipStreamSynchronize(s->st);
-    hipFree(dops);
     if (he != hipSuccess) {
         if (!reuse) hipFree(qnew);
         return hipfail(he, "regrid_remap");
@@ -1699,6 +1759,7 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     for (GhostProgram& g : s->gp) {
         g.nops = 0;
         g.nlaunch[0] = g.nlaunch[1] = 0;
+        g.layouts_built = false;
     }
     for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
     s->gp0_set = false;
