@@ -26,14 +26,25 @@ REGRID_SAME, REGRID_CHILD, REGRID_PARENT = 0, 1, 2
 SOLVER_SHAPES = {RP_ADVECTION: (1, 1, 2), RP_SHALLOW: (3, 3, 0), RP_EULER: (4, 4, 0)}
 
 _lib = None
+_user_libs = []      # loaded solver libraries (kept alive)
+_user_shapes = {}    # registered solver number -> (meqn, mwaves, maux)
+
+
+class HipclawSolverDesc(C.Structure):
+    """hipclaw_solver_t (include/hipclaw_solver.h)."""
+    _fields_ = [("abi", C.c_int), ("step_args_size", C.c_int), ("name", C.c_char_p),
+                ("meqn", C.c_int), ("mwaves", C.c_int), ("maux", C.c_int),
+                ("conservative", C.c_int), ("time_sync", C.c_int), ("tiles", C.c_int),
+                ("launch_fp64", C.c_void_p), ("launch_fp32", C.c_void_p)]
 
 
 class HipclawConfig(C.Structure):
     _fields_ = [("solver", C.c_int), ("precision", C.c_int), ("mx", C.c_int),
                 ("my", C.c_int), ("mbc", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
-                ("mwaves", C.c_int), ("order", C.c_int * 2), ("mthlim", C.c_int * 4),
+                ("mwaves", C.c_int), ("order", C.c_int * 2), ("mthlim", C.c_int * 10),
                 ("mthbc", C.c_int * 4), ("grav", C.c_double), ("gamma", C.c_double),
-                ("tile", C.c_int), ("time_sync", C.c_int), ("subcycle", C.c_int)]
+                ("tile", C.c_int), ("time_sync", C.c_int), ("subcycle", C.c_int),
+                ("rp_params", C.c_double * 8)]
 
 
 def _dp():
@@ -104,6 +115,7 @@ def lib():
         "hipclaw_ghost_pack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_ghost_unpack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_get_registers": ([vp, i, i, _dp()], i),
+        "hipclaw_register_solver": ([C.POINTER(HipclawSolverDesc), C.POINTER(i)], i),
         "hipclaw_step_batch_noalloc": ([C.POINTER(HipclawConfig), i, vp, vp, vp, vp, vp, _ip(), i,
                                         d, vp, C.POINTER(d)], i),
     }
@@ -117,6 +129,38 @@ def lib():
 
 class HipclawError(RuntimeError):
     pass
+
+
+def solver_shape(solver):
+    """(meqn, mwaves, maux) of a built-in or registered Riemann solver."""
+    if solver in SOLVER_SHAPES:
+        return SOLVER_SHAPES[solver]
+    if solver in _user_shapes:
+        return _user_shapes[solver]
+    raise ValueError("unknown Riemann solver %r" % (solver,))
+
+
+def register_solver(path, name):
+    """Load the shared library of an application's own Riemann solver (built
+    with csrc/hipclaw_user_solver.h, exporting `<name>_solver`) and register
+    it; returns the solver number for Solver(solver=...)."""
+    L = C.CDLL(os.path.abspath(path))
+    fn = getattr(L, name + "_solver")
+    fn.restype = C.POINTER(HipclawSolverDesc)
+    fn.argtypes = []
+    d = fn()
+    sid = C.c_int(-1)
+    _check(lib().hipclaw_register_solver(d, C.byref(sid)), "hipclaw_register_solver")
+    _user_libs.append(L)
+    _user_shapes[sid.value] = (d.contents.meqn, d.contents.mwaves, d.contents.maux)
+    return sid.value
+
+
+def example_solver(name):
+    """Register one of the example solvers built in user_solvers/
+    ("acoustics", "tracers")."""
+    from .build import user_solver_lib
+    return register_solver(user_solver_lib(name), name)
 
 
 def _check(rc, what):
@@ -243,18 +287,21 @@ class Solver:
 
     def __init__(self, solver, mx, my, mbc=2, precision=FP64, order=(2, 2), mthlim=None,
                  mthbc=(1, 1, 1, 1), grav=1.0, gamma=1.4, tile=0, max_patches=1,
-                 time_sync=False, subcycle=False):
-        meqn, mwaves, maux = SOLVER_SHAPES[solver]
+                 time_sync=False, subcycle=False, rp_params=None):
+        meqn, mwaves, maux = solver_shape(solver)
         cfg = HipclawConfig()
         cfg.solver, cfg.precision = solver, precision
         cfg.mx, cfg.my, cfg.mbc = mx, my, mbc
         cfg.meqn, cfg.maux, cfg.mwaves = meqn, maux, mwaves
         cfg.order[0], cfg.order[1] = order
         lim = list(mthlim) if mthlim is not None else [4] * mwaves
-        for k in range(4):
+        for k in range(10):
             cfg.mthlim[k] = lim[k] if k < len(lim) else 0
+        for k in range(4):
             cfg.mthbc[k] = mthbc[k]
         cfg.grav, cfg.gamma, cfg.tile = grav, gamma, tile
+        for k, v in enumerate(rp_params or ()):
+            cfg.rp_params[k] = v
         cfg.time_sync = int(bool(time_sync))
         cfg.subcycle = int(bool(subcycle))
         self.time_sync = bool(time_sync)

@@ -4,10 +4,16 @@ layer, linked with hipcc.  Run `python -m forestclaw_amd.build`.
 The FP64 kernels are compiled with -ffp-contract=off (bit-faithful to the CPU
 Clawpack-4.6 arithmetic); the FP32-bulk kernels allow contraction.  Packed
 FP32 instructions are disabled for all device code.
+
+Also builds the example solver libraries of user_solvers/ (an application's
+own Riemann solver compiled into the step kernel through
+csrc/hipclaw_user_solver.h): each source twice, FP64 and FP32-bulk, linked
+into user_solvers/lib<name>.so.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -28,6 +34,9 @@ HIP_SOURCES = {
     "hipclaw_capi.hip": [],
 }
 C_SOURCES = ["fclaw2d_forest.c", "fclaw2d_hip_host.c"]
+USER_DIR = os.path.join(HERE, "user_solvers")
+USER_SOLVERS = {"acoustics": "rp_acoustics.hip", "tracers": "rp_tracers.hip"}
+USER_PRECISION = {"64": ["-ffp-contract=off"], "32": []}
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function",
            "-I" + os.path.join(ROOT, "include")]
 
@@ -45,23 +54,44 @@ def _run(cmd, verbose):
         print(err)
 
 
-def build(verbose=False):
+def user_solver_lib(name):
+    return os.path.join(USER_DIR, "lib%s.so" % name)
+
+
+def build(verbose=False, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    compiles, objs = [], []
     for src, extra in HIP_SOURCES.items():
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        compiles.append([HIPCC] + HIP_COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in C_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
-        _run([CC] + C_FLAGS + ["-c", path, "-o", obj], verbose)
+        compiles.append([CC] + C_FLAGS + ["-c", path, "-o", obj])
         objs.append(obj)
+    user_objs = {}
+    for name, src in USER_SOLVERS.items():
+        user_objs[name] = []
+        for prec, extra in USER_PRECISION.items():
+            obj = os.path.join(BUILD, "user_%s_fp%s.o" % (name, prec))
+            compiles.append([HIPCC] + HIP_COMMON + extra +
+                            ["-I" + CSRC, "-DHIPCLAW_USER_PRECISION=" + prec,
+                             "-c", os.path.join(USER_DIR, src), "-o", obj])
+            user_objs[name].append(obj)
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd in compiles]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    for name, uo in user_objs.items():
+        out = user_solver_lib(name)
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out + ".tmp"] + uo, verbose)
+        os.replace(out + ".tmp", out)
     return LIB
 
 

@@ -620,15 +620,16 @@ void fc2d_hipclaw_allocate_buffers(fclaw2d_global_t *glob)
     fc2d_hipclaw_deallocate_buffers(glob);
     memset(&cfg, 0, sizeof(cfg));
     cfg.solver = vt->riemann_solver;
+    if (vt->rp_solver)   /* the application's own solver, as cuda_rpn2 / cuda_rpt2 */
+        device_check(hipclaw_register_solver(vt->rp_solver, &cfg.solver), "hipclaw_register_solver");
     cfg.precision = opt->precision;
     cfg.mx = o->mx; cfg.my = o->my; cfg.mbc = o->mbc;
     cfg.meqn = o->meqn; cfg.maux = o->maux; cfg.mwaves = opt->mwaves;
     cfg.order[0] = opt->order[0]; cfg.order[1] = opt->order[1];
-    for (k = 0; k < 4; k++) {
-        cfg.mthlim[k] = k < opt->mwaves ? opt->mthlim[k] : 0;
-        cfg.mthbc[k] = opt->mthbc[k];
-    }
+    for (k = 0; k < FC2D_HIPCLAW_MWAVES; k++) cfg.mthlim[k] = k < opt->mwaves ? opt->mthlim[k] : 0;
+    for (k = 0; k < 4; k++) cfg.mthbc[k] = opt->mthbc[k];
     cfg.grav = opt->grav; cfg.gamma = opt->gamma; cfg.tile = opt->tile;
+    for (k = 0; k < 8; k++) cfg.rp_params[k] = opt->rp_params[k];
     cfg.time_sync = glob->time_sync;
     cfg.subcycle = glob->subcycle;
     device_check(hipclaw_create(&cfg, d->local_num_patches, &glob->state), "hipclaw_create");

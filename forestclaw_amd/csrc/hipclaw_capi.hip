@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <cmath>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "../../include/hipclaw.h"
+#include "../../include/hipclaw_solver.h"
 #include "hipclaw_internal.h"
 
 using hipclaw::GhostOp;
@@ -77,6 +79,7 @@ struct hipclaw_state {
     hipclaw_config_t cfg;
     int tile = 16, ntx = 1, nty = 1;
     bool inplace = false;
+    bool conservative = false;   // the solver has the FP32-bulk flux form
     int max_patches = 0, npatch = 0;
     long long pstride = 0, astride = 0;
     double* q[2] = {nullptr, nullptr};
@@ -207,6 +210,95 @@ int hipclaw_initialize_device(int device, char* name, size_t name_len) {
     return HIPCLAW_OK;
 }
 
+// ---- Riemann solvers: the three built in (numbers 0..2), then those an
+// application registers (HIPCLAW_RP_USER on).  Registration publishes the
+// descriptor before the count, so readers need no lock.
+static std::mutex g_solvers_mu;
+static const hipclaw_solver_t* g_solvers[HIPCLAW_MAX_USER_SOLVERS];
+static std::atomic<int> g_nsolvers{0};
+
+static const hipclaw_solver_t* user_solver(int id) {
+    const int k = id - HIPCLAW_RP_USER;
+    if (k < 0 || k >= g_nsolvers.load(std::memory_order_acquire)) return nullptr;
+    return g_solvers[k];
+}
+
+int hipclaw_register_solver(const hipclaw_solver_t* d, int* id) {
+    if (!d || !id) return fail(HIPCLAW_EINVAL, "hipclaw_register_solver: bad arguments%s");
+    if (d->abi != HIPCLAW_SOLVER_ABI || d->step_args_size != (int) sizeof(hipclaw::StepArgs))
+        return fail(HIPCLAW_EINVAL, "hipclaw_register_solver: solver built against another "
+                                    "version of the step kernel%s");
+    if (d->meqn < 1 || d->mwaves < 1 || d->mwaves > 10 || d->maux < 0 || !d->launch_fp64 ||
+        !(d->tiles & (8 | 16)))
+        return fail(HIPCLAW_EINVAL, "hipclaw_register_solver: bad descriptor%s");
+    std::lock_guard<std::mutex> lock(g_solvers_mu);
+    const int n = g_nsolvers.load(std::memory_order_relaxed);
+    for (int k = 0; k < n; k++)
+        if (g_solvers[k] == d) { *id = HIPCLAW_RP_USER + k; return HIPCLAW_OK; }
+    if (n >= HIPCLAW_MAX_USER_SOLVERS) return fail(HIPCLAW_EINVAL, "hipclaw_register_solver: registry full%s");
+    g_solvers[n] = d;
+    g_nsolvers.store(n + 1, std::memory_order_release);
+    *id = HIPCLAW_RP_USER + n;
+    return HIPCLAW_OK;
+}
+
+const hipclaw_solver_t* hipclaw_registered_solver(int id) { return user_solver(id); }
+
+namespace {
+struct SolverShape {
+    int meqn, mwaves, maux;
+    bool conservative;   // FP32-bulk flux form
+    bool time_sync;      // keeps time-sync registers
+};
+}  // namespace
+
+static bool solver_shape(int solver, SolverShape* sh) {
+    static const SolverShape builtin[3] = {
+        {1, 1, 2, false, true}, {3, 3, 0, true, true}, {4, 4, 0, true, true}};
+    if (solver >= 0 && solver <= 2) { *sh = builtin[solver]; return true; }
+    const hipclaw_solver_t* d = user_solver(solver);
+    if (!d) return false;
+    *sh = {d->meqn, d->mwaves, d->maux, d->conservative != 0, d->time_sync != 0};
+    return true;
+}
+
+// Tile sizes instantiated for a solver in a precision.
+static bool solver_tile_ok(int solver, int precision, int tile) {
+    if (solver >= 0 && solver <= 2) return hipclaw::step_supported(solver, precision, tile);
+    const hipclaw_solver_t* d = user_solver(solver);
+    if (!d || (tile != 8 && tile != 16) || !(d->tiles & tile)) return false;
+    return precision == HIPCLAW_FP64 ? d->launch_fp64 != nullptr : d->launch_fp32 != nullptr;
+}
+
+static int solver_default_waves(int solver, int precision, int tile) {
+    if (solver >= 0 && solver <= 2)
+        return precision == HIPCLAW_FP64 ? hipclaw::default_waves_fp64(solver, tile)
+                                         : hipclaw::default_waves_fp32(solver, tile);
+    return tile != 16 ? 0 : precision == HIPCLAW_FP64 ? hipclaw::default_waves_t16<double>()
+                                                      : hipclaw::default_waves_t16<float>();
+}
+
+static int launch_solver_step(int solver, int precision, int tile, int waves,
+                              const hipclaw::StepArgs& a, const hipclaw::StepLaunch& L) {
+    if (solver >= 0 && solver <= 2)
+        return precision == HIPCLAW_FP64 ? hipclaw::launch_step_fp64(solver, tile, waves, a, L)
+                                         : hipclaw::launch_step_fp32(solver, tile, waves, a, L);
+    const hipclaw_solver_t* d = user_solver(solver);
+    const hipclaw_step_launcher_t fn = !d ? nullptr : precision == HIPCLAW_FP64 ? d->launch_fp64 : d->launch_fp32;
+    if (!fn) return (int) hipErrorInvalidValue;
+    return fn(tile, waves, &a, (void*) L.st, (void*) L.start, (void*) L.stop);
+}
+
+// The solver-dependent part of the step arguments.
+static void set_method(hipclaw::StepArgs& a, const hipclaw_config_t* c) {
+    a.order2 = c->order[0] == 2;
+    a.transverse = c->order[1];
+    for (int k = 0; k < 10; k++) a.mthlim[k] = k < c->mwaves ? c->mthlim[k] : 0;
+    a.pp.grav = c->grav;
+    a.pp.gamma = c->gamma;
+    for (int k = 0; k < 8; k++) a.pp.p[k] = c->rp_params[k];
+}
+
 // Tile size measured fastest on MI355X for the update kernel (the tile sweep of
 // the benchmark forest, mx = 16 and 32): 16 x 16 for both forms (four waves
 // per workgroup for FP64, two for the FP32-bulk form); smaller patches take
@@ -215,7 +307,7 @@ static int default_tile(const hipclaw_config_t* c) {
     const int m = std::min(c->mx, c->my);
     int t = 16;
     while (t > 8 && (t > m || c->mx % t != 0 || c->my % t != 0 ||
-                     !hipclaw::step_supported(c->solver, c->precision, t)))
+                     !solver_tile_ok(c->solver, c->precision, t)))
         t >>= 1;
     return t;
 }
@@ -223,15 +315,17 @@ static int default_tile(const hipclaw_config_t* c) {
 // Checks a configuration for the step kernels; *tile_out = the tile to use.
 static int check_config(const hipclaw_config_t* cfg, int* tile_out) {
     const int solver = cfg->solver;
-    static const int meqn_of[3] = {1, 3, 4}, mw_of[3] = {1, 3, 4}, maux_of[3] = {2, 0, 0};
-    if (solver < 0 || solver > 2) return fail(HIPCLAW_EINVAL, "unknown Riemann solver%s");
-    if (cfg->meqn != meqn_of[solver] || cfg->mwaves != mw_of[solver] || cfg->maux != maux_of[solver])
+    SolverShape sh;
+    if (!solver_shape(solver, &sh)) return fail(HIPCLAW_EINVAL, "unknown Riemann solver%s");
+    if (cfg->meqn != sh.meqn || cfg->mwaves != sh.mwaves || cfg->maux != sh.maux)
         return fail(HIPCLAW_EINVAL, "meqn/mwaves/maux do not match the Riemann solver%s");
+    if (cfg->time_sync && !sh.time_sync)
+        return fail(HIPCLAW_EINVAL, "the Riemann solver provides no conservative flux for time sync%s");
     if (cfg->mbc != 2) return fail(HIPCLAW_EINVAL, "the batched step requires mbc = 2%s");
     if (cfg->precision != HIPCLAW_FP64 && cfg->precision != HIPCLAW_FP32_BULK)
         return fail(HIPCLAW_EINVAL, "unknown precision%s");
     int tile = cfg->tile > 0 ? cfg->tile : default_tile(cfg);
-    if (!hipclaw::step_supported(solver, cfg->precision, tile))
+    if (!solver_tile_ok(solver, cfg->precision, tile))
         return fail(HIPCLAW_EINVAL, "tile size not available for this solver and precision%s");
     if (cfg->time_sync && cfg->precision == HIPCLAW_FP64 && tile == 32)
         return fail(HIPCLAW_EINVAL, "FP64 tiles of 32 leave no LDS for the time-sync edge buffers%s");
@@ -255,6 +349,11 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
     hipclaw_state* s = new hipclaw_state();
     s->cfg = *cfg;
     s->tile = tile;
+    {
+        SolverShape sh;
+        solver_shape(cfg->solver, &sh);
+        s->conservative = sh.conservative;
+    }
     s->ntx = cfg->mx / tile;
     s->nty = cfg->my / tile;
     s->inplace = (s->ntx == 1 && s->nty == 1);
@@ -944,7 +1043,7 @@ static int time_sync_async(hipclaw_state_t* s, int k) {
     a.pstride = s->pstride;
     a.rstride = s->rstride;
     a.mx = s->cfg.mx; a.mbc = s->cfg.mbc; a.meqn = s->cfg.meqn;
-    a.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->cfg.solver != HIPCLAW_RP_ADVECTION;
+    a.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
     if (int e = hipclaw::launch_time_sync(a, s->st)) return hipfail((hipError_t) e, "time_sync launch");
     return HIPCLAW_OK;
 }
@@ -979,7 +1078,7 @@ static int ghost_flat_async(hipclaw_state_t* s, bool full, bool sync) {
     f.rstride = s->rstride;
     f.qinc = s->qcorr;
     f.cstride = s->cstride;
-    f.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->cfg.solver != HIPCLAW_RP_ADVECTION;
+    f.fluxreg = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
     if (int e = hipclaw::launch_ghost_flat(f, s->st)) return hipfail((hipError_t) e, "ghost_flat launch");
     return HIPCLAW_OK;
 }
@@ -1123,7 +1222,7 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
 // Tile sizes a state can switch to: supported by its solver and precision,
 // dividing the patch, and (FP64 with time sync) leaving LDS for the registers.
 static bool tile_allowed(const hipclaw_state_t* s, int t) {
-    return hipclaw::step_supported(s->cfg.solver, s->cfg.precision, t) && s->cfg.mx % t == 0 &&
+    return solver_tile_ok(s->cfg.solver, s->cfg.precision, t) && s->cfg.mx % t == 0 &&
            s->cfg.my % t == 0 && !(s->cfg.time_sync && s->cfg.precision == HIPCLAW_FP64 && t == 32);
 }
 
@@ -1165,8 +1264,7 @@ int hipclaw_ghost_layout(const hipclaw_state_t* s) {
 
 static int effective_waves(const hipclaw_state_t* s) {
     if (s->waves) return s->waves;
-    return s->cfg.precision == HIPCLAW_FP64 ? hipclaw::default_waves_fp64(s->cfg.solver, s->tile)
-                                            : hipclaw::default_waves_fp32(s->cfg.solver, s->tile);
+    return solver_default_waves(s->cfg.solver, s->cfg.precision, s->tile);
 }
 
 int hipclaw_set_waves(hipclaw_state_t* s, int waves) {
@@ -1316,12 +1414,8 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
     a.nwork = n;
     a.mx = s->cfg.mx; a.my = s->cfg.my; a.mbc = s->cfg.mbc;
     a.ntx = s->ntx; a.nty = s->nty;
-    a.order2 = s->cfg.order[0] == 2;
-    a.transverse = s->cfg.order[1];
-    for (int k = 0; k < 4; k++) a.mthlim[k] = k < s->cfg.mwaves ? s->cfg.mthlim[k] : 0;
+    set_method(a, &s->cfg);
     a.dt = dt;
-    a.pp.grav = s->cfg.grav;
-    a.pp.gamma = s->cfg.gamma;
     a.reg = s->reg;
     a.rstride = s->rstride;
     a.reg_accumulate = s->reg_accumulate;
@@ -1333,9 +1427,7 @@ static hipclaw::StepArgs step_args(hipclaw_state_t* s, const double* src, double
 static int launch_step(hipclaw_state_t* s, const hipclaw::StepArgs& a, hipEvent_t ev0 = nullptr,
                        hipEvent_t ev1 = nullptr) {
     const hipclaw::StepLaunch L(s->st, ev0, ev1);
-    int e = (s->cfg.precision == HIPCLAW_FP64)
-                ? hipclaw::launch_step_fp64(s->cfg.solver, s->tile, s->waves, a, L)
-                : hipclaw::launch_step_fp32(s->cfg.solver, s->tile, s->waves, a, L);
+    int e = launch_solver_step(s->cfg.solver, s->cfg.precision, s->tile, s->waves, a, L);
     if (e) return hipfail((hipError_t) e, "step_tiles launch");
     return HIPCLAW_OK;
 }
@@ -1426,19 +1518,14 @@ int hipclaw_step_batch_noalloc(const hipclaw_config_t* cfg, int npatch, const do
     a.nwork = nslots;
     a.mx = cfg->mx; a.my = cfg->my; a.mbc = cfg->mbc;
     a.ntx = ntx; a.nty = nty;
-    a.order2 = cfg->order[0] == 2;
-    a.transverse = cfg->order[1];
-    for (int k = 0; k < 4; k++) a.mthlim[k] = k < cfg->mwaves ? cfg->mthlim[k] : 0;
+    set_method(a, cfg);
     a.dt = dt;
-    a.pp.grav = cfg->grav;
-    a.pp.gamma = cfg->gamma;
     a.reg = nullptr;
     a.rstride = 0;
     a.reg_accumulate = 0;
     a.dtscale = nullptr;
     const hipclaw::StepLaunch L(st);
-    int e = (cfg->precision == HIPCLAW_FP64) ? hipclaw::launch_step_fp64(cfg->solver, tile, 0, a, L)
-                                             : hipclaw::launch_step_fp32(cfg->solver, tile, 0, a, L);
+    int e = launch_solver_step(cfg->solver, cfg->precision, tile, 0, a, L);
     if (e) return hipfail((hipError_t) e, "step_tiles launch");
     unsigned long long bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, dcfl, sizeof(bits), hipMemcpyDeviceToHost, st));

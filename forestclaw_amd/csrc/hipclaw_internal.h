@@ -9,7 +9,7 @@
 
 #include "../../include/hipclaw.h"
 
-#include "hipclaw_step.h"
+#include "hipclaw_launch.h"
 
 namespace hipclaw {
 
@@ -116,15 +116,6 @@ int launch_ghost_flat(const FlatArgs& f, hipStream_t st);
 // qcorr -> the interior edge cells of q (slots 0..n-1, faces in cmask).
 int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, int n, long long pstride,
                        long long cstride, int mx, int my, int mbc, int meqn, hipStream_t st);
-// The stream of a step launch and, for a timed launch, two events that
-// hipExtLaunchKernelGGL binds to the kernel's own start and end, so their
-// elapsed time is the kernel's execution without the gaps around it.
-struct StepLaunch {
-    hipStream_t st;
-    hipEvent_t start, stop;
-    StepLaunch(hipStream_t s, hipEvent_t a = nullptr, hipEvent_t b = nullptr) : st(s), start(a), stop(b) {}
-};
-// waves: waves per workgroup of a 16 x 16 tile (2 or 4), 0 = the default.
 int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
 int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
 // The default waves per workgroup of a tile (0 where there is no choice).
@@ -151,48 +142,6 @@ int launch_copy_interior(const double* src, double* dst, const int* slots, int n
 int launch_mass(const double* q, const int* slots, int n, const double* dxs, const double* dys,
                 long long pstride, int mx, int my, int mbc, int meqn, double* out,
                 hipStream_t st);
-
-// Workgroups of `kernel` (block size `block`, static LDS only) that the
-// current device holds at once: the grid of a persistent launch.
-inline int resident_blocks(const void* kernel, int block) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) return 0;
-    return per_cu * cus;
-}
-
-// Launch a persistent step kernel: min(tiles, resident workgroups) of them.
-// `resident` caches the occupancy query; each kernel instantiation owns one.
-// Host threads may race to fill it: they compute the same value, and the
-// atomic makes the store and the loads well defined.
-template <typename K>
-inline int launch_persistent(K kernel, int block, const StepArgs& a, const StepLaunch& L,
-                             std::atomic<int>& resident) {
-    const int ntiles = a.nwork * a.ntx * a.nty;
-    if (ntiles <= 0) return 0;
-    int nres = resident.load(std::memory_order_relaxed);
-    if (nres <= 0) {
-        nres = resident_blocks((const void*) kernel, block);
-        resident.store(nres, std::memory_order_relaxed);
-    }
-    if (nres <= 0) return (int) hipErrorInvalidDeviceFunction;
-    const int nblocks = ntiles < nres ? ntiles : nres;
-    if (L.start || L.stop)
-        hipExtLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, L.st, L.start, L.stop, 0, a);
-    else
-        hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(block), 0, L.st, a);
-    return (int) hipGetLastError();
-}
-
-// Every wave family uses the MC limiter (mthlim 4), so the step may take the
-// instantiation with the limiter fixed at compile time.
-template <class RP>
-inline bool all_mc_limiter(const StepArgs& a) {
-    for (int mw = 0; mw < RP::MWAVES; mw++)
-        if (a.mthlim[mw] != 4) return false;
-    return true;
-}
 
 // Largest tile the FP64 / FP32 kernels are instantiated for, per solver.
 bool step_supported(int solver, int precision, int tile);

@@ -1057,51 +1057,15 @@ int launch_time_sync(const SyncArgs& a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- FP64 step
-template <class RP, int T, int NW, bool WITH_REGS = true>
-static int launch_one(const StepArgs& a, const StepLaunch& st) {
-    // kernels that keep time-sync registers are separate instantiations, so the
-    // plain update carries none of their code or register pressure; the MC
-    // limiter on every wave family has its own compile-time instantiation
-    static std::atomic<int> resident[2][2];   // zero-initialised (static storage)
-    if (all_mc_limiter<RP>(a)) {
-        if (a.reg) {
-            if constexpr (WITH_REGS)
-                return launch_persistent(step_tiles<RP, double, T, NW, false, true, 4>, NW * 64, a, st, resident[1][1]);
-            return (int) hipErrorInvalidValue;
-        }
-        return launch_persistent(step_tiles<RP, double, T, NW, false, false, 4>, NW * 64, a, st, resident[1][0]);
-    }
-    if (a.reg) {
-        if constexpr (WITH_REGS)
-            return launch_persistent(step_tiles<RP, double, T, NW, false, true>, NW * 64, a, st, resident[0][1]);
-        return (int) hipErrorInvalidValue;
-    }
-    return launch_persistent(step_tiles<RP, double, T, NW, false, false>, NW * 64, a, st, resident[0][0]);
-}
-
-template <class RP>
-// waves = 0: the default per tile (16 x 16: four waves per workgroup,
-// measured fastest for FP64); 2 or 4 select the 16 x 16 variant.
-static int launch_tile(int tile, int waves, const StepArgs& a, const StepLaunch& st) {
-    switch (tile) {
-    case 8: return launch_one<RP, 8, 2>(a, st);
-    case 16: return waves == 2 ? launch_one<RP, 16, 2>(a, st) : launch_one<RP, 16, 4>(a, st);
-    default: return (int) hipErrorInvalidValue;
-    }
-}
-
 int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st) {
     switch (solver) {
-    case 0: return launch_tile<RPAdvection<double>>(tile, waves, a, st);
-    case 1:
-        // T = 32 fits the LDS only without the time-sync edge buffers
-        if (tile == 32) return launch_one<RPShallow<double>, 32, 16, false>(a, st);
-        return launch_tile<RPShallow<double>>(tile, waves, a, st);
-    case 2: return launch_tile<RPEuler<double>>(tile, waves, a, st);
+    case 0: return launch_step_tile<RPAdvection<double>, double, true>(tile, waves, a, st);
+    case 1: return launch_step_tile<RPShallow<double>, double, true, true>(tile, waves, a, st);
+    case 2: return launch_step_tile<RPEuler<double>, double, true>(tile, waves, a, st);
     default: return (int) hipErrorInvalidValue;
     }
 }
 
-int default_waves_fp64(int /*solver*/, int tile) { return tile == 16 ? 4 : 0; }
+int default_waves_fp64(int /*solver*/, int tile) { return tile == 16 ? default_waves_t16<double>() : 0; }
 
 }  // namespace hipclaw

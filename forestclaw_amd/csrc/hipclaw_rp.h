@@ -24,6 +24,10 @@ namespace hipclaw {
 struct PhysParams {
     double grav;   // shallow water gravity
     double gamma;  // Euler ratio of specific heats
+    // A registered solver's own constants (what the reference keeps in a
+    // solver's common block or __constant__ memory, e.g. rho, bulk, cc, zz of
+    // the acoustics solver's common /cparam/).
+    double p[8];
 };
 
 template <typename R>
@@ -64,7 +68,8 @@ struct RPShallow {
     static constexpr int NROE = 3;  // u, v, a
 
     template <int IXY>
-    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxR*/,
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
+                                                const R* /*auxR*/,
                                                 const PhysParams& pp, R wave[MWAVES][MEQN],
                                                 R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
                                                 R roe[NROE]) {
@@ -133,7 +138,8 @@ struct RPShallow {
 
     template <int IXY>
     __device__ __forceinline__ static void rpt2(const R roe[NROE], const R asdq[MEQN],
-                                                R /*aux_lo*/, R /*aux_hi*/, R bm[MEQN],
+                                                const R* /*aux1*/, const R* /*aux2*/,
+                                                const R* /*aux3*/, const PhysParams&, R bm[MEQN],
                                                 R bp[MEQN]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R u = roe[0], v = roe[1], a = roe[2];
@@ -202,7 +208,8 @@ struct RPEuler {
     static constexpr int NROE = 7;  // u, v, a, enth, u2v2, g1a2, euv
 
     template <int IXY>
-    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxR*/,
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
+                                                const R* /*auxR*/,
                                                 const PhysParams& pp, R wave[MWAVES][MEQN],
                                                 R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
                                                 R roe[NROE]) {
@@ -299,7 +306,8 @@ struct RPEuler {
 
     template <int IXY>
     __device__ __forceinline__ static void rpt2(const R roe[NROE], const R asdq[MEQN],
-                                                R /*aux_lo*/, R /*aux_hi*/, R bm[MEQN],
+                                                const R* /*aux1*/, const R* /*aux2*/,
+                                                const R* /*aux3*/, const PhysParams&, R bm[MEQN],
                                                 R bp[MEQN]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R u = roe[0], v = roe[1], a = roe[2], enth = roe[3], u2v2 = roe[4];
@@ -369,8 +377,8 @@ struct RPEuler {
 
 // ----------------------------------------------------------------- advection
 // aux(:,:,1) = u, aux(:,:,2) = v.  The normal speed is the right cell's aux
-// component IXY; rpt2 receives the transverse velocity at the two cells it
-// needs (aux_lo = aux2(i1,kv), aux_hi = aux3(i1,kv)).
+// component IXY; rpt2 takes the transverse velocity from aux2 (bmasdq) and
+// aux3 (bpasdq) as clawpack46_rpt2adv.f:79-86.
 template <typename R>
 struct RPAdvection {
     static constexpr int MEQN = 1, MWAVES = 1, MAUX = 2;
@@ -380,7 +388,8 @@ struct RPAdvection {
     static constexpr int NROE = 1;
 
     template <int IXY>
-    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* auxR,
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
+                                                const R* auxR,
                                                 const PhysParams&, R wave[MWAVES][MEQN],
                                                 R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
                                                 R roe[NROE]) {
@@ -393,10 +402,12 @@ struct RPAdvection {
     }
 
     template <int IXY>
-    __device__ __forceinline__ static void rpt2(const R* /*roe*/, const R asdq[MEQN], R aux_lo,
-                                                R aux_hi, R bm[MEQN], R bp[MEQN]) {
-        bm[0] = rmin(aux_lo, R(0)) * asdq[0];
-        bp[0] = rmax(aux_hi, R(0)) * asdq[0];
+    __device__ __forceinline__ static void rpt2(const R* /*roe*/, const R asdq[MEQN],
+                                                const R* /*aux1*/, const R* aux2, const R* aux3,
+                                                const PhysParams&, R bm[MEQN], R bp[MEQN]) {
+        constexpr int kv = IXY == 1 ? 1 : 0;   // the transverse velocity component
+        bm[0] = rmin(aux2[kv], R(0)) * asdq[0];
+        bp[0] = rmax(aux3[kv], R(0)) * asdq[0];
     }
 
     template <int IXY>

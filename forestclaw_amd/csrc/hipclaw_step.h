@@ -49,7 +49,7 @@ struct StepArgs {
     int ntx, nty;              // tiles per patch in x and y
     int order2;                // method(2) == 2
     int transverse;            // method(3)
-    int mthlim[4];
+    int mthlim[10];            // limiter per wave family (mwaves <= 10)
     double dt;
     PhysParams pp;
     // Time-sync flux registers (null: not kept).  Per slot
@@ -197,8 +197,13 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
 // sweeps and, with the kernel's other uniform values, spills scalar
 // registers into vector lanes; the fixed form has neither (about 40 % fewer
 // vector instructions in each sweep for the MC limiter).
-template <typename R, int T>
+//
+// A solver with many equations and waves (more than 16 wave components per
+// interface) cannot keep them in the registers of 4 waves; it gets no target
+// and the allocator sizes the kernel itself.
+template <class RP, typename R, int T>
 constexpr int step_waves_per_eu() {
+    if (RP::MEQN * RP::MWAVES > 16) return 1;
 #ifdef HIPCLAW_WPE_F32_T16
     if (sizeof(R) == 4 && T == 16) return HIPCLAW_WPE_F32_T16;
 #endif
@@ -206,7 +211,7 @@ constexpr int step_waves_per_eu() {
 }
 
 template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS, int LIM = 0>
-__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<R, T>())))
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<RP, R, T>())))
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
     using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
@@ -269,15 +274,17 @@ step_tiles(StepArgs a) {
     // sx = 1 (left of the tile) / 2 (right) / 0, sy = 1 (below) / 2 (above) / 0,
     // four bits per element; s_pull[code] is the tile's offset for that side
     // (a patch face or corner the tile touches), 0 for the patch's own cells.
-    static_assert(KQ <= 16, "staging codes hold 16 elements");
-    unsigned long long pcode = 0;
+    constexpr int NPC = (KQ + 15) / 16;   // 16 codes per 64-bit word
+    unsigned long long pcode[NPC];
+#pragma unroll
+    for (int w = 0; w < NPC; w++) pcode[w] = 0;
 #pragma unroll
     for (int kk = 0; kk < KQ; kk++) {
         const int e = threadIdx.x + kk * NT;
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
         const int sx = qi < 2 ? 1 : qi > T + 1 ? 2 : 0, sy = qj < 2 ? 1 : qj > T + 1 ? 2 : 0;
-        if (e < NQ) pcode |= (unsigned long long) (sx + 3 * sy) << (4 * kk);
+        if (e < NQ) pcode[kk / 16] |= (unsigned long long) (sx + 3 * sy) << (4 * (kk % 16));
     }
     __shared__ int s_pull[9];
 
@@ -292,7 +299,7 @@ step_tiles(StepArgs a) {
         if (a.pull) {
 #pragma unroll
             for (int kk = 0; kk < KQ; kk++)
-                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode >> (4 * kk)) & 15]] : 0.0;
+                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode[kk / 16] >> (4 * (kk % 16))) & 15]] : 0.0;
         } else {
 #pragma unroll
             for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
@@ -309,6 +316,13 @@ step_tiles(StepArgs a) {
     auto SA = [&](int m, int qi, int qj) -> R {
         if constexpr (MAUX > 0) return sm.aux[m * QP + qj * QS + qi];
         else return R(0);
+    };
+    // the MAUX aux values of one staged cell (what a solver does not read is
+    // never loaded)
+    constexpr int MAD = MAUX > 0 ? MAUX : 1;
+    auto aux_cell = [&](R* v, int qi, int qj) {
+#pragma unroll
+        for (int m = 0; m < MAD; m++) v[m] = SA(m, qi, qj);
     };
     // ab[w][m][r][a]
     auto AB = [&](int w, int m, int r, int c) -> R& {
@@ -450,13 +464,13 @@ step_tiles(StepArgs a) {
             const bool act = (seg < SEG) && (r < T + 2);
             const int rc = r < T + 1 ? r : T + 1;
             const int kc = k < L ? k : L - 1;
-            R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
+            R qL[MEQN], qR[MEQN], auxL[MAD], auxR[MAD];
 #pragma unroll
             for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, kc, rc + 1); qR[m] = SQ(m, kc + 1, rc + 1); }
-#pragma unroll
-            for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, kc + 1, rc + 1);
+            aux_cell(auxL, kc, rc + 1);
+            aux_cell(auxR, kc + 1, rc + 1);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-            RP::template rpn2<1>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            RP::template rpn2<1>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, r, T, last_x, first_y, last_y)) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
@@ -509,9 +523,14 @@ step_tiles(StepArgs a) {
                     for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
                 }
                 R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
-                // rpt2adv reads aux(i1, j) and aux(i1, j+1) with i1 = i-1 (amdq) or i (apdq)
-                RP::template rpt2<1>(roe, amdq, SA(1, kc, rc + 1), SA(1, kc, rc + 2), bmA, bpA);
-                RP::template rpt2<1>(roe, apdq, SA(1, kc + 1, rc + 1), SA(1, kc + 1, rc + 2), bmP, bpP);
+                // aux1, aux2, aux3: rows j-1, j, j+1 of column i1 = i-1 (amdq) or i (apdq)
+                {
+                    R a1[MAD], a2[MAD], a3[MAD];
+                    aux_cell(a1, kc, rc); aux_cell(a2, kc, rc + 1); aux_cell(a3, kc, rc + 2);
+                    RP::template rpt2<1>(roe, amdq, a1, a2, a3, a.pp, bmA, bpA);
+                    aux_cell(a1, kc + 1, rc); aux_cell(a2, kc + 1, rc + 1); aux_cell(a3, kc + 1, rc + 2);
+                    RP::template rpt2<1>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
+                }
                 const R half = R(0.5) * dtdx;
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
@@ -548,13 +567,13 @@ step_tiles(StepArgs a) {
             const bool act = (seg < SEG) && (c < T + 2);
             const int cc = c < T + 1 ? c : T + 1;
             const int kc = k < L ? k : L - 1;
-            R qL[MEQN], qR[MEQN], auxR[MAUX > 0 ? MAUX : 1];
+            R qL[MEQN], qR[MEQN], auxL[MAD], auxR[MAD];
 #pragma unroll
             for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, cc + 1, kc); qR[m] = SQ(m, cc + 1, kc + 1); }
-#pragma unroll
-            for (int m = 0; m < (MAUX > 0 ? MAUX : 1); m++) auxR[m] = SA(m, cc + 1, kc + 1);
+            aux_cell(auxL, cc + 1, kc);
+            aux_cell(auxR, cc + 1, kc + 1);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-            RP::template rpn2<2>(qL, qR, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            RP::template rpn2<2>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, c, T, last_y, first_x, last_x)) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
@@ -638,9 +657,14 @@ step_tiles(StepArgs a) {
                     for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
                 }
                 R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
-                // y sweep: aux2 = column i, aux3 = column i+1, at j1 = j-1 (amdq) or j (apdq)
-                RP::template rpt2<2>(roe, amdq, SA(0, cc + 1, kc), SA(0, cc + 2, kc), bmA, bpA);
-                RP::template rpt2<2>(roe, apdq, SA(0, cc + 1, kc + 1), SA(0, cc + 2, kc + 1), bmP, bpP);
+                // aux1, aux2, aux3: columns i-1, i, i+1 of row j1 = j-1 (amdq) or j (apdq)
+                {
+                    R a1[MAD], a2[MAD], a3[MAD];
+                    aux_cell(a1, cc, kc); aux_cell(a2, cc + 1, kc); aux_cell(a3, cc + 2, kc);
+                    RP::template rpt2<2>(roe, amdq, a1, a2, a3, a.pp, bmA, bpA);
+                    aux_cell(a1, cc, kc + 1); aux_cell(a2, cc + 1, kc + 1); aux_cell(a3, cc + 2, kc + 1);
+                    RP::template rpt2<2>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
+                }
                 const R half = R(0.5) * dtdy;
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import RP_ADVECTION, RP_EULER, RP_SHALLOW, SOLVER_SHAPES, Forest
+from . import RP_ADVECTION, RP_EULER, RP_SHALLOW, Forest, solver_shape
 
 
 @dataclass
@@ -40,14 +40,18 @@ class Problem:
     grav: float = 1.0
     gamma: float = 1.4
     params: dict = field(default_factory=dict)
+    rp_params: tuple = ()     # a registered solver's constants (hipclaw_config_t.rp_params)
+    mthlim: tuple = None      # limiter per wave family (None: MC for every family)
+    max_speed: float = None   # largest wave speed, for solvers stable_dt does not know
+    oracle_solver: int = None # the oracle's number for this solver (None: the same)
 
     @property
     def meqn(self):
-        return SOLVER_SHAPES[self.solver][0]
+        return solver_shape(self.solver)[0]
 
     @property
     def maux(self):
-        return SOLVER_SHAPES[self.solver][2]
+        return solver_shape(self.solver)[2]
 
 
 def advection_problem(minlevel=4, maxlevel=4, mx=16):
@@ -91,6 +95,52 @@ def shockbubble_problem(minlevel=2, maxlevel=7, mx=16, gamma=1.4, mach=1.5):
     return Problem("shockbubble", RP_EULER, (0.0, 2.0, 0.0, 0.5), 4, 1, (False, False),
                    (1, 1, 3, 3), minlevel, maxlevel, mx, 0.8, q, gamma=gamma,
                    params=dict(p2=p2, rho2=rho2, u2=u2))
+
+
+def acoustics_problem(minlevel=2, maxlevel=4, mx=16, rho=1.0, bulk=4.0):
+    """The reference's radial acoustics example
+    (applications/cudaclaw/acoustics/2d/radial: fclaw_options.ini, cuda.ini,
+    user_cuda/qinit.f90): a ring of pressure 1 + cos(pi (r - 0.5) / 0.2) about
+    r = 0.5 on [-1, 1]^2, at rest, rho = 1, bulk = 4, extrapolation
+    boundaries; run with the application's own Riemann solver (the example
+    solver library user_solvers/libacoustics.so)."""
+    from . import example_solver
+    cc = float(np.sqrt(bulk / rho))
+
+    def q(x, y):
+        r = np.sqrt(x * x + y * y)
+        p = np.where(np.abs(r - 0.5) <= 0.2, 1.0 + np.cos(np.pi * (r - 0.5) / 0.2), 0.0)
+        return np.stack([p, np.zeros_like(p), np.zeros_like(p)])
+
+    return Problem("acoustics", example_solver("acoustics"), (-1.0, 1.0, -1.0, 1.0), 1, 1,
+                   (False, False), (1, 1, 1, 1), minlevel, maxlevel, mx, 0.9, q, tag_threshold=1.0,
+                   rp_params=(rho, bulk, cc, rho * cc), max_speed=cc, oracle_solver=3,
+                   params=dict(rho=rho, bulk=bulk))
+
+
+def tracers_problem(minlevel=2, maxlevel=4, mx=16):
+    """Ten colour equations on the periodic unit square (the example solver
+    library user_solvers/libtracers.so): tracer k is a disk of radius 0.15
+    about a centre that turns with k, moved by c_k (u, v) with
+    u = 1 + 0.5 sin(2 pi y), v = 0.5 + 0.25 cos(2 pi x), c_k = (10 - k) / 10;
+    each wave family takes its own limiter (1..6, then 1..4)."""
+    from . import example_solver
+    n = 10
+
+    def q(x, y):
+        out = []
+        for k in range(n):
+            a = 2 * np.pi * k / n
+            cx, cy = 0.5 + 0.25 * np.cos(a), 0.5 + 0.25 * np.sin(a)
+            out.append(((x - cx) ** 2 + (y - cy) ** 2 < 0.15 ** 2).astype(np.float64))
+        return np.stack(out)
+
+    def aux(x, y):
+        return np.stack([1.0 + 0.5 * np.sin(2 * np.pi * y), 0.5 + 0.25 * np.cos(2 * np.pi * x)])
+
+    return Problem("tracers", example_solver("tracers"), (0.0, 1.0, 0.0, 1.0), 1, 1, (True, True),
+                   (2, 2, 2, 2), minlevel, maxlevel, mx, 0.9, q, aux, tag_threshold=0.5,
+                   mthlim=tuple(1 + k % 6 for k in range(n)), max_speed=1.5, oracle_solver=4)
 
 
 def ensemble_problem(nruns=64, mx=16, minlevel=2, maxlevels=(4, 7), seed=2026):
@@ -221,5 +271,5 @@ def stable_dt(prob, forest, q, mbc=2):
         c = np.sqrt(prob.gamma * p / rho)
         s = np.maximum(np.abs(u) + c, np.abs(v) + c).max(axis=(1, 2))
     else:
-        s = np.full(len(dx), 1.0)
+        s = np.full(len(dx), prob.max_speed if prob.max_speed is not None else 1.0)
     return prob.cfl * float(np.min(dx / s))

@@ -7,7 +7,9 @@
  * Differences a maintainer switching from fc2d_cudaclaw must know:
  *  - cuda_rpn2 / cuda_rpt2 device function pointers are replaced by
  *    `riemann_solver`, one of the HIPCLAW_RP_* functors compiled into the
- *    backend (FP32 and FP64 instantiations);
+ *    backend, or `rp_solver`, the descriptor of the application's own functor
+ *    compiled into the step kernel (hipclaw_solver.h; FP32 and FP64
+ *    instantiations);
  *  - patch data stay on the GPU between steps; host views are synchronised
  *    with fc2d_hipclaw_download / fc2d_hipclaw_upload;
  *  - options add `precision` (HIPCLAW_FP64 or HIPCLAW_FP32_BULK) and `tile`.
@@ -17,6 +19,7 @@
 
 #include "fclaw2d_hip_core.h"
 #include "hipclaw.h"
+#include "hipclaw_solver.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -40,6 +43,7 @@ typedef struct fc2d_hipclaw_options {
     int precision;         /* HIPCLAW_FP64 or HIPCLAW_FP32_BULK */
     int tile;              /* 0 = tuned default */
     double grav, gamma;    /* physical constants of the compiled Riemann solvers */
+    double rp_params[8];   /* constants of an application's own solver (vtable rp_solver) */
     int is_registered;
 } fc2d_hipclaw_options_t;
 
@@ -93,7 +97,9 @@ typedef struct fc2d_hipclaw_vtable {
     hipclaw_fort_qinit_t fort_qinit;
     hipclaw_fort_setaux_t fort_setaux;
     hipclaw_fort_src2_t fort_src2;
-    int riemann_solver;              /* HIPCLAW_RP_*: replaces cuda_rpn2 / cuda_rpt2 */
+    int riemann_solver;              /* HIPCLAW_RP_*: a built-in solver ... */
+    const hipclaw_solver_t *rp_solver; /* ... or the application's own (hipclaw_solver.h),
+                                          taking precedence: replaces cuda_rpn2 / cuda_rpt2 */
     int is_set;
 } fc2d_hipclaw_vtable_t;
 

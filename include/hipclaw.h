@@ -30,8 +30,10 @@ enum {
 };
 
 /* Riemann solvers compiled into the backend (device functors, FP32 and FP64).
- * They replace the device function pointers cuda_rpn2 / cuda_rpt2 of
- * fc2d_cudaclaw_vtable_t (src/solvers/fc2d_cudaclaw/fc2d_cudaclaw.h:159-162). */
+ * With those an application registers (hipclaw_solver.h, numbers from
+ * HIPCLAW_RP_USER on) they replace the device function pointers cuda_rpn2 /
+ * cuda_rpt2 of fc2d_cudaclaw_vtable_t
+ * (src/solvers/fc2d_cudaclaw/fc2d_cudaclaw.h:159-162). */
 enum {
     HIPCLAW_RP_ADVECTION = 0, /* clawpack46_rpn2adv.f / _rpt2adv.f, maux = 2 (u, v) */
     HIPCLAW_RP_SHALLOW = 1,   /* clawpack46_rpn2.f / _rpt2.f, Roe + entropy fix */
@@ -49,7 +51,7 @@ typedef struct hipclaw_config {
     int mx, my, mbc; /* cells per patch and ghost layers (mbc = 2) */
     int meqn, maux, mwaves;
     int order[2];    /* Clawpack method(2:3): order and transverse propagation */
-    int mthlim[4];   /* limiter per wave family */
+    int mthlim[10];  /* limiter per wave family (mwaves <= 10, FC2D_CUDACLAW_MWAVES) */
     int mthbc[4];    /* physical BC per domain face: 1 extrapolation, 3 wall */
     double grav;     /* shallow water */
     double gamma;    /* Euler */
@@ -58,6 +60,7 @@ typedef struct hipclaw_config {
                         (fclaw_options time-sync, src/fclaw_options.c:110); needs mx == my */
     int subcycle;    /* keep the state before each update and time-interpolated
                         copies for local time stepping (fclaw_options subcycle) */
+    double rp_params[8]; /* a registered solver's own constants (hipclaw_solver.h) */
 } hipclaw_config_t;
 
 /* Numbered ghost-fill and synchronisation programs per state; program 0 is

@@ -59,7 +59,7 @@ def _port_step(lib, solver, q, aux, dx, dt, mwaves, maux, grav, gamma):
     class Claw(C.Structure):
         _fields_ = [("solver", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
                     ("mwaves", C.c_int), ("method", C.c_int * 7), ("mthlim", C.c_int * 10),
-                    ("grav", C.c_double), ("gamma", C.c_double)]
+                    ("grav", C.c_double), ("gamma", C.c_double), ("p", C.c_double * 8)]
     c = Claw()
     c.solver, c.meqn, c.maux, c.mwaves = solver, q.shape[0], maux, mwaves
     for k, v in enumerate([0, 2, 2, 0, 0, 0, maux]):

@@ -24,7 +24,12 @@ extern "C" {
 enum {
     ORACLE_RP_ADVECTION = 0, /* applications/clawpack/advection/2d/all/clawpack46_rpn2adv.f, _rpt2adv.f */
     ORACLE_RP_SHALLOW   = 1, /* applications/clawpack/shallow/2d/rp/clawpack46_rpn2.f, _rpt2.f (efix on) */
-    ORACLE_RP_EULER     = 2  /* applications/clawpack/euler/2d/rp/clawpack46_rpn2_euler4.f, _rpt2_euler4.f */
+    ORACLE_RP_EULER     = 2, /* applications/clawpack/euler/2d/rp/clawpack46_rpn2_euler4.f, _rpt2_euler4.f */
+    ORACLE_RP_ACOUSTICS = 3, /* applications/clawpack/acoustics/2d/rp/clawpack46_rpn2_acoustics.f, _rpt2_acoustics.f */
+    ORACLE_RP_TRACERS   = 4  /* meqn = mwaves independent colour equations, tracer k advected with
+                                velocity c_k (u, v), c_k = (meqn - k) / meqn, k = 0..meqn-1: the
+                                advection solvers above applied per component (a test solver with
+                                up to 10 wave families; no reference counterpart) */
 };
 
 typedef struct oracle_claw {
@@ -34,6 +39,7 @@ typedef struct oracle_claw {
     int mthlim[10];   /* limiter per wave family (0..6) */
     double grav;      /* shallow water: common /cparam/ grav */
     double gamma;     /* Euler: common /cparam/ gamma, gamma1 = gamma - 1 */
+    double p[8];      /* acoustics: common /cparam/ rho, bulk, cc, zz */
 } oracle_claw_t;
 
 /* One patch step: clawpack46_step2_wrap.f (step2 + flux2 + q update).

@@ -369,6 +369,105 @@ static void rpt2_adv(int ixy, int maxm, int meqn, int mbc, int mx,
     }
 }
 
+/* ---------------- acoustics: clawpack46_rpn2_acoustics.f / _rpt2_acoustics.f ---------------- */
+
+/* applications/clawpack/acoustics/2d/rp/clawpack46_rpn2_acoustics.f:39-75:
+ * two waves, speeds -cc and cc, impedance zz */
+static void rpn2_acoustics(int ixy, int maxm, int meqn, int mwaves, int mbc, int mx,
+                           const double *ql, const double *qr, double cc, double zz,
+                           double *wave, double *s, double *amdq, double *apdq)
+{
+    const int N = maxm + 2 * mbc;
+    const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2;
+    int i, m;
+    for (i = 2 - mbc; i <= mx + mbc; i++) {
+        const double d1 = S1(ql, i, 1) - S1(qr, i - 1, 1);
+        const double d2 = S1(ql, i, mu) - S1(qr, i - 1, mu);
+        const double a1 = (-d1 + zz * d2) / (2.0 * zz);
+        const double a2 = (d1 + zz * d2) / (2.0 * zz);
+        WV(i, 1, 1) = -a1 * zz;
+        WV(i, mu, 1) = a1;
+        WV(i, mv, 1) = 0.0;
+        SP(i, 1) = -cc;
+        WV(i, 1, 2) = a2 * zz;
+        WV(i, mu, 2) = a2;
+        WV(i, mv, 2) = 0.0;
+        SP(i, 2) = cc;
+    }
+    for (m = 1; m <= meqn; m++)
+        for (i = 2 - mbc; i <= mx + mbc; i++) {
+            S1(amdq, i, m) = SP(i, 1) * WV(i, m, 1);
+            S1(apdq, i, m) = SP(i, 2) * WV(i, m, 2);
+        }
+}
+
+/* applications/clawpack/acoustics/2d/rp/clawpack46_rpt2_acoustics.f:36-61 */
+static void rpt2_acoustics(int ixy, int maxm, int mbc, int mx, double cc, double zz,
+                           const double *asdq, double *bmasdq, double *bpasdq)
+{
+    const int N = maxm + 2 * mbc;
+    const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2;
+    int i;
+    for (i = 2 - mbc; i <= mx + mbc; i++) {
+        const double a1 = (-S1(asdq, i, 1) + zz * S1(asdq, i, mv)) / (2.0 * zz);
+        const double a2 = (S1(asdq, i, 1) + zz * S1(asdq, i, mv)) / (2.0 * zz);
+        S1(bmasdq, i, 1) = cc * a1 * zz;
+        S1(bmasdq, i, mu) = 0.0;
+        S1(bmasdq, i, mv) = -cc * a1;
+        S1(bpasdq, i, 1) = cc * a2 * zz;
+        S1(bpasdq, i, mu) = 0.0;
+        S1(bpasdq, i, mv) = cc * a2;
+    }
+}
+
+/* ---------------- tracers: the advection solvers per component ---------------- */
+
+static double tracer_scale(int n, int k) { return (double) (n - k) / (double) n; }
+
+/* wave mw carries component mw only, at speed c_mw u_n; the fluctuations sum
+ * over every wave as rpn2adv does */
+static void rpn2_tracers(int ixy, int maxm, int meqn, int mwaves, int mbc, int mx,
+                         const double *ql, const double *qr, const double *auxl,
+                         double *wave, double *s, double *amdq, double *apdq)
+{
+    const int N = maxm + 2 * mbc;
+    int i, mq, mw;
+    for (i = 2 - mbc; i <= mx + mbc; i++) {
+        for (mw = 1; mw <= mwaves; mw++) {
+            for (mq = 1; mq <= meqn; mq++)
+                WV(i, mq, mw) = mq == mw ? S1(ql, i, mq) - S1(qr, i - 1, mq) : 0.0;
+            SP(i, mw) = tracer_scale(mwaves, mw - 1) * S1(auxl, i, ixy);
+        }
+        for (mq = 1; mq <= meqn; mq++) {
+            double am = 0.0, ap = 0.0;
+            for (mw = 1; mw <= mwaves; mw++) {
+                am = am + dmin(SP(i, mw), 0.0) * WV(i, mq, mw);
+                ap = ap + dmax(SP(i, mw), 0.0) * WV(i, mq, mw);
+            }
+            S1(amdq, i, mq) = am;
+            S1(apdq, i, mq) = ap;
+        }
+    }
+}
+
+/* rpt2adv per component with the component's transverse velocity */
+static void rpt2_tracers(int ixy, int maxm, int meqn, int mbc, int mx,
+                         const double *aux2, const double *aux3, int imp,
+                         const double *asdq, double *bmasdq, double *bpasdq)
+{
+    const int N = maxm + 2 * mbc;
+    const int kv = 3 - ixy;
+    int i, mq;
+    for (i = 2 - mbc; i <= mx + mbc; i++) {
+        const int i1 = i - 2 + imp;
+        for (mq = 1; mq <= meqn; mq++) {
+            const double c = tracer_scale(meqn, mq - 1);
+            S1(bmasdq, i, mq) = dmin(c * S1(aux2, i1, kv), 0.0) * S1(asdq, i, mq);
+            S1(bpasdq, i, mq) = dmax(c * S1(aux3, i1, kv), 0.0) * S1(asdq, i, mq);
+        }
+    }
+}
+
 /* ---------------- limiter: clawpack46_inlinelimiter.f:39-115 ---------------- */
 
 void oracle_inlinelimiter(int maxm, int meqn, int mwaves, int mbc, int mx,
@@ -466,6 +565,12 @@ static void flux2(const oracle_claw_t *claw, int ixy, int maxm, int meqn, int ma
     case ORACLE_RP_SHALLOW:
         rpn2_shallow(ixy, maxm, meqn, mwaves, mbc, mx, q1d, q1d, claw->grav, wave, s, amdq, apdq, roe);
         break;
+    case ORACLE_RP_ACOUSTICS:
+        rpn2_acoustics(ixy, maxm, meqn, mwaves, mbc, mx, q1d, q1d, claw->p[2], claw->p[3], wave, s, amdq, apdq);
+        break;
+    case ORACLE_RP_TRACERS:
+        rpn2_tracers(ixy, maxm, meqn, mwaves, mbc, mx, q1d, q1d, aux2, wave, s, amdq, apdq);
+        break;
     default:
         rpn2_euler(ixy, maxm, meqn, mwaves, mbc, mx, q1d, q1d, claw->gamma, wave, s, amdq, apdq, roe);
         break;
@@ -513,6 +618,12 @@ static void flux2(const oracle_claw_t *claw, int ixy, int maxm, int meqn, int ma
         case ORACLE_RP_SHALLOW:
             rpt2_shallow(ixy, maxm, meqn, mwaves, mbc, mx, amdq, bmasdq, bpasdq, roe);
             break;
+        case ORACLE_RP_ACOUSTICS:
+            rpt2_acoustics(ixy, maxm, mbc, mx, claw->p[2], claw->p[3], amdq, bmasdq, bpasdq);
+            break;
+        case ORACLE_RP_TRACERS:
+            rpt2_tracers(ixy, maxm, meqn, mbc, mx, aux2, aux3, 1, amdq, bmasdq, bpasdq);
+            break;
         default:
             rpt2_euler(ixy, maxm, meqn, mwaves, mbc, mx, amdq, bmasdq, bpasdq, roe);
             break;
@@ -534,6 +645,12 @@ static void flux2(const oracle_claw_t *claw, int ixy, int maxm, int meqn, int ma
             break;
         case ORACLE_RP_SHALLOW:
             rpt2_shallow(ixy, maxm, meqn, mwaves, mbc, mx, apdq, bmasdq, bpasdq, roe);
+            break;
+        case ORACLE_RP_ACOUSTICS:
+            rpt2_acoustics(ixy, maxm, mbc, mx, claw->p[2], claw->p[3], apdq, bmasdq, bpasdq);
+            break;
+        case ORACLE_RP_TRACERS:
+            rpt2_tracers(ixy, maxm, meqn, mbc, mx, aux2, aux3, 2, apdq, bmasdq, bpasdq);
             break;
         default:
             rpt2_euler(ixy, maxm, meqn, mwaves, mbc, mx, apdq, bmasdq, bpasdq, roe);
@@ -569,6 +686,12 @@ void oracle_rpn2(const oracle_claw_t *claw, int ixy, int maxm, int mbc, int mx,
         break;
     case ORACLE_RP_SHALLOW:
         rpn2_shallow(ixy, maxm, claw->meqn, claw->mwaves, mbc, mx, q1d, q1d, claw->grav, wave, s, amdq, apdq, &roe);
+        break;
+    case ORACLE_RP_ACOUSTICS:
+        rpn2_acoustics(ixy, maxm, claw->meqn, claw->mwaves, mbc, mx, q1d, q1d, claw->p[2], claw->p[3], wave, s, amdq, apdq);
+        break;
+    case ORACLE_RP_TRACERS:
+        rpn2_tracers(ixy, maxm, claw->meqn, claw->mwaves, mbc, mx, q1d, q1d, aux1d, wave, s, amdq, apdq);
         break;
     default:
         rpn2_euler(ixy, maxm, claw->meqn, claw->mwaves, mbc, mx, q1d, q1d, claw->gamma, wave, s, amdq, apdq, &roe);

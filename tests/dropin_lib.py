@@ -70,7 +70,7 @@ class Options(C.Structure):
                 ("src_term", C.c_int), ("use_fwaves", C.c_int), ("ascii_out", C.c_int),
                 ("vtk_out", C.c_int), ("buffer_len", C.c_int), ("precision", C.c_int),
                 ("tile", C.c_int), ("grav", C.c_double), ("gamma", C.c_double),
-                ("is_registered", C.c_int)]
+                ("rp_params", C.c_double * 8), ("is_registered", C.c_int)]
 
 
 FORT_QINIT = C.CFUNCTYPE(None, *([C.POINTER(C.c_int)] * 6 + [C.POINTER(C.c_double)] * 4 +
@@ -80,7 +80,8 @@ FORT_QINIT = C.CFUNCTYPE(None, *([C.POINTER(C.c_int)] * 6 + [C.POINTER(C.c_doubl
 class Vtable(C.Structure):
     _fields_ = [("b4step2", C.c_void_p), ("src2", C.c_void_p), ("fort_setprob", C.c_void_p),
                 ("fort_bc2", C.c_void_p), ("fort_qinit", C.c_void_p), ("fort_setaux", C.c_void_p),
-                ("fort_src2", C.c_void_p), ("riemann_solver", C.c_int), ("is_set", C.c_int)]
+                ("fort_src2", C.c_void_p), ("riemann_solver", C.c_int), ("rp_solver", C.c_void_p),
+                ("is_set", C.c_int)]
 
 
 def bind():

@@ -46,6 +46,15 @@ def claw_for(solver, mthlim=None, order=(2, 2), grav=1.0, gamma=1.4):
                      gamma=gamma)
 
 
+def prob_claw(prob, order=(2, 2)):
+    """The oracle's method parameters for a problem (its solver, constants and
+    limiters)."""
+    meqn, mwaves, maux = fa.solver_shape(prob.solver)
+    solver = prob.oracle_solver if prob.oracle_solver is not None else prob.solver
+    return make_claw(solver, meqn, maux, mwaves, order=order, mthlim=prob.mthlim,
+                     grav=prob.grav, gamma=prob.gamma, params=prob.rp_params)
+
+
 def oracle_domain(prob, forest, mx):
     lv, gx, gy = forest.leaves()
     return domain_struct(lv, gx, gy, prob.nbx, prob.nby, prob.periodic, mx, mx, 2, prob.meqn,
@@ -61,7 +70,7 @@ def oracle_ghost_update(prob, forest, q, mx):
 
 def oracle_advance(prob, forest, q, aux, mx, dt, nsteps):
     d = oracle_domain(prob, forest, mx)
-    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma)
+    claw = prob_claw(prob)
     out = np.ascontiguousarray(q, dtype=np.float64).copy()
     auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
     cfl = oracle().oracle_advance(C.byref(d), C.byref(claw), out.reshape(-1),
@@ -72,7 +81,7 @@ def oracle_advance(prob, forest, q, aux, mx, dt, nsteps):
 def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True, order=(2, 2)):
     """nsteps of (update all levels, ghost update, time sync) on the oracle."""
     d = oracle_domain(prob, forest, mx)
-    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma, order=order)
+    claw = prob_claw(prob, order=order)
     out = np.ascontiguousarray(q, dtype=np.float64).copy()
     auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
     cfl = oracle().oracle_advance_ts(C.byref(d), C.byref(claw), out.reshape(-1),
@@ -84,7 +93,7 @@ def oracle_advance_ts(prob, forest, q, aux, mx, dt, nsteps, time_sync=True, orde
 def oracle_advance_subcycle(prob, forest, q, aux, mx, dt, ncoarse=1, time_sync=True):
     """ncoarse calls of fclaw2d_advance_all_levels(dt) with subcycling on the oracle."""
     d = oracle_domain(prob, forest, mx)
-    claw = claw_for(prob.solver, grav=prob.grav, gamma=prob.gamma)
+    claw = prob_claw(prob)
     out = np.ascontiguousarray(q, dtype=np.float64).copy()
     auxp = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
     cfl = 0.0
@@ -113,7 +122,7 @@ def setup_problem(prob, precision=fa.FP64, tile=0, time_sync=False):
     q0 = oracle_ghost_update(prob, forest, q0, prob.mx)
     sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
                    grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
-                   time_sync=time_sync)
+                   time_sync=time_sync, rp_params=prob.rp_params, mthlim=prob.mthlim)
     sv.set_forest(forest, prob.extent)
     sv.set_q(q0)
     if aux is not None:

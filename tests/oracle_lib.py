@@ -12,6 +12,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle46.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref46.so")
+REF_ACOUSTICS_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_acoustics.so")
 REF_AMR_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_amr.so")
 REF_SYNC_SO = os.path.join(ROOT, "oracle", "_ref", "libref46_sync.so")
 
@@ -24,7 +25,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 class OracleClaw(C.Structure):
     _fields_ = [("solver", C.c_int), ("meqn", C.c_int), ("maux", C.c_int),
                 ("mwaves", C.c_int), ("method", C.c_int * 7), ("mthlim", C.c_int * 10),
-                ("grav", C.c_double), ("gamma", C.c_double)]
+                ("grav", C.c_double), ("gamma", C.c_double), ("p", C.c_double * 8)]
 
 
 class OracleDomain(C.Structure):
@@ -116,7 +117,8 @@ def ref():
     return _ref
 
 
-def make_claw(solver, meqn, maux, mwaves, order=(2, 2), mthlim=None, grav=1.0, gamma=1.4):
+def make_claw(solver, meqn, maux, mwaves, order=(2, 2), mthlim=None, grav=1.0, gamma=1.4,
+              params=None):
     c = OracleClaw()
     c.solver, c.meqn, c.maux, c.mwaves = solver, meqn, maux, mwaves
     method = [0, order[0], order[1], 0, 0, 0, maux]
@@ -126,6 +128,8 @@ def make_claw(solver, meqn, maux, mwaves, order=(2, 2), mthlim=None, grav=1.0, g
     for k in range(10):
         c.mthlim[k] = lim[k] if k < len(lim) else 0
     c.grav, c.gamma = grav, gamma
+    for k, v in enumerate(params or ()):
+        c.p[k] = v
     return c
 
 
@@ -165,6 +169,32 @@ def step2_ref(claw, q, aux, dx, dy, dt, want_fluxes=False):
     if want_fluxes:
         return qq, cfl.value, [fm, fp, gm, gp]
     return qq, cfl.value
+
+
+_ref_ac = None
+
+
+def step2_ref_acoustics(claw, q, dx, dy, dt):
+    """The reference's own patch step with its acoustics solvers
+    (oracle/_ref/libref46_acoustics.so); claw.p = (rho, bulk, ...)."""
+    global _ref_ac
+    if _ref_ac is None:
+        lib = C.CDLL(REF_ACOUSTICS_SO)
+        lib.ref46_acoustics_step2.argtypes = [C.c_int] * 3 + [_ip, _ip, _dp] + [C.c_double] * 5 + \
+            [C.POINTER(C.c_double), _dp, _dp, _dp, _dp]
+        lib.ref46_acoustics_step2.restype = C.c_int
+        _ref_ac = lib
+    meqn, ny, nx = q.shape
+    mbc = 2
+    qq = np.ascontiguousarray(q, dtype=np.float64).copy()
+    cfl = C.c_double(0.0)
+    fl = [np.zeros_like(qq) for _ in range(4)]
+    rc = _ref_ac.ref46_acoustics_step2(nx - 2 * mbc, ny - 2 * mbc, mbc,
+                                       np.array(list(claw.method), dtype=np.int32),
+                                       np.array(list(claw.mthlim), dtype=np.int32), qq,
+                                       dx, dy, dt, claw.p[0], claw.p[1], C.byref(cfl), *fl)
+    assert rc == 0
+    return qq, cfl.value, fl
 
 
 def domain_struct(level, gx, gy, nbx, nby, periodic, mx, my, mbc, meqn, maux, mthbc, extent):
