@@ -115,7 +115,7 @@ def lib():
         "hipclaw_ghost_pack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_ghost_unpack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_get_registers": ([vp, i, i, _dp()], i),
-        "hipclaw_register_solver": ([C.POINTER(HipclawSolverDesc), C.POINTER(i)], i),
+        "hipclaw_register_solver": ([vp, C.POINTER(i)], i),
         "hipclaw_step_batch_noalloc": ([C.POINTER(HipclawConfig), i, vp, vp, vp, vp, vp, _ip(), i,
                                         d, vp, C.POINTER(d)], i),
     }
@@ -150,7 +150,8 @@ def register_solver(path, name):
     fn.argtypes = []
     d = fn()
     sid = C.c_int(-1)
-    _check(lib().hipclaw_register_solver(d, C.byref(sid)), "hipclaw_register_solver")
+    _check(lib().hipclaw_register_solver(C.cast(d, C.c_void_p), C.byref(sid)),
+           "hipclaw_register_solver")
     _user_libs.append(L)
     _user_shapes[sid.value] = (d.contents.meqn, d.contents.mwaves, d.contents.maux)
     return sid.value

@@ -299,17 +299,26 @@ static void set_method(hipclaw::StepArgs& a, const hipclaw_config_t* c) {
     for (int k = 0; k < 8; k++) a.pp.p[k] = c->rp_params[k];
 }
 
+// Tiles per patch in one direction: the last tile of a patch whose size the
+// tile does not divide overlaps the one before it (the step kernel moves it
+// back to end at the patch edge).
+static int tiles_along(int n, int t) { return (n + t - 1) / t; }
+
 // Tile size measured fastest on MI355X for the update kernel (the tile sweep of
 // the benchmark forest, mx = 16 and 32): 16 x 16 for both forms (four waves
 // per workgroup for FP64, two for the FP32-bulk form); smaller patches take
-// 8 x 8.  hipclaw_autotune re-measures on the caller's patches.
+// 8 x 8.  Where neither divides the patch, the tile with the fewest Riemann
+// solves per patch, (T + 2) (T + 3) per tile, counting the overlapped tiles
+// (ties to the larger).  hipclaw_autotune re-measures on the caller's patches.
 static int default_tile(const hipclaw_config_t* c) {
-    const int m = std::min(c->mx, c->my);
-    int t = 16;
-    while (t > 8 && (t > m || c->mx % t != 0 || c->my % t != 0 ||
-                     !solver_tile_ok(c->solver, c->precision, t)))
-        t >>= 1;
-    return t;
+    int best = 0;
+    long long best_cost = 0;
+    for (int t = 16; t >= 8; t >>= 1) {
+        if (t > c->mx || t > c->my || !solver_tile_ok(c->solver, c->precision, t)) continue;
+        const long long cost = (long long) tiles_along(c->mx, t) * tiles_along(c->my, t) * (t + 2) * (t + 3);
+        if (!best || cost < best_cost) { best = t; best_cost = cost; }
+    }
+    return best ? best : 8;
 }
 
 // Checks a configuration for the step kernels; *tile_out = the tile to use.
@@ -329,13 +338,14 @@ static int check_config(const hipclaw_config_t* cfg, int* tile_out) {
         return fail(HIPCLAW_EINVAL, "tile size not available for this solver and precision%s");
     if (cfg->time_sync && cfg->precision == HIPCLAW_FP64 && tile == 32)
         return fail(HIPCLAW_EINVAL, "FP64 tiles of 32 leave no LDS for the time-sync edge buffers%s");
-    if (cfg->mx % tile != 0 || cfg->my % tile != 0)
-        return fail(HIPCLAW_EINVAL, "mx and my must be multiples of the tile size%s");
+    if (cfg->mx < tile || cfg->my < tile)
+        return fail(HIPCLAW_EINVAL, "mx and my must be at least the tile size (8)%s");
     if (cfg->order[0] < 1 || cfg->order[0] > 2 || cfg->order[1] < 0 || cfg->order[1] > 2)
         return fail(HIPCLAW_EINVAL, "order must be (1|2, 0|1|2)%s");
     for (int k = 0; k < cfg->mwaves; k++)
         if (cfg->mthlim[k] < 0 || cfg->mthlim[k] > 6) return fail(HIPCLAW_EINVAL, "mthlim out of range%s");
-    if (cfg->mx != cfg->my) return fail(HIPCLAW_EINVAL, "the batched step requires mx = my%s");
+    if (cfg->time_sync && cfg->mx != cfg->my)
+        return fail(HIPCLAW_EINVAL, "time sync requires mx = my%s");
     *tile_out = tile;
     return HIPCLAW_OK;
 }
@@ -354,8 +364,8 @@ int hipclaw_create(const hipclaw_config_t* cfg, int max_patches, hipclaw_state_t
         solver_shape(cfg->solver, &sh);
         s->conservative = sh.conservative;
     }
-    s->ntx = cfg->mx / tile;
-    s->nty = cfg->my / tile;
+    s->ntx = tiles_along(cfg->mx, tile);
+    s->nty = tiles_along(cfg->my, tile);
     s->inplace = (s->ntx == 1 && s->nty == 1);
     s->max_patches = max_patches;
     // measured on MI355X: per-destination groups pay off once a patch's ghost
@@ -717,6 +727,11 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         const int* op = ops + 8 * o;
         const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
         while (o >= phase_start[ph + 1]) ph++;
+        // the reference's coarse/fine exchanges bound their face loops by mx
+        // (fclaw2d_clawpatch46_interpolate.f, fclaw2d_clawpatch46_average.f:
+        // average2coarse offsets children by mx in both directions)
+        if (mx != my && type >= hipclaw::GHOST_AVERAGE_FACE && type <= hipclaw::GHOST_INTERP_CORNER)
+            return fail(HIPCLAW_EINVAL, "coarse/fine ghost exchange requires mx = my%s");
         if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
             return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
         if (!(phase_types[ph] & (1 << type)))
@@ -1220,14 +1235,15 @@ int hipclaw_restore_step(hipclaw_state_t* s) {
 static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool timed);
 
 // Tile sizes a state can switch to: supported by its solver and precision,
-// dividing the patch, and (FP64 with time sync) leaving LDS for the registers.
+// no larger than the patch, and (FP64 with time sync) leaving LDS for the
+// registers.
 static bool tile_allowed(const hipclaw_state_t* s, int t) {
-    return solver_tile_ok(s->cfg.solver, s->cfg.precision, t) && s->cfg.mx % t == 0 &&
-           s->cfg.my % t == 0 && !(s->cfg.time_sync && s->cfg.precision == HIPCLAW_FP64 && t == 32);
+    return solver_tile_ok(s->cfg.solver, s->cfg.precision, t) && t <= s->cfg.mx &&
+           t <= s->cfg.my && !(s->cfg.time_sync && s->cfg.precision == HIPCLAW_FP64 && t == 32);
 }
 
 static int set_tile(hipclaw_state_t* s, int t) {
-    const int ntx = s->cfg.mx / t, nty = s->cfg.my / t;
+    const int ntx = tiles_along(s->cfg.mx, t), nty = tiles_along(s->cfg.my, t);
     const bool inplace = ntx == 1 && nty == 1;
     if (!inplace && !s->q[1]) {   // a ping-pong tile needs the second slab
         if (hipMalloc((void**) &s->q[1], sizeof(double) * s->pstride * (size_t) s->max_patches) != hipSuccess)
@@ -1488,7 +1504,7 @@ int hipclaw_step_batch_noalloc(const hipclaw_config_t* cfg, int npatch, const do
     if (int rc = check_config(cfg, &tile)) return rc;
     if ((cfg->maux > 0) != (aux != nullptr))
         return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: aux must be given exactly when maux > 0%s");
-    const int ntx = cfg->mx / tile, nty = cfg->my / tile;
+    const int ntx = tiles_along(cfg->mx, tile), nty = tiles_along(cfg->my, tile);
     if (q_in == q_out && (ntx > 1 || nty > 1))
         return fail(HIPCLAW_EINVAL, "hipclaw_step_batch_noalloc: q_in and q_out must differ when a "
                                     "tile does not cover a patch%s");
@@ -1789,6 +1805,8 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
             : k == hipclaw::REGRID_PARENT ? !(ok(sp[0]) && ok(sp[1]) && ok(sp[2]) && ok(sp[3]))
             : true)
             return fail(HIPCLAW_EINVAL, "regrid source with bad kind or slot%s");
+        if (k != hipclaw::REGRID_SAME && s->cfg.mx != s->cfg.my)
+            return fail(HIPCLAW_EINVAL, "refining or coarsening a patch requires mx = my%s");
         op[0] = k;
         op[1] = p;
         for (int g = 0; g < 4; g++) op[2 + g] = sp[g];

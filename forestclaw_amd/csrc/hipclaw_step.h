@@ -244,6 +244,13 @@ step_tiles(StepArgs a) {
         ty = tix / a.ntx;
         return a.work[wid];
     };
+    // Cells before the tile in x (y): tiles step by T, and when T does not
+    // divide mx (my) the last tile is moved back to end at the patch edge, so
+    // it overlaps its neighbour.  Both compute the shared cells from the same
+    // inputs with the same code and write the same bits; registers along a
+    // patch edge are written by the tile that owns the position (the first).
+    auto origin_x = [&](int tx) { return min(tx * T, mx - T); };
+    auto origin_y = [&](int ty) { return min(ty * T, my - T); };
     // Staging map, the same for every tile: element e = threadIdx.x + kk*NT of
     // the (field m, tile row qj, tile column qi) halo tile sits at goff[kk] +
     // (tile origin) in the patch slab and at loff[kk] in LDS (-1: no element).
@@ -294,7 +301,7 @@ step_tiles(StepArgs a) {
     auto prefetch = [&](int tile) {
         int tx, ty;
         const int slot = tile_slot(tile, tx, ty);
-        const long long shift = (long long) (ty * T) * nxp + tx * T;  // tile origin in the slab
+        const long long shift = (long long) origin_y(ty) * nxp + origin_x(tx);  // tile origin in the slab
         const double* __restrict__ qp = a.qsrc + (long long) slot * a.pstride + shift;
         if (a.pull) {
 #pragma unroll
@@ -365,7 +372,9 @@ step_tiles(StepArgs a) {
     for (; tile < ntiles; tile += G) {
         int tx, ty;
         const int slot = tile_slot(tile, tx, ty);
-        const int i0 = 1 + tx * T, j0 = 1 + ty * T;  // first interior cell of the tile (1-based)
+        const int i0 = 1 + origin_x(tx), j0 = 1 + origin_y(ty);  // first interior cell of the tile (1-based)
+        // cells of an overlapping last tile that the tile before it owns
+        const int own_x = tx * T + 1 - i0, own_y = ty * T + 1 - j0;
         const double dt = a.dtscale ? a.dt * a.dtscale[slot] : a.dt;
         const double dtdx64 = dt / a.dxs[slot], dtdy64 = dt / a.dys[slot];
         const R dtdx = (R) dtdx64, dtdy = (R) dtdy64;
@@ -400,7 +409,7 @@ step_tiles(StepArgs a) {
                     if (!((tb >> face) & 1)) continue;
                     // patch cell along the strip: j (faces 0, 1) or i (faces 2, 3), 1-based
                     const int len = face < 2 ? my : mx;
-                    const int c = (face < 2 ? ty : tx) * T + q - 1;
+                    const int c = (face < 2 ? j0 : i0) + q - 2;
                     if (c < 1 || c > len) continue;
                     const int other = (c == 1 || c == len) ? (face < 2 ? (c == 1 ? 2 : 3) : (c == 1 ? 0 : 1)) : -1;
                     const bool other_on = other >= 0 && ((mask >> other) & 1);
@@ -436,7 +445,7 @@ step_tiles(StepArgs a) {
                 else if (face == 1) { on = last_x; qi = T + 1 + kio; qj = c + 2; }
                 else if (face == 2) { on = first_y; qi = c + 2; qj = 2 - kio; }
                 else { on = last_y; qi = c + 2; qj = T + 1 + kio; }
-                if (!on) continue;
+                if (!on || c < (face < 2 ? own_y : own_x)) continue;
                 double qcell[MEQN], acell[MAUX > 0 ? MAUX : 1], fl[MEQN];
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) qcell[m] = SQD(m, qi, qj);
@@ -711,7 +720,7 @@ step_tiles(StepArgs a) {
                 for (int e = threadIdx.x; e < 2 * MEQN * T; e += NT) {
                     const int side = e / (MEQN * T), rem = e - side * (MEQN * T);
                     const int m = rem / T, b = rem - m * T;
-                    if (side == 0 ? !first_x : !last_x) continue;
+                    if ((side == 0 ? !first_x : !last_x) || b < own_y) continue;
                     const int s0 = side ? T : 0;
                     const R fmv = ((R(0) + s_fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
                     const R fpv = ((R(0) + s_fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
@@ -723,7 +732,7 @@ step_tiles(StepArgs a) {
                 for (int e = threadIdx.x; e < 2 * MEQN * T; e += NT) {
                     const int side = e / (MEQN * T), rem = e - side * (MEQN * T);
                     const int m = rem / T, c = rem - m * T;
-                    if (side == 0 ? !first_y : !last_y) continue;
+                    if ((side == 0 ? !first_y : !last_y) || c < own_x) continue;
                     const int face = 2 + side, pos = i0 - 1 + c;
                     reg_add(rg + ((face * 4 + REG_FM) * MEQN + m) * mx + pos,
                             dtdx_el * (double) sm.eg[((side * 2 + 0) * MEQN + m) * T + c], a.reg_accumulate);
@@ -760,6 +769,7 @@ step_tiles(StepArgs a) {
                 for (int e = threadIdx.x; e < 4 * T; e += NT) {
                     const int face = e / T, c = e - face * T;
                     if (face == 0 ? !first_x : face == 1 ? !last_x : face == 2 ? !first_y : !last_y) continue;
+                    if (c < (face < 2 ? own_y : own_x)) continue;
                     double f[MEQN], g[MEQN];
                     if (face < 2) cell_fluxes(face == 0 ? 1 : T + 1, c + 2, f, g);
                     else cell_fluxes(c + 2, face == 2 ? 1 : T + 1, f, g);

@@ -44,6 +44,11 @@ class Problem:
     mthlim: tuple = None      # limiter per wave family (None: MC for every family)
     max_speed: float = None   # largest wave speed, for solvers stable_dt does not know
     oracle_solver: int = None # the oracle's number for this solver (None: the same)
+    my: int = None            # cells per patch in y (None: mx)
+
+    def __post_init__(self):
+        if self.my is None:
+            self.my = self.mx
 
     @property
     def meqn(self):
@@ -65,14 +70,14 @@ def advection_problem(minlevel=4, maxlevel=4, mx=16):
                    (2, 2, 2, 2), minlevel, maxlevel, mx, 0.9, q, aux, tag_threshold=0.5)
 
 
-def dambreak_problem(minlevel=3, maxlevel=6, mx=32, r0=0.2, hin=2.0, hout=1.0):
+def dambreak_problem(minlevel=3, maxlevel=6, mx=32, r0=0.2, hin=2.0, hout=1.0, my=None):
     def q(x, y):
         h = np.where((x - 0.5) ** 2 + (y - 0.5) ** 2 < r0 ** 2, hin, hout)
         return np.stack([h, np.zeros_like(h), np.zeros_like(h)])
 
     return Problem("dambreak", RP_SHALLOW, (0.0, 1.0, 0.0, 1.0), 1, 1, (False, False),
                    (1, 1, 1, 1), minlevel, maxlevel, mx, 0.9, q, grav=1.0,
-                   params=dict(r0=r0, hin=hin, hout=hout))
+                   params=dict(r0=r0, hin=hin, hout=hout), my=my)
 
 
 def shockbubble_problem(minlevel=2, maxlevel=7, mx=16, gamma=1.4, mach=1.5):
@@ -219,8 +224,8 @@ def cell_centres(xlower, ylower, dx, dy, mx, my, mbc):
 
 
 def initial_data(prob, forest, mbc=2):
-    xl, yl, dx, dy = forest.geometry(prob.extent, prob.mx, prob.mx)
-    x, y = cell_centres(xl, yl, dx, dy, prob.mx, prob.mx, mbc)
+    xl, yl, dx, dy = forest.geometry(prob.extent, prob.mx, prob.my)
+    x, y = cell_centres(xl, yl, dx, dy, prob.mx, prob.my, mbc)
     q = np.ascontiguousarray(np.moveaxis(prob.qfun(x, y), 0, 1))  # (npatch, meqn, ny, nx)
     aux = None
     if prob.auxfun is not None:
@@ -231,7 +236,7 @@ def initial_data(prob, forest, mbc=2):
 def tag_gradient(prob, forest, q, mbc=2):
     """Patches where the centred gradient of the tag field exceeds the
     threshold in some interior cell (fclaw2d_clawpatch_gradient_exceeds_th)."""
-    _, _, dx, dy = forest.geometry(prob.extent, prob.mx, prob.mx)
+    _, _, dx, dy = forest.geometry(prob.extent, prob.mx, prob.my)
     f = q[:, prob.tag_field]
     dqx = (f[:, mbc:-mbc, mbc + 1:f.shape[2] - mbc + 1] - f[:, mbc:-mbc, mbc - 1:f.shape[2] - mbc - 1]) \
         / (2 * dx[:, None, None])
@@ -258,7 +263,8 @@ def build_forest(prob, mbc=2):
 def stable_dt(prob, forest, q, mbc=2):
     """dt giving Courant number prob.cfl on the finest patches for the initial
     data (the reference sets initial_dt; here it is derived from the data)."""
-    _, _, dx, _ = forest.geometry(prob.extent, prob.mx, prob.mx)
+    _, _, dx, dy = forest.geometry(prob.extent, prob.mx, prob.my)
+    dx = np.minimum(dx, dy)
     inner = q[:, :, mbc:-mbc, mbc:-mbc]
     if prob.solver == RP_SHALLOW:
         h = inner[:, 0]

@@ -48,14 +48,18 @@ enum {
 typedef struct hipclaw_config {
     int solver;      /* HIPCLAW_RP_* */
     int precision;   /* HIPCLAW_FP64 or HIPCLAW_FP32_BULK */
-    int mx, my, mbc; /* cells per patch and ghost layers (mbc = 2) */
+    int mx, my, mbc; /* cells per patch (mx, my >= 8, any size as the reference's
+                        cudaclaw_flux2.cu:278-281; coarse/fine exchanges and time sync
+                        need mx == my) and ghost layers (mbc = 2) */
     int meqn, maux, mwaves;
     int order[2];    /* Clawpack method(2:3): order and transverse propagation */
     int mthlim[10];  /* limiter per wave family (mwaves <= 10, FC2D_CUDACLAW_MWAVES) */
     int mthbc[4];    /* physical BC per domain face: 1 extrapolation, 3 wall */
     double grav;     /* shallow water */
     double gamma;    /* Euler */
-    int tile;        /* cells per tile side (8, 16, 32); 0 = tuned default */
+    int tile;        /* cells per tile side (8, 16, 32; at most mx and my); 0 = tuned
+                        default.  A tile that does not divide the patch overlaps its
+                        last tile with the one before */
     int time_sync;   /* keep flux registers for the conservation fix-up
                         (fclaw_options time-sync, src/fclaw_options.c:110); needs mx == my */
     int subcycle;    /* keep the state before each update and time-interpolated

@@ -57,7 +57,8 @@ def prob_claw(prob, order=(2, 2)):
 
 def oracle_domain(prob, forest, mx):
     lv, gx, gy = forest.leaves()
-    return domain_struct(lv, gx, gy, prob.nbx, prob.nby, prob.periodic, mx, mx, 2, prob.meqn,
+    my = prob.my if mx == prob.mx else mx
+    return domain_struct(lv, gx, gy, prob.nbx, prob.nby, prob.periodic, mx, my, 2, prob.meqn,
                          prob.maux, prob.mthbc, prob.extent)
 
 
@@ -120,7 +121,7 @@ def setup_problem(prob, precision=fa.FP64, tile=0, time_sync=False):
     forest = P.build_forest(prob)
     q0, aux = P.initial_data(prob, forest)
     q0 = oracle_ghost_update(prob, forest, q0, prob.mx)
-    sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
+    sv = fa.Solver(prob.solver, prob.mx, prob.my, precision=precision, mthbc=prob.mthbc,
                    grav=prob.grav, gamma=prob.gamma, tile=tile, max_patches=forest.npatch,
                    time_sync=time_sync, rp_params=prob.rp_params, mthlim=prob.mthlim)
     sv.set_forest(forest, prob.extent)

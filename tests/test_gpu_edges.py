@@ -2,7 +2,8 @@
 
   * a patch width that is a multiple of 8 but not of 16 (mx = 24: three 8x8
     tiles per row) on an AMR forest, FP64 with time sync, against the oracle;
-  * a width the tiles cannot cover (mx = 20) is refused with a message;
+  * a patch narrower than the smallest tile (mx = 6) is refused with a
+    message (widths the tiles do not divide are covered in test_gpu_shapes);
   * a forest of one patch (level 0: every ghost cell comes from the physical
     boundary conditions), FP64 against the oracle and FP32-bulk within 1e-5;
   * an advance of zero steps leaves q unchanged and reports a Courant
@@ -31,9 +32,11 @@ def test_mx24_amr_fp64_time_sync(gpu):
     assert abs(cfl - cfl_o) <= 1e-12 * cfl_o
 
 
-def test_mx20_is_refused(gpu):
-    with pytest.raises(fa.HipclawError, match="multiples of the tile size"):
-        fa.Solver(fa.RP_SHALLOW, 20, 20, precision=fa.FP64, mthbc=(1, 1, 1, 1), max_patches=4)
+def test_patch_narrower_than_a_tile_is_refused(gpu):
+    with pytest.raises(fa.HipclawError, match="at least the tile size"):
+        fa.Solver(fa.RP_SHALLOW, 6, 6, precision=fa.FP64, mthbc=(1, 1, 1, 1), max_patches=4)
+    with pytest.raises(fa.HipclawError, match="at least the tile size"):
+        fa.Solver(fa.RP_SHALLOW, 32, 12, precision=fa.FP64, tile=16, max_patches=4)
 
 
 @pytest.mark.parametrize("precision", [fa.FP64, fa.FP32_BULK])

@@ -38,10 +38,12 @@ def test_library_exports_all_declared_symbols():
     assert not missing, missing
 
 
-def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+def test_missing_library_fails_loudly(tmp_path):
     import importlib
-    monkeypatch.setattr(fa, "LIB_PATH", str(tmp_path / "nope.so"))
-    monkeypatch.setattr(fa, "_lib", None)
+    # the reload below restores the module (its path, a fresh library handle
+    # and the struct classes the handle's signatures name)
+    fa.LIB_PATH = str(tmp_path / "nope.so")
+    fa._lib = None
     try:
         fa.lib()
     except RuntimeError as e:

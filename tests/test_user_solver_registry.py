@@ -43,13 +43,13 @@ def test_registration_refuses_a_foreign_build():
     bad = fa.HipclawSolverDesc.from_buffer_copy(a.contents)
     sid = C.c_int(-1)
     bad.abi = 99
-    assert fa.lib().hipclaw_register_solver(C.byref(bad), C.byref(sid)) != 0
+    assert fa.lib().hipclaw_register_solver(C.cast(C.pointer(bad), C.c_void_p), C.byref(sid)) != 0
     bad = fa.HipclawSolverDesc.from_buffer_copy(a.contents)
     bad.step_args_size += 8
-    assert fa.lib().hipclaw_register_solver(C.byref(bad), C.byref(sid)) != 0
+    assert fa.lib().hipclaw_register_solver(C.cast(C.pointer(bad), C.c_void_p), C.byref(sid)) != 0
     bad = fa.HipclawSolverDesc.from_buffer_copy(a.contents)
     bad.mwaves = 11
-    assert fa.lib().hipclaw_register_solver(C.byref(bad), C.byref(sid)) != 0
+    assert fa.lib().hipclaw_register_solver(C.cast(C.pointer(bad), C.c_void_p), C.byref(sid)) != 0
 
 
 def test_configuration_checks_against_the_registered_solver():
