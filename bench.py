@@ -83,6 +83,65 @@ def run_form(fa, prob, forest, q0, aux, precision, dt, steps, warmup, tile, use_
                 waves=sv.waves)
 
 
+def run_subcycle(fa, prob, forest, q0, aux, precision, dt_fine, ncoarse, warmup, use_graph,
+                 time_sync):
+    """The subcycled advance (fclaw2d_advance_all_levels with subcycle = T) on
+    the same forest: level L takes 2^(L - minlevel) steps per coarse step of
+    dt_fine * 2^(maxlevel - minlevel), with time interpolation, partial ghost
+    fills and time syncs between the finest steps, all enqueued on the device
+    (hipclaw_advance_subcycle) and timed as one captured graph of ncoarse
+    coarse steps.  Cell updates count each level's own steps."""
+    lv = forest.leaves()[0]
+    lo, hi = int(lv.min()), int(lv.max())
+    sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=precision, mthbc=prob.mthbc,
+                   grav=prob.grav, gamma=prob.gamma, max_patches=forest.npatch,
+                   time_sync=time_sync, subcycle=True)
+    sv.set_forest(forest, prob.extent)
+    sv.set_q(q0)
+    if aux is not None:
+        sv.set_aux(aux)
+    dt = dt_fine * 2 ** (hi - lo)
+    tuned = {"%d/%d/%d" % k: v for k, v in sv.autotune(dt, 1).items()}
+    mass0 = sv.mass()
+    if warmup > 0:
+        sv.advance_subcycle_async(dt, warmup)
+    sv.sync()
+    if use_graph:
+        sv.capture_subcycle_graph(dt, ncoarse)
+        sv.sync()
+    t0 = time.perf_counter()
+    if use_graph:
+        sv.replay_graph()
+    else:
+        sv.advance_subcycle_async(dt, ncoarse)
+    sv.sync()
+    wall = time.perf_counter() - t0
+    mass1 = sv.mass()
+    updates = int(np.sum(2 ** (lv.astype(np.int64) - lo))) * prob.mx * prob.mx * ncoarse
+    return dict(wall=wall, updates=updates, ncoarse=ncoarse, dt=dt, tile=sv.tile, waves=sv.waves,
+                ghost_layout=sv.ghost_layout, autotune=tuned, graph=use_graph,
+                mass_change=float((mass1[0][0] - mass0[0][0]) / mass0[0][0]),
+                fine_steps_per_coarse=2 ** (hi - lo))
+
+
+def subcycle_summary(r, cells, value_nosub):
+    """Throughput of the subcycled advance, and the simulated time it covers
+    per second of wall time against the same forest advanced without
+    subcycling (every level at the finest dt, value_nosub cell updates/s)."""
+    ms_per_coarse = r["wall"] * 1e3 / r["ncoarse"]
+    gain = None
+    if value_nosub:
+        gain = round(cells * r["fine_steps_per_coarse"] / value_nosub * 1e3 / ms_per_coarse, 3)
+    return {"value": round(r["updates"] / r["wall"], 1), "unit": "cell-updates/s",
+            "ms_per_coarse_step": round(ms_per_coarse, 4), "coarse_steps": r["ncoarse"],
+            "cell_updates_per_coarse_step": r["updates"] // r["ncoarse"],
+            "fine_steps_per_coarse_step": r["fine_steps_per_coarse"],
+            "coarse_dt": r["dt"], "tile": r["tile"], "waves": r["waves"],
+            "ghost_layout": r["ghost_layout"], "autotune_ms_per_coarse_step": r["autotune"],
+            "timed_with_hip_graph": r["graph"], "mass_change": r["mass_change"],
+            "simulated_time_per_s_vs_no_subcycle": gain}
+
+
 def run_regrid(fa, prob, forest, q, repeats=3):
     """Cost of one regrid of the benchmark forest from the state q (the FP64
     form's state after the timed steps): device gradient tags, host forest
@@ -529,6 +588,12 @@ def main():
                     help="the dam break at levels --minlevel..--maxlevel for mx = 8, 16, 32, "
                          "both forms autotuned: one JSON line per mx (no CPU baseline)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--subcycle", action="store_true",
+                    help="headline the subcycled advance (each level at its own dt)")
+    ap.add_argument("--no-subcycle", action="store_true",
+                    help="leave the subcycled advance out of the default line")
+    ap.add_argument("--subcycle-steps", type=int, default=8,
+                    help="coarse steps timed for the subcycled advance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-time-sync", action="store_true",
@@ -640,10 +705,44 @@ def main():
                     dt_scale)
         return
 
+    if args.subcycle:
+        if ensemble:
+            sys.exit("bench.py: --subcycle is for single runs, not --problem ensemble")
+        r32 = run_form(fa, prob, forest, q0, aux, fa.FP32_BULK, dt, args.steps, args.warmup,
+                       args.tile32, not args.no_graph, not args.no_time_sync)
+        value32 = cells * args.steps / r32["wall"]
+        subs = {name: run_subcycle(fa, prob, forest, q0, aux, precision, dt, args.subcycle_steps, 2,
+                                   not args.no_graph, not args.no_time_sync)
+                for name, precision in (("fp32", fa.FP32_BULK), ("fp64", fa.FP64))}
+        s32 = subcycle_summary(subs["fp32"], cells, value32)
+        s64 = subcycle_summary(subs["fp64"], cells, None)
+        print(json.dumps({
+            "metric": "cell-updates/s, %s, subcycled (each level at its own dt)" % workload,
+            "value": s32["value"], "unit": "cell-updates/s", "n_gpus": 1,
+            "steps": args.subcycle_steps, "warmup": 2, "ms_per_step": s32["ms_per_coarse_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": workload, "form": "fp32-bulk", "subcycle": True,
+                       "step": "one coarse step", "patches": npatch, "cells": cells,
+                       "patches_per_level": np.bincount(lv).tolist(), "device": device,
+                       "time_sync": not args.no_time_sync,
+                       "forest": grown if grown is not None else {"kind": "initial refinement at t = 0"}},
+            "subcycle": {"fp32": s32, "fp64": s64},
+            "no_subcycle_fp32": {"value": round(value32, 1), "ms_per_step": round(r32["wall"] * 1e3 / args.steps, 5)},
+            "roofline": None, "cpu_baseline": None}))
+        return
+
     r64 = run_form(fa, prob, forest, q0, aux, fa.FP64, dt, args.steps, args.warmup,
                    args.tile64, not args.no_graph, not args.no_time_sync, dt_scale)
     r32 = run_form(fa, prob, forest, q0, aux, fa.FP32_BULK, dt, args.steps, args.warmup,
                    args.tile32, not args.no_graph, not args.no_time_sync, dt_scale)
+    subcycled = None
+    if not args.no_subcycle and not ensemble:
+        subs = {name: run_subcycle(fa, prob, forest, q0, aux, precision, dt, args.subcycle_steps, 2,
+                                   not args.no_graph, not args.no_time_sync)
+                for name, precision in (("fp32", fa.FP32_BULK), ("fp64", fa.FP64))}
+        nosub = {"fp32": cells * args.steps / r32["wall"], "fp64": cells * args.steps / r64["wall"]}
+        subcycled = {name: subcycle_summary(r, cells, nosub[name]) for name, r in subs.items()}
 
     regrid = run_regrid(fa, prob, forest, r64["q"])
     configured = None
@@ -746,6 +845,7 @@ def main():
         },
         "regrid": regrid,
         "configured_run": configured,
+        "subcycle": subcycled,
         "cpu_baseline": cpu,
     }
     if runs is not None:

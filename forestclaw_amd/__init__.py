@@ -116,6 +116,14 @@ def lib():
         "hipclaw_ghost_unpack": ([vp, _ip(), i, i, vp], i),
         "hipclaw_get_registers": ([vp, i, i, _dp()], i),
         "hipclaw_register_solver": ([vp, C.POINTER(i)], i),
+        "fclaw2d_hip_forest_ghost_program_range": ([vp, i, i, i, _ip(), i, i, i, vp, i, _ip()], i),
+        "fclaw2d_hip_forest_sync_program_range": ([vp, i, i, i, vp, i, vp, i, C.POINTER(i)], i),
+        "hipclaw_set_ghost_program_k": ([vp, i, _ip(), i, _ip()], i),
+        "hipclaw_set_sync_program_k": ([vp, i, _ip(), i, _ip(), i], i),
+        "hipclaw_set_levels": ([vp, _ip(), i], i),
+        "hipclaw_advance_subcycle": ([vp, d, i, C.POINTER(d)], i),
+        "hipclaw_advance_subcycle_async": ([vp, d, i], i),
+        "hipclaw_capture_subcycle_graph": ([vp, d, i], i),
         "hipclaw_step_batch_noalloc": ([C.POINTER(HipclawConfig), i, vp, vp, vp, vp, vp, _ip(), i,
                                         d, vp, C.POINTER(d)], i),
     }
@@ -245,6 +253,34 @@ class Forest:
             assert n2 == n
         return ops[:n], ps
 
+    def ghost_program_range(self, mx, my, mbc, mthbc, minlevel, maxlevel, time_interp):
+        """Ghost fill of levels minlevel..maxlevel, level minlevel - 1 taken
+        from its time-interpolated copies when time_interp (the partial
+        fclaw2d_ghost_update of a subcycled step)."""
+        mth = np.ascontiguousarray(mthbc, dtype=np.int32)
+        ps = np.zeros(7, np.int32)
+        n = lib().fclaw2d_hip_forest_ghost_program_range(self._h, mx, my, mbc, mth, minlevel,
+                                                         maxlevel, int(time_interp), None, 0, ps)
+        if n < 0:
+            raise RuntimeError("ghost program failed")
+        ops = np.zeros((max(n, 1), 8), np.int32)
+        lib().fclaw2d_hip_forest_ghost_program_range(self._h, mx, my, mbc, mth, minlevel, maxlevel,
+                                                     int(time_interp), ops.ctypes.data, n, ps)
+        return ops[:n], ps
+
+    def sync_program_range(self, mx, my, minlevel):
+        """Time-sync ops of levels minlevel.. and their register resets."""
+        nres = C.c_int(0)
+        n = lib().fclaw2d_hip_forest_sync_program_range(self._h, mx, my, minlevel, None, 0, None, 0,
+                                                        C.byref(nres))
+        if n < 0:
+            raise RuntimeError("sync program failed")
+        ops = np.zeros((max(n, 1), 8), np.int32)
+        res = np.zeros((max(nres.value, 1), 2), np.int32)
+        lib().fclaw2d_hip_forest_sync_program_range(self._h, mx, my, minlevel, ops.ctypes.data, n,
+                                                    res.ctypes.data, nres.value, C.byref(nres))
+        return ops[:n], res[:nres.value]
+
     def adapt(self, refine, coarsen, minlevel, maxlevel):
         """One regrid of the leaves; returns (kind, src) provenance of the new
         leaves (see fclaw2d_hip_forest_adapt)."""
@@ -306,6 +342,7 @@ class Solver:
         cfg.time_sync = int(bool(time_sync))
         cfg.subcycle = int(bool(subcycle))
         self.time_sync = bool(time_sync)
+        self.subcycle = bool(subcycle)
         self.cfg = cfg
         self.meqn, self.maux, self.mwaves = meqn, maux, mwaves
         self.mx, self.my, self.mbc = mx, my, mbc
@@ -330,13 +367,46 @@ class Solver:
         return (self.meqn, self.my + 2 * self.mbc, self.mx + 2 * self.mbc)
 
     def set_forest(self, forest, extent):
-        """Geometry and ghost-fill program of `forest` over physical `extent`."""
+        """Geometry and ghost-fill program of `forest` over physical `extent`;
+        for a subcycling state also the levels and the programs of every level
+        range (as fc2d_hipclaw_allocate_buffers installs them)."""
         _, _, dx, dy = forest.geometry(extent, self.mx, self.my)
         self.set_geometry(dx, dy)
         ops, ps = forest.ghost_program(self.mx, self.my, self.mbc, self.mthbc)
         self.set_ghost_program(ops, ps)
+        if not self.subcycle:
+            if self.time_sync:
+                self.set_sync_program(forest.sync_program(self.mx, self.my))
+            return
+        lv = forest.leaves()[0]
+        lo, hi = int(lv.min()), int(lv.max())
+        _check(lib().hipclaw_set_levels(self._h, np.ascontiguousarray(lv, dtype=np.int32), len(lv)),
+               "hipclaw_set_levels")
+        for ti in range(lo, hi):
+            ops, ps = forest.ghost_program_range(self.mx, self.my, self.mbc, self.mthbc, ti + 1, hi, True)
+            _check(lib().hipclaw_set_ghost_program_k(self._h, 1 + ti - lo, ops.reshape(-1), len(ops), ps),
+                   "hipclaw_set_ghost_program_k")
         if self.time_sync:
-            self.set_sync_program(forest.sync_program(self.mx, self.my))
+            for minl in range(lo, hi + 1):
+                ops, res = forest.sync_program_range(self.mx, self.my, minl)
+                _check(lib().hipclaw_set_sync_program_k(self._h, minl - lo, ops.reshape(-1), len(ops),
+                                                        res.reshape(-1), len(res)),
+                       "hipclaw_set_sync_program_k")
+
+    def advance_subcycle(self, dt, ncoarse=1):
+        """ncoarse coarse steps of the subcycled advance, enqueued on the device
+        (hipclaw_advance_subcycle); returns the largest Courant number."""
+        cfl = C.c_double(0.0)
+        _check(lib().hipclaw_advance_subcycle(self._h, dt, ncoarse, C.byref(cfl)),
+               "hipclaw_advance_subcycle")
+        return cfl.value
+
+    def advance_subcycle_async(self, dt, ncoarse=1):
+        _check(lib().hipclaw_advance_subcycle_async(self._h, dt, ncoarse), "hipclaw_advance_subcycle")
+
+    def capture_subcycle_graph(self, dt, ncoarse=1):
+        _check(lib().hipclaw_capture_subcycle_graph(self._h, dt, ncoarse),
+               "hipclaw_capture_subcycle_graph")
 
     def set_sync_program(self, ops):
         ops = np.ascontiguousarray(ops, dtype=np.int32).reshape(-1)

@@ -468,6 +468,16 @@ double fclaw2d_advance_all_levels(fclaw2d_global_t *glob, double t_curr, double 
         device_check(hipclaw_advance(glob->state, dt / rf, rf, &maxcfl), "hipclaw_advance");
         glob->count_single_step += rf * d->local_num_patches;
         glob->count_ghost_update += rf;
+    } else if (glob->subcycle && glob->patch_vt.single_step_update == hipclaw_update) {
+        /* the same schedule as advance_all_levels_stepwise, enqueued on the
+         * device as one sequence: level steps, time interpolation, partial ghost
+         * fills and synchronisations, one Courant number read at the end */
+        int b, p;
+        device_check(hipclaw_advance_subcycle(glob->state, dt, 1, &maxcfl), "hipclaw_advance_subcycle");
+        for (b = 0; b < d->num_blocks; b++)
+            for (p = 0; p < d->blocks[b].num_patches; p++)
+                glob->count_single_step += 1 << (d->blocks[b].patches[p].level - d->global_minlevel);
+        glob->count_ghost_update += 1 << (d->global_maxlevel - d->global_minlevel);
     } else {
         maxcfl = advance_all_levels_stepwise(glob, t_curr, dt);
     }
@@ -552,6 +562,15 @@ static void install_programs(fclaw2d_global_t *glob)
                                                      nops, phase), "hipclaw_set_ghost_program");
             free(ops);
         }
+    }
+    if (glob->subcycle) {
+        /* the level of every slot, for the device-resident subcycled schedule */
+        int b, p, *level = (int *) malloc(sizeof(int) * (size_t) (d->local_num_patches + 1));
+        for (b = 0; b < d->num_blocks; b++)
+            for (p = 0; p < d->blocks[b].num_patches; p++)
+                level[d->blocks[b].patches[p].slot] = d->blocks[b].patches[p].level;
+        device_check(hipclaw_set_levels(glob->state, level, d->local_num_patches), "hipclaw_set_levels");
+        free(level);
     }
     if (glob->time_sync) {
         /* program 0: every level; with subcycling also one per level range */

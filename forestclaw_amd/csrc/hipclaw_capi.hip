@@ -64,6 +64,7 @@ struct GhostProgram {
     GhostLaunch launch[2][6];
     int nlaunch[2] = {0, 0};
     bool layouts_built = false;
+    bool set = false;             // installed since the last regrid
 };
 
 // One time-synchronisation program (hipclaw_set_sync_program_k).
@@ -73,6 +74,7 @@ struct SyncProgram {
     int* resets = nullptr;    // [nresets][2] (slot, face)
     int ngroups = 0, nresets = 0, ops_cap = 0, groups_cap = 0, resets_cap = 0;
     int maxgroup = 0;         // most ops of one destination
+    bool set = false;         // installed since the last regrid
 };
 
 struct hipclaw_state {
@@ -103,6 +105,12 @@ struct hipclaw_state {
     double* dtscale = nullptr;    // per-slot factor on dt, or null
     int ghost_layout = 0;         // launch layout of the ghost fill (hipclaw_set_ghost_layout)
     int waves = 0;                // waves per workgroup of 16 x 16 tiles, 0 = the default
+    // subcycled advance: slots grouped by level (hipclaw_set_levels); level
+    // lmin + l holds lslots[loff[l] .. loff[l + 1])
+    int* lslots = nullptr;
+    int lslots_cap = 0;
+    std::vector<int> loff;
+    int lmin = 0, lmax = -1;      // lmax < lmin: no levels set
     double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
     bool saved = false;           // qsave holds the current slots' state
     // ghost-patch message: slab offsets of one field's band, in message order
@@ -445,6 +453,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     }
     hipFree(s->qlast);
     hipFree(s->qti);
+    hipFree(s->lslots);
     hipFree(s->qsave);
     hipFree(s->cfl);
     hipFree(s->mass_buf);
@@ -468,6 +477,7 @@ int hipclaw_set_geometry(hipclaw_state_t* s, int npatch, const double* dx, const
         return fail(HIPCLAW_EINVAL, "hipclaw_set_geometry: bad arguments%s");
     for (int p = 0; p < npatch; p++)
         if (!(dx[p] > 0.0) || !(dy[p] > 0.0)) return fail(HIPCLAW_EINVAL, "non-positive mesh width%s");
+    if (npatch != s->npatch) s->lmax = s->lmin - 1;   // the levels name other slots
     s->npatch = npatch;
     if (npatch > 0) {
         HIP_TRY(hipMemcpyAsync(s->dxs, dx, sizeof(double) * npatch, hipMemcpyHostToDevice, s->st));
@@ -788,6 +798,7 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
     }
     GhostProgram& g = s->gp[k];
     g.layouts_built = false;
+    g.set = true;
     g.nops = 0;
     g.nlaunch[0] = g.nlaunch[1] = 0;
     if (k == 0) {
@@ -854,6 +865,7 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     y.ngroups = ngroups;
     y.nresets = nresets;
     y.maxgroup = 0;
+    y.set = true;
     for (int g = 0; g < ngroups; g++) y.maxgroup = std::max(y.maxgroup, start[g + 1] - start[g]);
     if (k == 0) {
         s->sp0_ops.assign(ops, ops + 8 * (size_t) nops);
@@ -1233,6 +1245,8 @@ int hipclaw_restore_step(hipclaw_state_t* s) {
 }
 
 static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool timed);
+static int subcycle_launches(hipclaw_state_t* s, double dt, int ncoarse);
+static int subcycle_ready(hipclaw_state_t* s);
 
 // Tile sizes a state can switch to: supported by its solver and precision,
 // no larger than the patch, and (FP64 with time sync) leaving LDS for the
@@ -1300,10 +1314,17 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
                      double* ms, int* ntrials) {
     if (!s || nsteps <= 0 || !(dt > 0.0) || !tiles || !waves || !layouts || !ms || !ntrials)
         return fail(HIPCLAW_EINVAL, "hipclaw_autotune: bad arguments%s");
-    if (s->reg_accumulate)
-        return fail(HIPCLAW_EINVAL, "hipclaw_autotune: tune before subcycling (registers accumulate)%s");
+    // a subcycling state with its levels set is tuned on its subcycled coarse
+    // steps (each ends with the full synchronisation, which clears the
+    // registers the substeps accumulate)
+    const bool sub = s->qlast && s->lmax >= s->lmin;
+    if (s->reg_accumulate && !sub)
+        return fail(HIPCLAW_EINVAL, "hipclaw_autotune: a subcycling state needs its levels set%s");
     *ntrials = 0;
     if (s->npatch == 0) return HIPCLAW_OK;
+    if (sub)
+        if (int rc = subcycle_ready(s)) return rc;
+    auto trial_steps = [&](int n) { return sub ? subcycle_launches(s, dt, n) : advance_launches(s, dt, n, false); };
     if (int rc = hipclaw_save_step(s)) return rc;
     hipEvent_t ev[2] = {nullptr, nullptr};
     HIP_TRY(hipEventCreate(&ev[0]));
@@ -1319,9 +1340,9 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
             if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay)) || (rc = hipclaw_set_waves(s, w)))
                 break;
             // one untimed step (first use of these kernels), then the timed ones
-            if ((rc = advance_launches(s, dt, 1, false))) break;
+            if ((rc = trial_steps(1))) break;
             hipError_t e = hipEventRecord(ev[0], s->st);
-            if (e == hipSuccess && (rc = advance_launches(s, dt, nsteps, false))) break;
+            if (e == hipSuccess && (rc = trial_steps(nsteps))) break;
             if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
             if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
             float elapsed = 0.0f;
@@ -1605,6 +1626,178 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
     return HIPCLAW_OK;
 }
 
+// ---- subcycled advance on the device ----
+//
+// fclaw2d_advance_all_levels with subcycle = T (src/fclaw2d_advance.c:54-246):
+// level L takes 2^(L - lmin) steps of dt / 2^(L - lmin) per coarse step; after
+// every finest step that is not the last, the ghost cells of the levels above
+// the coarsest level still between steps are filled from time-interpolated
+// copies (ghost program 1 + ti - lmin) and, with time sync, those levels are
+// synchronised.  Every launch goes on the library's stream in the order the
+// reference's host loop makes its calls, with no host synchronisation: the
+// Courant numbers of all substeps meet in the one device maximum, so the
+// whole coarse step can be captured in a graph.
+
+// One level's update: the state before it (fclaw2d_clawpatch_save_current_step)
+// copied whole into qlast, then the step reading that copy and writing the
+// new interiors straight into the current slab, which the other levels keep
+// reading (the same values as stepping the slab itself: qlast holds its
+// every cell).
+static int subcycle_update(hipclaw_state_t* s, int level, double dt) {
+    const int l = level - s->lmin, n = s->loff[l + 1] - s->loff[l];
+    if (n == 0) return HIPCLAW_OK;
+    const int* slots = s->lslots + s->loff[l];
+    int e = hipclaw::launch_copy_patches(s->q[s->cur], s->qlast, slots, n, s->pstride, s->st);
+    if (e) return hipfail((hipError_t) e, "copy_patches launch");
+    return launch_step(s, step_args(s, s->qlast, s->q[s->cur], slots, n, dt));
+}
+
+// The time-interpolated copies of a level (fclaw2d_timeinterp).
+static int subcycle_timeinterp(hipclaw_state_t* s, int level, double alpha) {
+    const int l = level - s->lmin, n = s->loff[l + 1] - s->loff[l];
+    if (n == 0) return HIPCLAW_OK;
+    int e = hipclaw::launch_timeinterp(s->q[s->cur], s->qlast, s->qti, s->lslots + s->loff[l], n,
+                                       s->pstride, s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn,
+                                       alpha, s->st);
+    if (e) return hipfail((hipError_t) e, "timeinterp launch");
+    return HIPCLAW_OK;
+}
+
+namespace {
+struct LevelCounter {
+    int last_step = 0, step_inc = 1, total_steps = 1;
+    double dt_step = 0.0;
+};
+}  // namespace
+
+// advance_level (src/fclaw2d_advance.c:112-150, 152-170): the level's step,
+// then, when the coarser level is due, its step and its time-interpolated
+// copies at the finer level's new time.
+static int subcycle_level(hipclaw_state_t* s, int level, int curr_fine_step, LevelCounter* tc) {
+    LevelCounter& c = tc[level - s->lmin];
+    if (int rc = subcycle_update(s, level, c.dt_step)) return rc;
+    c.last_step += c.step_inc;
+    if (level > s->lmin) {
+        LevelCounter& cc = tc[level - 1 - s->lmin];
+        if (cc.last_step == curr_fine_step) {
+            if (int rc = subcycle_level(s, level - 1, cc.last_step, tc)) return rc;
+            const double alpha = ((double) (c.last_step % cc.step_inc)) / cc.step_inc;
+            if (int rc = subcycle_timeinterp(s, level - 1, alpha)) return rc;
+        }
+    }
+    return HIPCLAW_OK;
+}
+
+static int subcycle_launches(hipclaw_state_t* s, double dt, int ncoarse) {
+    const int lmin = s->lmin, lmax = s->lmax, nl = lmax - lmin + 1;
+    if (int e = hipclaw::launch_reset_scalar(s->cfl, s->st)) return hipfail((hipError_t) e, "reset");
+    for (int c = 0; c < ncoarse; c++) {
+        LevelCounter tc[HIPCLAW_MAX_PROGRAMS];
+        for (int l = 0; l < nl; l++) {   // initialize_timestep_counters (:54-110)
+            tc[l].dt_step = dt / (double) (1 << l);
+            tc[l].total_steps = 1 << l;
+            tc[l].step_inc = 1 << (lmax - lmin - l);
+        }
+        const int total = tc[nl - 1].total_steps;
+        for (int nf = 0; nf < total; nf++) {
+            if (int rc = subcycle_level(s, lmax, nf, tc)) return rc;
+            const int last_step = tc[nl - 1].last_step;
+            if (last_step < total) {
+                int ti = lmax - 1;
+                while (last_step % tc[ti - lmin].step_inc == 0) ti--;
+                const int k = 1 + ti - lmin;   // levels ti + 1 .. lmax, ti time-interpolated
+                if (int rc = ghost_fill_async(s, k)) return rc;
+                if (s->reg)
+                    if (int rc = time_sync_async(s, k)) return rc;
+            }
+        }
+        if (int rc = ghost_fill_async(s, 0)) return rc;
+        if (s->reg)
+            if (int rc = time_sync_async(s, 0)) return rc;
+    }
+    return HIPCLAW_OK;
+}
+
+// What the subcycled advance needs: subcycle state, levels, and the programs
+// of every level range.
+static int subcycle_ready(hipclaw_state_t* s) {
+    if (!s->qti || !s->qlast) return fail(HIPCLAW_EINVAL, "the state was created without subcycle%s");
+    if (s->lmax < s->lmin || (int) s->loff.size() != s->lmax - s->lmin + 2)
+        return fail(HIPCLAW_EINVAL, "no levels set (hipclaw_set_levels)%s");
+    // program 0: every level; program k = 1 + ti - lmin: levels ti + 1 .. lmax
+    // with level ti time-interpolated (ghost fill), levels from ti + 1 (sync)
+    for (int k = 0; k <= s->lmax - s->lmin; k++) {
+        if (!s->gp[k].set) return fail(HIPCLAW_EINVAL, "a ghost program of the subcycled schedule is missing%s");
+        if (s->reg && !s->sp[k].set)
+            return fail(HIPCLAW_EINVAL, "a time-sync program of the subcycled schedule is missing%s");
+    }
+    if (s->reg && s->sp[0].nresets == 0 && s->sp[0].ngroups > 0)
+        return fail(HIPCLAW_EINVAL, "with subcycle, the full sync program needs its register resets%s");
+    return ensure_layouts(s);
+}
+
+int hipclaw_set_levels(hipclaw_state_t* s, const int* level, int npatch) {
+    if (!s || npatch != s->npatch || (npatch > 0 && !level))
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_levels: one level per patch%s");
+    if (npatch == 0) { s->lmax = s->lmin - 1; return HIPCLAW_OK; }
+    int lo = level[0], hi = level[0];
+    for (int p = 0; p < npatch; p++) {
+        if (level[p] < 0 || level[p] > 30) return fail(HIPCLAW_EINVAL, "hipclaw_set_levels: level out of range%s");
+        lo = std::min(lo, level[p]);
+        hi = std::max(hi, level[p]);
+    }
+    if (hi - lo + 1 > HIPCLAW_MAX_PROGRAMS || hi - lo > 20)
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_levels: more levels than programs%s");
+    std::vector<int> off(hi - lo + 2, 0), slots(npatch);
+    for (int p = 0; p < npatch; p++) off[level[p] - lo + 1]++;
+    for (int l = 0; l <= hi - lo; l++) off[l + 1] += off[l];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int p = 0; p < npatch; p++) slots[fill[level[p] - lo]++] = p;
+    if (int rc = upload_array(s, s->lslots, s->lslots_cap, slots.data(), npatch, "level slots")) return rc;
+    s->loff = off;
+    s->lmin = lo;
+    s->lmax = hi;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_advance_subcycle_async(hipclaw_state_t* s, double dt, int ncoarse) {
+    if (!s || ncoarse < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance_subcycle: bad arguments%s");
+    if (int rc = subcycle_ready(s)) return rc;
+    HIP_TRY(hipEventRecord(s->region[0], s->st));
+    if (int rc = subcycle_launches(s, dt, ncoarse)) return rc;
+    HIP_TRY(hipEventRecord(s->region[1], s->st));
+    s->timed_steps = 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_advance_subcycle(hipclaw_state_t* s, double dt, int ncoarse, double* maxcfl) {
+    if (int rc = hipclaw_advance_subcycle_async(s, dt, ncoarse)) return rc;
+    int rc;
+    const double c = read_cfl(s, &rc);
+    if (rc) return rc;
+    if (maxcfl) *maxcfl = c;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_capture_subcycle_graph(hipclaw_state_t* s, double dt, int ncoarse) {
+    if (!s || ncoarse <= 0 || !(dt > 0.0))
+        return fail(HIPCLAW_EINVAL, "hipclaw_capture_subcycle_graph: bad arguments%s");
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (int rc = subcycle_ready(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    int rc = subcycle_launches(s, dt, ncoarse);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->st, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+    s->graph = g;
+    HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    s->graph_steps = ncoarse;
+    return HIPCLAW_OK;
+}
+
 int hipclaw_advance_async(hipclaw_state_t* s, double dt, int nsteps) {
     if (!s || nsteps < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance: bad arguments%s");
     if (s->reg_accumulate && s->sp[0].nresets == 0 && s->sp[0].ngroups > 0)
@@ -1865,8 +2058,10 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
         g.nops = 0;
         g.nlaunch[0] = g.nlaunch[1] = 0;
         g.layouts_built = false;
+        g.set = false;
     }
-    for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; }
+    for (SyncProgram& y : s->sp) { y.ngroups = 0; y.nresets = 0; y.set = false; }
+    s->lmax = s->lmin - 1;
     s->gp0_set = false;
     s->gp0_ops.clear();
     s->sp0_ops.clear();

@@ -354,6 +354,28 @@ int hipclaw_mass(hipclaw_state_t *s, double *sum_hi, double *sum_lo);
  * update kernel's own start and end, as a kernel trace reports it). */
 int hipclaw_timing(hipclaw_state_t *s, int nsteps, double *total_ms, double *step_kernel_ms);
 
+/* Levels of slots 0..npatch-1 (npatch = the current patch count), for the
+ * subcycled advance.  Regrid and a change of the patch count forget them. */
+int hipclaw_set_levels(hipclaw_state_t *s, const int *level, int npatch);
+
+/* ncoarse calls of fclaw2d_advance_all_levels(dt) with subcycle = T
+ * (src/fclaw2d_advance.c:54-246, the level loop of :112-170 and the ghost
+ * updates / time syncs of :292-330): level L takes 2^(L - lmin) steps of
+ * dt / 2^(L - lmin), the state before each step is kept, coarse levels are
+ * time-interpolated for the finer levels' ghost cells, and partial ghost
+ * fills (program 1 + ti - lmin: levels ti+1..lmax with ti time-interpolated)
+ * and time syncs (program L - lmin: levels L..lmax) run between finest steps,
+ * all enqueued on the device with no host synchronisation; *maxcfl receives
+ * the largest Courant number of every substep.  Needs a state created with
+ * subcycle = 1, hipclaw_set_levels, and ghost (and, with time sync, sync)
+ * programs 0 .. lmax - lmin. */
+int hipclaw_advance_subcycle(hipclaw_state_t *s, double dt, int ncoarse, double *maxcfl);
+int hipclaw_advance_subcycle_async(hipclaw_state_t *s, double dt, int ncoarse);
+
+/* Capture ncoarse subcycled coarse steps into a graph (replayed with
+ * hipclaw_replay_graph). */
+int hipclaw_capture_subcycle_graph(hipclaw_state_t *s, double dt, int ncoarse);
+
 #ifdef __cplusplus
 }
 #endif

@@ -280,13 +280,18 @@ def test_autotune_keeps_the_state(gpu, precision):
         assert rel_max_err(sv.get_q(), qo) <= TOL
 
 
-def test_autotune_refuses_subcycling(gpu):
+def test_autotune_refuses_subcycling_without_levels(gpu):
+    """A subcycling state is tuned on its subcycled coarse steps
+    (test_gpu_subcycle_device), which need the patches' levels."""
     prob = P.dambreak_problem(2, 3, mx=16)
     forest = P.build_forest(prob)
     sv = fa.Solver(prob.solver, 16, 16, mthbc=prob.mthbc, max_patches=forest.npatch,
                    time_sync=True, subcycle=True)
-    sv.set_forest(forest, prob.extent)
+    _, _, dx, dy = forest.geometry(prob.extent, 16, 16)
+    sv.set_geometry(dx, dy)
+    ops, ps = forest.ghost_program(16, 16, 2, prob.mthbc)
+    sv.set_ghost_program(ops, ps)
     q0, _ = P.initial_data(prob, forest)
     sv.set_q(q0)
-    with pytest.raises(fa.HipclawError):
+    with pytest.raises(fa.HipclawError, match="levels"):
         sv.autotune(1e-4, 2)
