@@ -25,17 +25,24 @@ typedef struct {
     int ti[2], tj[2];
 } nbinfo_t;
 
+/* One slot of the leaf map: key and leaf side by side, so a probe touches
+ * one cache line.  An empty slot has val < 0 (key 0 is a valid key). */
+typedef struct {
+    uint64_t key;
+    int64_t val;
+} hentry_t;
+
 struct fclaw2d_hip_forest {
     int nbx, nby, per_x, per_y;
     int iso;   /* blocks are separate domains (an ensemble of runs) */
+    int bal;   /* the leaves are known to be 2:1 balanced */
     int n, cap;
     int *level, *gx, *gy;
     /* provenance of the leaves after the last adapt: kind and 4 sources each */
     int *pkind, *psrc, pn;
     /* open-addressing map (level, gx, gy) -> leaf */
     size_t hcap;
-    uint64_t *hkey;
-    int *hval;
+    hentry_t *hash;
     /* face (4) and corner (4) relations of every leaf for one (mx, my), built
      * on first use and dropped whenever the leaves change */
     nbinfo_t *rel;
@@ -59,25 +66,23 @@ static int rehash(fclaw2d_hip_forest_t *f)
     size_t cap = 64, p;
     int i;
     f->rel_valid = 0;
-    while (cap < (size_t) f->n * 3) cap <<= 1;
+    while (cap < (size_t) f->n * 2) cap <<= 1;   /* load factor <= 1/2 */
     if (cap != f->hcap) {
-        free(f->hkey);
-        free(f->hval);
-        f->hkey = (uint64_t *) malloc(cap * sizeof(uint64_t));
-        f->hval = (int *) malloc(cap * sizeof(int));
-        if (!f->hkey || !f->hval) return -1;
+        free(f->hash);
+        f->hash = (hentry_t *) malloc(cap * sizeof(hentry_t));
+        if (!f->hash) { f->hcap = 0; return -1; }
         f->hcap = cap;
     }
-    for (p = 0; p < cap; p++) f->hval[p] = -1;
+    for (p = 0; p < cap; p++) f->hash[p].val = -1;
     for (i = 0; i < f->n; i++) {
         uint64_t k = fkey(f->level[i], f->gx[i], f->gy[i]);
         p = fslot(k, cap);
-        while (f->hval[p] >= 0) {
-            if (f->hkey[p] == k) return -1; /* duplicate leaf */
+        while (f->hash[p].val >= 0) {
+            if (f->hash[p].key == k) return -1; /* duplicate leaf */
             p = (p + 1) & (cap - 1);
         }
-        f->hkey[p] = k;
-        f->hval[p] = i;
+        f->hash[p].key = k;
+        f->hash[p].val = i;
     }
     return 0;
 }
@@ -89,8 +94,8 @@ static int lookup(const fclaw2d_hip_forest_t *f, int level, int64_t gx, int64_t 
     if (level < 0 || gx < 0 || gy < 0) return -1;
     k = fkey(level, gx, gy);
     p = fslot(k, f->hcap);
-    while (f->hval[p] >= 0) {
-        if (f->hkey[p] == k) return f->hval[p];
+    while (f->hash[p].val >= 0) {
+        if (f->hash[p].key == k) return (int) f->hash[p].val;
         p = (p + 1) & (f->hcap - 1);
     }
     return -1;
@@ -157,33 +162,57 @@ static uint64_t spread(uint32_t v)
     return x;
 }
 
-typedef struct { uint64_t block, morton; int level, gx, gy; } leaf_key_t;
-
-static int cmp_leaf(const void *a, const void *b)
+/* One stable counting pass of an LSD radix sort: order idx by byte `shift`
+ * of key[idx]. */
+static void radix_pass(const uint64_t *key, const int *in, int *out, int n, int shift)
 {
-    const leaf_key_t *x = (const leaf_key_t *) a, *y = (const leaf_key_t *) b;
-    if (x->block != y->block) return x->block < y->block ? -1 : 1;
-    if (x->morton != y->morton) return x->morton < y->morton ? -1 : 1;
-    return x->level - y->level;
+    int count[257] = {0}, i;
+    for (i = 0; i < n; i++) count[((key[in[i]] >> shift) & 255) + 1]++;
+    for (i = 0; i < 256; i++) count[i + 1] += count[i];
+    for (i = 0; i < n; i++) out[count[(key[in[i]] >> shift) & 255]++] = in[i];
 }
 
+/* p4est order: by block (row-major), then the Morton index of the leaf's
+ * lower-left corner at the finest level, then level (a parent's corner is its
+ * first child's).  LSD radix sort over (level, Morton, block), as stable
+ * counting passes over the bytes that vary. */
 static int sort_leaves(fclaw2d_hip_forest_t *f)
 {
-    leaf_key_t *k = (leaf_key_t *) malloc((size_t) f->n * sizeof(leaf_key_t));
-    int i;
-    if (!k) return -1;
-    for (i = 0; i < f->n; i++) {
+    const int n = f->n;
+    uint64_t *mort = (uint64_t *) malloc(sizeof(uint64_t) * (size_t) (n + 1));
+    uint64_t *lvbk = (uint64_t *) malloc(sizeof(uint64_t) * (size_t) (n + 1));
+    int *a = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    int *b = (int *) malloc(sizeof(int) * (size_t) (n + 1));
+    int *tmp = (int *) malloc(sizeof(int) * 3 * (size_t) (n + 1));
+    uint64_t mor_or = 0, bk_or = 0;
+    int i, sh;
+    if (!mort || !lvbk || !a || !b || !tmp) {
+        free(mort); free(lvbk); free(a); free(b); free(tmp);
+        return -1;
+    }
+    for (i = 0; i < n; i++) {
         const int L = f->level[i];
         const uint32_t bx = (uint32_t) (f->gx[i] >> L), by = (uint32_t) (f->gy[i] >> L);
         const uint32_t lx = (uint32_t) (f->gx[i] - ((int64_t) bx << L));
         const uint32_t ly = (uint32_t) (f->gy[i] - ((int64_t) by << L));
-        k[i].block = (uint64_t) by * f->nbx + bx;
-        k[i].morton = spread(lx << (28 - L)) | (spread(ly << (28 - L)) << 1);
-        k[i].level = L; k[i].gx = f->gx[i]; k[i].gy = f->gy[i];
+        const uint64_t block = (uint64_t) by * (uint64_t) f->nbx + bx;
+        mort[i] = spread(lx << (28 - L)) | (spread(ly << (28 - L)) << 1);
+        lvbk[i] = (uint64_t) L | (block << 8);
+        mor_or |= mort[i];
+        bk_or |= block;
+        a[i] = i;
     }
-    qsort(k, (size_t) f->n, sizeof(leaf_key_t), cmp_leaf);
-    for (i = 0; i < f->n; i++) { f->level[i] = k[i].level; f->gx[i] = k[i].gx; f->gy[i] = k[i].gy; }
-    free(k);
+    radix_pass(lvbk, a, b, n, 0);                       /* level (< 256) */
+    { int *t = a; a = b; b = t; }
+    for (sh = 0; sh < 64; sh += 8)                      /* Morton bytes in use */
+        if ((mor_or >> sh) & 255) { radix_pass(mort, a, b, n, sh); int *t = a; a = b; b = t; }
+    for (sh = 8; sh < 64; sh += 8)                      /* block bytes in use */
+        if ((bk_or >> (sh - 8)) & 255) { radix_pass(lvbk, a, b, n, sh); int *t = a; a = b; b = t; }
+    for (i = 0; i < n; i++) {
+        tmp[3 * i] = f->level[a[i]]; tmp[3 * i + 1] = f->gx[a[i]]; tmp[3 * i + 2] = f->gy[a[i]];
+    }
+    for (i = 0; i < n; i++) { f->level[i] = tmp[3 * i]; f->gx[i] = tmp[3 * i + 1]; f->gy[i] = tmp[3 * i + 2]; }
+    free(mort); free(lvbk); free(a); free(b); free(tmp);
     return rehash(f);
 }
 
@@ -225,15 +254,18 @@ static int split_marked(fclaw2d_hip_forest_t *f, const char *mark)
 }
 
 /* Restore the 2:1 face and corner balance by splitting leaves that are more
- * than one level coarser than a neighbour.  The first pass checks every
- * leaf; later passes check only the leaves a split created and the leaves
- * that asked for it (a leaf no pass revisits saw no coarser neighbour, and
- * splits only make its neighbours finer). */
-static int balance(fclaw2d_hip_forest_t *f)
+ * than one level coarser than a neighbour.  The first pass checks the leaves
+ * of work0 (all leaves when work0 is null); later passes check only the
+ * leaves a split created and the leaves that asked for it (a leaf no pass
+ * revisits saw no coarser neighbour, and splits only make its neighbours
+ * finer).  On a balanced forest whose leaves work0 were just split, only
+ * their children can see a neighbour two levels coarser, so work0 = the
+ * children suffices. */
+static int balance_from(fclaw2d_hip_forest_t *f, const int *work0, int nwork0)
 {
     static const int dx8[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     static const int dy8[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-    int *work = NULL, nwork = f->n, first = 1, rc = 0;
+    int *work = NULL, nwork = work0 ? nwork0 : f->n, first = 1, rc = 0;
     for (;;) {
         char *mark, *trig;
         int *next, nnext = 0, w, d, any = 0, n0;
@@ -242,7 +274,7 @@ static int balance(fclaw2d_hip_forest_t *f)
         trig = (char *) calloc((size_t) f->n + 1, 1);
         if (!mark || !trig) { free(mark); free(trig); rc = -1; break; }
         for (w = 0; w < nwork; w++) {
-            const int i = first ? w : work[w];
+            const int i = first ? (work0 ? work0[w] : w) : work[w];
             const int L = f->level[i];
             if (L < 2) continue;
             for (d = 0; d < 8; d++) {
@@ -270,7 +302,28 @@ static int balance(fclaw2d_hip_forest_t *f)
     }
     free(work);
     if (rc) return rc;
+    f->bal = 1;
     return sort_leaves(f);
+}
+
+static int balance(fclaw2d_hip_forest_t *f) { return balance_from(f, NULL, 0); }
+
+/* Split the marked leaves and restore the balance, checking only the new
+ * children when the forest was balanced before. */
+static int split_and_balance(fclaw2d_hip_forest_t *f, const char *mark)
+{
+    const int n0 = f->n;
+    int *kids, nk = 0, i, rc;
+    if (split_marked(f, mark)) return -1;
+    if (!f->bal) return balance(f);
+    kids = (int *) malloc(sizeof(int) * (size_t) (f->n + 1));
+    if (!kids) return -1;
+    for (i = 0; i < n0; i++)
+        if (mark[i]) kids[nk++] = i;           /* the first child keeps the parent's index */
+    for (i = n0; i < f->n; i++) kids[nk++] = i;
+    rc = balance_from(f, kids, nk);
+    free(kids);
+    return rc;
 }
 
 /* ---------------------------------------------------------------- public */
@@ -298,6 +351,7 @@ fclaw2d_hip_forest_t *fclaw2d_hip_forest_new(int nbx, int nby, int periodic_x, i
         for (x = 0; x < (nbx << minlevel); x++) {
             f->level[f->n] = minlevel; f->gx[f->n] = x; f->gy[f->n] = y; f->n++;
         }
+    f->bal = 1;   /* one level */
     if (sort_leaves(f)) { fclaw2d_hip_forest_destroy(f); return NULL; }
     return f;
 }
@@ -345,7 +399,7 @@ int fclaw2d_hip_forest_set_isolated(fclaw2d_hip_forest_t *f, int isolated)
 void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
 {
     if (!f) return;
-    free(f->level); free(f->gx); free(f->gy); free(f->hkey); free(f->hval);
+    free(f->level); free(f->gx); free(f->gy); free(f->hash);
     free(f->pkind); free(f->psrc); free(f->rel);
     free(f);
 }
@@ -382,10 +436,9 @@ int fclaw2d_hip_forest_refine(fclaw2d_hip_forest_t *f, const int *flags, int max
     mark = (char *) calloc((size_t) f->n, 1);
     if (!mark) return -1;
     for (i = 0; i < f->n; i++) mark[i] = flags[i] != 0 && f->level[i] < maxlevel;
-    rc = split_marked(f, mark);
+    rc = split_and_balance(f, mark);
     free(mark);
-    if (rc || balance(f)) return -1;
-    return f->n;
+    return rc ? -1 : f->n;
 }
 
 /* ---------------------------------------------------------------- adapt */
@@ -406,6 +459,54 @@ static int coarsen_keeps_balance(const fclaw2d_hip_forest_t *f, int L, int64_t X
     return 1;
 }
 
+static void face_rel(const fclaw2d_hip_forest_t *f, int p, int iface, int mx, int my, nbinfo_t *r);
+static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int my, nbinfo_t *r);
+
+/* The relation cache of the adapted leaves from the one of the leaves before
+ * the adapt: a leaf that stayed (provenance SAME) keeps each relation whose
+ * neighbours all stayed, with their new numbers (the neighbour still covers
+ * the same region, so the kind and the index translation are unchanged);
+ * every other relation, and every relation of a new leaf, is recomputed.  A
+ * corner without a neighbour (a hanging node) names no leaf, so it is always
+ * recomputed. */
+static int update_relations(fclaw2d_hip_forest_t *f, const nbinfo_t *orel, int n0, int mx, int my)
+{
+    int *o2n = (int *) malloc(sizeof(int) * (size_t) (n0 + 1));
+    nbinfo_t *rel = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * (size_t) (f->n > 0 ? f->n : 1));
+    int i, k, m;
+    if (!o2n || !rel) { free(o2n); free(rel); return -1; }
+    for (i = 0; i < n0; i++) o2n[i] = -1;
+    for (i = 0; i < f->n; i++)
+        if (f->pkind[i] == FCLAW2D_HIP_REGRID_SAME) o2n[f->psrc[4 * i]] = i;
+    for (i = 0; i < f->n; i++) {
+        const int q = f->pkind[i] == FCLAW2D_HIP_REGRID_SAME ? f->psrc[4 * i] : -1;
+        for (k = 0; k < 8; k++) {
+            nbinfo_t *r = &rel[8 * (size_t) i + k];
+            int keep = q >= 0;
+            if (keep) {
+                *r = orel[8 * (size_t) q + k];
+                if (r->rel == REL_NONE) keep = 0;
+                else if (r->rel != REL_BOUNDARY) {
+                    const int nn = (r->rel == REL_HALF && k < 4) ? 2 : 1;
+                    for (m = 0; m < nn && keep; m++) {
+                        r->nb[m] = o2n[r->nb[m]];
+                        keep = r->nb[m] >= 0;
+                    }
+                }
+            }
+            if (!keep) {
+                if (k < 4) face_rel(f, i, k, mx, my, r);
+                else corner_rel(f, i, k - 4, mx, my, r);
+            }
+        }
+    }
+    free(o2n);
+    free(f->rel);
+    f->rel = rel;
+    f->rel_mx = mx; f->rel_my = my; f->rel_valid = 1;
+    return 0;
+}
+
 int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const int *coarsen,
                              int minlevel, int maxlevel)
 {
@@ -413,20 +514,28 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
     char *mark;
     int i, g, n0, ncoarse = 0;
     int *fam = NULL;   /* parent (level, X, Y) of families to coarsen, 3 ints each */
+    nbinfo_t *orel = NULL;   /* the relation cache of the leaves before the adapt */
+    int omx = 0, omy = 0;
     if (!f || !refine || !coarsen) return -1;
     n0 = f->n;
+    if (f->rel_valid) {
+        orel = f->rel;
+        omx = f->rel_mx; omy = f->rel_my;
+        f->rel = NULL;
+        f->rel_valid = 0;
+    }
     /* keep the old leaves for the provenance */
     old = alloc_forest(f->nbx, f->nby, f->per_x, f->per_y);
-    if (!old || reserve(old, n0)) { fclaw2d_hip_forest_destroy(old); return -1; }
+    if (!old || reserve(old, n0)) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     memcpy(old->level, f->level, (size_t) n0 * sizeof(int));
     memcpy(old->gx, f->gx, (size_t) n0 * sizeof(int));
     memcpy(old->gy, f->gy, (size_t) n0 * sizeof(int));
     old->n = n0;
-    if (rehash(old)) { fclaw2d_hip_forest_destroy(old); return -1; }
+    if (rehash(old)) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     /* families to coarsen: all four siblings flagged, none flagged to refine
      * (cb_regrid_tag4coarsening, src/fclaw2d_regrid.c:61-93) */
     fam = (int *) malloc(sizeof(int) * 3 * (size_t) (n0 / 4 + 1));
-    if (!fam) { fclaw2d_hip_forest_destroy(old); return -1; }
+    if (!fam) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     for (i = 0; i < n0; i++) {
         const int L = f->level[i], x = f->gx[i], y = f->gy[i];
         int sib[4], ok = L > minlevel && L > 0 && !(x & 1) && !(y & 1);
@@ -438,16 +547,16 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
     }
     /* refine (cb_fclaw2d_regrid_tag4refinement) and restore the balance */
     mark = (char *) calloc((size_t) n0 + 1, 1);
-    if (!mark) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+    if (!mark) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     for (i = 0; i < n0; i++) mark[i] = refine[i] != 0 && f->level[i] < maxlevel;
-    if (split_marked(f, mark) || balance(f)) { free(mark); free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+    if (split_and_balance(f, mark)) { free(mark); free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     free(mark);
     /* coarsen the families that are still intact and keep the balance */
     if (ncoarse > 0) {
         int k, w;
         char *drop = (char *) calloc((size_t) f->n + 1, 1);
         int nadd = 0;
-        if (!drop) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+        if (!drop) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
         for (k = 0; k < ncoarse; k++) {
             const int L = fam[3 * k], X = fam[3 * k + 1], Y = fam[3 * k + 2];
             int kids[4], ok = 1;
@@ -463,20 +572,20 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
             if (!drop[i]) { f->level[w] = f->level[i]; f->gx[w] = f->gx[i]; f->gy[w] = f->gy[i]; w++; }
         f->n = w;
         free(drop);
-        if (reserve(f, f->n + nadd)) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+        if (reserve(f, f->n + nadd)) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
         for (k = 0; k < ncoarse; k++)
             if (fam[3 * k] >= 0) {
                 f->level[f->n] = fam[3 * k] - 1; f->gx[f->n] = fam[3 * k + 1]; f->gy[f->n] = fam[3 * k + 2];
                 f->n++;
             }
-        if (sort_leaves(f)) { free(fam); fclaw2d_hip_forest_destroy(old); return -1; }
+        if (sort_leaves(f)) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     }
     free(fam);
     /* provenance of every new leaf (cb_fclaw2d_regrid_repopulate, src/fclaw2d_regrid.c:95-230) */
     free(f->pkind); free(f->psrc);
     f->pkind = (int *) malloc(sizeof(int) * (size_t) f->n);
     f->psrc = (int *) malloc(sizeof(int) * 4 * (size_t) f->n);
-    if (!f->pkind || !f->psrc) { fclaw2d_hip_forest_destroy(old); return -1; }
+    if (!f->pkind || !f->psrc) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     f->pn = f->n;
     for (i = 0; i < f->n; i++) {
         const int L = f->level[i], x = f->gx[i], y = f->gy[i];
@@ -491,10 +600,15 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
         f->pkind[i] = FCLAW2D_HIP_REGRID_PARENT;
         for (g = 0; g < 4; g++) {
             src[g] = lookup(old, L + 1, 2 * x + (g & 1), 2 * y + (g >> 1));
-            if (src[g] < 0) { fclaw2d_hip_forest_destroy(old); return -1; }
+            if (src[g] < 0) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
         }
     }
     fclaw2d_hip_forest_destroy(old);
+    if (orel) {
+        const int rc = update_relations(f, orel, n0, omx, omy);
+        free(orel);
+        if (rc) return -1;
+    }
     return f->n;
 }
 

@@ -111,6 +111,9 @@ struct hipclaw_state {
     int lslots_cap = 0;
     std::vector<int> loff;
     int lmin = 0, lmax = -1;      // lmax < lmin: no levels set
+    void* hstage = nullptr;       // pinned staging buffer of host-to-device uploads
+    size_t hstage_cap = 0;
+    bool hstage_busy = false;     // a copy out of hstage may still run
     double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
     bool saved = false;           // qsave holds the current slots' state
     // ghost-patch message: slab offsets of one field's band, in message order
@@ -454,6 +457,10 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->qlast);
     hipFree(s->qti);
     hipFree(s->lslots);
+    if (s->hstage) {
+        if (s->st) hipStreamSynchronize(s->st);
+        hipHostFree(s->hstage);
+    }
     hipFree(s->qsave);
     hipFree(s->cfl);
     hipFree(s->mass_buf);
@@ -877,6 +884,29 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
 }  // extern "C"
 
 // Upload n elements of `host` into a device array grown as needed.
+// Host-to-device copy of a program or table through the state's pinned
+// staging buffer (pageable copies run several times slower).  The buffer is
+// reused by the next upload, which first waits for the copy out of it.
+static int staged_upload(hipclaw_state_t* s, void* dev, const void* host, size_t bytes) {
+    if (s->hstage_busy) {
+        HIP_TRY(hipStreamSynchronize(s->st));
+        s->hstage_busy = false;
+    }
+    if (bytes > s->hstage_cap) {
+        if (s->hstage) hipHostFree(s->hstage);
+        s->hstage = nullptr;
+        s->hstage_cap = 0;
+        const size_t cap = bytes + bytes / 4;
+        if (hipHostMalloc(&s->hstage, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipHostMalloc of the upload staging buffer failed%s");
+        s->hstage_cap = cap;
+    }
+    std::memcpy(s->hstage, host, bytes);
+    HIP_TRY(hipMemcpyAsync(dev, s->hstage, bytes, hipMemcpyHostToDevice, s->st));
+    s->hstage_busy = true;
+    return HIPCLAW_OK;
+}
+
 template <typename T>
 static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, int n, const char* what) {
     if (n > cap) {
@@ -887,7 +917,7 @@ static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, in
             return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
         cap = n;
     }
-    if (n > 0) HIP_TRY(hipMemcpyAsync(dev, host, sizeof(T) * (size_t) n, hipMemcpyHostToDevice, s->st));
+    if (n > 0) return staged_upload(s, dev, host, sizeof(T) * (size_t) n);
     return HIPCLAW_OK;
 }
 

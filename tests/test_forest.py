@@ -254,3 +254,35 @@ def test_isolated_blocks_hold_independent_runs():
         fb = fa.Forest(1, 1, (False, False), leaves=(lv, gx, gy))
         qo = oracle_ghost_update(mp, fb, q[idx], 8)
         np.testing.assert_array_equal(qe[idx], qo)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_adapt_keeps_relations_equal_to_a_fresh_forest(seed):
+    """After an adapt the forest reuses the relations of the leaves that
+    stayed; every ghost and sync program must equal the one of a forest built
+    from scratch on the same leaves (brick, periodic and isolated blocks)."""
+    rng = np.random.default_rng(100 + seed)
+    per = (seed % 2 == 0, seed % 3 == 0)
+    f = fa.Forest(2, 1 + seed % 2, per, 2, isolated=(seed % 4 == 3))
+    for _ in range(3):
+        f.refine((rng.random(f.npatch) < 0.35).astype(np.int32), 5)
+    mthbc = (1, 3, 1, 3)
+    for _ in range(5):
+        f.ghost_program(16, 16, 2, mthbc)          # relations cached for (16, 16)
+        refine = (rng.random(f.npatch) < 0.2).astype(np.int32)
+        coarsen = (rng.random(f.npatch) < 0.7).astype(np.int32)
+        f.adapt(refine, coarsen, 1, 5)
+        g = fa.Forest(f.nbx, f.nby, f.periodic, leaves=f.leaves(), isolated=f.isolated,
+                      balance=False)
+        for x, y in zip(f.leaves(), g.leaves()):
+            np.testing.assert_array_equal(x, y)
+        a, pa = f.ghost_program(16, 16, 2, mthbc)
+        b, pb = g.ghost_program(16, 16, 2, mthbc)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(f.sync_program(16, 16), g.sync_program(16, 16))
+        lo, hi = int(f.leaves()[0].min()), int(f.leaves()[0].max())
+        for ti in range(lo, hi):
+            a, pa = f.ghost_program_range(16, 16, 2, mthbc, ti + 1, hi, True)
+            b, pb = g.ghost_program_range(16, 16, 2, mthbc, ti + 1, hi, True)
+            np.testing.assert_array_equal(a, b)
