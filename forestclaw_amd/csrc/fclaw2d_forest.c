@@ -47,6 +47,9 @@ struct fclaw2d_hip_forest {
      * on first use and dropped whenever the leaves change */
     nbinfo_t *rel;
     int rel_mx, rel_my, rel_valid;
+    size_t rel_cap;          /* leaves rel has room for */
+    nbinfo_t *rel_spare;     /* a second buffer, for carrying relations over an adapt */
+    size_t spare_cap;
 };
 
 /* ---------------------------------------------------------------- hashing */
@@ -313,7 +316,9 @@ static int balance(fclaw2d_hip_forest_t *f) { return balance_from(f, NULL, 0); }
 static int split_and_balance(fclaw2d_hip_forest_t *f, const char *mark)
 {
     const int n0 = f->n;
-    int *kids, nk = 0, i, rc;
+    int *kids, nk = 0, i, rc, any = 0;
+    for (i = 0; i < n0 && !any; i++) any = mark[i] != 0;
+    if (!any && f->bal) return 0;   /* nothing split: the leaves, their order and map stay */
     if (split_marked(f, mark)) return -1;
     if (!f->bal) return balance(f);
     kids = (int *) malloc(sizeof(int) * (size_t) (f->n + 1));
@@ -400,7 +405,7 @@ void fclaw2d_hip_forest_destroy(fclaw2d_hip_forest_t *f)
 {
     if (!f) return;
     free(f->level); free(f->gx); free(f->gy); free(f->hash);
-    free(f->pkind); free(f->psrc); free(f->rel);
+    free(f->pkind); free(f->psrc); free(f->rel); free(f->rel_spare);
     free(f);
 }
 
@@ -472,21 +477,34 @@ static void corner_rel(const fclaw2d_hip_forest_t *f, int p, int ic, int mx, int
 static int update_relations(fclaw2d_hip_forest_t *f, const nbinfo_t *orel, int n0, int mx, int my)
 {
     int *o2n = (int *) malloc(sizeof(int) * (size_t) (n0 + 1));
-    nbinfo_t *rel = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * (size_t) (f->n > 0 ? f->n : 1));
+    nbinfo_t *rel;
     int i, k, m;
-    if (!o2n || !rel) { free(o2n); free(rel); return -1; }
+    if (!o2n) return -1;
+    if (f->spare_cap < (size_t) f->n) {   /* grown with headroom, kept across adapts */
+        free(f->rel_spare);
+        f->spare_cap = (size_t) f->n + (size_t) f->n / 4 + 16;
+        f->rel_spare = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * f->spare_cap);
+        if (!f->rel_spare) { f->spare_cap = 0; free(o2n); return -1; }
+    }
+    rel = f->rel_spare;
     for (i = 0; i < n0; i++) o2n[i] = -1;
     for (i = 0; i < f->n; i++)
         if (f->pkind[i] == FCLAW2D_HIP_REGRID_SAME) o2n[f->psrc[4 * i]] = i;
     for (i = 0; i < f->n; i++) {
         const int q = f->pkind[i] == FCLAW2D_HIP_REGRID_SAME ? f->psrc[4 * i] : -1;
+        int kept[4] = {0, 0, 0, 0};
         for (k = 0; k < 8; k++) {
             nbinfo_t *r = &rel[8 * (size_t) i + k];
             int keep = q >= 0;
             if (keep) {
                 *r = orel[8 * (size_t) q + k];
-                if (r->rel == REL_NONE) keep = 0;
-                else if (r->rel != REL_BOUNDARY) {
+                if (r->rel == REL_NONE) {
+                    /* a hanging corner lies inside the side of a coarser leaf,
+                     * which is the face neighbour on one of the corner's two
+                     * faces: while both face relations stand, so does it */
+                    const int ic = k - 4;
+                    keep = k >= 4 && kept[(ic & 1) ? 1 : 0] && kept[(ic & 2) ? 3 : 2];
+                } else if (r->rel != REL_BOUNDARY) {
                     const int nn = (r->rel == REL_HALF && k < 4) ? 2 : 1;
                     for (m = 0; m < nn && keep; m++) {
                         r->nb[m] = o2n[r->nb[m]];
@@ -498,11 +516,15 @@ static int update_relations(fclaw2d_hip_forest_t *f, const nbinfo_t *orel, int n
                 if (k < 4) face_rel(f, i, k, mx, my, r);
                 else corner_rel(f, i, k - 4, mx, my, r);
             }
+            if (k < 4) kept[k] = keep;
         }
     }
     free(o2n);
-    free(f->rel);
+    free(f->rel);                 /* (null: the adapt holds the old buffer) */
     f->rel = rel;
+    f->rel_cap = f->spare_cap;
+    f->rel_spare = NULL;
+    f->spare_cap = 0;
     f->rel_mx = mx; f->rel_my = my; f->rel_valid = 1;
     return 0;
 }
@@ -515,13 +537,16 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
     int i, g, n0, ncoarse = 0;
     int *fam = NULL;   /* parent (level, X, Y) of families to coarsen, 3 ints each */
     nbinfo_t *orel = NULL;   /* the relation cache of the leaves before the adapt */
+    size_t orel_cap = 0;
     int omx = 0, omy = 0;
     if (!f || !refine || !coarsen) return -1;
     n0 = f->n;
     if (f->rel_valid) {
         orel = f->rel;
+        orel_cap = f->rel_cap;
         omx = f->rel_mx; omy = f->rel_my;
         f->rel = NULL;
+        f->rel_cap = 0;
         f->rel_valid = 0;
     }
     /* keep the old leaves for the provenance */
@@ -606,8 +631,11 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
     fclaw2d_hip_forest_destroy(old);
     if (orel) {
         const int rc = update_relations(f, orel, n0, omx, omy);
-        free(orel);
-        if (rc) return -1;
+        if (rc) { free(orel); return -1; }
+        /* the old buffer is the spare of the next adapt */
+        free(f->rel_spare);
+        f->rel_spare = orel;
+        f->spare_cap = orel_cap;
     }
     return f->n;
 }
@@ -713,9 +741,12 @@ static const nbinfo_t *relations(const fclaw2d_hip_forest_t *cf, int p, int mx, 
     fclaw2d_hip_forest_t *f = (fclaw2d_hip_forest_t *) cf;   /* the cache is not part of the forest's value */
     if (!f->rel_valid || f->rel_mx != mx || f->rel_my != my) {
         int q, k;
-        free(f->rel);
-        f->rel = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * (size_t) (f->n > 0 ? f->n : 1));
-        if (!f->rel) return NULL;
+        if (f->rel_cap < (size_t) f->n || !f->rel) {
+            free(f->rel);
+            f->rel_cap = (size_t) f->n + (size_t) f->n / 4 + 16;
+            f->rel = (nbinfo_t *) malloc(sizeof(nbinfo_t) * 8 * f->rel_cap);
+            if (!f->rel) { f->rel_cap = 0; return NULL; }
+        }
         for (q = 0; q < f->n; q++) {
             for (k = 0; k < 4; k++) face_rel(f, q, k, mx, my, &f->rel[8 * (size_t) q + k]);
             for (k = 0; k < 4; k++) corner_rel(f, q, k, mx, my, &f->rel[8 * (size_t) q + 4 + k]);
