@@ -61,11 +61,30 @@ template <>
 __device__ __forceinline__ float sq<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 // ----------------------------------------------------------------- shallow water
+// Wave components that are zero by construction.  A sum over the waves that
+// starts at +0 may skip those terms without changing its bits (in
+// round-to-nearest such a sum is never -0, so adding x * 0 = ±0 leaves it
+// as it is), so the step kernel and the solvers skip them.
+template <class RP, int IXY, class = void>
+struct wave_zeros {
+    __host__ __device__ static constexpr bool at(int, int) { return false; }
+};
+template <class RP, int IXY>
+struct wave_zeros<RP, IXY, decltype((void) RP::template wave_zero<IXY>(0, 0))> {
+    __host__ __device__ static constexpr bool at(int mw, int m) { return RP::template wave_zero<IXY>(mw, m); }
+};
+
 template <typename R>
 struct RPShallow {
     static constexpr int MEQN = 3, MWAVES = 3, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;  // amdq + apdq = f(qR) - f(qL)
     static constexpr int NROE = 3;  // u, v, a
+
+    // the shear wave carries no depth and no normal momentum
+    template <int IXY>
+    __host__ __device__ static constexpr bool wave_zero(int mw, int m) {
+        return mw == 1 && (m == 0 || m == (IXY == 1 ? 1 : 2));
+    }
 
     template <int IXY>
     __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
@@ -131,7 +150,8 @@ struct RPShallow {
         for (int m = 0; m < 3; m++) {
             R df = R(0);
 #pragma unroll
-            for (int mw = 0; mw < 3; mw++) df = df + s[mw] * wave[mw][m];
+            for (int mw = 0; mw < 3; mw++)
+                if (!wave_zero<IXY>(mw, m)) df = df + s[mw] * wave[mw][m];
             apdq[m] = df - amdq[m];
         }
     }
@@ -155,6 +175,7 @@ struct RPShallow {
             R m_ = R(0), p_ = R(0);
 #pragma unroll
             for (int mw = 0; mw < 3; mw++) {
+                if (mw == 1 && m != mu) continue;   // wb[1][0], wb[1][mv] are zero
                 m_ = m_ + rmin(sb[mw], R(0)) * wb[mw][m];
                 p_ = p_ + rmax(sb[mw], R(0)) * wb[mw][m];
             }
@@ -206,6 +227,12 @@ struct RPEuler {
     static constexpr int MEQN = 4, MWAVES = 4, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;
     static constexpr int NROE = 7;  // u, v, a, enth, u2v2, g1a2, euv
+
+    // the shear wave carries no density and no normal momentum
+    template <int IXY>
+    __host__ __device__ static constexpr bool wave_zero(int mw, int m) {
+        return mw == 1 && (m == 0 || m == (IXY == 1 ? 1 : 2));
+    }
 
     template <int IXY>
     __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
@@ -299,7 +326,8 @@ struct RPEuler {
         for (int m = 0; m < 4; m++) {
             R df = R(0);
 #pragma unroll
-            for (int mw = 0; mw < 4; mw++) df = df + s[mw] * wave[mw][m];
+            for (int mw = 0; mw < 4; mw++)
+                if (!wave_zero<IXY>(mw, m)) df = df + s[mw] * wave[mw][m];
             apdq[m] = df - amdq[m];
         }
     }

@@ -214,6 +214,9 @@ template <class RP, typename R, int T, int NW, bool FLUXFORM, bool REGS, int LIM
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<RP, R, T>())))
 step_tiles(StepArgs a) {
     constexpr int MEQN = RP::MEQN, MW = RP::MWAVES, MAUX = RP::MAUX, NROE = RP::NROE;
+    // wave components zero by construction, skipped in the sums over waves
+    using WZ1 = wave_zeros<RP, 1>;
+    using WZ2 = wave_zeros<RP, 2>;
     using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
     constexpr int QW = Smem::QW, QS = Smem::QS, QP = Smem::QP, GS = Smem::GS;
     constexpr int L = T + 3;               // lanes per row/column of interfaces
@@ -494,9 +497,11 @@ step_tiles(StepArgs a) {
                     if (mth == 0) continue;
                     R wn = R(0), dotl = R(0);
 #pragma unroll
-                    for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
+                    for (int m = 0; m < MEQN; m++)
+                        if (!WZ1::at(mw, m)) wn = wn + wave_[mw][m] * wave_[mw][m];
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
+                        if (WZ1::at(mw, m)) continue;
                         const R wl = prev_lane(wave_[mw][m]);
                         dotl = dotl + wl * wave_[mw][m];
                     }
@@ -505,7 +510,8 @@ step_tiles(StepArgs a) {
                         const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                         const R phi = philim<R>(mth, rr);
 #pragma unroll
-                        for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
+                        for (int m = 0; m < MEQN; m++)
+                            if (!WZ1::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
                     }
                 }
 #pragma unroll
@@ -513,7 +519,7 @@ step_tiles(StepArgs a) {
                     R c = R(0);
 #pragma unroll
                     for (int mw = 0; mw < MW; mw++)
-                        c = c + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdx) * wave_[mw][m];
+                        if (!WZ1::at(mw, m)) c = c + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdx) * wave_[mw][m];
                     cqxx[m] = c;
                 }
             }
@@ -597,9 +603,11 @@ step_tiles(StepArgs a) {
                     if (mth == 0) continue;
                     R wn = R(0), dotl = R(0);
 #pragma unroll
-                    for (int m = 0; m < MEQN; m++) wn = wn + wave_[mw][m] * wave_[mw][m];
+                    for (int m = 0; m < MEQN; m++)
+                        if (!WZ2::at(mw, m)) wn = wn + wave_[mw][m] * wave_[mw][m];
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
+                        if (WZ2::at(mw, m)) continue;
                         const R wl = prev_lane(wave_[mw][m]);
                         dotl = dotl + wl * wave_[mw][m];
                     }
@@ -608,7 +616,8 @@ step_tiles(StepArgs a) {
                         const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
                         const R phi = philim<R>(mth, rr);
 #pragma unroll
-                        for (int m = 0; m < MEQN; m++) wave_[mw][m] = phi * wave_[mw][m];
+                        for (int m = 0; m < MEQN; m++)
+                            if (!WZ2::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
                     }
                 }
 #pragma unroll
@@ -616,7 +625,7 @@ step_tiles(StepArgs a) {
                     R cq = R(0);
 #pragma unroll
                     for (int mw = 0; mw < MW; mw++)
-                        cq = cq + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdy) * wave_[mw][m];
+                        if (!WZ2::at(mw, m)) cq = cq + fabs(s[mw]) * (R(1) - fabs(s[mw]) * dtdy) * wave_[mw][m];
                     cqxx[m] = cq;
                 }
             }
