@@ -86,20 +86,48 @@ struct RPShallow {
         return mw == 1 && (m == 0 || m == (IXY == 1 ? 1 : 2));
     }
 
+    // What rpn2 computes from one cell alone: sqrt(h), hu_n / sqrt(h),
+    // hu_t / sqrt(h), u_n = hu_n / h and sqrt(g h) (n: the sweep's normal
+    // direction).  The step kernel evaluates them once per cell and hands each
+    // interface its two cells' values (the same operations, so the same bits).
+    static constexpr int NCELL = 5;
     template <int IXY>
-    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
-                                                const R* /*auxR*/,
-                                                const PhysParams& pp, R wave[MWAVES][MEQN],
-                                                R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
-                                                R roe[NROE]) {
+    __device__ __forceinline__ static void cell(const R* q, const PhysParams& pp, R c[NCELL]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R hs = sq(q[0]);
+        c[0] = hs;
+        c[1] = dv(q[mu], hs);
+        c[2] = dv(q[mv], hs);
+        c[3] = dv(q[mu], q[0]);
+        c[4] = sq((R) pp.grav * q[0]);
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* auxL,
+                                                const R* auxR, const PhysParams& pp,
+                                                R wave[MWAVES][MEQN], R s[MWAVES], R amdq[MEQN],
+                                                R apdq[MEQN], R roe[NROE]) {
+        R cL[NCELL], cR[NCELL];
+        cell<IXY>(qL, pp, cL);
+        cell<IXY>(qR, pp, cR);
+        rpn2c<IXY>(qL, qR, cL, cR, auxL, auxR, pp, wave, s, amdq, apdq, roe);
+    }
+
+    // rpn2 from the two cells' values of `cell`
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2c(const R* qL, const R* qR, const R* cL,
+                                                 const R* cR, const R* /*auxL*/,
+                                                 const R* /*auxR*/, const PhysParams& pp,
+                                                 R wave[MWAVES][MEQN], R s[MWAVES], R amdq[MEQN],
+                                                 R apdq[MEQN], R roe[NROE]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R grav = (R) pp.grav;
         const R h = dv(qL[0] + qR[0], R(2));
-        const R hsqrtl = sq(qL[0]);
-        const R hsqrtr = sq(qR[0]);
+        const R hsqrtl = cL[0];
+        const R hsqrtr = cR[0];
         const R hsq2 = hsqrtl + hsqrtr;
-        const R u = dv(dv(qL[mu], hsqrtl) + dv(qR[mu], hsqrtr), hsq2);
-        const R v = dv(dv(qL[mv], hsqrtl) + dv(qR[mv], hsqrtr), hsq2);
+        const R u = dv(cL[1] + cR[1], hsq2);
+        const R v = dv(cL[2] + cR[2], hsq2);
         const R a = sq(grav * h);
         roe[0] = u; roe[1] = v; roe[2] = a;
 
@@ -113,8 +141,7 @@ struct RPShallow {
         wave[2][0] = a3;   wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;
 
         // entropy fix
-        const R him1 = qL[0];
-        const R s0 = dv(qL[mu], him1) - sq(grav * him1);
+        const R s0 = cL[3] - cL[4];   // dv(qL[mu], qL[0]) - sq(grav * qL[0])
         if (s0 > R(0) && s[0] > R(0)) {
 #pragma unroll
             for (int m = 0; m < 3; m++) amdq[m] = R(0);
@@ -131,8 +158,7 @@ struct RPShallow {
             if (!(s[1] > R(0))) {
 #pragma unroll
                 for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
-                const R hi = qR[0];
-                const R s03 = dv(qR[mu], hi) + sq(grav * hi);
+                const R s03 = cR[3] + cR[4];   // dv(qR[mu], qR[0]) + sq(grav * qR[0])
                 const R h3 = qR[0] - wave[2][0];
                 const R hu3 = qR[mu] - wave[2][mu];
                 const R s3 = dv(hu3, h3) + sq(grav * h3);
@@ -234,23 +260,56 @@ struct RPEuler {
         return mw == 1 && (m == 0 || m == (IXY == 1 ? 1 : 2));
     }
 
+    // What rpn2 computes from one cell alone (n: the sweep's normal
+    // direction): sqrt(rho), rho u_n / sqrt(rho), rho u_t / sqrt(rho),
+    // (E + p) / sqrt(rho) with p from (rho u)^2 + (rho v)^2, u_n and the sound
+    // speed of the entropy fix, whose p sums the momenta in the sweep's order.
+    // The step kernel evaluates them once per cell (the same operations, so
+    // the same bits).
+    static constexpr int NCELL = 6;
     template <int IXY>
-    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* /*auxL*/,
-                                                const R* /*auxR*/,
-                                                const PhysParams& pp, R wave[MWAVES][MEQN],
-                                                R s[MWAVES], R amdq[MEQN], R apdq[MEQN],
-                                                R roe[NROE]) {
+    __device__ __forceinline__ static void cell(const R* q, const PhysParams& pp, R c[NCELL]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R gamma = (R) pp.gamma;
         const R gamma1 = (R) (pp.gamma - 1.0);
-        const R rhsqrtl = sq(qL[0]);
-        const R rhsqrtr = sq(qR[0]);
-        const R pl = gamma1 * (qL[3] - dv(R(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]), qL[0]));
-        const R pr = gamma1 * (qR[3] - dv(R(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]), qR[0]));
+        const R rs = sq(q[0]);
+        const R p = gamma1 * (q[3] - dv(R(0.5) * (q[1] * q[1] + q[2] * q[2]), q[0]));
+        const R pn = gamma1 * (q[3] - dv(R(0.5) * (q[mu] * q[mu] + q[mv] * q[mv]), q[0]));
+        c[0] = rs;
+        c[1] = dv(q[mu], rs);
+        c[2] = dv(q[mv], rs);
+        c[3] = dv(q[3] + p, rs);
+        c[4] = dv(q[mu], q[0]);
+        c[5] = sq(dv(gamma * pn, q[0]));
+    }
+
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2(const R* qL, const R* qR, const R* auxL,
+                                                const R* auxR, const PhysParams& pp,
+                                                R wave[MWAVES][MEQN], R s[MWAVES], R amdq[MEQN],
+                                                R apdq[MEQN], R roe[NROE]) {
+        R cL[NCELL], cR[NCELL];
+        cell<IXY>(qL, pp, cL);
+        cell<IXY>(qR, pp, cR);
+        rpn2c<IXY>(qL, qR, cL, cR, auxL, auxR, pp, wave, s, amdq, apdq, roe);
+    }
+
+    // rpn2 from the two cells' values of `cell`
+    template <int IXY>
+    __device__ __forceinline__ static void rpn2c(const R* qL, const R* qR, const R* cL,
+                                                 const R* cR, const R* /*auxL*/,
+                                                 const R* /*auxR*/, const PhysParams& pp,
+                                                 R wave[MWAVES][MEQN], R s[MWAVES], R amdq[MEQN],
+                                                 R apdq[MEQN], R roe[NROE]) {
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const R gamma = (R) pp.gamma;
+        const R gamma1 = (R) (pp.gamma - 1.0);
+        const R rhsqrtl = cL[0];
+        const R rhsqrtr = cR[0];
         const R rhsq2 = rhsqrtl + rhsqrtr;
-        const R u = dv(dv(qL[mu], rhsqrtl) + dv(qR[mu], rhsqrtr), rhsq2);
-        const R v = dv(dv(qL[mv], rhsqrtl) + dv(qR[mv], rhsqrtr), rhsq2);
-        const R enth = dv(dv(qL[3] + pl, rhsqrtl) + dv(qR[3] + pr, rhsqrtr), rhsq2);
+        const R u = dv(cL[1] + cR[1], rhsq2);
+        const R v = dv(cL[2] + cR[2], rhsq2);
+        const R enth = dv(cL[3] + cR[3], rhsq2);
         const R u2v2 = u * u + v * v;
         const R a2s = gamma1 * (enth - R(.5) * u2v2);
         const R a = sq(a2s);
@@ -274,10 +333,7 @@ struct RPEuler {
         wave[3][3] = a4 * (enth + u * a); s[3] = u + a;
 
         // entropy fix
-        const R rhoim1 = qL[0];
-        const R pim1 = gamma1 * (qL[3] - dv(R(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]), rhoim1));
-        const R cim1 = sq(dv(gamma * pim1, rhoim1));
-        const R s0 = dv(qL[mu], rhoim1) - cim1;
+        const R s0 = cL[4] - cL[5];   // u_n - c of the left cell
         if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
             for (int m = 0; m < 4; m++) amdq[m] = R(0);
@@ -301,10 +357,7 @@ struct RPEuler {
                     amdq[m] = amdq[m] + s[1] * wave[1][m];
                     amdq[m] = amdq[m] + s[2] * wave[2][m];
                 }
-                const R rhoi = qR[0];
-                const R pi = gamma1 * (qR[3] - dv(R(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]), rhoi));
-                const R ci = sq(dv(gamma * pi, rhoi));
-                const R s3 = dv(qR[mu], rhoi) + ci;
+                const R s3 = cR[4] + cR[5];   // u_n + c of the right cell
                 const R rho2 = qR[0] - wave[3][0];
                 const R rhou2 = qR[mu] - wave[3][mu];
                 const R rhov2 = qR[mv] - wave[3][mv];

@@ -187,6 +187,37 @@ __device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool
     return kn && rt;
 }
 
+// Per-cell values a Riemann functor can take from the step kernel
+// (RP::NCELL, RP::cell, RP::rpn2c), 0 if it has none.
+template <class RP, class = void>
+struct cell_values {
+    static constexpr int N = 0;
+};
+template <class RP>
+struct cell_values<RP, decltype((void) RP::NCELL)> {
+    static constexpr int N = RP::NCELL;
+};
+
+// One interface's Riemann solve.  With per-cell values, each lane evaluates
+// its own (right) cell's and takes the left cell's from the lane before: the
+// kernel lays a row's cells on consecutive lanes, so every cell is evaluated
+// once per sweep instead of once per side of each interface.
+template <class RP, int IXY, int NC, typename R>
+__device__ __forceinline__ void riemann(const R* qL, const R* qR, const R* auxL, const R* auxR,
+                                        const PhysParams& pp, R wave[RP::MWAVES][RP::MEQN],
+                                        R s[RP::MWAVES], R amdq[RP::MEQN], R apdq[RP::MEQN],
+                                        R roe[RP::NROE]) {
+    if constexpr (NC > 0) {
+        R cL[NC], cR[NC];
+        RP::template cell<IXY>(qR, pp, cR);
+#pragma unroll
+        for (int i = 0; i < NC; i++) cL[i] = prev_lane(cR[i]);
+        RP::template rpn2c<IXY>(qL, qR, cL, cR, auxL, auxR, pp, wave, s, amdq, apdq, roe);
+    } else {
+        RP::template rpn2<IXY>(qL, qR, auxL, auxR, pp, wave, s, amdq, apdq, roe);
+    }
+}
+
 // Waves per SIMD the register allocation aims for: 4, except FP64 16 x 16
 // tiles, which need more than the 128 VGPRs of 4 waves and are held to the 168
 // of 3 (without the target the allocator let them grow to 2 waves).
@@ -219,8 +250,9 @@ step_tiles(StepArgs a) {
     using WZ2 = wave_zeros<RP, 2>;
     using Smem = TileSmem<T, MEQN, MAUX, R, FLUXFORM, REGS>;
     constexpr int QW = Smem::QW, QS = Smem::QS, QP = Smem::QP, GS = Smem::GS;
-    constexpr int L = T + 3;               // lanes per row/column of interfaces
+    constexpr int L = T + 4;               // lanes per row/column: one per staged cell
     constexpr int SEG = 64 / L;            // rows (columns) per wave
+    constexpr int NC = cell_values<RP>::N; // per-cell values of the Riemann solver
     constexpr int NT = NW * 64;
     constexpr int NQ = MEQN * QW * QW;     // q values staged per tile
     constexpr int NA = MAUX * QW * QW;     // aux values staged per tile
@@ -237,7 +269,10 @@ step_tiles(StepArgs a) {
     const int mx = a.mx, my = a.my, mbc = a.mbc;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int seg = lane / L, k = lane - seg * L;
+    // lane (seg, kp): cell kp of a staged row (column) and the interface
+    // k = kp - 1 between cells kp - 1 and kp; the left cell's values come from
+    // the lane before (lane kp = 0 has no interface of its own)
+    const int seg = lane / L, kp = lane - seg * L, k = kp - 1;
     double cfl = 0.0;
 
     auto tile_slot = [&](int tile, int& tx, int& ty) {
@@ -475,14 +510,17 @@ step_tiles(StepArgs a) {
             const int r = rbase + seg;
             const bool act = (seg < SEG) && (r < T + 2);
             const int rc = r < T + 1 ? r : T + 1;
-            const int kc = k < L ? k : L - 1;
+            const int kc = kp < L ? kp : L - 1;     // the lane's cell (column)
+            const int kl = kc > 0 ? kc - 1 : 0;     // the cell left of its interface
             R qL[MEQN], qR[MEQN], auxL[MAD], auxR[MAD];
 #pragma unroll
-            for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, kc, rc + 1); qR[m] = SQ(m, kc + 1, rc + 1); }
-            aux_cell(auxL, kc, rc + 1);
-            aux_cell(auxR, kc + 1, rc + 1);
+            for (int m = 0; m < MEQN; m++) qR[m] = SQ(m, kc, rc + 1);
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) qL[m] = prev_lane(qR[m]);
+            aux_cell(auxL, kl, rc + 1);
+            aux_cell(auxR, kc, rc + 1);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-            RP::template rpn2<1>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            riemann<RP, 1, NC>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, r, T, last_x, first_y, last_y)) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
@@ -541,9 +579,9 @@ step_tiles(StepArgs a) {
                 // aux1, aux2, aux3: rows j-1, j, j+1 of column i1 = i-1 (amdq) or i (apdq)
                 {
                     R a1[MAD], a2[MAD], a3[MAD];
-                    aux_cell(a1, kc, rc); aux_cell(a2, kc, rc + 1); aux_cell(a3, kc, rc + 2);
+                    aux_cell(a1, kl, rc); aux_cell(a2, kl, rc + 1); aux_cell(a3, kl, rc + 2);
                     RP::template rpt2<1>(roe, amdq, a1, a2, a3, a.pp, bmA, bpA);
-                    aux_cell(a1, kc + 1, rc); aux_cell(a2, kc + 1, rc + 1); aux_cell(a3, kc + 1, rc + 2);
+                    aux_cell(a1, kc, rc); aux_cell(a2, kc, rc + 1); aux_cell(a3, kc, rc + 2);
                     RP::template rpt2<1>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
                 }
                 const R half = R(0.5) * dtdx;
@@ -581,14 +619,17 @@ step_tiles(StepArgs a) {
             const int c = cbase + seg;
             const bool act = (seg < SEG) && (c < T + 2);
             const int cc = c < T + 1 ? c : T + 1;
-            const int kc = k < L ? k : L - 1;
+            const int kc = kp < L ? kp : L - 1;     // the lane's cell (row)
+            const int kl = kc > 0 ? kc - 1 : 0;     // the cell below its interface
             R qL[MEQN], qR[MEQN], auxL[MAD], auxR[MAD];
 #pragma unroll
-            for (int m = 0; m < MEQN; m++) { qL[m] = SQ(m, cc + 1, kc); qR[m] = SQ(m, cc + 1, kc + 1); }
-            aux_cell(auxL, cc + 1, kc);
-            aux_cell(auxR, cc + 1, kc + 1);
+            for (int m = 0; m < MEQN; m++) qR[m] = SQ(m, cc + 1, kc);
+#pragma unroll
+            for (int m = 0; m < MEQN; m++) qL[m] = prev_lane(qR[m]);
+            aux_cell(auxL, cc + 1, kl);
+            aux_cell(auxR, cc + 1, kc);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
-            RP::template rpn2<2>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
+            riemann<RP, 2, NC>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
             if (act && cfl_owned(k, c, T, last_y, first_x, last_x)) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
@@ -678,9 +719,9 @@ step_tiles(StepArgs a) {
                 // aux1, aux2, aux3: columns i-1, i, i+1 of row j1 = j-1 (amdq) or j (apdq)
                 {
                     R a1[MAD], a2[MAD], a3[MAD];
-                    aux_cell(a1, cc, kc); aux_cell(a2, cc + 1, kc); aux_cell(a3, cc + 2, kc);
+                    aux_cell(a1, cc, kl); aux_cell(a2, cc + 1, kl); aux_cell(a3, cc + 2, kl);
                     RP::template rpt2<2>(roe, amdq, a1, a2, a3, a.pp, bmA, bpA);
-                    aux_cell(a1, cc, kc + 1); aux_cell(a2, cc + 1, kc + 1); aux_cell(a3, cc + 2, kc + 1);
+                    aux_cell(a1, cc, kc); aux_cell(a2, cc + 1, kc); aux_cell(a3, cc + 2, kc);
                     RP::template rpt2<2>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
                 }
                 const R half = R(0.5) * dtdy;
