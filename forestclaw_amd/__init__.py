@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libforestclaw_amd.so")
 
 RP_ADVECTION, RP_SHALLOW, RP_EULER = 0, 1, 2
 FP64, FP32_BULK = 0, 1
-AUTOTUNE_MAX_TRIALS = 12   # HIPCLAW_AUTOTUNE_MAX_TRIALS
+AUTOTUNE_MAX_TRIALS = 16   # HIPCLAW_AUTOTUNE_MAX_TRIALS
 # provenance of a leaf after a forest adapt (fclaw2d_hip_forest_provenance)
 REGRID_SAME, REGRID_CHILD, REGRID_PARENT = 0, 1, 2
 SOLVER_SHAPES = {RP_ADVECTION: (1, 1, 2), RP_SHALLOW: (3, 3, 0), RP_EULER: (4, 4, 0)}

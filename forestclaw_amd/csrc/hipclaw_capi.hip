@@ -1328,8 +1328,8 @@ static int effective_waves(const hipclaw_state_t* s) {
 }
 
 int hipclaw_set_waves(hipclaw_state_t* s, int waves) {
-    if (!s || (waves != 0 && waves != 2 && waves != 4))
-        return fail(HIPCLAW_EINVAL, "hipclaw_set_waves: waves must be 0, 2 or 4%s");
+    if (!s || (waves != 0 && waves != 2 && waves != 3 && waves != 4))
+        return fail(HIPCLAW_EINVAL, "hipclaw_set_waves: waves must be 0, 2, 3 or 4%s");
     s->waves = waves;
     // a captured graph holds the launches of the old kernel
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
@@ -1365,8 +1365,8 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
     for (int t : candidates) {
         if (!tile_allowed(s, t)) continue;
         const int nlay = s->fused_ok ? 3 : 2;
-        for (int trial = 0; trial < (t == 16 ? 2 * nlay : nlay) && !rc; trial++) {
-            const int lay = trial % nlay, w = t == 16 ? (trial < nlay ? 2 : 4) : 0;
+        for (int trial = 0; trial < (t == 16 ? 3 * nlay : nlay) && !rc; trial++) {
+            const int lay = trial % nlay, w = t == 16 ? 2 + trial / nlay : 0;
             if ((rc = set_tile(s, t)) || (rc = hipclaw_set_ghost_layout(s, lay)) || (rc = hipclaw_set_waves(s, w)))
                 break;
             // one untimed step (first use of these kernels), then the timed ones

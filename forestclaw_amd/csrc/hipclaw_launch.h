@@ -120,7 +120,7 @@ inline int launch_step_shape(const StepArgs& a, const StepLaunch& st) {
 template <typename R>
 constexpr int default_waves_t16() { return sizeof(R) == 8 ? 4 : 2; }
 
-// waves = 0: the default for 16 x 16 tiles; 2 or 4 select the variant.
+// waves = 0: the default for 16 x 16 tiles; 2, 3 or 4 select the variant.
 // T = 32 only where WITH32 (its LDS budget is checked per solver).
 template <class RP, typename R, bool WITH_REGS, bool WITH32 = false>
 inline int launch_step_tile(int tile, int waves, const StepArgs& a, const StepLaunch& st) {
@@ -129,6 +129,7 @@ inline int launch_step_tile(int tile, int waves, const StepArgs& a, const StepLa
     case 8: return launch_step_shape<RP, R, 8, 2, WITH_REGS>(a, st);
     case 16:
         return waves == 2 ? launch_step_shape<RP, R, 16, 2, WITH_REGS>(a, st)
+             : waves == 3 ? launch_step_shape<RP, R, 16, 3, WITH_REGS>(a, st)
                           : launch_step_shape<RP, R, 16, 4, WITH_REGS>(a, st);
     case 32:
         if constexpr (WITH32) return launch_step_shape<RP, R, 32, 16, WITH_REGS>(a, st);

@@ -285,28 +285,29 @@ int hipclaw_restore_step(hipclaw_state_t *s);
 int hipclaw_set_ghost_layout(hipclaw_state_t *s, int layout);
 int hipclaw_ghost_layout(const hipclaw_state_t *s);
 
-/* Waves per workgroup of the 16 x 16 update tile: 2 or 4 (0 = the default:
- * 2 for the FP32-bulk form, 4 for FP64).  Two waves take three
- * row groups of a sweep each, four take two, two, one and one; fewer waves
- * also let more workgroups share a CU when the tile's LDS allows.
+/* Waves per workgroup of the 16 x 16 update tile: 2, 3 or 4 (0 = the
+ * default: 2 for the FP32-bulk form, 4 for FP64).  A sweep has six row
+ * groups: two waves take three each, three take two, four take two, two, one
+ * and one; fewer waves per workgroup hold fewer staging registers, and more
+ * workgroups share a CU when the tile's LDS allows.
  * hipclaw_waves returns the waves the next update uses (0 for tiles without
  * a choice).  No counterpart in the reference (128 threads per patch). */
 int hipclaw_set_waves(hipclaw_state_t *s, int waves);
 int hipclaw_waves(const hipclaw_state_t *s);
 
 /* Most trials of one hipclaw_autotune call. */
-#define HIPCLAW_AUTOTUNE_MAX_TRIALS 12
+#define HIPCLAW_AUTOTUNE_MAX_TRIALS 16
 
 /* Autotuner: time nsteps full steps (update, ghost fill, time sync) of dt on
  * the current patches for every tile size the state's solver, precision and
- * patch size allow, with 2 and 4 waves per workgroup for 16 x 16 tiles, in
+ * patch size allow, with 2, 3 and 4 waves per workgroup for 16 x 16 tiles, in
  * every ghost-fill layout the program allows, each trial from the state saved
  * at entry and restored after it (the saved state of hipclaw_save_step is
  * overwritten), and keep the fastest.  tiles[k], waves[k], layouts[k], ms[k]
  * (room for HIPCLAW_AUTOTUNE_MAX_TRIALS)
- * receive each trial and its mean step time, *ntrials their count.  Not
- * with subcycling.  No counterpart in the reference, whose CUDA kernels have
- * a fixed block size. */
+ * receive each trial and its mean step time, *ntrials their count.  A
+ * subcycling state (levels set) is timed on subcycled coarse steps.  No
+ * counterpart in the reference, whose CUDA kernels have a fixed block size. */
 int hipclaw_autotune(hipclaw_state_t *s, double dt, int nsteps, int *tiles, int *waves,
                      int *layouts, double *ms, int *ntrials);
 

@@ -173,14 +173,14 @@ def test_advance_two_slabs(gpu, layout):
 
 @pytest.mark.parametrize("solver_case", ["shallow", "euler"])
 def test_waves_per_workgroup(gpu, solver_case):
-    """16 x 16 tiles with 2 and 4 waves per workgroup: FP64 bit-identical to the
+    """16 x 16 tiles with 2, 3 and 4 waves per workgroup: FP64 bit-identical to the
     oracle with time sync, and the FP32-bulk form the same bits either way
     (the waves only change which lanes do the same arithmetic)."""
     from helpers import oracle_advance_ts
     prob = P.dambreak_problem(2, 5, mx=16) if solver_case == "shallow" else P.shockbubble_problem(1, 4, mx=16)
     results = {}
     for precision in (fa.FP64, fa.FP32_BULK):
-        for waves in (2, 4):
+        for waves in (2, 3, 4):
             forest, q0, aux, sv = setup_problem(prob, precision=precision, tile=16, time_sync=True)
             sv.set_waves(waves)
             assert sv.waves == waves
@@ -189,7 +189,8 @@ def test_waves_per_workgroup(gpu, solver_case):
             results[(precision, waves)] = sv.get_q()
         if precision == fa.FP64:
             qo, _ = oracle_advance_ts(prob, forest, q0, aux, prob.mx, dt, 6, time_sync=True)
-            for waves in (2, 4):
+            for waves in (2, 3, 4):
                 assert rel_max_err(results[(precision, waves)], qo) <= TOL, waves
         else:
             np.testing.assert_array_equal(results[(precision, 2)], results[(precision, 4)])
+            np.testing.assert_array_equal(results[(precision, 2)], results[(precision, 3)])
