@@ -604,10 +604,19 @@ step_tiles(StepArgs a) {
         __syncthreads();
 
         // gm(i,j) = gp(i,j) after the x sweeps: row j-1 adds gadd(.,2), then row j adds gadd(.,1)
-        for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NT) {
-            const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
-            const int t = rem / T, c = rem - t * T;
-            GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+        if constexpr (NT % T == 0) {
+            // thread (t0, c): column c, rows t0, t0 + NT/T, ... (no index division)
+            constexpr int TS = NT / T;
+            const int c = threadIdx.x % T, t0 = threadIdx.x / T;
+#pragma unroll
+            for (int m = 0; m < MEQN; m++)
+                for (int t = t0; t <= T; t += TS) GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+        } else {
+            for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NT) {
+                const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
+                const int t = rem / T, c = rem - t * T;
+                GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+            }
         }
         __syncthreads();
         cfl = fmax(cfl, dtdx64 * (double) smax);
