@@ -315,11 +315,15 @@ step_tiles(StepArgs a) {
         laoff[kk] = (MAUX > 0 && e < NA) ? m * QP + qj * QS + qi : -1;
     }
 
-    // Side of the tile's halo each staged element lies on: code = sx + 3 sy,
-    // sx = 1 (left of the tile) / 2 (right) / 0, sy = 1 (below) / 2 (above) / 0,
-    // four bits per element; s_pull[code] is the tile's offset for that side
-    // (a patch face or corner the tile touches), 0 for the patch's own cells.
-    constexpr int NPC = (KQ + 15) / 16;   // 16 codes per 64-bit word
+    // Halo column and row of each staged element: code = cx + 5 cy, cx = 1, 2
+    // for tile columns 0, 1 (left of the tile), 3, 4 for columns T+2, T+3
+    // (right of it), 0 inside; cy likewise for rows.  Five bits per element;
+    // s_pull[code] is the tile's offset for that column and row: a patch face's
+    // or corner's where the cell is a ghost cell of the patch, 0 for the patch's
+    // own cells.  Any tile's halo can reach ghost cells, not only the first and
+    // last tile's (an overlapped last tile leaves the one before it within two
+    // cells of the patch edge).
+    constexpr int NPC = (KQ + 11) / 12;   // 12 codes per 64-bit word
     unsigned long long pcode[NPC];
 #pragma unroll
     for (int w = 0; w < NPC; w++) pcode[w] = 0;
@@ -328,10 +332,11 @@ step_tiles(StepArgs a) {
         const int e = threadIdx.x + kk * NT;
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
-        const int sx = qi < 2 ? 1 : qi > T + 1 ? 2 : 0, sy = qj < 2 ? 1 : qj > T + 1 ? 2 : 0;
-        if (e < NQ) pcode[kk / 16] |= (unsigned long long) (sx + 3 * sy) << (4 * (kk % 16));
+        const int cx = qi < 2 ? 1 + qi : qi > T + 1 ? qi - T + 1 : 0;
+        const int cy = qj < 2 ? 1 + qj : qj > T + 1 ? qj - T + 1 : 0;
+        if (e < NQ) pcode[kk / 12] |= (unsigned long long) (cx + 5 * cy) << (5 * (kk % 12));
     }
-    __shared__ int s_pull[9];
+    __shared__ int s_pull[25];
 
     // prefetch registers for the next tile
     double pq[KQ];
@@ -344,7 +349,7 @@ step_tiles(StepArgs a) {
         if (a.pull) {
 #pragma unroll
             for (int kk = 0; kk < KQ; kk++)
-                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode[kk / 16] >> (4 * (kk % 16))) & 15]] : 0.0;
+                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode[kk / 12] >> (5 * (kk % 12))) & 31]] : 0.0;
         } else {
 #pragma unroll
             for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
@@ -380,21 +385,24 @@ step_tiles(StepArgs a) {
     // that, in every round of the persistent loop, each XCD works on one
     // contiguous run of tiles: neighbouring tiles of a patch share halo cells,
     // which then come from the same L2.
-    // The offsets of tile t for s_pull (threads 0..8, one side each): a side
-    // on a patch face takes that face's offset, one on a patch corner the
-    // corner's, a side inside the patch 0.
+    // The offsets of tile t for s_pull (threads 0..24, one halo column and
+    // row class each): a ghost cell beside a patch face takes that face's
+    // offset, one beside a patch corner the corner's, a cell of the patch 0.
     auto pull_table = [&](int t) {
-        if (threadIdx.x < 9) {
+        if (threadIdx.x < 25) {
             int tx, ty;
             const int slot = tile_slot(t, tx, ty);
             const int* __restrict__ pt = a.pull + 8 * (long long) slot;
-            const int sx = threadIdx.x % 3, sy = threadIdx.x / 3;
-            const bool ex = (sx == 1 && tx == 0) || (sx == 2 && tx == a.ntx - 1);
-            const bool ey = (sy == 1 && ty == 0) || (sy == 2 && ty == a.nty - 1);
+            const int cx = threadIdx.x % 5, cy = threadIdx.x / 5;
+            // the class's patch column and row (1-based; cells 1..mx x 1..my)
+            const int i = origin_x(tx) + (cx == 0 ? 1 : cx < 3 ? cx - 2 : T + cx - 2);
+            const int j = origin_y(ty) + (cy == 0 ? 1 : cy < 3 ? cy - 2 : T + cy - 2);
+            const bool gx = i < 1 || i > mx, gy = j < 1 || j > my;
+            const int hx = i > mx ? 1 : 0, hy = j > my ? 1 : 0;
             int d = 0;
-            if (ex && ey) d = pt[4 + (sx == 2 ? 1 : 0) + (sy == 2 ? 2 : 0)];
-            else if (ex) d = pt[sx == 2 ? 1 : 0];
-            else if (ey) d = pt[sy == 2 ? 3 : 2];
+            if (gx && gy) d = pt[4 + hx + 2 * hy];
+            else if (gx) d = pt[hx];
+            else if (gy) d = pt[2 + hy];
             s_pull[threadIdx.x] = d;
         }
     };
@@ -435,9 +443,13 @@ step_tiles(StepArgs a) {
             // the pending fix-up of the last synchronisation: each corrected edge
             // cell of the staged tile plus its increments, in the reference's
             // order (one thread per cell; a corner cell belongs to its x-face
-            // entry when that face is corrected)
+            // entry when that face is corrected).  A face's edge cells are staged
+            // by every tile whose halo reaches them: the first and last tile
+            // along it, and the one before an overlapped last tile when the
+            // overlap leaves it within two cells of the edge.
             const int mask = a.cmask[slot];
-            const int tb = ((int) first_x | ((int) last_x << 1) | ((int) first_y << 2) | ((int) last_y << 3)) & mask;
+            const int tb = ((int) (i0 <= 3) | ((int) (i0 + T + 1 >= mx) << 1) | ((int) (j0 <= 3) << 2) |
+                            ((int) (j0 + T + 1 >= my) << 3)) & mask;
             if (tb) {
                 const double* __restrict__ qc = a.qcorr + (long long) slot * a.cstride;
 #pragma unroll 1
@@ -452,8 +464,9 @@ step_tiles(StepArgs a) {
                     const int other = (c == 1 || c == len) ? (face < 2 ? (c == 1 ? 2 : 3) : (c == 1 ? 0 : 1)) : -1;
                     const bool other_on = other >= 0 && ((mask >> other) & 1);
                     if (face >= 2 && other_on) continue;
-                    const int qi = face == 0 ? 2 : face == 1 ? T + 1 : q;
-                    const int qj = face == 2 ? 2 : face == 3 ? T + 1 : q;
+                    // the edge cell's tile column (faces 0, 1) or row (2, 3)
+                    const int qi = face == 0 ? 3 - i0 : face == 1 ? mx - i0 + 2 : q;
+                    const int qj = face == 2 ? 3 - j0 : face == 3 ? my - j0 + 2 : q;
                     double& cell = sm.q[m * QP + qj * QS + qi];
                     const double v = qc[(face * MEQN + m) * len + c - 1];
                     double y = cell;

@@ -70,14 +70,17 @@ def test_default_tile_for_other_sizes(gpu):
         assert fa.Solver(fa.RP_SHALLOW, mx, my).tile == want, (mx, my)
 
 
+@pytest.mark.parametrize("mx", [18, 20])
 @pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("tile", [8, 16])
-def test_amr_time_sync_mx20(gpu, layout, tile):
-    """AMR dam break with 20 x 20 patches (neither tile divides them), time
-    sync on, every ghost layout, against the oracle's step / ghost fill / sync
-    sequence."""
-    prob = P.dambreak_problem(2, 4, mx=20)
-    print("case mx 20 layout", layout, "tile", tile, flush=True)
+def test_amr_time_sync_mx20(gpu, layout, tile, mx):
+    """AMR dam break with 20 x 20 and 18 x 18 patches (neither tile divides
+    them), time sync on, every ghost layout, against the oracle's step / ghost
+    fill / sync sequence.  With mx = 18 the overlapped last tile leaves the
+    tile before it two cells from the patch's edge, so that tile stages the
+    edge cells' deferred time-sync increments too (layout 2)."""
+    prob = P.dambreak_problem(2, 4, mx=mx)
+    print("case mx", mx, "layout", layout, "tile", tile, flush=True)
     forest, q0, aux, sv = setup_problem(prob, tile=tile, time_sync=True)
     assert forest.maxlevel > forest.minlevel
     sv.set_ghost_layout(layout)
@@ -107,6 +110,27 @@ def test_uniform_forest_mx_ne_my(gpu, precision):
     err = rel_max_err(interior(qg), interior(qo))
     assert err <= (TOL64 if precision == fa.FP64 else TOL32), "rel err %.3e" % err
     assert abs(cfl - cfl_o) <= (TOL64 if precision == fa.FP64 else 1e-5) * cfl_o
+
+
+@pytest.mark.parametrize("layout", [0, 2])
+@pytest.mark.parametrize("tile", [8, 16])
+def test_uniform_forest_odd_mx(gpu, tile, layout):
+    """A uniform forest of 17 x 17 patches: the tile before the overlapped
+    last one reads the patch's ghost column and row, which the fused layout
+    (2) stages straight from the same-size neighbours."""
+    prob = P.dambreak_problem(3, 3, mx=17)
+    print("case 17 x 17 tile", tile, "layout", layout, flush=True)
+    forest, q0, aux, sv = setup_problem(prob, tile=tile)
+    sv.set_ghost_layout(layout)
+    assert sv.tile == tile
+    dt = P.stable_dt(prob, forest, q0)
+    nsteps = 6
+    cfl = sv.advance(dt, nsteps)
+    qg = sv.get_q()
+    qo, cfl_o = oracle_advance(prob, forest, q0, aux, prob.mx, dt, nsteps)
+    err = rel_max_err(interior(qg), interior(qo))
+    assert err <= TOL64, "rel err %.3e" % err
+    assert abs(cfl - cfl_o) <= TOL64 * cfl_o
 
 
 def test_coarse_fine_work_refused_when_mx_ne_my(gpu):
