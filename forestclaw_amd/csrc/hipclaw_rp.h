@@ -78,7 +78,8 @@ template <typename R>
 struct RPShallow {
     static constexpr int MEQN = 3, MWAVES = 3, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;  // amdq + apdq = f(qR) - f(qL)
-    static constexpr int NROE = 3;  // u, v, a
+    // u, v, a (and in FP32 1 / (2 a), which rpt2 then takes from the normal solve)
+    static constexpr int NROE = sizeof(R) == 4 ? 4 : 3;
 
     // the shear wave carries no depth and no normal momentum
     template <int IXY>
@@ -91,15 +92,47 @@ struct RPShallow {
     // direction).  The step kernel evaluates them once per cell and hands each
     // interface its two cells' values (the same operations, so the same bits).
     static constexpr int NCELL = 5;
+    //
+    // FP32 takes all five from one hardware reciprocal square root (v_rsq_f32,
+    // 1 ulp): sqrt(h) = h / sqrt(h), 1 / h = (1 / sqrt(h))^2, sqrt(g h) =
+    // sqrt(g) sqrt(h); FP64 keeps the reference's divisions and square roots.
     template <int IXY>
     __device__ __forceinline__ static void cell(const R* q, const PhysParams& pp, R c[NCELL]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
-        const R hs = sq(q[0]);
-        c[0] = hs;
-        c[1] = dv(q[mu], hs);
-        c[2] = dv(q[mv], hs);
-        c[3] = dv(q[mu], q[0]);
-        c[4] = sq((R) pp.grav * q[0]);
+        if constexpr (sizeof(R) == 4) {
+            const float rs = __builtin_amdgcn_rsqf(q[0]);
+            const float hs = q[0] * rs;
+            c[0] = hs;
+            c[1] = q[mu] * rs;
+            c[2] = q[mv] * rs;
+            c[3] = c[1] * rs;
+            c[4] = sqrt_grav(pp) * hs;
+        } else {
+            const R hs = sq(q[0]);
+            c[0] = hs;
+            c[1] = dv(q[mu], hs);
+            c[2] = dv(q[mv], hs);
+            c[3] = dv(q[mu], q[0]);
+            c[4] = sq((R) pp.grav * q[0]);
+        }
+    }
+
+    // sqrt(g) in FP32 (uniform; the compiler keeps it out of the sweeps' loops)
+    __device__ __forceinline__ static float sqrt_grav(const PhysParams& pp) {
+        return __builtin_amdgcn_sqrtf((float) pp.grav);
+    }
+
+    // u_n - sqrt(g h) (SIGN = -1) or u_n + sqrt(g h) (SIGN = +1) of the
+    // intermediate state h, hu_n of the entropy fix
+    template <int SIGN>
+    __device__ __forceinline__ static R char_speed(R hu, R h, const PhysParams& pp) {
+        if constexpr (sizeof(R) == 4) {
+            const float rs = __builtin_amdgcn_rsqf(h);
+            const float hs = h * rs;
+            return SIGN < 0 ? hu * rs * rs - sqrt_grav(pp) * hs : hu * rs * rs + sqrt_grav(pp) * hs;
+        } else {
+            return SIGN < 0 ? dv(hu, h) - sq((R) pp.grav * h) : dv(hu, h) + sq((R) pp.grav * h);
+        }
     }
 
     template <int IXY>
@@ -126,15 +159,28 @@ struct RPShallow {
         const R hsqrtl = cL[0];
         const R hsqrtr = cR[0];
         const R hsq2 = hsqrtl + hsqrtr;
-        const R u = dv(cL[1] + cR[1], hsq2);
-        const R v = dv(cL[2] + cR[2], hsq2);
-        const R a = sq(grav * h);
-        roe[0] = u; roe[1] = v; roe[2] = a;
-
+        R u, v, a, a1, a3;
         const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv];
-        const R a1 = dv((u + a) * d0 - d1, R(2) * a);
+        if constexpr (sizeof(R) == 4) {
+            // one reciprocal and one reciprocal square root: 1 / (2 a) = rsq(g h) / 2
+            const float rh = __builtin_amdgcn_rcpf(hsq2);
+            u = (cL[1] + cR[1]) * rh;
+            v = (cL[2] + cR[2]) * rh;
+            const float gh = grav * h, ra = __builtin_amdgcn_rsqf(gh);
+            a = gh * ra;
+            const float inv2a = 0.5f * ra;
+            roe[3] = inv2a;
+            a1 = ((u + a) * d0 - d1) * inv2a;
+            a3 = (-((u - a) * d0) + d1) * inv2a;
+        } else {
+            u = dv(cL[1] + cR[1], hsq2);
+            v = dv(cL[2] + cR[2], hsq2);
+            a = sq(grav * h);
+            a1 = dv((u + a) * d0 - d1, R(2) * a);
+            a3 = dv(-((u - a) * d0) + d1, R(2) * a);
+        }
+        roe[0] = u; roe[1] = v; roe[2] = a;
         const R a2 = -(v * d0) + d2;
-        const R a3 = dv(-((u - a) * d0) + d1, R(2) * a);
 
         wave[0][0] = a1;   wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
         wave[1][0] = R(0); wave[1][mu] = R(0);         wave[1][mv] = a2;     s[1] = u;
@@ -148,7 +194,7 @@ struct RPShallow {
         } else {
             const R h1 = qL[0] + wave[0][0];
             const R hu1 = qL[mu] + wave[0][mu];
-            const R s1 = dv(hu1, h1) - sq(grav * h1);
+            const R s1 = char_speed<-1>(hu1, h1, pp);
             R sfract;
             if (s0 < R(0) && s1 > R(0)) sfract = s0 * dv(s1 - s[0], s1 - s0);
             else if (s[0] < R(0)) sfract = s[0];
@@ -161,7 +207,7 @@ struct RPShallow {
                 const R s03 = cR[3] + cR[4];   // dv(qR[mu], qR[0]) + sq(grav * qR[0])
                 const R h3 = qR[0] - wave[2][0];
                 const R hu3 = qR[mu] - wave[2][mu];
-                const R s3 = dv(hu3, h3) + sq(grav * h3);
+                const R s3 = char_speed<1>(hu3, h3, pp);
                 bool add = true;
                 if (s3 < R(0) && s03 > R(0)) sfract = s3 * dv(s03 - s[2], s03 - s3);
                 else if (s[2] < R(0)) sfract = s[2];
@@ -189,9 +235,15 @@ struct RPShallow {
                                                 R bp[MEQN]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R u = roe[0], v = roe[1], a = roe[2];
-        const R a1 = dv((v + a) * asdq[0] - asdq[mv], R(2) * a);
+        R a1, a3;
+        if constexpr (sizeof(R) == 4) {
+            a1 = ((v + a) * asdq[0] - asdq[mv]) * roe[3];
+            a3 = (-((v - a) * asdq[0]) + asdq[mv]) * roe[3];
+        } else {
+            a1 = dv((v + a) * asdq[0] - asdq[mv], R(2) * a);
+            a3 = dv(-((v - a) * asdq[0]) + asdq[mv], R(2) * a);
+        }
         const R a2 = asdq[mu] - u * asdq[0];
-        const R a3 = dv(-((v - a) * asdq[0]) + asdq[mv], R(2) * a);
         R wb[3][3], sb[3];
         wb[0][0] = a1;   wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2;     wb[1][mv] = R(0);         sb[1] = v;
