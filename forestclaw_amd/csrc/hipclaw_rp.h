@@ -30,10 +30,25 @@ struct PhysParams {
     double p[8];
 };
 
+// min / max.  FP64 selects as the reference's MIN / MAX (a compare and a
+// select); FP32 uses the hardware v_min_f32 / v_max_f32, which differ only
+// for NaN operands.
 template <typename R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <>
+__device__ __forceinline__ float rmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <>
+__device__ __forceinline__ float rmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+
+// Sums that start at zero: 0 + x and 0 - x.  FP64 keeps the reference's
+// accumulations from zero, whose only effect is the sign of a zero result;
+// FP32 drops them.
+template <typename R>
+__device__ __forceinline__ R zadd(R x) { return sizeof(R) == 4 ? x : R(0) + x; }
+template <typename R>
+__device__ __forceinline__ R zsub(R x) { return sizeof(R) == 4 ? -x : R(0) - x; }
 
 // Division and square root.  FP64 keeps IEEE-correct operations (the parity
 // form); the FP32-bulk form uses the hardware reciprocal and square root
@@ -304,7 +319,9 @@ template <typename R>
 struct RPEuler {
     static constexpr int MEQN = 4, MWAVES = 4, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;
-    static constexpr int NROE = 7;  // u, v, a, enth, u2v2, g1a2, euv
+    // u, v, a, enth, u2v2, g1a2, euv (and in FP32 1 / (2 a), which rpt2 then
+    // takes from the normal solve)
+    static constexpr int NROE = sizeof(R) == 4 ? 8 : 7;
 
     // the shear wave carries no density and no normal momentum
     template <int IXY>
@@ -317,13 +334,26 @@ struct RPEuler {
     // (E + p) / sqrt(rho) with p from (rho u)^2 + (rho v)^2, u_n and the sound
     // speed of the entropy fix, whose p sums the momenta in the sweep's order.
     // The step kernel evaluates them once per cell (the same operations, so
-    // the same bits).
+    // the same bits).  FP32 takes sqrt(rho), 1 / sqrt(rho) and 1 / rho from one
+    // hardware reciprocal square root; FP64 keeps the reference's divisions.
     static constexpr int NCELL = 6;
     template <int IXY>
     __device__ __forceinline__ static void cell(const R* q, const PhysParams& pp, R c[NCELL]) {
         constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
         const R gamma = (R) pp.gamma;
         const R gamma1 = (R) (pp.gamma - 1.0);
+        if constexpr (sizeof(R) == 4) {
+            const float r = __builtin_amdgcn_rsqf(q[0]), rr = r * r;
+            // one pressure (the two summation orders differ only in rounding)
+            const float p = gamma1 * (q[3] - 0.5f * (q[1] * q[1] + q[2] * q[2]) * rr);
+            c[0] = q[0] * r;
+            c[1] = q[mu] * r;
+            c[2] = q[mv] * r;
+            c[3] = (q[3] + p) * r;
+            c[4] = c[1] * r;
+            c[5] = __builtin_amdgcn_sqrtf(gamma * p) * r;
+            return;
+        }
         const R rs = sq(q[0]);
         const R p = gamma1 * (q[3] - dv(R(0.5) * (q[1] * q[1] + q[2] * q[2]), q[0]));
         const R pn = gamma1 * (q[3] - dv(R(0.5) * (q[mu] * q[mu] + q[mv] * q[mv]), q[0]));
@@ -359,20 +389,40 @@ struct RPEuler {
         const R rhsqrtl = cL[0];
         const R rhsqrtr = cR[0];
         const R rhsq2 = rhsqrtl + rhsqrtr;
-        const R u = dv(cL[1] + cR[1], rhsq2);
-        const R v = dv(cL[2] + cR[2], rhsq2);
-        const R enth = dv(cL[3] + cR[3], rhsq2);
+        R u, v, enth, a, g1a2, inv2a = R(0);
+        if constexpr (sizeof(R) == 4) {
+            // one reciprocal and one reciprocal square root: a = a2s rsq(a2s),
+            // gamma1 / a2s = gamma1 rsq(a2s)^2, 1 / (2 a) = rsq(a2s) / 2
+            const float rh = __builtin_amdgcn_rcpf(rhsq2);
+            u = (cL[1] + cR[1]) * rh;
+            v = (cL[2] + cR[2]) * rh;
+            enth = (cL[3] + cR[3]) * rh;
+        } else {
+            u = dv(cL[1] + cR[1], rhsq2);
+            v = dv(cL[2] + cR[2], rhsq2);
+            enth = dv(cL[3] + cR[3], rhsq2);
+        }
         const R u2v2 = u * u + v * v;
         const R a2s = gamma1 * (enth - R(.5) * u2v2);
-        const R a = sq(a2s);
-        const R g1a2 = dv(gamma1, a2s);
+        if constexpr (sizeof(R) == 4) {
+            const float ra = __builtin_amdgcn_rsqf(a2s);
+            a = a2s * ra;
+            g1a2 = gamma1 * (ra * ra);
+            inv2a = 0.5f * ra;
+            roe[7] = inv2a;
+        } else {
+            a = sq(a2s);
+            g1a2 = dv(gamma1, a2s);
+        }
         const R euv = enth - u2v2;
         roe[0] = u; roe[1] = v; roe[2] = a; roe[3] = enth; roe[4] = u2v2; roe[5] = g1a2; roe[6] = euv;
 
         const R d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[3] - qL[3];
         const R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
         const R a2 = d2 - v * d0;
-        const R a4 = dv(d1 + (a - u) * d0 - a * a3, R(2) * a);
+        R a4;
+        if constexpr (sizeof(R) == 4) a4 = (d1 + (a - u) * d0 - a * a3) * inv2a;
+        else a4 = dv(d1 + (a - u) * d0 - a * a3, R(2) * a);
         const R a1 = d0 - a3 - a4;
 
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
@@ -447,7 +497,9 @@ struct RPEuler {
         const R g1a2 = roe[5], euv = roe[6];
         const R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         const R a2 = asdq[mu] - u * asdq[0];
-        const R a4 = dv(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
+        R a4;
+        if constexpr (sizeof(R) == 4) a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * roe[7];
+        else a4 = dv(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
         const R a1 = asdq[0] - a3 - a4;
         R wb[3][4], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);

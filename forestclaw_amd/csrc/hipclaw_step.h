@@ -12,10 +12,11 @@
 //  * while a tile is being computed, the next tile's cells are already in
 //    flight from HBM into registers (software pipelining across tiles), so
 //    the global-load latency is hidden behind the sweeps;
-//  * each lane owns ONE interface: a row of T+3 x-interfaces (or a column of
-//    T+3 y-interfaces) lives in T+3 consecutive lanes of a wave64, so the
-//    limiter's neighbour waves and the transverse fluctuations of the next
-//    interface are exchanged with cross-lane shuffles, not memory;
+//  * each lane owns ONE staged cell and the interface on its left: a row (or
+//    column) of the T+4 staged cells lives in T+4 consecutive lanes of a
+//    wave64, so the left cell's values, the limiter's neighbour waves and the
+//    transverse fluctuations of the next interface are exchanged with
+//    cross-lane DPP moves, not memory;
 //  * no global scratch: normal fluxes and transverse corrections are
 //    accumulated in LDS in exactly the order the reference adds them, so the
 //    FP64 instantiation (compiled with FMA contraction off) is bit-identical to
@@ -174,17 +175,6 @@ struct TileSmem {
 template <class RP, typename R, int T, bool FLUXFORM>
 constexpr size_t tile_smem_bytes() {
     return sizeof(TileSmem<T, RP::MEQN, RP::MAUX, R, FLUXFORM>);
-}
-
-// Interface-in-CFL-set test.  The reference takes the CFL over x-interfaces
-// i = 1..mx+1 of rows j = 2-mbc..my+mbc-1 (clawpack46_flux2.f:129-137,
-// clawpack46_step2.f:113) and symmetrically in y; each tile owns a disjoint
-// share of that set.
-__device__ __forceinline__ bool cfl_owned(int k, int r, int T, bool last_n, bool first_t,
-                                          bool last_t) {
-    const bool kn = (k >= 1 && k <= T) || (k == T + 1 && last_n);
-    const bool rt = (r >= 1 && r <= T) || (r == 0 && first_t) || (r == T + 1 && last_t);
-    return kn && rt;
 }
 
 // Per-cell values a Riemann functor can take from the step kernel
@@ -534,7 +524,12 @@ step_tiles(StepArgs a) {
             aux_cell(auxR, kc, rc + 1);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
             riemann<RP, 1, NC>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
-            if (act && cfl_owned(k, r, T, last_x, first_y, last_y)) {
+            // The reference takes the CFL over x-interfaces i = 1..mx+1 of rows
+            // j = 2-mbc..my+mbc-1 (clawpack46_flux2.f:129-137, clawpack46_step2.f:113).
+            // A tile's interfaces k = 1..T+1 of its rows r = 0..T+1 lie in that set
+            // and the tiles cover it; an interface two tiles share has the same
+            // speeds in both, so the maximum is the reference's.
+            if (act && k >= 1 && k <= T + 1) {
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
             }
@@ -557,13 +552,14 @@ step_tiles(StepArgs a) {
                         dotl = dotl + wl * wave_[mw][m];
                     }
                     const R dotr = next_lane(dotl);
-                    if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
-                        const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
-                        const R phi = philim<R>(mth, rr);
+                    // phi = 1 where the reference leaves the wave as it is (zero
+                    // norm, or an interface whose flux no cell of the tile reads):
+                    // phi * w = w bit for bit
+                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
+                    const R phi = (k >= 1 && k <= T + 1 && wave_nonzero(wn)) ? philim<R>(mth, rr) : R(1);
 #pragma unroll
-                        for (int m = 0; m < MEQN; m++)
-                            if (!WZ1::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
-                    }
+                    for (int m = 0; m < MEQN; m++)
+                        if (!WZ1::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
                 }
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
@@ -577,9 +573,9 @@ step_tiles(StepArgs a) {
             if (act && r >= 1 && r <= T && k >= 1 && k <= T + 1) {
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    s_fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) + amdq[m]) + R(0.5) * cqxx[m];
+                    s_fm[(m * T + (r - 1)) * (T + 1) + (k - 1)] = zadd(amdq[m]) + R(0.5) * cqxx[m];
                     if constexpr (!FLUXFORM)
-                        s_fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = (R(0) - apdq[m]) + R(0.5) * cqxx[m];
+                        s_fp[(m * T + (r - 1)) * (T + 1) + (k - 1)] = zsub(apdq[m]) + R(0.5) * cqxx[m];
                 }
             }
             R tx1[MEQN], tx2[MEQN];
@@ -602,8 +598,8 @@ step_tiles(StepArgs a) {
                 for (int m = 0; m < MEQN; m++) {
                     const R gAd = half * bmA[m], gAu = half * bpA[m];
                     const R gPd = half * bmP[m], gPu = half * bpP[m];
-                    tx1[m] = (R(0) - next_lane(gAd)) - gPd;
-                    tx2[m] = (R(0) - next_lane(gAu)) - gPu;
+                    tx1[m] = zsub(next_lane(gAd)) - gPd;
+                    tx2[m] = zsub(next_lane(gAu)) - gPu;
                 }
             } else {
 #pragma unroll
@@ -623,12 +619,12 @@ step_tiles(StepArgs a) {
             const int c = threadIdx.x % T, t0 = threadIdx.x / T;
 #pragma unroll
             for (int m = 0; m < MEQN; m++)
-                for (int t = t0; t <= T; t += TS) GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+                for (int t = t0; t <= T; t += TS) GT(m, t, c) = zadd(AB(1, m, t, c)) + AB(0, m, t + 1, c);
         } else {
             for (int e = threadIdx.x; e < MEQN * (T + 1) * T; e += NT) {
                 const int m = e / ((T + 1) * T), rem = e - m * (T + 1) * T;
                 const int t = rem / T, c = rem - t * T;
-                GT(m, t, c) = (R(0) + AB(1, m, t, c)) + AB(0, m, t + 1, c);
+                GT(m, t, c) = zadd(AB(1, m, t, c)) + AB(0, m, t + 1, c);
             }
         }
         __syncthreads();
@@ -652,7 +648,7 @@ step_tiles(StepArgs a) {
             aux_cell(auxR, cc + 1, kc);
             R wave_[MW][MEQN], s[MW], amdq[MEQN], apdq[MEQN], roe[NROE];
             riemann<RP, 2, NC>(qL, qR, auxL, auxR, a.pp, wave_, s, amdq, apdq, roe);
-            if (act && cfl_owned(k, c, T, last_y, first_x, last_x)) {
+            if (act && k >= 1 && k <= T + 1) {   // as in the x sweep
 #pragma unroll
                 for (int mw = 0; mw < MW; mw++) smax = fmax(smax, fabs(s[mw]));
             }
@@ -675,13 +671,14 @@ step_tiles(StepArgs a) {
                         dotl = dotl + wl * wave_[mw][m];
                     }
                     const R dotr = next_lane(dotl);
-                    if (k >= 1 && k <= T + 1 && wave_nonzero(wn)) {
-                        const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
-                        const R phi = philim<R>(mth, rr);
+                    // phi = 1 where the reference leaves the wave as it is (zero
+                    // norm, or an interface whose flux no cell of the tile reads):
+                    // phi * w = w bit for bit
+                    const R rr = dv(s[mw] > R(0) ? dotl : dotr, wn);
+                    const R phi = (k >= 1 && k <= T + 1 && wave_nonzero(wn)) ? philim<R>(mth, rr) : R(1);
 #pragma unroll
-                        for (int m = 0; m < MEQN; m++)
-                            if (!WZ2::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
-                    }
+                    for (int m = 0; m < MEQN; m++)
+                        if (!WZ2::at(mw, m)) wave_[mw][m] = phi * wave_[mw][m];
                 }
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
@@ -699,7 +696,7 @@ step_tiles(StepArgs a) {
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) {
                         R& g = GT(m, k - 1, c - 1);
-                        g = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
+                        g = g + (zadd(amdq[m]) + R(0.5) * cqxx[m]);
                     }
                 }
             } else {
@@ -709,8 +706,8 @@ step_tiles(StepArgs a) {
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
                     const R g = GT(m, kk, ccol);
-                    gm[m] = g + ((R(0) + amdq[m]) + R(0.5) * cqxx[m]);
-                    gp[m] = g + ((R(0) - apdq[m]) + R(0.5) * cqxx[m]);
+                    gm[m] = g + (zadd(amdq[m]) + R(0.5) * cqxx[m]);
+                    gp[m] = g + (zsub(apdq[m]) + R(0.5) * cqxx[m]);
                 }
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
@@ -751,8 +748,8 @@ step_tiles(StepArgs a) {
                 for (int m = 0; m < MEQN; m++) {
                     const R gAd = half * bmA[m], gAu = half * bpA[m];
                     const R gPd = half * bmP[m], gPu = half * bpP[m];
-                    ty1[m] = (R(0) - next_lane(gAd)) - gPd;
-                    ty2[m] = (R(0) - next_lane(gAu)) - gPu;
+                    ty1[m] = zsub(next_lane(gAd)) - gPd;
+                    ty2[m] = zsub(next_lane(gAu)) - gPu;
                 }
             } else {
 #pragma unroll
@@ -772,8 +769,8 @@ step_tiles(StepArgs a) {
             for (int e = threadIdx.x; e < MEQN * T * T; e += NT) {
                 const int m = e / (T * T), rem = e - m * T * T;
                 const int b = rem / T, c = rem - b * T;   // cell i = i0+c, j = j0+b
-                const R fmR = ((R(0) + s_fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
-                const R fpL = ((R(0) + s_fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
+                const R fmR = (zadd(s_fm[(m * T + b) * (T + 1) + c + 1]) + AB(1, m, c + 1, b)) + AB(0, m, c + 2, b);
+                const R fpL = (zadd(s_fp[(m * T + b) * (T + 1) + c]) + AB(1, m, c, b)) + AB(0, m, c + 1, b);
                 const R Dx = fmR - fpL;
                 const R Dy = GT(m, b, c);
                 const double qc = SQD(m, c + 2, b + 2);
@@ -794,8 +791,8 @@ step_tiles(StepArgs a) {
                     const int m = rem / T, b = rem - m * T;
                     if ((side == 0 ? !first_x : !last_x) || b < own_y) continue;
                     const int s0 = side ? T : 0;
-                    const R fmv = ((R(0) + s_fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
-                    const R fpv = ((R(0) + s_fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    const R fmv = (zadd(s_fm[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
+                    const R fpv = (zadd(s_fp[(m * T + b) * (T + 1) + s0]) + AB(1, m, s0, b)) + AB(0, m, s0 + 1, b);
                     reg_add(rg + ((side * 4 + REG_FM) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fmv, a.reg_accumulate);
                     reg_sub(rg + ((side * 4 + REG_FP) * MEQN + m) * mx + j0 - 1 + b, dtdy_el * (double) fpv, a.reg_accumulate);
                 }
@@ -820,7 +817,7 @@ step_tiles(StepArgs a) {
             // LDS values and the FP64 update telescopes exactly; no flux array is
             // kept in LDS.
             auto xflux = [&](int sx, int b, int m, double f) -> double {  // x-interface sx at row b
-                const R fmv = ((R(0) + s_fm[(m * T + b) * (T + 1) + sx]) + AB(1, m, sx, b)) + AB(0, m, sx + 1, b);
+                const R fmv = (zadd(s_fm[(m * T + b) * (T + 1) + sx]) + AB(1, m, sx, b)) + AB(0, m, sx + 1, b);
                 return f + (double) fmv;
             };
             auto yflux = [&](int t, int c, int m, double g) -> double {   // y-interface t at column c
