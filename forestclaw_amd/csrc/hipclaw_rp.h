@@ -17,6 +17,8 @@
 // compiler can keep waves, speeds and fluctuations in VGPRs.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 namespace hipclaw {
@@ -75,6 +77,14 @@ __device__ __forceinline__ R sq(R a) { return sqrt(a); }
 template <>
 __device__ __forceinline__ float sq<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
+// A solver whose rpt2 is linear in asdq (RP::RPT2_LINEAR): the FP32 form then
+// scales asdq by dt / (2 dx) before the transverse solve instead of scaling
+// both results after it.  Solvers without the flag keep the reference's order.
+template <class RP, class = void>
+struct rpt2_linear : std::false_type {};
+template <class RP>
+struct rpt2_linear<RP, std::enable_if_t<RP::RPT2_LINEAR>> : std::true_type {};
+
 // ----------------------------------------------------------------- shallow water
 // Wave components that are zero by construction.  A sum over the waves that
 // starts at +0 may skip those terms without changing its bits (in
@@ -93,6 +103,7 @@ template <typename R>
 struct RPShallow {
     static constexpr int MEQN = 3, MWAVES = 3, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;  // amdq + apdq = f(qR) - f(qL)
+    static constexpr bool RPT2_LINEAR = true;   // bm, bp linear in asdq
     // u, v, a (and in FP32 1 / (2 a), which rpt2 then takes from the normal solve)
     static constexpr int NROE = sizeof(R) == 4 ? 4 : 3;
 
@@ -319,6 +330,7 @@ template <typename R>
 struct RPEuler {
     static constexpr int MEQN = 4, MWAVES = 4, MAUX = 0;
     static constexpr bool CONSERVATIVE = true;
+    static constexpr bool RPT2_LINEAR = true;   // bm, bp linear in asdq
     // u, v, a, enth, u2v2, g1a2, euv (and in FP32 1 / (2 a), which rpt2 then
     // takes from the normal solve)
     static constexpr int NROE = sizeof(R) == 4 ? 8 : 7;
@@ -570,6 +582,7 @@ struct RPAdvection {
     // colour equation q_t + u q_x + v q_y = 0: the fluctuations are not a flux
     // difference unless the velocity is uniform, so no flux form exists
     static constexpr bool CONSERVATIVE = false;
+    static constexpr bool RPT2_LINEAR = true;   // bm, bp linear in asdq
     static constexpr int NROE = 1;
 
     template <int IXY>

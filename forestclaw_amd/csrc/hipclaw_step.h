@@ -281,7 +281,12 @@ step_tiles(StepArgs a) {
     auto origin_y = [&](int ty) { return min(ty * T, my - T); };
     // Staging map, the same for every tile: element e = threadIdx.x + kk*NT of
     // the (field m, tile row qj, tile column qi) halo tile sits at goff[kk] +
-    // (tile origin) in the patch slab and at loff[kk] in LDS (-1: no element).
+    // (tile origin) in the patch slab and at loff[kk] in LDS.  Every chunk kk
+    // but a partial last one holds an element in every thread (the test is
+    // then a compile-time constant, and no per-chunk lane mask lives through
+    // the tile loop).
+    auto has_q = [&](int kk) { return (kk + 1) * NT <= NQ || (int) threadIdx.x + kk * NT < NQ; };
+    auto has_a = [&](int kk) { return MAUX > 0 && ((kk + 1) * NT <= NA || (int) threadIdx.x + kk * NT < NA); };
     int goff[KQ], loff[KQ];
 #pragma unroll
     for (int kk = 0; kk < KQ; kk++) {
@@ -289,8 +294,8 @@ step_tiles(StepArgs a) {
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
         // cell (i, j) = (i0 - 2 + qi, j0 - 2 + qj) of field m
-        goff[kk] = e < NQ ? (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2 : -1;
-        loff[kk] = e < NQ ? m * QP + qj * QS + qi : -1;
+        goff[kk] = (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2;
+        loff[kk] = m * QP + qj * QS + qi;
     }
     // Deferred fix-up: entries (face, field m, position 0..QW-1 along the strip)
     // of the tile's four interior edge strips.
@@ -301,8 +306,8 @@ step_tiles(StepArgs a) {
         const int e = threadIdx.x + kk * NT;
         const int m = e / (QW * QW), rem = e - m * QW * QW;
         const int qj = rem / QW, qi = rem - qj * QW;
-        gaoff[kk] = (MAUX > 0 && e < NA) ? (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2 : -1;
-        laoff[kk] = (MAUX > 0 && e < NA) ? m * QP + qj * QS + qi : -1;
+        gaoff[kk] = (m * nyp + qj + mbc - 2) * nxp + qi + mbc - 2;
+        laoff[kk] = m * QP + qj * QS + qi;
     }
 
     // Halo column and row of each staged element: code = cx + 5 cy, cx = 1, 2
@@ -339,15 +344,15 @@ step_tiles(StepArgs a) {
         if (a.pull) {
 #pragma unroll
             for (int kk = 0; kk < KQ; kk++)
-                pq[kk] = goff[kk] >= 0 ? qp[goff[kk] + s_pull[(pcode[kk / 12] >> (5 * (kk % 12))) & 31]] : 0.0;
+                pq[kk] = has_q(kk) ? qp[goff[kk] + s_pull[(pcode[kk / 12] >> (5 * (kk % 12))) & 31]] : 0.0;
         } else {
 #pragma unroll
-            for (int kk = 0; kk < KQ; kk++) pq[kk] = goff[kk] >= 0 ? qp[goff[kk]] : 0.0;
+            for (int kk = 0; kk < KQ; kk++) pq[kk] = has_q(kk) ? qp[goff[kk]] : 0.0;
         }
         if constexpr (MAUX > 0) {
             const double* __restrict__ ap = a.aux + (long long) slot * a.astride + shift;
 #pragma unroll
-            for (int kk = 0; kk < KA; kk++) pa[kk] = gaoff[kk] >= 0 ? (R) ap[gaoff[kk]] : R(0);
+            for (int kk = 0; kk < KA; kk++) pa[kk] = has_a(kk) ? (R) ap[gaoff[kk]] : R(0);
         }
     };
 
@@ -422,11 +427,11 @@ step_tiles(StepArgs a) {
         // ---- stage 0: the prefetched tile + 2-cell halo into LDS ----
 #pragma unroll
         for (int kk = 0; kk < KQ; kk++)
-            if (loff[kk] >= 0) sm.q[loff[kk]] = pq[kk];
+            if (has_q(kk)) sm.q[loff[kk]] = pq[kk];
         if constexpr (MAUX > 0) {
 #pragma unroll
             for (int kk = 0; kk < KA; kk++)
-                if (laoff[kk] >= 0) sm.aux[laoff[kk]] = pa[kk];
+                if (has_a(kk)) sm.aux[laoff[kk]] = pa[kk];
         }
         __syncthreads();
         if (a.qcorr) {
@@ -584,6 +589,13 @@ step_tiles(StepArgs a) {
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
                 }
+                // FP32 with a linear rpt2: the factor dt / (2 dx) on asdq
+                constexpr bool PRE = sizeof(R) == 4 && rpt2_linear<RP>::value;
+                const R half = R(0.5) * dtdx;
+                if constexpr (PRE) {
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) { amdq[m] = half * amdq[m]; apdq[m] = half * apdq[m]; }
+                }
                 R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
                 // aux1, aux2, aux3: rows j-1, j, j+1 of column i1 = i-1 (amdq) or i (apdq)
                 {
@@ -593,11 +605,10 @@ step_tiles(StepArgs a) {
                     aux_cell(a1, kc, rc); aux_cell(a2, kc, rc + 1); aux_cell(a3, kc, rc + 2);
                     RP::template rpt2<1>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
                 }
-                const R half = R(0.5) * dtdx;
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    const R gAd = half * bmA[m], gAu = half * bpA[m];
-                    const R gPd = half * bmP[m], gPu = half * bpP[m];
+                    const R gAd = PRE ? bmA[m] : half * bmA[m], gAu = PRE ? bpA[m] : half * bpA[m];
+                    const R gPd = PRE ? bmP[m] : half * bmP[m], gPu = PRE ? bpP[m] : half * bpP[m];
                     tx1[m] = zsub(next_lane(gAd)) - gPd;
                     tx2[m] = zsub(next_lane(gAu)) - gPu;
                 }
@@ -734,6 +745,13 @@ step_tiles(StepArgs a) {
 #pragma unroll
                     for (int m = 0; m < MEQN; m++) { amdq[m] = amdq[m] + cqxx[m]; apdq[m] = apdq[m] - cqxx[m]; }
                 }
+                // FP32 with a linear rpt2: the factor dt / (2 dx) on asdq
+                constexpr bool PRE = sizeof(R) == 4 && rpt2_linear<RP>::value;
+                const R half = R(0.5) * dtdy;
+                if constexpr (PRE) {
+#pragma unroll
+                    for (int m = 0; m < MEQN; m++) { amdq[m] = half * amdq[m]; apdq[m] = half * apdq[m]; }
+                }
                 R bmA[MEQN], bpA[MEQN], bmP[MEQN], bpP[MEQN];
                 // aux1, aux2, aux3: columns i-1, i, i+1 of row j1 = j-1 (amdq) or j (apdq)
                 {
@@ -743,11 +761,10 @@ step_tiles(StepArgs a) {
                     aux_cell(a1, cc, kc); aux_cell(a2, cc + 1, kc); aux_cell(a3, cc + 2, kc);
                     RP::template rpt2<2>(roe, apdq, a1, a2, a3, a.pp, bmP, bpP);
                 }
-                const R half = R(0.5) * dtdy;
 #pragma unroll
                 for (int m = 0; m < MEQN; m++) {
-                    const R gAd = half * bmA[m], gAu = half * bpA[m];
-                    const R gPd = half * bmP[m], gPu = half * bpP[m];
+                    const R gAd = PRE ? bmA[m] : half * bmA[m], gAu = PRE ? bpA[m] : half * bpA[m];
+                    const R gPd = PRE ? bmP[m] : half * bmP[m], gPu = PRE ? bpP[m] : half * bpP[m];
                     ty1[m] = zsub(next_lane(gAd)) - gPd;
                     ty2[m] = zsub(next_lane(gAu)) - gPu;
                 }
