@@ -40,9 +40,12 @@ struct fclaw2d_hip_forest {
     int *level, *gx, *gy;
     /* provenance of the leaves after the last adapt: kind and 4 sources each */
     int *pkind, *psrc, pn;
-    /* open-addressing map (level, gx, gy) -> leaf */
+    /* open-addressing map (level, gx, gy) -> leaf; hok: it holds the current
+     * leaves (kept up to date through splits, coarsenings and sorts, rebuilt
+     * only when it would fill beyond half) */
     size_t hcap;
     hentry_t *hash;
+    int hok;
     /* face (4) and corner (4) relations of every leaf for one (mx, my), built
      * on first use and dropped whenever the leaves change */
     nbinfo_t *rel;
@@ -69,6 +72,7 @@ static int rehash(fclaw2d_hip_forest_t *f)
     size_t cap = 64, p;
     int i;
     f->rel_valid = 0;
+    f->hok = 0;
     while (cap < (size_t) f->n * 2) cap <<= 1;   /* load factor <= 1/2 */
     if (cap != f->hcap) {
         free(f->hash);
@@ -87,7 +91,49 @@ static int rehash(fclaw2d_hip_forest_t *f)
         f->hash[p].key = k;
         f->hash[p].val = i;
     }
+    f->hok = 1;
     return 0;
+}
+
+/* Add leaf v with key k to the map (-1: k is there already). */
+static int hash_put(fclaw2d_hip_forest_t *f, uint64_t k, int v)
+{
+    size_t p = fslot(k, f->hcap);
+    while (f->hash[p].val >= 0) {
+        if (f->hash[p].key == k) return -1;
+        p = (p + 1) & (f->hcap - 1);
+    }
+    f->hash[p].key = k;
+    f->hash[p].val = v;
+    return 0;
+}
+
+/* Remove key k from the map (-1: not there).  Linear probing without
+ * tombstones: the entries after the hole in its run move back into it when
+ * their home slot does not lie cyclically after the hole. */
+static int hash_del(fclaw2d_hip_forest_t *f, uint64_t k)
+{
+    const size_t mask = f->hcap - 1;
+    size_t p = fslot(k, f->hcap), q;
+    while (f->hash[p].val >= 0 && f->hash[p].key != k) p = (p + 1) & mask;
+    if (f->hash[p].val < 0) return -1;
+    for (q = (p + 1) & mask; f->hash[q].val >= 0; q = (q + 1) & mask) {
+        const size_t h = fslot(f->hash[q].key, f->hcap);
+        if (q > p ? (h <= p || h > q) : (h <= p && h > q)) {
+            f->hash[p] = f->hash[q];
+            p = q;
+        }
+    }
+    f->hash[p].val = -1;
+    return 0;
+}
+
+/* Renumber the map's leaves: leaf i becomes perm[i] (perm[i] >= 0). */
+static void hash_renumber(fclaw2d_hip_forest_t *f, const int *perm)
+{
+    size_t p;
+    for (p = 0; p < f->hcap; p++)
+        if (f->hash[p].val >= 0) f->hash[p].val = perm[f->hash[p].val];
 }
 
 static int lookup(const fclaw2d_hip_forest_t *f, int level, int64_t gx, int64_t gy)
@@ -215,8 +261,13 @@ static int sort_leaves(fclaw2d_hip_forest_t *f)
         tmp[3 * i] = f->level[a[i]]; tmp[3 * i + 1] = f->gx[a[i]]; tmp[3 * i + 2] = f->gy[a[i]];
     }
     for (i = 0; i < n; i++) { f->level[i] = tmp[3 * i]; f->gx[i] = tmp[3 * i + 1]; f->gy[i] = tmp[3 * i + 2]; }
+    f->rel_valid = 0;
+    if (f->hok) {   /* the map holds these leaves: renumber it (b: old -> new) */
+        for (i = 0; i < n; i++) b[a[i]] = i;
+        hash_renumber(f, b);
+    }
     free(mort); free(lvbk); free(a); free(b); free(tmp);
-    return rehash(f);
+    return f->hok ? 0 : rehash(f);
 }
 
 static int reserve(fclaw2d_hip_forest_t *f, int n)
@@ -239,20 +290,29 @@ static int reserve(fclaw2d_hip_forest_t *f, int n)
     return 0;
 }
 
-/* Replace every marked leaf by its four children. */
+/* Replace every marked leaf by its four children (the first child keeps the
+ * leaf's index, the others are appended). */
 static int split_marked(fclaw2d_hip_forest_t *f, const char *mark)
 {
-    int i, add = 0, n0 = f->n;
+    int i, add = 0, n0 = f->n, keep;
     for (i = 0; i < n0; i++) add += mark[i] ? 3 : 0;
     if (reserve(f, n0 + add)) return -1;
+    keep = f->hok && (size_t) (n0 + add) * 2 <= f->hcap;   /* the map stays at most half full */
+    f->hok = 0;
+    f->rel_valid = 0;
     for (i = 0; i < n0; i++) {
         if (!mark[i]) continue;
         const int L = f->level[i] + 1, x = 2 * f->gx[i], y = 2 * f->gy[i];
+        if (keep && hash_del(f, fkey(L - 1, f->gx[i], f->gy[i]))) return -1;
         f->level[i] = L; f->gx[i] = x; f->gy[i] = y;
         f->level[f->n] = L; f->gx[f->n] = x + 1; f->gy[f->n] = y;     f->n++;
         f->level[f->n] = L; f->gx[f->n] = x;     f->gy[f->n] = y + 1; f->n++;
         f->level[f->n] = L; f->gx[f->n] = x + 1; f->gy[f->n] = y + 1; f->n++;
+        if (keep && (hash_put(f, fkey(L, x, y), i) || hash_put(f, fkey(L, x + 1, y), f->n - 3) ||
+                     hash_put(f, fkey(L, x, y + 1), f->n - 2) || hash_put(f, fkey(L, x + 1, y + 1), f->n - 1)))
+            return -1;
     }
+    f->hok = keep;
     return 0;
 }
 
@@ -272,7 +332,7 @@ static int balance_from(fclaw2d_hip_forest_t *f, const int *work0, int nwork0)
     for (;;) {
         char *mark, *trig;
         int *next, nnext = 0, w, d, any = 0, n0;
-        if (rehash(f)) { rc = -1; break; }
+        if (!f->hok && rehash(f)) { rc = -1; break; }
         mark = (char *) calloc((size_t) f->n + 1, 1);
         trig = (char *) calloc((size_t) f->n + 1, 1);
         if (!mark || !trig) { free(mark); free(trig); rc = -1; break; }
@@ -549,14 +609,22 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
         f->rel_cap = 0;
         f->rel_valid = 0;
     }
-    /* keep the old leaves for the provenance */
+    /* keep the old leaves for the provenance (with a copy of their map) */
     old = alloc_forest(f->nbx, f->nby, f->per_x, f->per_y);
     if (!old || reserve(old, n0)) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     memcpy(old->level, f->level, (size_t) n0 * sizeof(int));
     memcpy(old->gx, f->gx, (size_t) n0 * sizeof(int));
     memcpy(old->gy, f->gy, (size_t) n0 * sizeof(int));
     old->n = n0;
-    if (rehash(old)) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
+    if (f->hok) {
+        old->hash = (hentry_t *) malloc(f->hcap * sizeof(hentry_t));
+        if (!old->hash) { fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
+        memcpy(old->hash, f->hash, f->hcap * sizeof(hentry_t));
+        old->hcap = f->hcap;
+        old->hok = 1;
+    } else if (rehash(old) || rehash(f)) {
+        fclaw2d_hip_forest_destroy(old); free(orel); return -1;
+    }
     /* families to coarsen: all four siblings flagged, none flagged to refine
      * (cb_regrid_tag4coarsening, src/fclaw2d_regrid.c:61-93) */
     fam = (int *) malloc(sizeof(int) * 3 * (size_t) (n0 / 4 + 1));
@@ -578,10 +646,11 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
     free(mark);
     /* coarsen the families that are still intact and keep the balance */
     if (ncoarse > 0) {
-        int k, w;
+        int k, w, keep = f->hok;
         char *drop = (char *) calloc((size_t) f->n + 1, 1);
+        int *cmap = (int *) malloc(sizeof(int) * ((size_t) f->n + 1));
         int nadd = 0;
-        if (!drop) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
+        if (!drop || !cmap) { free(drop); free(cmap); free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
         for (k = 0; k < ncoarse; k++) {
             const int L = fam[3 * k], X = fam[3 * k + 1], Y = fam[3 * k + 2];
             int kids[4], ok = 1;
@@ -593,16 +662,26 @@ int fclaw2d_hip_forest_adapt(fclaw2d_hip_forest_t *f, const int *refine, const i
             for (g = 0; g < 4; g++) drop[kids[g]] = 1;
             nadd++;
         }
+        /* the map: the children leave, the kept leaves are renumbered by the
+         * compaction, the parents join */
+        f->hok = 0;
+        for (i = 0; i < f->n && keep; i++)
+            if (drop[i]) keep = hash_del(f, fkey(f->level[i], f->gx[i], f->gy[i])) == 0;
         for (i = 0, w = 0; i < f->n; i++)
-            if (!drop[i]) { f->level[w] = f->level[i]; f->gx[w] = f->gx[i]; f->gy[w] = f->gy[i]; w++; }
+            if (!drop[i]) { cmap[i] = w; f->level[w] = f->level[i]; f->gx[w] = f->gx[i]; f->gy[w] = f->gy[i]; w++; }
+        if (keep) hash_renumber(f, cmap);
         f->n = w;
         free(drop);
+        free(cmap);
         if (reserve(f, f->n + nadd)) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
         for (k = 0; k < ncoarse; k++)
             if (fam[3 * k] >= 0) {
                 f->level[f->n] = fam[3 * k] - 1; f->gx[f->n] = fam[3 * k + 1]; f->gy[f->n] = fam[3 * k + 2];
+                if (keep) keep = hash_put(f, fkey(f->level[f->n], f->gx[f->n], f->gy[f->n]), f->n) == 0;
                 f->n++;
             }
+        f->hok = keep;
+        f->rel_valid = 0;
         if (sort_leaves(f)) { free(fam); fclaw2d_hip_forest_destroy(old); free(orel); return -1; }
     }
     free(fam);
@@ -788,13 +867,35 @@ static int corner_interior(const int phys[4], int ic)
     return !phys[fa] && !phys[fb];
 }
 
-static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, const int mthbc[4],
-                        int tflag)
+/* The leaves of each level in forest order: leaves byl[start[L] ..
+ * start[L + 1]) have level L (levels 0..MAXL). */
+enum { MAXL = 30 };
+typedef struct { int start[MAXL + 2]; int *byl; } levels_t;
+
+static int level_buckets(const fclaw2d_hip_forest_t *f, levels_t *lv)
 {
-    int p, k;
-    for (p = 0; p < f->n; p++) {
+    int p, L;
+    lv->byl = (int *) malloc(sizeof(int) * ((size_t) f->n + 1));
+    if (!lv->byl) return -1;
+    memset(lv->start, 0, sizeof(lv->start));
+    for (p = 0; p < f->n; p++) lv->start[f->level[p] + 1]++;
+    for (L = 0; L <= MAXL; L++) lv->start[L + 1] += lv->start[L];
+    {
+        int fill[MAXL + 1];
+        memcpy(fill, lv->start, sizeof(fill));
+        for (p = 0; p < f->n; p++) lv->byl[fill[f->level[p]]++] = p;
+    }
+    return 0;
+}
+
+static void emit_physbc(const fclaw2d_hip_forest_t *f, const levels_t *lv, opbuf_t *b, int level,
+                        const int mthbc[4], int tflag)
+{
+    int w, k;
+    if (level < 0 || level > MAXL) return;
+    for (w = lv->start[level]; w < lv->start[level + 1]; w++) {
+        const int p = lv->byl[w];
         int phys[4], bc = 0, any = 0;
-        if (f->level[p] != level) continue;
         physical_faces(f, p, phys);
         for (k = 0; k < 4; k++) {
             const int m = phys[k] ? mthbc[k] : -1;
@@ -809,14 +910,15 @@ static void emit_physbc(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, co
  * neighbours, as cb_face_fill / cb_corner_fill do for one exchange type.
  * `tflag` marks this side's slots as time-interpolated copies; for
  * interpolation the op writes the fine neighbour and reads this patch. */
-static void emit_exchanges(const fclaw2d_hip_forest_t *f, opbuf_t *b, int level, int mx, int my,
-                           int kind, int tflag)
+static void emit_exchanges(const fclaw2d_hip_forest_t *f, const levels_t *lv, opbuf_t *b, int level,
+                           int mx, int my, int kind, int tflag)
 {
-    int p, k;
-    for (p = 0; p < f->n; p++) {
+    int w, k;
+    if (level < 0 || level > MAXL) return;
+    for (w = lv->start[level]; w < lv->start[level + 1]; w++) {
+        const int p = lv->byl[w];
         int phys[4];
         const nbinfo_t *rp;
-        if (f->level[p] != level) continue;
         rp = relations(f, p, mx, my);
         physical_faces(f, p, phys);
         for (k = 0; k < 4; k++) {
@@ -853,36 +955,39 @@ int fclaw2d_hip_forest_ghost_program_range(const fclaw2d_hip_forest_t *f, int mx
                                            int phase_start[7])
 {
     opbuf_t b;
+    levels_t lv;
     const int ti = minlevel - 1, TI = FCLAW2D_HIP_TIME_INTERP_SLOT;
     int L;
     (void) mbc;
     if (!f || f->n == 0 || !phase_start || !mthbc || minlevel > maxlevel) return -1;
     if (time_interp && ti < fclaw2d_hip_forest_min_level(f)) return -1;
     if (!relations(f, 0, mx, my)) return -1;   /* builds the relation cache used below */
+    if (level_buckets(f, &lv)) return -1;
     b.ops = ops; b.cap = ops ? cap : 0; b.n = 0;
 
     /* phase 0: average fine -> coarse ghost cells (coarse levels, finest first) */
     phase_start[0] = b.n;
-    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_AVG_FACE, 0);
-    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_AVG_FACE, TI);
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &lv, &b, L, mx, my, OP_AVG_FACE, 0);
+    if (time_interp) emit_exchanges(f, &lv, &b, ti, mx, my, OP_AVG_FACE, TI);
     /* phase 1: copy between same-size neighbours */
     phase_start[1] = b.n;
-    for (L = maxlevel; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_COPY_FACE, 0);
-    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_COPY_FACE, TI);
+    for (L = maxlevel; L >= minlevel; L--) emit_exchanges(f, &lv, &b, L, mx, my, OP_COPY_FACE, 0);
+    if (time_interp) emit_exchanges(f, &lv, &b, ti, mx, my, OP_COPY_FACE, TI);
     /* phase 2: physical BC on coarse levels */
     phase_start[2] = b.n;
-    for (L = maxlevel - 1; L >= minlevel; L--) emit_physbc(f, &b, L, mthbc, 0);
-    if (time_interp) emit_physbc(f, &b, ti, mthbc, TI);
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_physbc(f, &lv, &b, L, mthbc, 0);
+    if (time_interp) emit_physbc(f, &lv, &b, ti, mthbc, TI);
     /* phase 3: interpolate coarse -> fine ghost cells */
     phase_start[3] = b.n;
-    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &b, L, mx, my, OP_INTERP_FACE, 0);
-    if (time_interp) emit_exchanges(f, &b, ti, mx, my, OP_INTERP_FACE, TI);
+    for (L = maxlevel - 1; L >= minlevel; L--) emit_exchanges(f, &lv, &b, L, mx, my, OP_INTERP_FACE, 0);
+    if (time_interp) emit_exchanges(f, &lv, &b, ti, mx, my, OP_INTERP_FACE, TI);
     /* phase 4: physical BC on fine levels; phase 5: on all levels */
     phase_start[4] = b.n;
-    for (L = maxlevel; L >= minlevel + 1; L--) emit_physbc(f, &b, L, mthbc, 0);
+    for (L = maxlevel; L >= minlevel + 1; L--) emit_physbc(f, &lv, &b, L, mthbc, 0);
     phase_start[5] = b.n;
-    for (L = maxlevel; L >= minlevel; L--) emit_physbc(f, &b, L, mthbc, 0);
+    for (L = maxlevel; L >= minlevel; L--) emit_physbc(f, &lv, &b, L, mthbc, 0);
     phase_start[6] = b.n;
+    free(lv.byl);
     return b.n;
 }
 

@@ -199,6 +199,74 @@ static void* scratch_ptr(int* rc) {
 static int build_fused(hipclaw_state_t* s);
 static int build_fused_sync(hipclaw_state_t* s);
 
+// Host-to-device copies of programs and tables go through the state's pinned
+// staging buffer (pageable copies run several times slower).  The buffer is
+// reused by the next upload, which first waits for the copies out of it.
+
+// The staging buffer with room for `bytes`, once no copy out of it runs any
+// more (the caller may then fill it and copy out of it).
+static int stage_reserve(hipclaw_state_t* s, size_t bytes, char** base) {
+    if (s->hstage_busy) {
+        HIP_TRY(hipStreamSynchronize(s->st));
+        s->hstage_busy = false;
+    }
+    if (bytes > s->hstage_cap) {
+        if (s->hstage) hipHostFree(s->hstage);
+        s->hstage = nullptr;
+        s->hstage_cap = 0;
+        const size_t cap = bytes + bytes / 4;
+        if (hipHostMalloc(&s->hstage, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipHostMalloc of the upload staging buffer failed%s");
+        s->hstage_cap = cap;
+    }
+    *base = (char*) s->hstage;
+    return HIPCLAW_OK;
+}
+
+// A device array of at least n elements (grown, contents dropped, as needed).
+template <typename T>
+static int ensure_dev(T*& dev, int& cap, int n, const char* what) {
+    if (n > cap) {
+        hipFree(dev);
+        dev = nullptr;
+        cap = 0;
+        if (hipMalloc((void**) &dev, sizeof(T) * (size_t) (n > 0 ? n : 1)) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
+        cap = n;
+    }
+    return HIPCLAW_OK;
+}
+
+// Copy n elements out of the staging buffer (at `host`, inside it) into a
+// device array grown as needed; the copy is asynchronous on the state's stream.
+template <typename T>
+static int stage_copy(hipclaw_state_t* s, T*& dev, int& cap, const T* host, int n, const char* what) {
+    if (int rc = ensure_dev(dev, cap, n, what)) return rc;
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(dev, host, sizeof(T) * (size_t) n, hipMemcpyHostToDevice, s->st));
+        s->hstage_busy = true;
+    }
+    return HIPCLAW_OK;
+}
+
+// Copy `bytes` from host memory to `dev` through the staging buffer.
+static int staged_upload(hipclaw_state_t* s, void* dev, const void* host, size_t bytes) {
+    char* base = nullptr;
+    if (int rc = stage_reserve(s, bytes, &base)) return rc;
+    std::memcpy(base, host, bytes);
+    HIP_TRY(hipMemcpyAsync(dev, base, bytes, hipMemcpyHostToDevice, s->st));
+    s->hstage_busy = true;
+    return HIPCLAW_OK;
+}
+
+// Upload n elements of `host` into a device array grown as needed.
+template <typename T>
+static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, int n, const char* what) {
+    if (int rc = ensure_dev(dev, cap, n, what)) return rc;
+    if (n > 0) return staged_upload(s, dev, host, sizeof(T) * (size_t) n);
+    return HIPCLAW_OK;
+}
+
 extern "C" {
 
 const char* hipclaw_last_error(void) { return g_err; }
@@ -883,44 +951,6 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
 
 }  // extern "C"
 
-// Upload n elements of `host` into a device array grown as needed.
-// Host-to-device copy of a program or table through the state's pinned
-// staging buffer (pageable copies run several times slower).  The buffer is
-// reused by the next upload, which first waits for the copy out of it.
-static int staged_upload(hipclaw_state_t* s, void* dev, const void* host, size_t bytes) {
-    if (s->hstage_busy) {
-        HIP_TRY(hipStreamSynchronize(s->st));
-        s->hstage_busy = false;
-    }
-    if (bytes > s->hstage_cap) {
-        if (s->hstage) hipHostFree(s->hstage);
-        s->hstage = nullptr;
-        s->hstage_cap = 0;
-        const size_t cap = bytes + bytes / 4;
-        if (hipHostMalloc(&s->hstage, cap, hipHostMallocDefault) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipHostMalloc of the upload staging buffer failed%s");
-        s->hstage_cap = cap;
-    }
-    std::memcpy(s->hstage, host, bytes);
-    HIP_TRY(hipMemcpyAsync(dev, s->hstage, bytes, hipMemcpyHostToDevice, s->st));
-    s->hstage_busy = true;
-    return HIPCLAW_OK;
-}
-
-template <typename T>
-static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, int n, const char* what) {
-    if (n > cap) {
-        hipFree(dev);
-        dev = nullptr;
-        cap = 0;
-        if (hipMalloc((void**) &dev, sizeof(T) * (size_t) (n > 0 ? n : 1)) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
-        cap = n;
-    }
-    if (n > 0) return staged_upload(s, dev, host, sizeof(T) * (size_t) n);
-    return HIPCLAW_OK;
-}
-
 // The fused step of program 0 (ghost layout 2).  Every ghost op becomes one
 // wave of ghost_flat that reads interior cells only: copies and averages do
 // already; interpolation and the physical BC recompute the ghost cells they
@@ -944,8 +974,15 @@ static int build_fused(hipclaw_state_t* s) {
     const int nops = (int) s->gp0_ops.size() / 8;
     const int* ops = s->gp0_ops.data();
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
-    std::vector<FaceRel> frel((size_t) 4 * np);
-    for (auto& r : frel) r = FaceRel{hipclaw::FREL_NONE, -1, 0, 0, -1, 0, 0, 0};
+    // the three tables are built in place in the pinned staging buffer, then
+    // copied out together: face table, flat op list, pull offsets
+    const size_t frel_bytes = sizeof(FaceRel) * 4 * (size_t) np;
+    const size_t off_fops = (frel_bytes + 255) & ~(size_t) 255;
+    const size_t off_pull = off_fops + ((sizeof(GhostOp) * (size_t) nops + 255) & ~(size_t) 255);
+    char* stage = nullptr;
+    if (int rc = stage_reserve(s, off_pull + sizeof(int) * 8 * (size_t) np, &stage)) return rc;
+    FaceRel* frel = reinterpret_cast<FaceRel*>(stage);
+    for (size_t k = 0; k < 4 * (size_t) np; k++) frel[k] = FaceRel{hipclaw::FREL_NONE, -1, 0, 0, -1, 0, 0, 0};
     std::vector<int> bcop(np, -1);
     for (int o = 0; o < nops; o++) {
         const int* op = ops + 8 * o;
@@ -982,14 +1019,15 @@ static int build_fused(hipclaw_state_t* s) {
         if (type == hipclaw::GHOST_INTERP_FACE) {
             if (c < 0 || c >= np || !(frel[4 * c + which].rel == hipclaw::FREL_HALF && frel[4 * c + which].nb1 >= 0))
                 return HIPCLAW_OK;
-            bool lo = false, hi = false;
-            for (int l = 1; l <= mx; l++) {   // the kernel bounds the loop by mx (mx == my)
+            // whether the op writes the fine cells at either end of the face (the
+            // kernel bounds its loop by mx, mx == my, and skips fine cells
+            // outside the patch)
+            auto writes = [&](int l) {
                 const int ic = which < 2 ? (which == 0 ? 1 : mx) : l, jc = which < 2 ? l : (which == 2 ? 1 : my);
                 const int fi = 2 * ic - 1 + ti, fj = 2 * jc - 1 + tj;
-                if (fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc) continue;
-                if (l == 1) lo = true;
-                if (l == mx) hi = true;
-            }
+                return !(fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc);
+            };
+            const bool lo = writes(1), hi = writes(mx);
             const int flo = which < 2 ? 2 : 0, fhi = which < 2 ? 3 : 1;
             if ((lo && !simple(c, flo)) || (hi && !simple(c, fhi))) return HIPCLAW_OK;
         } else if (type == hipclaw::GHOST_INTERP_CORNER) {
@@ -1001,8 +1039,8 @@ static int build_fused(hipclaw_state_t* s) {
         }
     }
     // flat op list: everything but the copies, then the copies
-    std::vector<GhostOp> fops;
-    fops.reserve((size_t) nops);
+    GhostOp* fops = reinterpret_cast<GhostOp*>(stage + off_fops);
+    int nf = 0;
     auto to_op = [&](int o) {
         const int* q = ops + 8 * o;
         GhostOp op;
@@ -1014,16 +1052,17 @@ static int build_fused(hipclaw_state_t* s) {
         const int type = ops[8 * o];
         if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) continue;
         if (type == hipclaw::GHOST_PHYSBC && bcop[ops[8 * o + 1]] != o) continue;   // once per patch
-        fops.push_back(to_op(o));
+        fops[nf++] = to_op(o);
     }
-    const int nlite = (int) fops.size();
+    const int nlite = nf;
     for (int o = 0; o < nops; o++) {
         const int type = ops[8 * o];
-        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) fops.push_back(to_op(o));
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) fops[nf++] = to_op(o);
     }
     // same-size strips and corners the update stages from the neighbours
     // (offsets in doubles must fit an int)
-    std::vector<int> pull((size_t) 8 * np, 0);
+    int* pull = reinterpret_cast<int*>(stage + off_pull);
+    std::fill(pull, pull + 8 * (size_t) np, 0);
     bool pull_ok = (double) s->pstride * s->max_patches < 2.0e9;
     for (int o = 0; o < nops && pull_ok; o++) {
         const int* op = ops + 8 * o;
@@ -1031,13 +1070,14 @@ static int build_fused(hipclaw_state_t* s) {
         const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
         pull[8 * (size_t) op[1] + (op[0] == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
     }
-    if (int rc = upload_array(s, s->fops, s->fops_cap, fops.data(), (int) fops.size(), "fused ghost ops")) return rc;
-    if (int rc = upload_array(s, s->frel, s->frel_cap, frel.data(), (int) frel.size(), "face table")) return rc;
+    if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
+    if (int rc = stage_copy(s, s->frel, s->frel_cap, frel, 4 * np, "face table")) return rc;
     if (pull_ok)
-        if (int rc = upload_array(s, s->pull, s->pull_cap, pull.data(), (int) pull.size(), "pull offsets")) return rc;
+        if (int rc = stage_copy(s, s->pull, s->pull_cap, pull, 8 * np, "pull offsets")) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
+    s->hstage_busy = false;
     s->nlite = nlite;
-    s->nall = (int) fops.size();
+    s->nall = nf;
     s->pull_ok = pull_ok;
     s->fused_ok = true;
     return build_fused_sync(s);
@@ -2046,10 +2086,8 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     if (reuse) qnew = s->q[1 - s->cur];
     else if (hipMalloc((void**) &qnew, sizeof(double) * s->pstride * (size_t) cap) != hipSuccess)
         return fail(HIPCLAW_ENOMEM, "hipclaw_regrid: device allocation failed%s");
-    // cells the transfer does not write (ghost cells of new children and
-    // parents) start at zero until the ghost fill
-    hipError_t he = hipMemsetAsync(qnew, 0, sizeof(double) * s->pstride * (size_t) npatch_new, s->st);
-    if (he == hipSuccess) he = hipMemcpyAsync(dops, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, s->st);
+    // (regrid_remap zeroes the ghost cells of new children and parents)
+    hipError_t he = hipMemcpyAsync(dops, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, s->st);
     if (he == hipSuccess)
         he = (hipError_t) hipclaw::launch_regrid_remap(s->q[s->cur], qnew, dops, npatch_new, s->pstride,
                                                        s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn, s->st);

@@ -593,7 +593,13 @@ regrid_remap(const double* __restrict__ qold, double* __restrict__ qnew,
     if (kind == REGRID_SAME) {
         const double* qs = qold + (long long) op[2] * pstride;
         for (long long e = threadIdx.x; e < pstride; e += blockDim.x) qd[e] = qs[e];
-    } else if (kind == REGRID_CHILD) {
+    } else {
+        // the cells the transfer does not write (ghost cells of new children
+        // and parents) start at zero until the ghost fill
+        for (long long e = threadIdx.x; e < pstride; e += blockDim.x) qd[e] = 0.0;
+        __syncthreads();
+    }
+    if (kind == REGRID_CHILD) {
         const double* qc = qold + (long long) op[2] * pstride;
         const int igrid = op[3], ig = igrid % 2, jg = (igrid - ig) / 2;
         const int i1 = 1 - ig, ni = mx / 2 + 1, ic_add = ig * mx / 2;
@@ -612,7 +618,7 @@ regrid_remap(const double* __restrict__ qold, double* __restrict__ qnew,
                     qd[QIX((i - 1) * 2 + ii, (j - 1) * 2 + jj, m)] = c + shiftx * gradx + shifty * grady;
                 }
         }
-    } else {
+    } else if (kind != REGRID_SAME) {
         for (int e = threadIdx.x; e < meqn * mx * my; e += blockDim.x) {
             const int m = e / (mx * my), r = e - m * mx * my;
             const int jc = r / mx + 1, ic = r - (jc - 1) * mx + 1;
