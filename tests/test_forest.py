@@ -286,3 +286,30 @@ def test_adapt_keeps_relations_equal_to_a_fresh_forest(seed):
             a, pa = f.ghost_program_range(16, 16, 2, mthbc, ti + 1, hi, True)
             b, pb = g.ghost_program_range(16, 16, 2, mthbc, ti + 1, hi, True)
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_long_adapt_sequences_keep_the_leaf_map(seed):
+    """Twelve adapts of a forest of a few thousand leaves, alternating heavy
+    refinement (the leaf map outgrows its table and is rebuilt) with heavy
+    coarsening (children leave the map, parents join, leaves are renumbered):
+    after each, tiling, balance, provenance and the programs equal those of a
+    forest built from the same leaves."""
+    rng = np.random.default_rng(300 + seed)
+    f = fa.Forest(2, 2, (seed == 1, seed == 2), 3)
+    mthbc = (1, 3, 1, 3)
+    for it in range(12):
+        f.ghost_program(16, 16, 2, mthbc)
+        old = [a.copy() for a in f.leaves()]
+        p_ref, p_coa = (0.25, 0.1) if it % 3 == 0 else (0.02, 0.9)
+        refine = (rng.random(f.npatch) < p_ref).astype(np.int32)
+        coarsen = (rng.random(f.npatch) < p_coa).astype(np.int32)
+        kind, src = f.adapt(refine, coarsen, 2, 6)
+        _check_tiling_and_balance(f)
+        _check_provenance(old, f.leaves(), kind, src)
+        g = fa.Forest(f.nbx, f.nby, f.periodic, leaves=f.leaves(), balance=False)
+        a, pa = f.ghost_program(16, 16, 2, mthbc)
+        b, pb = g.ghost_program(16, 16, 2, mthbc)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(f.sync_program(16, 16), g.sync_program(16, 16))
