@@ -115,7 +115,12 @@ struct hipclaw_state {
     size_t hstage_cap = 0;
     bool hstage_busy = false;     // a copy out of hstage may still run
     double* qsave = nullptr;      // saved state (hipclaw_save_step), max_patches slabs
-    bool saved = false;           // qsave holds the current slots' state
+    bool saved = false;           // a saved state of the current slots exists
+    // With two slabs the saved state stays in slab save_slab, uncopied, while
+    // nothing writes that slab (a one-step advance writes the other one);
+    // anything else that may write it copies it into qsave first.
+    bool save_lazy = false;
+    int save_slab = 0;
     // ghost-patch message: slab offsets of one field's band, in message order
     int* band = nullptr;
     int nband = 0;
@@ -123,6 +128,7 @@ struct hipclaw_state {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
+    int graph_cur = 0;            // the slab a captured advance graph starts from
     // timing of the update kernel inside advance_async
     bool time_kernels = true;
     std::vector<hipEvent_t> kev;  // pairs (kernel start, kernel end) per step
@@ -265,6 +271,28 @@ static int upload_array(hipclaw_state_t* s, T*& dev, int& cap, const T* host, in
     if (int rc = ensure_dev(dev, cap, n, what)) return rc;
     if (n > 0) return staged_upload(s, dev, host, sizeof(T) * (size_t) n);
     return HIPCLAW_OK;
+}
+
+// Copy slab `from` into qsave (allocated on first use).
+static int copy_to_save(hipclaw_state_t* s, int from) {
+    const size_t bytes = sizeof(double) * (size_t) s->pstride * (size_t) s->npatch;
+    if (!s->qsave) {
+        if (hipMalloc((void**) &s->qsave, sizeof(double) * (size_t) s->pstride * (size_t) s->max_patches) != hipSuccess)
+            return fail(HIPCLAW_ENOMEM, "hipMalloc of the saved state failed%s");
+    }
+    if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->qsave, s->q[from], bytes, hipMemcpyDeviceToDevice, s->st));
+    return HIPCLAW_OK;
+}
+
+// Before anything that may write slab save_slab: a lazily saved state is
+// copied into qsave.  A one-step advance from the saved slab writes only the
+// other one (the update's destination, then that slab's ghost cells and
+// fix-up), so it leaves the saved state in place.
+static int guard_save(hipclaw_state_t* s, bool one_step_advance = false) {
+    if (!s->save_lazy) return HIPCLAW_OK;
+    if (one_step_advance && s->cur == s->save_slab) return HIPCLAW_OK;
+    s->save_lazy = false;
+    return copy_to_save(s, s->save_slab);
 }
 
 extern "C" {
@@ -570,6 +598,7 @@ static int check_range(hipclaw_state_t* s, const void* p, int first, int count) 
 
 int hipclaw_set_q(hipclaw_state_t* s, const double* q, int first, int count) {
     if (int rc = check_range(s, q, first, count)) return rc;
+    if (int rc = guard_save(s)) return rc;
     const size_t bytes = sizeof(double) * s->pstride * count;
     HIP_TRY(hipMemcpyAsync(s->q[s->cur] + s->pstride * first, q, bytes, hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -1147,6 +1176,7 @@ static int time_sync_async(hipclaw_state_t* s, int k) {
 
 int hipclaw_time_sync_k(hipclaw_state_t* s, int k) {
     if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS) return fail(HIPCLAW_EINVAL, "hipclaw_time_sync: bad arguments%s");
+    if (int rc = guard_save(s)) return rc;
     if (int rc = time_sync_async(s, k)) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
     return HIPCLAW_OK;
@@ -1211,6 +1241,7 @@ static int ghost_fill_async(hipclaw_state_t* s, int k) {
 
 int hipclaw_ghost_fill_k(hipclaw_state_t* s, int k) {
     if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS) return fail(HIPCLAW_EINVAL, "hipclaw_ghost_fill: bad arguments%s");
+    if (int rc = guard_save(s)) return rc;
     if (int rc = ghost_fill_async(s, k)) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
     return HIPCLAW_OK;
@@ -1290,17 +1321,21 @@ int hipclaw_ghost_pack(hipclaw_state_t* s, const int* slots, int n, int time_int
 
 int hipclaw_ghost_unpack(hipclaw_state_t* s, const int* slots, int n, int time_interp,
                          const double* buf) {
+    if (s && !time_interp)
+        if (int rc = guard_save(s)) return rc;
     return ghost_message(s, slots, n, time_interp, const_cast<double*>(buf), 1);
 }
 
 int hipclaw_save_step(hipclaw_state_t* s) {
     if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_save_step: bad arguments%s");
-    const size_t bytes = sizeof(double) * (size_t) s->pstride * (size_t) s->npatch;
-    if (!s->qsave) {
-        if (hipMalloc((void**) &s->qsave, sizeof(double) * (size_t) s->pstride * (size_t) s->max_patches) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of the saved state failed%s");
+    if (!s->inplace) {   // fclaw2d_clawpatch_save_step without the copy
+        s->save_lazy = true;
+        s->save_slab = s->cur;
+        s->saved = true;
+        return HIPCLAW_OK;
     }
-    if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->qsave, s->q[s->cur], bytes, hipMemcpyDeviceToDevice, s->st));
+    s->save_lazy = false;
+    if (int rc = copy_to_save(s, s->cur)) return rc;
     s->saved = true;
     return HIPCLAW_OK;
 }
@@ -1308,6 +1343,12 @@ int hipclaw_save_step(hipclaw_state_t* s) {
 int hipclaw_restore_step(hipclaw_state_t* s) {
     if (!s) return fail(HIPCLAW_EINVAL, "hipclaw_restore_step: bad arguments%s");
     if (!s->saved) return fail(HIPCLAW_EINVAL, "no saved state for the current patches%s");
+    if (s->save_lazy) {
+        // the saved slab was not written since: it becomes the current one again
+        s->cur = s->save_slab;
+        HIP_TRY(hipStreamSynchronize(s->st));
+        return HIPCLAW_OK;
+    }
     const size_t bytes = sizeof(double) * (size_t) s->pstride * (size_t) s->npatch;
     if (bytes > 0) HIP_TRY(hipMemcpyAsync(s->q[s->cur], s->qsave, bytes, hipMemcpyDeviceToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -1384,6 +1425,7 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
                      double* ms, int* ntrials) {
     if (!s || nsteps <= 0 || !(dt > 0.0) || !tiles || !waves || !layouts || !ms || !ntrials)
         return fail(HIPCLAW_EINVAL, "hipclaw_autotune: bad arguments%s");
+    if (int rc = guard_save(s)) return rc;
     // a subcycling state with its levels set is tuned on its subcycled coarse
     // steps (each ends with the full synchronisation, which clears the
     // registers the substeps accumulate)
@@ -1395,7 +1437,10 @@ int hipclaw_autotune(hipclaw_state_t* s, double dt, int nsteps, int* tiles, int*
     if (sub)
         if (int rc = subcycle_ready(s)) return rc;
     auto trial_steps = [&](int n) { return sub ? subcycle_launches(s, dt, n) : advance_launches(s, dt, n, false); };
-    if (int rc = hipclaw_save_step(s)) return rc;
+    // the trials step through both slabs: the state is copied, not kept in its slab
+    if (int rc = copy_to_save(s, s->cur)) return rc;
+    s->save_lazy = false;
+    s->saved = true;
     hipEvent_t ev[2] = {nullptr, nullptr};
     HIP_TRY(hipEventCreate(&ev[0]));
     HIP_TRY(hipEventCreate(&ev[1]));
@@ -1557,6 +1602,7 @@ int hipclaw_step_batch(hipclaw_state_t* s, const int* slots, int nslots, double 
         return fail(HIPCLAW_EINVAL, "hipclaw_step_batch: bad arguments%s");
     for (int k = 0; k < nslots; k++)
         if (slots[k] < 0 || slots[k] >= s->npatch) return fail(HIPCLAW_EINVAL, "slot out of range%s");
+    if (int rc = guard_save(s)) return rc;   // the batch writes both slabs
     if (maxcfl) *maxcfl = 0.0;
     if (nslots == 0) return HIPCLAW_OK;
     HIP_TRY(hipMemcpyAsync(s->batch_slots, slots, sizeof(int) * nslots, hipMemcpyHostToDevice, s->st));
@@ -1833,6 +1879,7 @@ int hipclaw_set_levels(hipclaw_state_t* s, const int* level, int npatch) {
 int hipclaw_advance_subcycle_async(hipclaw_state_t* s, double dt, int ncoarse) {
     if (!s || ncoarse < 0 || !(dt > 0.0)) return fail(HIPCLAW_EINVAL, "hipclaw_advance_subcycle: bad arguments%s");
     if (int rc = subcycle_ready(s)) return rc;
+    if (int rc = guard_save(s)) return rc;
     HIP_TRY(hipEventRecord(s->region[0], s->st));
     if (int rc = subcycle_launches(s, dt, ncoarse)) return rc;
     HIP_TRY(hipEventRecord(s->region[1], s->st));
@@ -1855,7 +1902,9 @@ int hipclaw_capture_subcycle_graph(hipclaw_state_t* s, double dt, int ncoarse) {
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
     if (int rc = subcycle_ready(s)) return rc;
+    if (int rc = guard_save(s)) return rc;   // (replays write the slabs)
     HIP_TRY(hipStreamSynchronize(s->st));
+    s->graph_cur = s->cur;   // the subcycled steps write the current slab in place
     HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     int rc = subcycle_launches(s, dt, ncoarse);
     hipGraph_t g = nullptr;
@@ -1874,6 +1923,7 @@ int hipclaw_advance_async(hipclaw_state_t* s, double dt, int nsteps) {
         return fail(HIPCLAW_EINVAL, "with subcycle, the full sync program needs its register resets%s");
     if (!(s->ghost_layout == 2 && s->fused_ok && !s->reg_accumulate))
         if (int rc = ensure_layouts(s)) return rc;
+    if (int rc = guard_save(s, nsteps == 1)) return rc;
     HIP_TRY(hipEventRecord(s->region[0], s->st));
     if (int rc = advance_launches(s, dt, nsteps, s->time_kernels)) return rc;
     HIP_TRY(hipEventRecord(s->region[1], s->st));
@@ -1902,10 +1952,12 @@ int hipclaw_capture_graph(hipclaw_state_t* s, double dt, int nsteps) {
         return fail(HIPCLAW_EINVAL, "ping-pong states need an even number of captured steps%s");
     if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (int rc = guard_save(s)) return rc;   // (replays write both slabs)
     HIP_TRY(hipStreamSynchronize(s->st));
     if (!(s->ghost_layout == 2 && s->fused_ok && !s->reg_accumulate))
         if (int rc = ensure_layouts(s)) return rc;
     const int cur0 = s->cur;
+    s->graph_cur = cur0;
     HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     int rc = advance_launches(s, dt, nsteps, false);
     hipGraph_t g = nullptr;
@@ -1921,6 +1973,10 @@ int hipclaw_capture_graph(hipclaw_state_t* s, double dt, int nsteps) {
 
 int hipclaw_replay_graph(hipclaw_state_t* s) {
     if (!s || !s->gexec) return fail(HIPCLAW_EINVAL, "no captured graph%s");
+    if (s->cur != s->graph_cur)
+        return fail(HIPCLAW_EINVAL, "the captured graph starts from the other slab (an odd number of "
+                                    "steps or a restored step since the capture)%s");
+    if (int rc = guard_save(s)) return rc;
     HIP_TRY(hipEventRecord(s->region[0], s->st));
     HIP_TRY(hipGraphLaunch(s->gexec, s->st));
     HIP_TRY(hipEventRecord(s->region[1], s->st));
@@ -2112,6 +2168,7 @@ int hipclaw_regrid(hipclaw_state_t* s, int npatch_new, const int* kind, const in
     }
     // a saved state names old slots (and is sized for the old capacity)
     s->saved = false;
+    s->save_lazy = false;
     if (cap > s->max_patches) {
         hipFree(s->qsave);
         s->qsave = nullptr;

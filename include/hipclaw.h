@@ -264,7 +264,11 @@ int hipclaw_set_dt_scale(hipclaw_state_t *s, const double *scale);
  * for all patches at once, as save_time_step / restore_time_step of the run
  * loop use them to reject a step whose Courant number exceeds max_cfl
  * (src/fclaw2d_run.c:47-80).  Restoring without a saved state, or after a
- * regrid (the slots changed), is an error. */
+ * regrid (the slots changed), is an error.  With two slabs (tile smaller
+ * than the patch) the saved state stays in its slab without a copy while
+ * only a one-step advance runs (it writes the other slab); any other call
+ * that may write that slab copies it first.  A restore may so make the
+ * other slab current: a graph captured before then refuses to replay. */
 int hipclaw_save_step(hipclaw_state_t *s);
 int hipclaw_restore_step(hipclaw_state_t *s);
 
@@ -339,7 +343,9 @@ int hipclaw_advance_async(hipclaw_state_t *s, double dt, int nsteps);
 int hipclaw_sync(hipclaw_state_t *s);
 
 /* Capture nsteps of hipclaw_advance_async in a HIP graph (replaces any graph
- * captured before) and replay it. */
+ * captured before) and replay it.  The graph starts from the slab current at
+ * its capture; a replay when the other slab is current (after an odd number
+ * of steps, or a restore) is refused with HIPCLAW_EINVAL. */
 int hipclaw_capture_graph(hipclaw_state_t *s, double dt, int nsteps);
 int hipclaw_replay_graph(hipclaw_state_t *s);
 
