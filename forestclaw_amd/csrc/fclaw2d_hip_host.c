@@ -533,18 +533,32 @@ void fc2d_hipclaw_initialize_GPUs(fclaw2d_global_t *glob)
     device_check(hipclaw_initialize_device(0, name, sizeof(name)), "hipclaw_initialize_device");
 }
 
-/* Ghost-fill and time-sync programs of the current domain. */
+/* Ghost-fill and time-sync programs of the current domain.  The op buffer
+ * is sized for the usual forest (at most 16 ghost and 8 sync ops per leaf),
+ * so each program is built once; a larger one is built again into a buffer
+ * of the size the first call reported. */
 static void install_programs(fclaw2d_global_t *glob)
 {
     const fc2d_hipclaw_options_t *opt = fc2d_hipclaw_get_options(glob);
     fclaw2d_domain_t *d = glob->domain;
     const fclaw2d_clawpatch_options_t *o = &glob->clawpatch_opt;
-    int *ops, phase[7], nops;
-    nops = fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, NULL, 0, phase);
-    ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
-    fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, nops, phase);
+    int *ops, phase[7], nops, cap = 16 * fclaw2d_hip_forest_num_patches(d->forest) + 16;
+    ops = (int *) malloc(sizeof(int) * 8 * (size_t) cap);
+    if (!ops) { fprintf(stderr, "install_programs: out of memory\n"); exit(1); }
+#define GROW_OPS(n) do {                                                              \
+        if ((n) > cap) {                                                              \
+            free(ops);                                                                \
+            cap = (n);                                                                \
+            ops = (int *) malloc(sizeof(int) * 8 * (size_t) cap);                     \
+            if (!ops) { fprintf(stderr, "install_programs: out of memory\n"); exit(1); } \
+        }                                                                             \
+    } while (0)
+    nops = fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, cap, phase);
+    if (nops > cap) {
+        GROW_OPS(nops);
+        fclaw2d_hip_forest_ghost_program(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ops, cap, phase);
+    }
     device_check(hipclaw_set_ghost_program(glob->state, ops, nops, phase), "hipclaw_set_ghost_program");
-    free(ops);
     if (glob->subcycle) {
         /* one program per time-interpolated level ti: levels ti+1..maxlevel */
         int ti;
@@ -554,13 +568,14 @@ static void install_programs(fclaw2d_global_t *glob)
         }
         for (ti = d->global_minlevel; ti < d->global_maxlevel; ti++) {
             nops = fclaw2d_hip_forest_ghost_program_range(d->forest, o->mx, o->my, o->mbc, opt->mthbc,
-                                                          ti + 1, d->global_maxlevel, 1, NULL, 0, phase);
-            ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
-            fclaw2d_hip_forest_ghost_program_range(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ti + 1,
-                                                   d->global_maxlevel, 1, ops, nops, phase);
+                                                          ti + 1, d->global_maxlevel, 1, ops, cap, phase);
+            if (nops > cap) {
+                GROW_OPS(nops);
+                fclaw2d_hip_forest_ghost_program_range(d->forest, o->mx, o->my, o->mbc, opt->mthbc, ti + 1,
+                                                       d->global_maxlevel, 1, ops, cap, phase);
+            }
             device_check(hipclaw_set_ghost_program_k(glob->state, 1 + ti - d->global_minlevel, ops,
                                                      nops, phase), "hipclaw_set_ghost_program");
-            free(ops);
         }
     }
     if (glob->subcycle) {
@@ -575,24 +590,35 @@ static void install_programs(fclaw2d_global_t *glob)
     if (glob->time_sync) {
         /* program 0: every level; with subcycling also one per level range */
         int minl;
+        int rcap = 4 * fclaw2d_hip_forest_num_patches(d->forest) + 4;   /* one per face */
+        int *res = (int *) malloc(sizeof(int) * 2 * (size_t) rcap);
+        if (!res) { fprintf(stderr, "install_programs: out of memory\n"); exit(1); }
         for (minl = d->global_minlevel; minl <= (glob->subcycle ? d->global_maxlevel : d->global_minlevel); minl++) {
-            int nres = 0, *res;
-            nops = fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, NULL, 0, NULL, 0, &nres);
+            int nres = 0;
+            nops = fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, ops, cap, res, rcap, &nres);
             if (nops < 0) {
                 fprintf(stderr, "fc2d_hipclaw_allocate_buffers: time sync needs mx = my\n");
                 exit(1);
             }
-            ops = (int *) malloc(sizeof(int) * 8 * (size_t) (nops > 0 ? nops : 1));
-            res = (int *) malloc(sizeof(int) * 2 * (size_t) (nres > 0 ? nres : 1));
-            fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, ops, nops, res, nres, &nres);
+            if (nops > cap || nres > rcap) {
+                GROW_OPS(nops);
+                if (nres > rcap) {
+                    free(res);
+                    rcap = nres;
+                    res = (int *) malloc(sizeof(int) * 2 * (size_t) rcap);
+                    if (!res) { fprintf(stderr, "install_programs: out of memory\n"); exit(1); }
+                }
+                fclaw2d_hip_forest_sync_program_range(d->forest, o->mx, o->my, minl, ops, cap, res, rcap, &nres);
+            }
             /* without subcycling every register is rewritten by every update */
             device_check(hipclaw_set_sync_program_k(glob->state, minl == d->global_minlevel ? 0 : minl - d->global_minlevel,
                                                     ops, nops, res, glob->subcycle ? nres : 0),
                          "hipclaw_set_sync_program");
-            free(ops);
-            free(res);
         }
+        free(res);
     }
+#undef GROW_OPS
+    free(ops);
 }
 
 /* Host views of q and aux for every patch (zeroed). */
