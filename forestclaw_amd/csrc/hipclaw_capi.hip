@@ -137,6 +137,11 @@ struct hipclaw_state {
     // host copies of program 0 (ghost fill with its phases, time sync): the
     // fused step is built from them
     std::vector<int> gp0_ops, sp0_ops;
+    // the fused step's sync ops, FP32 flux form (same-size ops paired);
+    // other forms run sync program 0 as it is
+    int* fsops = nullptr;
+    int fsops_cap = 0, nfsync = 0;
+    bool fpaired = false;
     int gp0_phase[7] = {0, 0, 0, 0, 0, 0, 0};
     bool gp0_set = false;
     // fused step (ghost layout 2): the update pulls same-size ghost strips, one
@@ -564,6 +569,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->band);
     hipFree(s->dtscale);
     hipFree(s->fops);
+    hipFree(s->fsops);
     hipFree(s->pull);
     hipFree(s->iwork);
     hipFree(s->frel);
@@ -1128,6 +1134,37 @@ static int build_fused_sync(hipclaw_state_t* s) {
             cm[op[1]] |= (1 << op[3]) | (op[0] == hipclaw::SYNC_F2C ? 1 << (4 + op[3]) : 0);
         }
     }
+    s->nfsync = 0;
+    s->fpaired = false;
+    const bool pair = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
+    if (sync && pair) {
+        // FP32 flux form: one wave per same-size face pair, the op with the
+        // smaller (dst, face) key writing both increments
+        const int nso = (int) s->sp0_ops.size() / 8;
+        const int* ops = s->sp0_ops.data();
+        std::vector<int> fs;
+        fs.reserve(8 * (size_t) nso);
+        std::vector<int> same(4 * (size_t) np, -1);   // same-size op of (dst, face), -1 if none
+        for (int o = 0; o < nso; o++)
+            if (ops[8 * o] == hipclaw::SYNC_SAMESIZE) same[4 * (size_t) ops[8 * o + 1] + ops[8 * o + 3]] = o;
+        for (int o = 0; o < nso; o++) {
+            const int* op = ops + 8 * o;
+            int type = op[0];
+            if (type == hipclaw::SYNC_SAMESIZE) {
+                const long long key = 4LL * op[1] + op[3], pkey = 4LL * op[2] + (op[3] ^ 1);
+                const int q = same[(size_t) pkey];
+                if (q >= 0 && ops[8 * q + 2] == op[1]) {
+                    if (pkey < key) continue;          // the partner writes this one
+                    type = hipclaw::SYNC_SAMESIZE_PAIR;
+                }
+            }
+            fs.insert(fs.end(), op, op + 8);
+            fs[fs.size() - 8] = type;
+        }
+        if (int rc = upload_array(s, s->fsops, s->fsops_cap, fs.data(), (int) fs.size(), "fused sync ops")) return rc;
+        s->nfsync = (int) fs.size() / 8;
+        s->fpaired = true;
+    }
     if (sync) {
         if (int rc = upload_array(s, s->cmask, s->cmask_cap, cm.data(), np, "sync face masks")) return rc;
         const long long cstride = 4LL * s->cfg.meqn * mx;
@@ -1197,8 +1234,8 @@ static int ghost_flat_async(hipclaw_state_t* s, bool full, bool sync) {
     f.g.mx = s->cfg.mx; f.g.my = s->cfg.my; f.g.mbc = s->cfg.mbc; f.g.meqn = s->cfg.meqn;
     f.nghost = full ? s->nall : s->nlite;
     f.frel = s->frel;
-    f.sops = sync ? s->sp[0].ops : nullptr;
-    f.nsync = sync ? (int) s->sp0_ops.size() / 8 : 0;
+    f.sops = !sync ? nullptr : s->fpaired ? s->fsops : s->sp[0].ops;
+    f.nsync = !sync ? 0 : s->fpaired ? s->nfsync : (int) s->sp0_ops.size() / 8;
     f.reg = s->reg;
     f.dxs = s->dxs;
     f.dys = s->dys;

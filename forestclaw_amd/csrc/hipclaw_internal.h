@@ -45,7 +45,11 @@ struct GhostArgs {
 };
 
 // One time-synchronisation op (int32 x 8), grouped by destination slot.
-enum { SYNC_SAMESIZE = 0, SYNC_F2C = 1 };
+enum { SYNC_SAMESIZE = 0, SYNC_F2C = 1,
+       // fused step, FP32 flux form only: a same-size op that also writes its
+       // partner's increment (dst's face is src's face ^ 1; both are the same
+       // value there, since both read the same two shared-flux registers)
+       SYNC_SAMESIZE_PAIR = 2 };
 
 struct SyncArgs {
     const int* ops;      // [nops][8]: type, dst, src, face, ti, tj, -, -

@@ -447,11 +447,19 @@ __device__ void fused_sync_op(const FlatArgs& f, const int* __restrict__ op, int
         const double ein = R(rt, face, REG_EFIN, c - 1);
         const double ft = f.fluxreg ? 0.0 : R(rt, face, kfl, c - 1);
         double v;
-        if (type == SYNC_SAMESIZE) {
+        if (type != SYNC_F2C) {
             const double ef = R(rn, kn, kout, c - 1);
             const double fn = f.fluxreg ? 0.0 : R(rn, kn, kfl, c - 1);
             const double delta = (face == 0 || face == 2) ? (ef - ein) - (fn - ft) : (ein - ef) - (fn - ft);
             v = 0.5 * delta / area;
+            // the partner's op (dst and src swapped, face ^ 1) would compute the
+            // same delta: it reads these two registers and takes their
+            // difference in the same order (the FP32 flux form stores no
+            // fluctuations, fn = ft = 0)
+            if (type == SYNC_SAMESIZE_PAIR) {
+                const double area_src = f.dxs[src] * f.dys[src];
+                f.qinc[(long long) src * f.cstride + (kn * meqn + m) * mx + c - 1] = 0.0 + 0.5 * delta / area_src;
+            }
         } else {
             const int f0 = 2 * c - 1 + (face < 2 ? tj : ti);
             if (f0 < 1 || f0 + 1 > mx) continue;
