@@ -1124,26 +1124,36 @@ static int build_fused_sync(hipclaw_state_t* s) {
     s->fsync = false;
     if (!s->fused_ok) return HIPCLAW_OK;
     const int np = s->npatch, mx = s->cfg.mx;
-    std::vector<int> cm(np, 0);
     const bool sync = s->reg && !s->reg_accumulate && !s->sp0_ops.empty();
-    if (sync) {
-        const int nso = (int) s->sp0_ops.size() / 8;
-        for (int o = 0; o < nso; o++) {
-            const int* op = s->sp0_ops.data() + 8 * o;
-            if (op[1] < 0 || op[1] >= np || op[2] < 0 || op[2] >= np) return HIPCLAW_OK;
-            cm[op[1]] |= (1 << op[3]) | (op[0] == hipclaw::SYNC_F2C ? 1 << (4 + op[3]) : 0);
-        }
-    }
+    const int nso = (int) s->sp0_ops.size() / 8;
+    const int* ops = s->sp0_ops.data();
+    if (sync)
+        for (int o = 0; o < nso; o++)
+            if (ops[8 * o + 1] < 0 || ops[8 * o + 1] >= np || ops[8 * o + 2] < 0 || ops[8 * o + 2] >= np)
+                return HIPCLAW_OK;
     s->nfsync = 0;
     s->fpaired = false;
+    if (!sync) {
+        HIP_TRY(hipStreamSynchronize(s->st));
+        return HIPCLAW_OK;
+    }
+    // the faces each patch corrects and, in the FP32 flux form, the paired
+    // sync list, built in the pinned staging buffer and copied out together
     const bool pair = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
-    if (sync && pair) {
-        // FP32 flux form: one wave per same-size face pair, the op with the
-        // smaller (dst, face) key writing both increments
-        const int nso = (int) s->sp0_ops.size() / 8;
-        const int* ops = s->sp0_ops.data();
-        std::vector<int> fs;
-        fs.reserve(8 * (size_t) nso);
+    const size_t off_fs = ((sizeof(int) * (size_t) np) + 255) & ~(size_t) 255;
+    char* stage = nullptr;
+    if (int rc = stage_reserve(s, off_fs + (pair ? sizeof(int) * 8 * (size_t) nso : 0), &stage)) return rc;
+    int* cm = reinterpret_cast<int*>(stage);
+    std::fill(cm, cm + np, 0);
+    for (int o = 0; o < nso; o++) {
+        const int* op = ops + 8 * o;
+        cm[op[1]] |= (1 << op[3]) | (op[0] == hipclaw::SYNC_F2C ? 1 << (4 + op[3]) : 0);
+    }
+    if (pair) {
+        // one wave per same-size face pair, the op with the smaller
+        // (dst, face) key writing both increments
+        int* fs = reinterpret_cast<int*>(stage + off_fs);
+        int nf = 0;
         std::vector<int> same(4 * (size_t) np, -1);   // same-size op of (dst, face), -1 if none
         for (int o = 0; o < nso; o++)
             if (ops[8 * o] == hipclaw::SYNC_SAMESIZE) same[4 * (size_t) ops[8 * o + 1] + ops[8 * o + 3]] = o;
@@ -1158,15 +1168,16 @@ static int build_fused_sync(hipclaw_state_t* s) {
                     type = hipclaw::SYNC_SAMESIZE_PAIR;
                 }
             }
-            fs.insert(fs.end(), op, op + 8);
-            fs[fs.size() - 8] = type;
+            int* d = fs + 8 * (size_t) nf++;
+            std::memcpy(d, op, 8 * sizeof(int));
+            d[0] = type;
         }
-        if (int rc = upload_array(s, s->fsops, s->fsops_cap, fs.data(), (int) fs.size(), "fused sync ops")) return rc;
-        s->nfsync = (int) fs.size() / 8;
+        if (int rc = stage_copy(s, s->fsops, s->fsops_cap, fs, 8 * nf, "fused sync ops")) return rc;
+        s->nfsync = nf;
         s->fpaired = true;
     }
-    if (sync) {
-        if (int rc = upload_array(s, s->cmask, s->cmask_cap, cm.data(), np, "sync face masks")) return rc;
+    if (int rc = stage_copy(s, s->cmask, s->cmask_cap, cm, np, "sync face masks")) return rc;
+    {
         const long long cstride = 4LL * s->cfg.meqn * mx;
         if (!s->qcorr || s->cstride != cstride || s->qcorr_cap < s->max_patches) {
             hipFree(s->qcorr);
@@ -1179,6 +1190,7 @@ static int build_fused_sync(hipclaw_state_t* s) {
         }
     }
     HIP_TRY(hipStreamSynchronize(s->st));
+    s->hstage_busy = false;
     s->fsync = sync;
     return HIPCLAW_OK;
 }
