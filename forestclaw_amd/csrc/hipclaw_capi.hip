@@ -300,6 +300,15 @@ static int guard_save(hipclaw_state_t* s, bool one_step_advance = false) {
     return copy_to_save(s, s->save_slab);
 }
 
+// A captured graph holds its launches' arguments: op counts, table and
+// buffer addresses, patch counts.  Whatever changes those drops the graph
+// (replaying it is then an error until the next capture).
+static void drop_graph(hipclaw_state_t* s) {
+    if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_steps = 0;
+}
+
 extern "C" {
 
 const char* hipclaw_last_error(void) { return g_err; }
@@ -586,6 +595,7 @@ int hipclaw_set_geometry(hipclaw_state_t* s, int npatch, const double* dx, const
         return fail(HIPCLAW_EINVAL, "hipclaw_set_geometry: bad arguments%s");
     for (int p = 0; p < npatch; p++)
         if (!(dx[p] > 0.0) || !(dy[p] > 0.0)) return fail(HIPCLAW_EINVAL, "non-positive mesh width%s");
+    if (npatch != s->npatch) drop_graph(s);
     if (npatch != s->npatch) s->lmax = s->lmin - 1;   // the levels name other slots
     s->npatch = npatch;
     if (npatch > 0) {
@@ -830,6 +840,7 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
     if (phase_start[0] != 0 || phase_start[6] != nops)
         return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
+    drop_graph(s);
     for (int k = 0; k < 6; k++)
         if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
     // validate every op on the host: a bad slot or translation would fault the GPU
@@ -945,6 +956,7 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) ||
         nresets < 0 || (nresets > 0 && !resets))
         return fail(HIPCLAW_EINVAL, "hipclaw_set_sync_program: bad arguments%s");
+    drop_graph(s);
     if (!s->reg && (nops > 0 || nresets > 0))
         return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
     // validate on the host and group by destination (groups must be contiguous)
@@ -1904,6 +1916,7 @@ static int subcycle_ready(hipclaw_state_t* s) {
 int hipclaw_set_levels(hipclaw_state_t* s, const int* level, int npatch) {
     if (!s || npatch != s->npatch || (npatch > 0 && !level))
         return fail(HIPCLAW_EINVAL, "hipclaw_set_levels: one level per patch%s");
+    drop_graph(s);
     if (npatch == 0) { s->lmax = s->lmin - 1; return HIPCLAW_OK; }
     int lo = level[0], hi = level[0];
     for (int p = 0; p < npatch; p++) {

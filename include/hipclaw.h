@@ -345,7 +345,9 @@ int hipclaw_sync(hipclaw_state_t *s);
 /* Capture nsteps of hipclaw_advance_async in a HIP graph (replaces any graph
  * captured before) and replay it.  The graph starts from the slab current at
  * its capture; a replay when the other slab is current (after an odd number
- * of steps, or a restore) is refused with HIPCLAW_EINVAL. */
+ * of steps, or a restore) is refused with HIPCLAW_EINVAL.  Installing a ghost
+ * or sync program, the levels, or geometry for another number of patches
+ * drops the graph (its launches hold the old tables and counts). */
 int hipclaw_capture_graph(hipclaw_state_t *s, double dt, int nsteps);
 int hipclaw_replay_graph(hipclaw_state_t *s);
 

@@ -85,3 +85,19 @@ def test_graph_refuses_to_start_from_the_other_slab(gpu):
     sv.restore_step()
     sv.replay_graph()
     sv.sync()
+
+
+def test_graph_dropped_when_the_programs_change(gpu):
+    """A graph's launches hold the ghost program's tables and counts: after a
+    program is installed again, replaying the old graph is refused."""
+    prob, forest, q0, sv = _solver(16)
+    dt = P.stable_dt(prob, forest, q0)
+    sv.capture_graph(dt, 2)
+    sv.replay_graph()
+    sv.sync()
+    sv.set_forest(forest, prob.extent)
+    with pytest.raises(fa.HipclawError, match="no captured graph"):
+        sv.replay_graph()
+    sv.capture_graph(dt, 2)
+    sv.replay_graph()
+    sv.sync()
