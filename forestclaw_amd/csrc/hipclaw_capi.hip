@@ -1021,8 +1021,8 @@ static int build_fused(hipclaw_state_t* s) {
     const int nops = (int) s->gp0_ops.size() / 8;
     const int* ops = s->gp0_ops.data();
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
-    // the three tables are built in place in the pinned staging buffer, then
-    // copied out together: face table, flat op list, pull offsets
+    // the three tables are built in place in the pinned staging buffer: face
+    // table, flat op list, pull offsets
     const size_t frel_bytes = sizeof(FaceRel) * 4 * (size_t) np;
     const size_t off_fops = (frel_bytes + 255) & ~(size_t) 255;
     const size_t off_pull = off_fops + ((sizeof(GhostOp) * (size_t) nops + 255) & ~(size_t) 255);
@@ -1085,6 +1085,9 @@ static int build_fused(hipclaw_state_t* s) {
                 if (frel[4 * op[1] + f].rel == hipclaw::FREL_NONE) return HIPCLAW_OK;
         }
     }
+    // each table is copied out as soon as it is built, the copies running
+    // while the host builds the next one
+    if (int rc = stage_copy(s, s->frel, s->frel_cap, frel, 4 * np, "face table")) return rc;
     // flat op list: everything but the copies, then the copies
     GhostOp* fops = reinterpret_cast<GhostOp*>(stage + off_fops);
     int nf = 0;
@@ -1106,6 +1109,7 @@ static int build_fused(hipclaw_state_t* s) {
         const int type = ops[8 * o];
         if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) fops[nf++] = to_op(o);
     }
+    if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
     // same-size strips and corners the update stages from the neighbours
     // (offsets in doubles must fit an int)
     int* pull = reinterpret_cast<int*>(stage + off_pull);
@@ -1117,8 +1121,6 @@ static int build_fused(hipclaw_state_t* s) {
         const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
         pull[8 * (size_t) op[1] + (op[0] == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
     }
-    if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
-    if (int rc = stage_copy(s, s->frel, s->frel_cap, frel, 4 * np, "face table")) return rc;
     if (pull_ok)
         if (int rc = stage_copy(s, s->pull, s->pull_cap, pull, 8 * np, "pull offsets")) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
