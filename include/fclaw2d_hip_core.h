@@ -1,6 +1,6 @@
 /*
  * The parts of ForestClaw's core C interface that the batched patch update
- * plugs into, declared for this build (the reference's own headers need
+ * plugs into, declared for this library (the reference's own headers need
  * p4est/libsc).  Names, argument order and meaning follow the reference:
  *
  *   fclaw2d_patch_t / fclaw2d_block_t / fclaw2d_domain_t
@@ -141,7 +141,7 @@ typedef struct fclaw2d_single_step_buffer_data {
 
 /* sizeof of fclaw2d_global_t, fclaw2d_domain_t, fclaw2d_block_t, fclaw2d_patch_t
  * (out[0..3]) and of fc2d_hipclaw_options_t, fc2d_hipclaw_vtable_t (out[4..5]),
- * for bindings to check their struct copies against this build. */
+ * for bindings to check their struct copies against the compiled library. */
 void fclaw2d_hip_struct_sizes(size_t out[6]);
 
 /* ---- globals, domains ---- */

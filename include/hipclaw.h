@@ -79,7 +79,7 @@ typedef struct hipclaw_state hipclaw_state_t;
 const char *hipclaw_last_error(void);
 
 /* sizeof(hipclaw_config_t): lets a foreign-language binding check that its
- * copy of the struct matches this build (no counterpart in the reference). */
+ * copy of the struct matches the compiled library (no counterpart in the reference). */
 size_t hipclaw_config_size(void);
 
 /* Select the GPU and report its name.

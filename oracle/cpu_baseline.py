@@ -89,8 +89,8 @@ def _worker_port(args):
     return time.perf_counter() - t0
 
 
-# Host cores a GPU box allots to one GPU (its worker pools are sized to this
-# share, whatever os.cpu_count() reports there).
+# Host cores the machine allots to one GPU: a host shared by several GPUs
+# gives each GPU's processes this share, whatever os.cpu_count() reports.
 HOST_CORES_PER_GPU = 16
 
 
