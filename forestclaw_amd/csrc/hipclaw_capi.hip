@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <cmath>
@@ -207,7 +208,20 @@ static void* scratch_ptr(int* rc) {
     return g_scratch_ptr[dev];
 }
 
-static int build_fused(hipclaw_state_t* s);
+// Program 0's fused tables are built in two passes over its ops: the first
+// checks every op and fills the face table in the staging buffer, the second
+// (once the state has taken the program) builds the flat op list and the pull
+// offsets beside it.
+struct FusedBuild {
+    bool fusable = false;
+    int nlite = 0;              // ops of the flat list before the copies
+    std::vector<int> bcop;      // per patch: the physical-BC op kept (the last), -1 if none
+    char* stage = nullptr;      // face table, flat op list, pull offsets
+    size_t off_fops = 0, off_pull = 0;
+};
+static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start,
+                                 FusedBuild& b);
+static int fused_finish(hipclaw_state_t* s, const int* ops, int nops, FusedBuild& b);
 static int build_fused_sync(hipclaw_state_t* s);
 
 // Host-to-device copies of programs and tables go through the state's pinned
@@ -234,6 +248,13 @@ static int stage_reserve(hipclaw_state_t* s, size_t bytes, char** base) {
     return HIPCLAW_OK;
 }
 
+// Room for n elements plus a quarter: the tables of a growing forest are not
+// reallocated (hipFree waits for the device) at every regrid.
+static int grown_cap(int n) {
+    const long long c = (long long) n + n / 4 + 16;
+    return c > INT_MAX ? INT_MAX : (int) c;
+}
+
 // A device array of at least n elements (grown, contents dropped, as needed).
 template <typename T>
 static int ensure_dev(T*& dev, int& cap, int n, const char* what) {
@@ -241,9 +262,10 @@ static int ensure_dev(T*& dev, int& cap, int n, const char* what) {
         hipFree(dev);
         dev = nullptr;
         cap = 0;
-        if (hipMalloc((void**) &dev, sizeof(T) * (size_t) (n > 0 ? n : 1)) != hipSuccess)
+        const int c = grown_cap(n);
+        if (hipMalloc((void**) &dev, sizeof(T) * (size_t) c) != hipSuccess)
             return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
-        cap = n;
+        cap = c;
     }
     return HIPCLAW_OK;
 }
@@ -645,6 +667,81 @@ static bool op_slot_ok(const hipclaw_state_t* s, int slot) {
     return slot >= 0 && p < s->npatch;
 }
 
+// The op types each phase of a ghost program may hold (bit mask over GHOST_*).
+static const int kPhaseTypes[6] = {
+    (1 << hipclaw::GHOST_AVERAGE_FACE) | (1 << hipclaw::GHOST_AVERAGE_CORNER),
+    (1 << hipclaw::GHOST_COPY_FACE) | (1 << hipclaw::GHOST_COPY_CORNER),
+    1 << hipclaw::GHOST_PHYSBC,
+    (1 << hipclaw::GHOST_INTERP_FACE) | (1 << hipclaw::GHOST_INTERP_CORNER),
+    1 << hipclaw::GHOST_PHYSBC,
+    1 << hipclaw::GHOST_PHYSBC};
+
+// Host check of one ghost op of phase ph (a bad slot or translation would
+// fault the GPU).
+static int check_ghost_op(const hipclaw_state_t* s, const int* op, int ph) {
+    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
+    // the reference's coarse/fine exchanges bound their face loops by mx
+    // (fclaw2d_clawpatch46_interpolate.f, fclaw2d_clawpatch46_average.f:
+    // average2coarse offsets children by mx in both directions)
+    if (mx != my && type >= hipclaw::GHOST_AVERAGE_FACE && type <= hipclaw::GHOST_INTERP_CORNER)
+        return fail(HIPCLAW_EINVAL, "coarse/fine ghost exchange requires mx = my%s");
+    if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
+        return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
+    if (!(kPhaseTypes[ph] & (1 << type)))
+        return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
+    if (type == hipclaw::GHOST_PHYSBC) {
+        // per face: -1 (not a physical face), 1 (extrapolation) or 3 (wall).  The
+        // reference stops on mthbc 0 and 2 and leaves other codes' ghost cells
+        // unwritten (clawpack46_bc2_default.f:17-25, 41-43), which with two slabs
+        // would leave them to whatever the spare slab held.
+        for (int f = 0; f < 4; f++) {
+            const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
+            if (b != -1 && b != 1 && b != 3)
+                return fail(HIPCLAW_EINVAL,
+                            "physical boundary condition code must be 1 (extrapolation) or 3 (wall)%s");
+        }
+        return HIPCLAW_OK;
+    }
+    if (!op_slot_ok(s, src) || which < 0 || which > 3)
+        return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
+    // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
+    // neighbour's array; check the extreme cells the op can touch
+    int lo_i, hi_i, lo_j, hi_j;  // range of the translated index
+    if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
+        lo_i = 1 - mbc + ti; hi_i = mx + mbc + ti; lo_j = 1 - mbc + tj; hi_j = my + mbc + tj;
+        // copy reads the neighbour's interior band only
+        int ci0, ci1, cj0, cj1;
+        if (type == hipclaw::GHOST_COPY_FACE) {
+            if (which == 0) { ci0 = 1 - mbc; ci1 = 0; cj0 = 1; cj1 = my; }
+            else if (which == 1) { ci0 = mx + 1; ci1 = mx + mbc; cj0 = 1; cj1 = my; }
+            else if (which == 2) { ci0 = 1; ci1 = mx; cj0 = 1 - mbc; cj1 = 0; }
+            else { ci0 = 1; ci1 = mx; cj0 = my + 1; cj1 = my + mbc; }
+        } else {
+            ci0 = (which & 1) ? mx + 1 : 1 - mbc; ci1 = ci0 + mbc - 1;
+            cj0 = (which & 2) ? my + 1 : 1 - mbc; cj1 = cj0 + mbc - 1;
+        }
+        lo_i = ci0 + ti; hi_i = ci1 + ti; lo_j = cj0 + tj; hi_j = cj1 + tj;
+        if (lo_i < 1 || hi_i > mx || lo_j < 1 || hi_j > my)
+            return fail(HIPCLAW_EINVAL, "copy op reads outside the neighbour's interior%s");
+    } else if (type == hipclaw::GHOST_AVERAGE_CORNER) {
+        const int ic0 = (which & 1) ? mx + 1 : 1 - mbc, jc0 = (which & 2) ? my + 1 : 1 - mbc;
+        lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + mbc - 1) + ti;
+        lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + mbc - 1) + tj;
+        if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+            return fail(HIPCLAW_EINVAL, "average-corner op reads outside the fine patch%s");
+    } else if (type == hipclaw::GHOST_INTERP_CORNER) {
+        const int hb = mbc / 2;
+        const int ic0 = (which & 1) ? mx - hb + 1 : 1, jc0 = (which & 2) ? my - hb + 1 : 1;
+        lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + hb - 1) + ti;
+        lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + hb - 1) + tj;
+        if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
+            return fail(HIPCLAW_EINVAL, "interpolate-corner op writes outside the fine patch%s");
+    }
+    // average/interpolate faces test every cell against the neighbour's bounds in the kernel
+    return HIPCLAW_OK;
+}
+
 // The two launch layouts of ghost program k (hipclaw_set_ghost_layout 0 and 1)
 // from its validated ops.
 static int build_layouts(hipclaw_state_t* s, int k, const int* ops, int nops, const int* phase_start) {
@@ -795,26 +892,12 @@ static int build_layouts(hipclaw_state_t* s, int k, const int* ops, int nops, co
             add_physbc(phase_start[5], phase_start[6], nullptr);
         }
     }
-    if ((int) dev.size() > g.cap) {
-        hipFree(g.ops);
-        g.ops = nullptr;
-        g.cap = 0;
-        if (hipMalloc((void**) &g.ops, sizeof(GhostOp) * dev.size()) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost ops failed%s");
-        g.cap = (int) dev.size();
-    }
+    if (int rc = ensure_dev(g.ops, g.cap, (int) dev.size(), "ghost ops")) return rc;
     if (!dev.empty()) {
         HIP_TRY(hipMemcpyAsync(g.ops, dev.data(), sizeof(GhostOp) * dev.size(), hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
     }
-    if ((int) groups.size() > g.groups_cap) {
-        hipFree(g.groups);
-        g.groups = nullptr;
-        g.groups_cap = 0;
-        if (hipMalloc((void**) &g.groups, sizeof(int4) * groups.size()) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of ghost groups failed%s");
-        g.groups_cap = (int) groups.size();
-    }
+    if (int rc = ensure_dev(g.groups, g.groups_cap, (int) groups.size(), "ghost groups")) return rc;
     if (!groups.empty()) {
         HIP_TRY(hipMemcpyAsync(g.groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipStreamSynchronize(s->st));
@@ -843,79 +926,18 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
     drop_graph(s);
     for (int k = 0; k < 6; k++)
         if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
-    // validate every op on the host: a bad slot or translation would fault the GPU
-    const int mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
-    // the op types each phase may hold (bit mask over GHOST_*)
-    static const int phase_types[6] = {
-        (1 << hipclaw::GHOST_AVERAGE_FACE) | (1 << hipclaw::GHOST_AVERAGE_CORNER),
-        (1 << hipclaw::GHOST_COPY_FACE) | (1 << hipclaw::GHOST_COPY_CORNER),
-        1 << hipclaw::GHOST_PHYSBC,
-        (1 << hipclaw::GHOST_INTERP_FACE) | (1 << hipclaw::GHOST_INTERP_CORNER),
-        1 << hipclaw::GHOST_PHYSBC,
-        1 << hipclaw::GHOST_PHYSBC};
-    int ph = 0;
-    for (int o = 0; o < nops; o++) {
-        const int* op = ops + 8 * o;
-        const int type = op[0], dst = op[1], src = op[2], which = op[3], ti = op[4], tj = op[5];
-        while (o >= phase_start[ph + 1]) ph++;
-        // the reference's coarse/fine exchanges bound their face loops by mx
-        // (fclaw2d_clawpatch46_interpolate.f, fclaw2d_clawpatch46_average.f:
-        // average2coarse offsets children by mx in both directions)
-        if (mx != my && type >= hipclaw::GHOST_AVERAGE_FACE && type <= hipclaw::GHOST_INTERP_CORNER)
-            return fail(HIPCLAW_EINVAL, "coarse/fine ghost exchange requires mx = my%s");
-        if (type < 0 || type > hipclaw::GHOST_PHYSBC || !op_slot_ok(s, dst))
-            return fail(HIPCLAW_EINVAL, "ghost op with bad type or destination%s");
-        if (!(phase_types[ph] & (1 << type)))
-            return fail(HIPCLAW_EINVAL, "ghost op of a type its phase does not hold%s");
-        if (type == hipclaw::GHOST_PHYSBC) {
-            // per face: -1 (not a physical face), 1 (extrapolation) or 3 (wall).  The
-            // reference stops on mthbc 0 and 2 and leaves other codes' ghost cells
-            // unwritten (clawpack46_bc2_default.f:17-25, 41-43), which with two slabs
-            // would leave them to whatever the spare slab held.
-            for (int f = 0; f < 4; f++) {
-                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
-                if (b != -1 && b != 1 && b != 3)
-                    return fail(HIPCLAW_EINVAL,
-                                "physical boundary condition code must be 1 (extrapolation) or 3 (wall)%s");
-            }
-            continue;
+    // every op is checked on the host before anything of the state changes (a
+    // bad slot or translation would fault the GPU); program 0's face table is
+    // built in the same pass
+    FusedBuild fb;
+    if (k == 0) {
+        if (int rc = fused_check_and_faces(s, ops, nops, phase_start, fb)) return rc;
+    } else {
+        int ph = 0;
+        for (int o = 0; o < nops; o++) {
+            while (o >= phase_start[ph + 1]) ph++;
+            if (int rc = check_ghost_op(s, ops + 8 * o, ph)) return rc;
         }
-        if (!op_slot_ok(s, src) || which < 0 || which > 3)
-            return fail(HIPCLAW_EINVAL, "ghost op with bad source or face%s");
-        // every read index i + ti (same size) or 2*i - 1 + ti .. +1 (half size) must lie in the
-        // neighbour's array; check the extreme cells the op can touch
-        int lo_i, hi_i, lo_j, hi_j;  // range of the translated index
-        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
-            lo_i = 1 - mbc + ti; hi_i = mx + mbc + ti; lo_j = 1 - mbc + tj; hi_j = my + mbc + tj;
-            // copy reads the neighbour's interior band only
-            int ci0, ci1, cj0, cj1;
-            if (type == hipclaw::GHOST_COPY_FACE) {
-                if (which == 0) { ci0 = 1 - mbc; ci1 = 0; cj0 = 1; cj1 = my; }
-                else if (which == 1) { ci0 = mx + 1; ci1 = mx + mbc; cj0 = 1; cj1 = my; }
-                else if (which == 2) { ci0 = 1; ci1 = mx; cj0 = 1 - mbc; cj1 = 0; }
-                else { ci0 = 1; ci1 = mx; cj0 = my + 1; cj1 = my + mbc; }
-            } else {
-                ci0 = (which & 1) ? mx + 1 : 1 - mbc; ci1 = ci0 + mbc - 1;
-                cj0 = (which & 2) ? my + 1 : 1 - mbc; cj1 = cj0 + mbc - 1;
-            }
-            lo_i = ci0 + ti; hi_i = ci1 + ti; lo_j = cj0 + tj; hi_j = cj1 + tj;
-            if (lo_i < 1 || hi_i > mx || lo_j < 1 || hi_j > my)
-                return fail(HIPCLAW_EINVAL, "copy op reads outside the neighbour's interior%s");
-        } else if (type == hipclaw::GHOST_AVERAGE_CORNER) {
-            const int ic0 = (which & 1) ? mx + 1 : 1 - mbc, jc0 = (which & 2) ? my + 1 : 1 - mbc;
-            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + mbc - 1) + ti;
-            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + mbc - 1) + tj;
-            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
-                return fail(HIPCLAW_EINVAL, "average-corner op reads outside the fine patch%s");
-        } else if (type == hipclaw::GHOST_INTERP_CORNER) {
-            const int hb = mbc / 2;
-            const int ic0 = (which & 1) ? mx - hb + 1 : 1, jc0 = (which & 2) ? my - hb + 1 : 1;
-            lo_i = 2 * ic0 - 1 + ti; hi_i = 2 * (ic0 + hb - 1) + ti;
-            lo_j = 2 * jc0 - 1 + tj; hi_j = 2 * (jc0 + hb - 1) + tj;
-            if (lo_i < 1 - mbc || hi_i > mx + mbc || lo_j < 1 - mbc || hi_j > my + mbc)
-                return fail(HIPCLAW_EINVAL, "interpolate-corner op writes outside the fine patch%s");
-        }
-        // average/interpolate faces test every cell against the neighbour's bounds in the kernel
     }
     GhostProgram& g = s->gp[k];
     g.layouts_built = false;
@@ -926,7 +948,7 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         s->gp0_ops.assign(ops, ops + 8 * (size_t) nops);
         for (int p = 0; p < 7; p++) s->gp0_phase[p] = phase_start[p];
         s->gp0_set = true;
-        if (int rc = build_fused(s)) return rc;
+        if (int rc = fused_finish(s, ops, nops, fb)) return rc;
         // the fused step needs no launch layout of its own; the two-launch
         // layouts are built when one is asked for (ensure_layouts)
         if (s->ghost_layout == 2 && s->fused_ok) return HIPCLAW_OK;
@@ -939,14 +961,7 @@ int hipclaw_set_ghost_program(hipclaw_state_t* s, const int* ops, int nops, cons
 }
 
 static int upload(hipclaw_state_t* s, int*& dev, int& cap, const int* host, int n, const char* what) {
-    if (n > cap) {
-        hipFree(dev);
-        dev = nullptr;
-        cap = 0;
-        if (hipMalloc((void**) &dev, sizeof(int) * (size_t) n) != hipSuccess)
-            return fail(HIPCLAW_ENOMEM, "hipMalloc of %s failed", what);
-        cap = n;
-    }
+    if (int rc = ensure_dev(dev, cap, n, what)) return rc;
     if (n > 0) HIP_TRY(hipMemcpyAsync(dev, host, sizeof(int) * (size_t) n, hipMemcpyHostToDevice, s->st));
     return HIPCLAW_OK;
 }
@@ -1010,49 +1025,87 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
 // same-size ghost strips and corners straight from the neighbours (pull), and
 // the copy ops run only in the last step of an advance, so that the slab's
 // ghost cells are complete for readers.
-static int build_fused(hipclaw_state_t* s) {
+static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start,
+                                 FusedBuild& b) {
     using hipclaw::FaceRel;
-    s->fused_ok = false;
-    s->fsync = false;
-    s->pull_ok = false;
-    if (!s->gp0_set || s->npatch <= 0) return HIPCLAW_OK;
-    const int np = s->npatch, mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
-    const int nxp = mx + 2 * mbc;
-    const int nops = (int) s->gp0_ops.size() / 8;
-    const int* ops = s->gp0_ops.data();
+    const int np = s->npatch;
     const int TI = HIPCLAW_TIME_INTERP_SLOT;
-    // the three tables are built in place in the pinned staging buffer: face
-    // table, flat op list, pull offsets
-    const size_t frel_bytes = sizeof(FaceRel) * 4 * (size_t) np;
-    const size_t off_fops = (frel_bytes + 255) & ~(size_t) 255;
-    const size_t off_pull = off_fops + ((sizeof(GhostOp) * (size_t) nops + 255) & ~(size_t) 255);
-    char* stage = nullptr;
-    if (int rc = stage_reserve(s, off_pull + sizeof(int) * 8 * (size_t) np, &stage)) return rc;
-    FaceRel* frel = reinterpret_cast<FaceRel*>(stage);
-    for (size_t k = 0; k < 4 * (size_t) np; k++) frel[k] = FaceRel{hipclaw::FREL_NONE, -1, 0, 0, -1, 0, 0, 0};
-    std::vector<int> bcop(np, -1);
+    b.fusable = np > 0;
+    FaceRel* frel = nullptr;
+    if (b.fusable) {
+        const size_t frel_bytes = sizeof(FaceRel) * 4 * (size_t) np;
+        b.off_fops = (frel_bytes + 255) & ~(size_t) 255;
+        b.off_pull = b.off_fops + ((sizeof(GhostOp) * (size_t) nops + 255) & ~(size_t) 255);
+        if (int rc = stage_reserve(s, b.off_pull + sizeof(int) * 8 * (size_t) np, &b.stage)) return rc;
+        frel = reinterpret_cast<FaceRel*>(b.stage);
+        for (size_t k = 0; k < 4 * (size_t) np; k++) frel[k] = FaceRel{hipclaw::FREL_NONE, -1, 0, 0, -1, 0, 0, 0};
+        b.bcop.assign((size_t) np, -1);
+    }
+    int ph = 0;
     for (int o = 0; o < nops; o++) {
         const int* op = ops + 8 * o;
+        while (o >= phase_start[ph + 1]) ph++;
+        if (int rc = check_ghost_op(s, op, ph)) return rc;
+        if (!b.fusable) continue;
         const int type = op[0], dst = op[1], src = op[2], which = op[3];
-        if ((dst & TI) || (type != hipclaw::GHOST_PHYSBC && (src & TI))) return HIPCLAW_OK;
-        if (dst < 0 || dst >= np) return HIPCLAW_OK;
+        // a program with time-interpolated slots stays out of the fused step
+        if ((dst & TI) || (type != hipclaw::GHOST_PHYSBC && (src & TI))) {
+            b.fusable = false;
+            continue;
+        }
         if (type == hipclaw::GHOST_COPY_FACE) {
             frel[4 * dst + which] = FaceRel{hipclaw::FREL_SAME, src, op[4], op[5], -1, 0, 0, 0};
         } else if (type == hipclaw::GHOST_AVERAGE_FACE) {
             FaceRel& r = frel[4 * dst + which];
             if (r.rel != hipclaw::FREL_HALF) r = FaceRel{hipclaw::FREL_HALF, src, op[4], op[5], -1, 0, 0, 0};
             else { r.nb1 = src; r.ti1 = op[4]; r.tj1 = op[5]; }
+            b.nlite++;
         } else if (type == hipclaw::GHOST_INTERP_FACE) {
             frel[4 * dst + (which ^ 1)] = FaceRel{hipclaw::FREL_DOUBLE, src, op[4], op[5], -1, 0, 0, which};
+            b.nlite++;
         } else if (type == hipclaw::GHOST_PHYSBC) {
-            if (bcop[dst] >= 0 && ops[8 * bcop[dst] + 6] != op[6]) return HIPCLAW_OK;
-            bcop[dst] = o;
-            for (int f = 0; f < 4; f++) {
-                const int b = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
-                if (b == 1 || b == 3) frel[4 * dst + f] = FaceRel{hipclaw::FREL_PHYS, -1, 0, 0, -1, 0, 0, b};
+            // one op per patch in the flat list: every op of a patch must hold the same codes
+            const int prev = b.bcop[dst];
+            if (prev >= 0 && ops[8 * prev + 6] != op[6]) {
+                b.fusable = false;
+                continue;
             }
+            if (prev < 0) b.nlite++;
+            b.bcop[dst] = o;
+            for (int f = 0; f < 4; f++) {
+                const int c = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
+                if (c == 1 || c == 3) frel[4 * dst + f] = FaceRel{hipclaw::FREL_PHYS, -1, 0, 0, -1, 0, 0, c};
+            }
+        } else if (type != hipclaw::GHOST_COPY_CORNER) {
+            b.nlite++;   // average and interpolate corners
         }
     }
+    return HIPCLAW_OK;
+}
+
+// The fused step of program 0 (ghost layout 2).  Every ghost op becomes one
+// wave of ghost_flat that reads interior cells only: copies and averages do
+// already; interpolation and the physical BC recompute the ghost cells they
+// read from the face table (same-size neighbour, fine children, physical BC,
+// or for a physical BC beside a face filled from a coarser patch, that
+// interpolation).  The program is fused only if every such cell has one of
+// those sources (which the 2:1 balance of the forest guarantees) and it has no
+// time-interpolated slots; otherwise fused_ok stays false and the advance
+// keeps the two-launch layouts.  With two slabs the update also stages the
+// same-size ghost strips and corners straight from the neighbours (pull), and
+// the copy ops run only in the last step of an advance, so that the slab's
+// ghost cells are complete for readers.
+static int fused_finish(hipclaw_state_t* s, const int* ops, int nops, FusedBuild& b) {
+    using hipclaw::FaceRel;
+    s->fused_ok = false;
+    s->fsync = false;
+    s->pull_ok = false;
+    if (!b.fusable) return HIPCLAW_OK;
+    const int np = s->npatch, mx = s->cfg.mx, my = s->cfg.my, mbc = s->cfg.mbc;
+    const int nxp = mx + 2 * mbc;
+    const FaceRel* frel = reinterpret_cast<const FaceRel*>(b.stage);
+    // the face table travels while the host builds the other two
+    if (int rc = stage_copy(s, s->frel, s->frel_cap, frel, 4 * np, "face table")) return rc;
     // cells read from a coarse patch's face strips (interpolation) need a source
     // that is not itself an interpolation; a physical BC may read any face strip
     auto simple = [&](int c, int f) {
@@ -1060,12 +1113,34 @@ static int build_fused(hipclaw_state_t* s) {
         return r.rel == hipclaw::FREL_SAME || r.rel == hipclaw::FREL_PHYS ||
                (r.rel == hipclaw::FREL_HALF && r.nb1 >= 0);
     };
+    auto to_op = [&](const int* q) {
+        GhostOp op;
+        op.type = q[0]; op.dst = q[1]; op.src = q[2]; op.which = q[3];
+        op.ti = q[4]; op.tj = q[5]; op.bc = q[6]; op.reps = 1;
+        return op;
+    };
+    // flat op list: everything but the copies, then the copies; the same-size
+    // strips and corners the update stages from the neighbours (pull offsets in
+    // doubles must fit an int)
+    GhostOp* fops = reinterpret_cast<GhostOp*>(b.stage + b.off_fops);
+    int* pull = reinterpret_cast<int*>(b.stage + b.off_pull);
+    const bool pull_ok = (double) s->pstride * s->max_patches < 2.0e9;
+    if (pull_ok) std::fill(pull, pull + 8 * (size_t) np, 0);
+    int nl = 0, nf = b.nlite;
     for (int o = 0; o < nops; o++) {
         const int* op = ops + 8 * o;
-        const int type = op[0], c = op[2], which = op[3], ti = op[4], tj = op[5];
+        const int type = op[0];
+        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
+            fops[nf++] = to_op(op);
+            if (pull_ok) {
+                const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
+                pull[8 * (size_t) op[1] + (type == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
+            }
+            continue;
+        }
+        const int c = op[2], which = op[3], ti = op[4], tj = op[5];
         if (type == hipclaw::GHOST_INTERP_FACE) {
-            if (c < 0 || c >= np || !(frel[4 * c + which].rel == hipclaw::FREL_HALF && frel[4 * c + which].nb1 >= 0))
-                return HIPCLAW_OK;
+            if (!(frel[4 * c + which].rel == hipclaw::FREL_HALF && frel[4 * c + which].nb1 >= 0)) return HIPCLAW_OK;
             // whether the op writes the fine cells at either end of the face (the
             // kernel bounds its loop by mx, mx == my, and skips fine cells
             // outside the patch)
@@ -1074,58 +1149,24 @@ static int build_fused(hipclaw_state_t* s) {
                 const int fi = 2 * ic - 1 + ti, fj = 2 * jc - 1 + tj;
                 return !(fi < 1 - mbc || fi + 1 > mx + mbc || fj < 1 - mbc || fj + 1 > my + mbc);
             };
-            const bool lo = writes(1), hi = writes(mx);
             const int flo = which < 2 ? 2 : 0, fhi = which < 2 ? 3 : 1;
-            if ((lo && !simple(c, flo)) || (hi && !simple(c, fhi))) return HIPCLAW_OK;
+            if ((writes(1) && !simple(c, flo)) || (writes(mx) && !simple(c, fhi))) return HIPCLAW_OK;
         } else if (type == hipclaw::GHOST_INTERP_CORNER) {
-            if (c < 0 || c >= np || !simple(c, (which & 1) ? 1 : 0) || !simple(c, (which & 2) ? 3 : 2))
-                return HIPCLAW_OK;
+            if (!simple(c, (which & 1) ? 1 : 0) || !simple(c, (which & 2) ? 3 : 2)) return HIPCLAW_OK;
         } else if (type == hipclaw::GHOST_PHYSBC) {
             for (int f = 0; f < 4; f++)
                 if (frel[4 * op[1] + f].rel == hipclaw::FREL_NONE) return HIPCLAW_OK;
+            if (b.bcop[op[1]] != o) continue;   // once per patch
         }
+        fops[nl++] = to_op(op);
     }
-    // each table is copied out as soon as it is built, the copies running
-    // while the host builds the next one
-    if (int rc = stage_copy(s, s->frel, s->frel_cap, frel, 4 * np, "face table")) return rc;
-    // flat op list: everything but the copies, then the copies
-    GhostOp* fops = reinterpret_cast<GhostOp*>(stage + off_fops);
-    int nf = 0;
-    auto to_op = [&](int o) {
-        const int* q = ops + 8 * o;
-        GhostOp op;
-        op.type = q[0]; op.dst = q[1]; op.src = q[2]; op.which = q[3];
-        op.ti = q[4]; op.tj = q[5]; op.bc = q[6]; op.reps = 1;
-        return op;
-    };
-    for (int o = 0; o < nops; o++) {
-        const int type = ops[8 * o];
-        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) continue;
-        if (type == hipclaw::GHOST_PHYSBC && bcop[ops[8 * o + 1]] != o) continue;   // once per patch
-        fops[nf++] = to_op(o);
-    }
-    const int nlite = nf;
-    for (int o = 0; o < nops; o++) {
-        const int type = ops[8 * o];
-        if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) fops[nf++] = to_op(o);
-    }
+    if (nl != b.nlite) return fail(HIPCLAW_EINVAL, "internal: fused ghost op count mismatch%s");
     if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
-    // same-size strips and corners the update stages from the neighbours
-    // (offsets in doubles must fit an int)
-    int* pull = reinterpret_cast<int*>(stage + off_pull);
-    std::fill(pull, pull + 8 * (size_t) np, 0);
-    bool pull_ok = (double) s->pstride * s->max_patches < 2.0e9;
-    for (int o = 0; o < nops && pull_ok; o++) {
-        const int* op = ops + 8 * o;
-        if (op[0] != hipclaw::GHOST_COPY_FACE && op[0] != hipclaw::GHOST_COPY_CORNER) continue;
-        const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
-        pull[8 * (size_t) op[1] + (op[0] == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
-    }
     if (pull_ok)
         if (int rc = stage_copy(s, s->pull, s->pull_cap, pull, 8 * np, "pull offsets")) return rc;
     HIP_TRY(hipStreamSynchronize(s->st));
     s->hstage_busy = false;
-    s->nlite = nlite;
+    s->nlite = b.nlite;
     s->nall = nf;
     s->pull_ok = pull_ok;
     s->fused_ok = true;
