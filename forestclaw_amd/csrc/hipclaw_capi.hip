@@ -1164,8 +1164,8 @@ static int fused_finish(hipclaw_state_t* s, const int* ops, int nops, FusedBuild
     if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
     if (pull_ok)
         if (int rc = stage_copy(s, s->pull, s->pull_cap, pull, 8 * np, "pull offsets")) return rc;
-    HIP_TRY(hipStreamSynchronize(s->st));
-    s->hstage_busy = false;
+    // the copies run on while the host goes on (the next user of the staging
+    // buffer waits for them; launches on the stream come after them)
     s->nlite = b.nlite;
     s->nall = nf;
     s->pull_ok = pull_ok;
@@ -1188,10 +1188,7 @@ static int build_fused_sync(hipclaw_state_t* s) {
                 return HIPCLAW_OK;
     s->nfsync = 0;
     s->fpaired = false;
-    if (!sync) {
-        HIP_TRY(hipStreamSynchronize(s->st));
-        return HIPCLAW_OK;
-    }
+    if (!sync) return HIPCLAW_OK;
     // the faces each patch corrects and, in the FP32 flux form, the paired
     // sync list, built in the pinned staging buffer and copied out together
     const bool pair = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
@@ -1244,8 +1241,6 @@ static int build_fused_sync(hipclaw_state_t* s) {
             s->qcorr_cap = s->max_patches;
         }
     }
-    HIP_TRY(hipStreamSynchronize(s->st));
-    s->hstage_busy = false;
     s->fsync = sync;
     return HIPCLAW_OK;
 }
