@@ -142,19 +142,23 @@ def subcycle_summary(r, cells, value_nosub):
             "simulated_time_per_s_vs_no_subcycle": gain}
 
 
-def run_regrid(fa, prob, forest, q, repeats=3):
+def run_regrid(fa, prob, forest, q, repeats=5):
     """Cost of one regrid of the benchmark forest from the state q (the FP64
     form's state after the timed steps): device gradient tags, host forest
     adapt, device transfer onto the new leaves (copy / interpolate2fine /
-    average2coarse) with the new ghost-fill program, and the ghost fill of
-    every level.  Median over `repeats` regrids, each from the same state."""
+    average2coarse) with the new ghost-fill and time-sync programs, and the
+    ghost fill of every level.  The state is set up as the run loop sets it up
+    (time sync on, slabs for twice the leaves) and put back before each of
+    the `repeats` regrids: the first grows the tables and buffers a forest of
+    that size needs (`first_ms`), the others reuse them, as the regrids of a
+    run do.  Median over the repeats."""
     times = []
     run_maxlevel = prob.params.get("maxlevel") if forest.isolated else None
+    sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=fa.FP64, mthbc=prob.mthbc,
+                   grav=prob.grav, gamma=prob.gamma, max_patches=2 * forest.npatch, time_sync=True)
     for _ in range(repeats):
         f = fa.Forest(forest.nbx, forest.nby, forest.periodic, leaves=forest.leaves(),
                       isolated=forest.isolated)
-        sv = fa.Solver(prob.solver, prob.mx, prob.mx, precision=fa.FP64, mthbc=prob.mthbc,
-                       grav=prob.grav, gamma=prob.gamma, max_patches=f.npatch)
         sv.set_forest(f, prob.extent)
         sv.set_q(q)
         sv.sync()
@@ -176,12 +180,14 @@ def run_regrid(fa, prob, forest, q, repeats=3):
         times.append((t4 - t0, t1 - t0, t2 - t1, t4 - t2, f.npatch,
                       int(np.sum(kind == fa.REGRID_CHILD)), int(np.sum(kind == fa.REGRID_PARENT)),
                       parts["transfer"], parts["programs"], t4 - t3))
-        del sv
+    del sv
+    first = times[0][0]
     tot, tag, adapt, move, n, nchild, nparent, transfer, programs, fill = sorted(times)[len(times) // 2]
     return {"ms": round(tot * 1e3, 3), "tag_ms": round(tag * 1e3, 3),
             "adapt_ms": round(adapt * 1e3, 3), "transfer_and_ghost_fill_ms": round(move * 1e3, 3),
             "transfer_ms": round(transfer * 1e3, 3), "programs_ms": round(programs * 1e3, 3),
-            "ghost_fill_ms": round(fill * 1e3, 3),
+            "ghost_fill_ms": round(fill * 1e3, 3), "first_ms": round(first * 1e3, 3),
+            "repeats": repeats,
             "patches_before": forest.npatch, "patches_after": n, "new_children": nchild,
             "new_parents": nparent, "refine_threshold": prob.tag_threshold,
             "coarsen_threshold": 0.5 * prob.tag_threshold}
