@@ -118,7 +118,10 @@ int hipclaw_set_aux(hipclaw_state_t *s, const double *aux, int first, int count)
  *   0 average fine->coarse ghost, 1 copy same level, 2 physical BC on coarse
  *   levels, 3 interpolate coarse->fine ghost, 4 physical BC on fine levels,
  *   5 physical BC on all levels
- * (the order of fclaw2d_ghost_update_async, src/fclaw2d_ghost_fill.c:611-830). */
+ * (the order of fclaw2d_ghost_update_async, src/fclaw2d_ghost_fill.c:611-830).
+ * Every op is checked on the host first: a bad one returns HIPCLAW_EINVAL and
+ * leaves the installed program as it was.  The library keeps its own copy, so
+ * the caller may reuse ops on return. */
 int hipclaw_set_ghost_program(hipclaw_state_t *s, const int *ops, int nops,
                               const int *phase_start);
 

@@ -7,7 +7,9 @@
   * a forest of one patch (level 0: every ghost cell comes from the physical
     boundary conditions), FP64 against the oracle and FP32-bulk within 1e-5;
   * an advance of zero steps leaves q unchanged and reports a Courant
-    number of 0.
+    number of 0;
+  * a ghost program with a bad op is refused and leaves the installed
+    program (and the fused step's tables built from it) as it was.
 """
 import numpy as np
 import pytest
@@ -65,3 +67,34 @@ def test_zero_steps(gpu):
     before = sv.get_q()
     assert sv.advance(0.01, 0) == 0.0
     assert np.array_equal(sv.get_q(), before)
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32_BULK])
+@pytest.mark.parametrize("layout", [1, 2])
+def test_bad_ghost_program_leaves_the_installed_one(gpu, layout, precision):
+    """Ops are checked on the host before anything of the state changes: a
+    copy from a slot out of range, or one that reads outside its neighbour's
+    interior, in the middle of the list, raises and the next steps run the
+    program installed before, bit for bit as a state that never saw it."""
+    prob = P.dambreak_problem(2, 4, mx=16)
+    print("case layout", layout, "precision", precision, flush=True)
+    forest, q0, aux, sv = setup_problem(prob, precision=precision, time_sync=True)
+    _, _, _, ref = setup_problem(prob, precision=precision, time_sync=True)
+    for s in (sv, ref):
+        s.set_ghost_layout(layout)
+    ops, ps = forest.ghost_program(prob.mx, prob.my, 2, prob.mthbc)
+    assert ps[2] - ps[1] >= 2   # copies (phase 1)
+    k = (ps[1] + ps[2]) // 2
+    bad = ops.copy()
+    bad[k, 2] = forest.npatch + 3
+    with pytest.raises(fa.HipclawError, match="bad source"):
+        sv.set_ghost_program(bad, ps)
+    bad = ops.copy()
+    bad[k, 4] += 4 * prob.mx
+    with pytest.raises(fa.HipclawError, match="outside"):
+        sv.set_ghost_program(bad, ps)
+    dt = P.stable_dt(prob, forest, q0)
+    c1 = sv.advance(dt, 4)
+    c2 = ref.advance(dt, 4)
+    np.testing.assert_array_equal(sv.get_q(), ref.get_q())
+    assert c1 == c2
