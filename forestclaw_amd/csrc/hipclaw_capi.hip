@@ -149,8 +149,12 @@ struct hipclaw_state {
     // ghost_flat launch fills the rest and computes the time-sync increments,
     // which the next update adds (deferred through qcorr)
     bool fused_ok = false;
-    GhostOp* fops = nullptr;      // ops [0, nlite): not copies; [nlite, nall): copies
-    int fops_cap = 0, nlite = 0, nall = 0;
+    // ops: corners of every step (averages, interpolations) [0, fc0), the
+    // other ops of every step [fc0, fc0 + fbig), same-size face copies
+    // [.., + fcopy), corner copies [.., + fc1) (copies only in an advance's
+    // last step)
+    GhostOp* fops = nullptr;
+    int fops_cap = 0, fc0 = 0, fbig = 0, fcopy = 0, fc1 = 0;
     int* pull = nullptr;          // [slot][8] offsets of same-size ghost strips (StepArgs::pull)
     int pull_cap = 0;
     bool pull_ok = false;
@@ -215,6 +219,8 @@ static void* scratch_ptr(int* rc) {
 struct FusedBuild {
     bool fusable = false;
     int nlite = 0;              // ops of the flat list before the copies
+    int ncorner = 0;            // of them: average and interpolate corners
+    int ncopy = 0, ncopyc = 0;  // face and corner copies
     std::vector<int> bcop;      // per patch: the physical-BC op kept (the last), -1 if none
     char* stage = nullptr;      // face table, flat op list, pull offsets
     size_t off_fops = 0, off_pull = 0;
@@ -1055,6 +1061,7 @@ static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, c
         }
         if (type == hipclaw::GHOST_COPY_FACE) {
             frel[4 * dst + which] = FaceRel{hipclaw::FREL_SAME, src, op[4], op[5], -1, 0, 0, 0};
+            b.ncopy++;
         } else if (type == hipclaw::GHOST_AVERAGE_FACE) {
             FaceRel& r = frel[4 * dst + which];
             if (r.rel != hipclaw::FREL_HALF) r = FaceRel{hipclaw::FREL_HALF, src, op[4], op[5], -1, 0, 0, 0};
@@ -1076,8 +1083,11 @@ static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, c
                 const int c = (int) (signed char) ((op[6] >> (8 * f)) & 0xff);
                 if (c == 1 || c == 3) frel[4 * dst + f] = FaceRel{hipclaw::FREL_PHYS, -1, 0, 0, -1, 0, 0, c};
             }
-        } else if (type != hipclaw::GHOST_COPY_CORNER) {
+        } else if (type == hipclaw::GHOST_COPY_CORNER) {
+            b.ncopyc++;
+        } else {
             b.nlite++;   // average and interpolate corners
+            b.ncorner++;
         }
     }
     return HIPCLAW_OK;
@@ -1119,19 +1129,19 @@ static int fused_finish(hipclaw_state_t* s, const int* ops, int nops, FusedBuild
         op.ti = q[4]; op.tj = q[5]; op.bc = q[6]; op.reps = 1;
         return op;
     };
-    // flat op list: everything but the copies, then the copies; the same-size
-    // strips and corners the update stages from the neighbours (pull offsets in
-    // doubles must fit an int)
+    // flat op list: the corners of every step, its other ops, the face copies,
+    // the corner copies; the same-size strips and corners the update stages
+    // from the neighbours (pull offsets in doubles must fit an int)
     GhostOp* fops = reinterpret_cast<GhostOp*>(b.stage + b.off_fops);
     int* pull = reinterpret_cast<int*>(b.stage + b.off_pull);
     const bool pull_ok = (double) s->pstride * s->max_patches < 2.0e9;
     if (pull_ok) std::fill(pull, pull + 8 * (size_t) np, 0);
-    int nl = 0, nf = b.nlite;
+    int nc = 0, nl = b.ncorner, nf = b.nlite, nfc = b.nlite + b.ncopy;
     for (int o = 0; o < nops; o++) {
         const int* op = ops + 8 * o;
         const int type = op[0];
         if (type == hipclaw::GHOST_COPY_FACE || type == hipclaw::GHOST_COPY_CORNER) {
-            fops[nf++] = to_op(op);
+            fops[type == hipclaw::GHOST_COPY_FACE ? nf++ : nfc++] = to_op(op);
             if (pull_ok) {
                 const long long d = ((long long) op[2] - op[1]) * s->pstride + op[4] + (long long) op[5] * nxp;
                 pull[8 * (size_t) op[1] + (type == hipclaw::GHOST_COPY_FACE ? 0 : 4) + op[3]] = (int) d;
@@ -1158,16 +1168,20 @@ static int fused_finish(hipclaw_state_t* s, const int* ops, int nops, FusedBuild
                 if (frel[4 * op[1] + f].rel == hipclaw::FREL_NONE) return HIPCLAW_OK;
             if (b.bcop[op[1]] != o) continue;   // once per patch
         }
-        fops[nl++] = to_op(op);
+        const bool corner = type == hipclaw::GHOST_AVERAGE_CORNER || type == hipclaw::GHOST_INTERP_CORNER;
+        fops[corner ? nc++ : nl++] = to_op(op);
     }
-    if (nl != b.nlite) return fail(HIPCLAW_EINVAL, "internal: fused ghost op count mismatch%s");
-    if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nf, "fused ghost ops")) return rc;
+    if (nc != b.ncorner || nl != b.nlite || nf != b.nlite + b.ncopy || nfc != nf + b.ncopyc)
+        return fail(HIPCLAW_EINVAL, "internal: fused ghost op count mismatch%s");
+    if (int rc = stage_copy(s, s->fops, s->fops_cap, fops, nfc, "fused ghost ops")) return rc;
     if (pull_ok)
         if (int rc = stage_copy(s, s->pull, s->pull_cap, pull, 8 * np, "pull offsets")) return rc;
     // the copies run on while the host goes on (the next user of the staging
     // buffer waits for them; launches on the stream come after them)
-    s->nlite = b.nlite;
-    s->nall = nf;
+    s->fc0 = b.ncorner;
+    s->fbig = b.nlite - b.ncorner;
+    s->fcopy = b.ncopy;
+    s->fc1 = b.ncopyc;
     s->pull_ok = pull_ok;
     s->fused_ok = true;
     return build_fused_sync(s);
@@ -1294,7 +1308,9 @@ static int ghost_flat_async(hipclaw_state_t* s, bool full, bool sync) {
     f.g.qti = nullptr;
     f.g.pstride = s->pstride;
     f.g.mx = s->cfg.mx; f.g.my = s->cfg.my; f.g.mbc = s->cfg.mbc; f.g.meqn = s->cfg.meqn;
-    f.nghost = full ? s->nall : s->nlite;
+    f.ncorner0 = s->fc0;
+    f.nbig = full ? s->fbig + s->fcopy : s->fbig;
+    f.ncorner1 = full ? s->fc1 : 0;
     f.frel = s->frel;
     f.sops = !sync ? nullptr : s->fpaired ? s->fsops : s->sp[0].ops;
     f.nsync = !sync ? 0 : s->fpaired ? s->nfsync : (int) s->sp0_ops.size() / 8;

@@ -81,11 +81,13 @@ struct FaceRel {
                          // coarse patch's face toward this one
 };
 
-// One launch of the fused ghost fill and time-sync increments, one wave per
-// op: ghost ops [0, nghost) of g.ops, then sync ops [0, nsync) of sops.
+// One launch of the fused ghost fill and time-sync increments: ghost ops
+// [ncorner0, ncorner0 + nbig) of g.ops one wave each, the corner ops
+// [0, ncorner0) and [ncorner0 + nbig, ... + ncorner1) four to a wave, then
+// sync ops [0, nsync) of sops one wave each.
 struct FlatArgs {
     GhostArgs g;
-    int nghost;
+    int ncorner0, nbig, ncorner1;
     const FaceRel* frel;       // [slot][4]
     const int* sops;           // [nsync][8] as SyncArgs::ops
     int nsync;

@@ -42,6 +42,9 @@ __device__ __forceinline__ void interp_cell(const double* qc, double* qf, int m,
 // so each lane issues the loads of up to U elements before their stores: the
 // source and destination slabs are the same allocation, and without this the
 // compiler must wait for every store before the next load.
+// W lanes (t = 0..W-1) work on the op: a whole wave, or a quarter of one for
+// the corner ops of the fused launch.
+template <int W = 64>
 __device__ __forceinline__ void ghost_exchange_op(const GhostArgs& g, const GhostOp& op, int t) {
     const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
     const int nxp = mx + 2 * mbc, nyp = my + 2 * mbc;
@@ -56,12 +59,12 @@ __device__ __forceinline__ void ghost_exchange_op(const GhostArgs& g, const Ghos
         const bool face = op.type == GHOST_COPY_FACE;
         const int len = (op.which < 2) ? my : mx;
         const int n = face ? meqn * mbc * len : meqn * mbc * mbc;
-        for (int base = t; base < n; base += 64 * U) {
+        for (int base = t; base < n; base += W * U) {
             double v[U];
             long long di[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int e = base + 64 * u;
+                const int e = base + W * u;
                 di[u] = -1;
                 if (e >= n) continue;
                 int m, i1, j1;
@@ -91,12 +94,12 @@ __device__ __forceinline__ void ghost_exchange_op(const GhostArgs& g, const Ghos
         const bool face = op.type == GHOST_AVERAGE_FACE;
         const int len = (op.which < 2) ? my : mx;
         const int n = face ? meqn * mbc * len : meqn * mbc * mbc;
-        for (int base = t; base < n; base += 64 * U) {
+        for (int base = t; base < n; base += W * U) {
             double v[U];
             long long di[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int e = base + 64 * u;
+                const int e = base + W * u;
                 di[u] = -1;
                 if (e >= n) continue;
                 int m, ic, jc;
@@ -133,12 +136,12 @@ __device__ __forceinline__ void ghost_exchange_op(const GhostArgs& g, const Ghos
         const int hb = mbc / 2;
         const int n = face ? meqn * hb * mx : meqn * hb * hb;  // the reference bounds the face loop by mx
         constexpr int UI = 2;
-        for (int base = t; base < n; base += 64 * UI) {
+        for (int base = t; base < n; base += W * UI) {
             double c[UI], gx[UI], gy[UI];
             long long d0[UI];
 #pragma unroll
             for (int u = 0; u < UI; u++) {
-                const int e = base + 64 * u;
+                const int e = base + W * u;
                 d0[u] = -1;
                 if (e >= n) continue;
                 int m, ic, jc;
@@ -349,6 +352,7 @@ __device__ double fused_cell(const GhostArgs& g, const FaceRel* __restrict__ fre
 
 // Interpolation op (dst = fine, src = coarse) as in ghost_exchange_op, its
 // coarse-side reads through fused_cell_simple.
+template <int W = 64>
 __device__ void fused_interp_op(const GhostArgs& g, const FaceRel* __restrict__ frel, const GhostOp& op,
                                 int t) {
     const int mx = g.mx, my = g.my, mbc = g.mbc, meqn = g.meqn;
@@ -357,7 +361,7 @@ __device__ void fused_interp_op(const GhostArgs& g, const FaceRel* __restrict__ 
     const bool face = op.type == GHOST_INTERP_FACE;
     const int hb = mbc / 2;
     const int n = face ? meqn * hb * mx : meqn * hb * hb;
-    for (int e = t; e < n; e += 64) {
+    for (int e = t; e < n; e += W) {
         int m, ic, jc;
         if (face) {
             m = e / (hb * mx);
@@ -472,25 +476,46 @@ __device__ void fused_sync_op(const FlatArgs& f, const int* __restrict__ op, int
     }
 }
 
-// The fused launch, one wave per op, no op depending on another: ghost ops
-// (copies and averages read interior cells; interpolations and physical BCs
-// recompute the ghost cells they read), then time-sync increments.  The
-// interior cells keep their values before the time sync, which every ghost op
-// reads (the reference fills ghost cells before it synchronises,
-// src/fclaw2d_advance.c:330-341); the next update adds the increments to the
-// cells it stages.
-constexpr int FLAT_OPW = 4;
+// The fused launch, no op depending on another: ghost ops (copies and
+// averages read interior cells; interpolations and physical BCs recompute the
+// ghost cells they read), then time-sync increments.  A face, physical-BC or
+// sync op takes a wave; the corner ops, a few cells each, take a quarter of
+// one.  The interior cells keep their values before the time sync, which
+// every ghost op reads (the reference fills ghost cells before it
+// synchronises, src/fclaw2d_advance.c:330-341); the next update adds the
+// increments to the cells it stages.
+constexpr int FLAT_OPW = 4;           // waves per workgroup
+constexpr int FLAT_CORNER_LANES = 16; // lanes of one corner op
+constexpr int FLAT_CORNERS = 64 / FLAT_CORNER_LANES;
+
+__device__ __forceinline__ int flat_corner_waves(int n) { return (n + FLAT_CORNERS - 1) / FLAT_CORNERS; }
+
 __global__ void __launch_bounds__(64 * FLAT_OPW) ghost_flat(FlatArgs f) {
-    const int o = blockIdx.x * FLAT_OPW + (int) (threadIdx.x >> 6);
-    const int t = threadIdx.x & 63;
-    if (o < f.nghost) {
-        const GhostOp op = f.g.ops[o];
-        if (op.type == GHOST_INTERP_FACE || op.type == GHOST_INTERP_CORNER) fused_interp_op(f.g, f.frel, op, t);
-        else if (op.type == GHOST_PHYSBC) fused_physbc_op(f.g, f.frel, op, t);
-        else ghost_exchange_op(f.g, op, t);
-    } else if (o - f.nghost < f.nsync) {
-        fused_sync_op(f, f.sops + 8 * (o - f.nghost), t);
+    int w = blockIdx.x * FLAT_OPW + (int) (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w < f.nbig) {
+        const GhostOp op = f.g.ops[f.ncorner0 + w];
+        if (op.type == GHOST_INTERP_FACE || op.type == GHOST_INTERP_CORNER) fused_interp_op(f.g, f.frel, op, lane);
+        else if (op.type == GHOST_PHYSBC) fused_physbc_op(f.g, f.frel, op, lane);
+        else ghost_exchange_op(f.g, op, lane);
+        return;
     }
+    w -= f.nbig;
+    const int nw0 = flat_corner_waves(f.ncorner0), nw1 = flat_corner_waves(f.ncorner1);
+    if (w < nw0 + nw1) {
+        // corner ops [0, ncorner0) and [ncorner0 + nbig, ... + ncorner1)
+        const bool first = w < nw0;
+        const int k = (first ? w : w - nw0) * FLAT_CORNERS + lane / FLAT_CORNER_LANES;
+        if (k < (first ? f.ncorner0 : f.ncorner1)) {
+            const GhostOp op = f.g.ops[(first ? 0 : f.ncorner0 + f.nbig) + k];
+            const int t = lane % FLAT_CORNER_LANES;
+            if (op.type == GHOST_INTERP_CORNER) fused_interp_op<FLAT_CORNER_LANES>(f.g, f.frel, op, t);
+            else ghost_exchange_op<FLAT_CORNER_LANES>(f.g, op, t);
+        }
+        return;
+    }
+    w -= nw0 + nw1;
+    if (w < f.nsync) fused_sync_op(f, f.sops + 8 * w, lane);
 }
 
 // The pending increments added into the slab (after the last step of an
@@ -528,7 +553,8 @@ __global__ void apply_qcorr(double* __restrict__ q, const double* __restrict__ q
 }
 
 int launch_ghost_flat(const FlatArgs& f, hipStream_t st) {
-    const int n = f.nghost + f.nsync;
+    const int n = f.nbig + (f.ncorner0 + FLAT_CORNERS - 1) / FLAT_CORNERS +
+                  (f.ncorner1 + FLAT_CORNERS - 1) / FLAT_CORNERS + f.nsync;
     if (n <= 0) return 0;
     hipLaunchKernelGGL(ghost_flat, dim3((n + FLAT_OPW - 1) / FLAT_OPW), dim3(64 * FLAT_OPW), 0, st, f);
     return (int) hipGetLastError();
