@@ -1019,18 +1019,9 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
 
 }  // extern "C"
 
-// The fused step of program 0 (ghost layout 2).  Every ghost op becomes one
-// wave of ghost_flat that reads interior cells only: copies and averages do
-// already; interpolation and the physical BC recompute the ghost cells they
-// read from the face table (same-size neighbour, fine children, physical BC,
-// or for a physical BC beside a face filled from a coarser patch, that
-// interpolation).  The program is fused only if every such cell has one of
-// those sources (which the 2:1 balance of the forest guarantees) and it has no
-// time-interpolated slots; otherwise fused_ok stays false and the advance
-// keeps the two-launch layouts.  With two slabs the update also stages the
-// same-size ghost strips and corners straight from the neighbours (pull), and
-// the copy ops run only in the last step of an advance, so that the slab's
-// ghost cells are complete for readers.
+// Pass one over program 0: every op checked (nothing of the state changes
+// before all have passed), the face table filled in the staging buffer, and
+// the ops of each part of the flat list counted.
 static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, const int* phase_start,
                                  FusedBuild& b) {
     using hipclaw::FaceRel;
@@ -1094,7 +1085,8 @@ static int fused_check_and_faces(hipclaw_state_t* s, const int* ops, int nops, c
 }
 
 // The fused step of program 0 (ghost layout 2).  Every ghost op becomes one
-// wave of ghost_flat that reads interior cells only: copies and averages do
+// wave of ghost_flat (a corner op a quarter of one) that reads interior cells
+// only: copies and averages do
 // already; interpolation and the physical BC recompute the ghost cells they
 // read from the face table (same-size neighbour, fine children, physical BC,
 // or for a physical BC beside a face filled from a coarser patch, that
