@@ -162,6 +162,8 @@ struct hipclaw_state {
     int frel_cap = 0;
     int* cmask = nullptr;
     int cmask_cap = 0;
+    int* qslots = nullptr;        // the patches with a corrected face (apply_qcorr)
+    int qslots_cap = 0, nqslots = 0;
     double* qcorr = nullptr;
     long long cstride = 0;
     int qcorr_cap = 0;            // patches qcorr holds
@@ -611,6 +613,7 @@ void hipclaw_destroy(hipclaw_state_t* s) {
     hipFree(s->iwork);
     hipFree(s->frel);
     hipFree(s->cmask);
+    hipFree(s->qslots);
     hipFree(s->qcorr);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
@@ -1193,14 +1196,16 @@ static int build_fused_sync(hipclaw_state_t* s) {
             if (ops[8 * o + 1] < 0 || ops[8 * o + 1] >= np || ops[8 * o + 2] < 0 || ops[8 * o + 2] >= np)
                 return HIPCLAW_OK;
     s->nfsync = 0;
+    s->nqslots = 0;
     s->fpaired = false;
     if (!sync) return HIPCLAW_OK;
     // the faces each patch corrects and, in the FP32 flux form, the paired
     // sync list, built in the pinned staging buffer and copied out together
     const bool pair = s->cfg.precision == HIPCLAW_FP32_BULK && s->conservative;
     const size_t off_fs = ((sizeof(int) * (size_t) np) + 255) & ~(size_t) 255;
+    const size_t off_qs = off_fs + (((pair ? sizeof(int) * 8 * (size_t) nso : 0) + 255) & ~(size_t) 255);
     char* stage = nullptr;
-    if (int rc = stage_reserve(s, off_fs + (pair ? sizeof(int) * 8 * (size_t) nso : 0), &stage)) return rc;
+    if (int rc = stage_reserve(s, off_qs + sizeof(int) * (size_t) np, &stage)) return rc;
     int* cm = reinterpret_cast<int*>(stage);
     std::fill(cm, cm + np, 0);
     for (int o = 0; o < nso; o++) {
@@ -1235,6 +1240,14 @@ static int build_fused_sync(hipclaw_state_t* s) {
         s->fpaired = true;
     }
     if (int rc = stage_copy(s, s->cmask, s->cmask_cap, cm, np, "sync face masks")) return rc;
+    {
+        int* qs = reinterpret_cast<int*>(stage + off_qs);
+        int nq = 0;
+        for (int p = 0; p < np; p++)
+            if (cm[p] & 0xf) qs[nq++] = p;
+        if (int rc = stage_copy(s, s->qslots, s->qslots_cap, qs, nq, "corrected patches")) return rc;
+        s->nqslots = nq;
+    }
     {
         const long long cstride = 4LL * s->cfg.meqn * mx;
         if (!s->qcorr || s->cstride != cstride || s->qcorr_cap < s->max_patches) {
@@ -1830,9 +1843,9 @@ static int advance_launches(hipclaw_state_t* s, double dt, int nsteps, bool time
             if (int rc = ghost_flat_async(s, !pull || n == nsteps - 1, s->fsync)) return rc;
         }
         if (s->fsync && nsteps > 0) {
-            int e = hipclaw::launch_apply_qcorr(s->q[s->cur], s->qcorr, s->cmask, s->npatch, s->pstride,
-                                                s->cstride, s->cfg.mx, s->cfg.my, s->cfg.mbc, s->cfg.meqn,
-                                                s->st);
+            int e = hipclaw::launch_apply_qcorr(s->q[s->cur], s->qcorr, s->cmask, s->qslots, s->nqslots,
+                                                s->pstride, s->cstride, s->cfg.mx, s->cfg.my, s->cfg.mbc,
+                                                s->cfg.meqn, s->st);
             if (e) return hipfail((hipError_t) e, "apply_qcorr launch");
         }
         return HIPCLAW_OK;

@@ -120,8 +120,9 @@ int launch_ghost_groups(const GhostArgs& g, const int4* groups, int ngroups, hip
 // The fused ghost fill (+ time-sync increments into qinc) of one step.
 int launch_ghost_flat(const FlatArgs& f, hipStream_t st);
 // qcorr -> the interior edge cells of q (slots 0..n-1, faces in cmask).
-int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, int n, long long pstride,
-                       long long cstride, int mx, int my, int mbc, int meqn, hipStream_t st);
+int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, const int* slots, int n,
+                       long long pstride, long long cstride, int mx, int my, int mbc, int meqn,
+                       hipStream_t st);
 int launch_step_fp64(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
 int launch_step_fp32(int solver, int tile, int waves, const StepArgs& a, const StepLaunch& st);
 // The default waves per workgroup of a tile (0 where there is no choice).

@@ -519,16 +519,18 @@ __global__ void __launch_bounds__(64 * FLAT_OPW) ghost_flat(FlatArgs f) {
 }
 
 // The pending increments added into the slab (after the last step of an
-// advance, so the state is the synchronised one), one thread per edge cell in
-// the update's order: a corner cell belongs to its x-face when that face is
+// advance, so the state is the synchronised one), one thread per edge cell of
+// the patches listed in slots (those with a corrected face), in the update's
+// order: a corner cell belongs to its x-face when that face is
 // corrected, and takes both faces' increments in the reference's order.
 __global__ void apply_qcorr(double* __restrict__ q, const double* __restrict__ qcorr,
-                            const int* __restrict__ cmask, int n, long long pstride, long long cstride,
-                            int mx, int my, int mbc, int meqn) {
+                            const int* __restrict__ cmask, const int* __restrict__ slots, int n,
+                            long long pstride, long long cstride, int mx, int my, int mbc, int meqn) {
     const int per = 4 * meqn * mx;
     const long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long long) n * per) return;
-    const int slot = (int) (e / per), r = (int) (e - (long long) slot * per);
+    const int k = (int) (e / per), r = (int) (e - (long long) k * per);
+    const int slot = slots[k];
     const int fc = r / (meqn * mx), r2 = r - fc * meqn * mx, m = r2 / mx, c = r2 - m * mx + 1;
     const int mask = cmask[slot];
     if (!((mask >> fc) & 1)) return;
@@ -560,12 +562,13 @@ int launch_ghost_flat(const FlatArgs& f, hipStream_t st) {
     return (int) hipGetLastError();
 }
 
-int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, int n, long long pstride,
-                       long long cstride, int mx, int my, int mbc, int meqn, hipStream_t st) {
+int launch_apply_qcorr(double* q, const double* qcorr, const int* cmask, const int* slots, int n,
+                       long long pstride, long long cstride, int mx, int my, int mbc, int meqn,
+                       hipStream_t st) {
     const long long total = (long long) n * 4 * meqn * mx;
     if (total <= 0) return 0;
     hipLaunchKernelGGL(apply_qcorr, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, st, q, qcorr, cmask,
-                       n, pstride, cstride, mx, my, mbc, meqn);
+                       slots, n, pstride, cstride, mx, my, mbc, meqn);
     return (int) hipGetLastError();
 }
 
