@@ -932,7 +932,6 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
         return fail(HIPCLAW_EINVAL, "hipclaw_set_ghost_program: bad arguments%s");
     if (phase_start[0] != 0 || phase_start[6] != nops)
         return fail(HIPCLAW_EINVAL, "phase_start must run from 0 to nops%s");
-    drop_graph(s);
     for (int k = 0; k < 6; k++)
         if (phase_start[k + 1] < phase_start[k]) return fail(HIPCLAW_EINVAL, "phase_start not monotone%s");
     // every op is checked on the host before anything of the state changes (a
@@ -948,6 +947,8 @@ int hipclaw_set_ghost_program_k(hipclaw_state_t* s, int k, const int* ops, int n
             if (int rc = check_ghost_op(s, ops + 8 * o, ph)) return rc;
         }
     }
+    // a captured graph holds the tables about to be replaced
+    drop_graph(s);
     GhostProgram& g = s->gp[k];
     g.layouts_built = false;
     g.set = true;
@@ -980,7 +981,6 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     if (!s || k < 0 || k >= HIPCLAW_MAX_PROGRAMS || nops < 0 || (nops > 0 && !ops) ||
         nresets < 0 || (nresets > 0 && !resets))
         return fail(HIPCLAW_EINVAL, "hipclaw_set_sync_program: bad arguments%s");
-    drop_graph(s);
     if (!s->reg && (nops > 0 || nresets > 0))
         return fail(HIPCLAW_EINVAL, "the state was created without time_sync%s");
     // validate on the host and group by destination (groups must be contiguous)
@@ -1002,6 +1002,7 @@ int hipclaw_set_sync_program_k(hipclaw_state_t* s, int k, const int* ops, int no
     for (int r = 0; r < nresets; r++)
         if (resets[2 * r] < 0 || resets[2 * r] >= s->npatch || resets[2 * r + 1] < 0 || resets[2 * r + 1] > 3)
             return fail(HIPCLAW_EINVAL, "register reset with bad slot or face%s");
+    drop_graph(s);
     SyncProgram& y = s->sp[k];
     const int ngroups = (int) start.size() - 1;
     if (int rc = upload(s, y.ops, y.ops_cap, ops, 8 * nops, "sync ops")) return rc;

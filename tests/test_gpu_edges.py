@@ -9,7 +9,8 @@
   * an advance of zero steps leaves q unchanged and reports a Courant
     number of 0;
   * a ghost program with a bad op is refused and leaves the installed
-    program (and the fused step's tables built from it) as it was.
+    program (and the fused step's tables built from it) as it was, and a
+    graph captured before stays valid.
 """
 import numpy as np
 import pytest
@@ -98,3 +99,14 @@ def test_bad_ghost_program_leaves_the_installed_one(gpu, layout, precision):
     c2 = ref.advance(dt, 4)
     np.testing.assert_array_equal(sv.get_q(), ref.get_q())
     assert c1 == c2
+    # a refused program leaves a captured graph in place
+    for s in (sv, ref):
+        s.capture_graph(dt, 2)
+    with pytest.raises(fa.HipclawError, match="bad source"):
+        bad = ops.copy()
+        bad[k, 2] = forest.npatch + 3
+        sv.set_ghost_program(bad, ps)
+    for s in (sv, ref):
+        s.replay_graph()
+        s.sync()
+    np.testing.assert_array_equal(sv.get_q(), ref.get_q())
